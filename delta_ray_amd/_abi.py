@@ -1,0 +1,155 @@
+"""ctypes mirror of include/delta_ray.h and the loader of the HIP library.
+
+The library is built in-tree by ``__graft_entry__.build()`` into
+``delta_ray_amd/libdelta_ray_amd.so``.  There is no CPU fallback: when the
+library (or a GPU) is missing, ``load_library`` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdelta_ray_amd.so")
+LTC_PATH = os.path.join(_HERE, "data", "ltc_tabM.f32")
+
+DR_OK = 0
+
+DR_QUIRK_JITTER_OVER_SPP = 1 << 0
+DR_QUIRK_TEXEL_SWAP = 1 << 1
+DR_QUIRK_NORMAL_DN0 = 1 << 2
+DR_QUIRK_DP_LAST = 1 << 3
+DR_SKIP_INTERIOR = 1 << 8
+DR_SKIP_SECONDARY_EDGES = 1 << 9
+DR_SKIP_PRIMARY_EDGES = 1 << 10
+DR_WRITE_DX_IMAGE = 1 << 11
+DR_FLAGS_PARITY = (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP |
+                   DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST)
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class dr_camera(ctypes.Structure):
+    _fields_ = [("cam_to_world", ctypes.c_float * 16),
+                ("world_to_cam", ctypes.c_float * 16),
+                ("sample_to_cam", ctypes.c_float * 16),
+                ("cam_to_sample", ctypes.c_float * 16),
+                ("fov_factor", ctypes.c_float),
+                ("aspect_ratio", ctypes.c_float),
+                ("clip_near", ctypes.c_float),
+                ("fisheye", ctypes.c_int)]
+
+
+class dr_texture(ctypes.Structure):
+    _fields_ = [("data", _fp), ("height", ctypes.c_int), ("width", ctypes.c_int),
+                ("channels", ctypes.c_int)]
+
+
+class dr_material(ctypes.Structure):
+    _fields_ = [("diffuse_reflectance", dr_texture),
+                ("specular_reflectance", dr_texture),
+                ("roughness", dr_texture),
+                ("diffuse_uv_scale", ctypes.c_float * 2),
+                ("specular_uv_scale", ctypes.c_float * 2),
+                ("roughness_uv_scale", ctypes.c_float * 2),
+                ("two_sided", ctypes.c_int)]
+
+
+class dr_shape(ctypes.Structure):
+    _fields_ = [("vertices", _fp), ("num_vertices", ctypes.c_int),
+                ("indices", _ip), ("num_triangles", ctypes.c_int),
+                ("uvs", _fp), ("normals", _fp),
+                ("material_id", ctypes.c_int), ("light_id", ctypes.c_int)]
+
+
+class dr_light(ctypes.Structure):
+    _fields_ = [("shape_id", ctypes.c_int), ("intensity", ctypes.c_float * 3)]
+
+
+class dr_scene_desc(ctypes.Structure):
+    _fields_ = [("camera", dr_camera),
+                ("shapes", ctypes.POINTER(dr_shape)), ("num_shapes", ctypes.c_int),
+                ("materials", ctypes.POINTER(dr_material)), ("num_materials", ctypes.c_int),
+                ("lights", ctypes.POINTER(dr_light)), ("num_lights", ctypes.c_int)]
+
+
+class dr_render_options(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("num_samples", ctypes.c_int), ("max_bounces", ctypes.c_int),
+                ("seed", ctypes.c_uint64), ("flags", ctypes.c_uint32),
+                ("num_primary_edge_samples", ctypes.c_int64)]
+
+
+class dr_stats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_uint64), ("nodes_visited", ctypes.c_uint64),
+                ("tris_visited", ctypes.c_uint64), ("secondary_edges", ctypes.c_uint64),
+                ("primary_edge_samples", ctypes.c_uint64)]
+
+
+class dr_scene_info(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_int), ("num_vertices", ctypes.c_int),
+                ("bvh_nodes", ctypes.c_int), ("num_edges", ctypes.c_int),
+                ("num_primary_edges", ctypes.c_int),
+                ("silhouette_tree_nodes", ctypes.c_int),
+                ("normal_tree_nodes", ctypes.c_int), ("build_ms", ctypes.c_double)]
+
+
+# entry points declared in include/delta_ray.h: (name, restype, argtypes)
+ENTRY_POINTS = [
+    ("dr_scene_create", ctypes.c_int, [ctypes.POINTER(dr_scene_desc), ctypes.POINTER(ctypes.c_void_p)]),
+    ("dr_scene_destroy", None, [ctypes.c_void_p]),
+    ("dr_scene_set_camera", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_camera)]),
+    ("dr_scene_grad_size", ctypes.c_size_t, [ctypes.c_void_p]),
+    ("dr_render", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_render_options), ctypes.c_void_p,
+                                 ctypes.c_void_p]),
+    ("dr_render_adjoint", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_render_options),
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
+    ("dr_intersect", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                    ctypes.c_int, ctypes.c_void_p]),
+    ("dr_scene_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(dr_stats)]),
+    ("dr_scene_get_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_scene_info)]),
+    ("dr_last_error", ctypes.c_char_p, []),
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library; raise if it is not built."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("delta_ray_amd: %s is not built (run __graft_entry__.build())" % path)
+    lib = ctypes.CDLL(path)
+    for name, res, args in ENTRY_POINTS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def check(lib, status, what):
+    if status != DR_OK:
+        msg = lib.dr_last_error()
+        raise RuntimeError("%s failed (%d): %s" % (what, status, msg.decode() if msg else "?"))
+
+
+def fptr(a):
+    return a.ctypes.data_as(_fp) if a is not None else ctypes.cast(None, _fp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip) if a is not None else ctypes.cast(None, _ip)
+
+
+def f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))

@@ -1,0 +1,244 @@
+"""Synthetic scenes of the benchmark workloads, built with seeded numpy RNGs.
+
+single_triangle  1 diffuse triangle (albedo 0.5), vertices (x, y) ~ U[-1,1]^2,
+                 z = 0; camera at (0,0,5) looking at the origin, fov 45; one
+                 triangle area light at z = 7 facing -z, intensity 20
+                 (cf. the reference's test/test_single_triangle.py:28-41).
+shadow           ground quad [-5,5]^2 at y = 0, a blocker triangle with
+                 (x, z) ~ U[-1,1]^2, y ~ U[1,2]; a 0.2 x 0.2 light quad at
+                 y = 5 facing -y, intensity 10; camera (0,2,-5) -> origin
+                 (cf. test/test_shadow.py:12-47).
+heightfield      128 x 128 grid on [-5,5]^2 with heights 0.2 * fBm value
+                 noise (6 octaves, lacunarity 2, gain 0.5), angle-weighted
+                 vertex normals, uvs = grid coordinates, Blinn-Phong
+                 material with F0 ~ U[0.04, 1], diffuse 0.5 (1 - F0), a
+                 128 x 128 roughness texture ~ U[0.05, 0.5]; two 1 x 1 light
+                 quads (intensity 50) at (+-3, 6, 0) facing the origin;
+                 camera (0,3,-6) -> origin, fov 45.
+
+Each builder returns a dict with the reference-API objects (delta_ray.Camera,
+Shape, Material, Light) plus the resolution / spp / bounce budget of the
+workload.  Camera matrices follow the reference's camera.py / transform.py
+(look-at, perspective, cam_to_sample = scale @ translate @ perspective).
+"""
+import math
+
+import numpy as np
+
+from . import delta_ray as dr
+
+
+# ------------------------------------------------------------ camera ------
+def look_at_matrix(pos, look, up):
+    """transform.py:12-25: columns left, new_up, d, pos."""
+    pos, look, up = (np.asarray(a, np.float64) for a in (pos, look, up))
+
+    def nrm(v):
+        return v / np.linalg.norm(v)
+    d = nrm(look - pos)
+    left = nrm(np.cross(nrm(up), d))
+    new_up = nrm(np.cross(d, left))
+    m = np.eye(4)
+    m[:3, 0], m[:3, 1], m[:3, 2], m[:3, 3] = left, new_up, d, pos
+    return m
+
+
+def perspective_matrix(fov, clip_near, clip_far):
+    """transform.py:39-49"""
+    clip_dist = clip_far - clip_near
+    cot = 1.0 / math.tan(math.radians(fov / 2.0))
+    return np.array([[cot, 0, 0, 0], [0, cot, 0, 0],
+                     [0, 0, 1.0 / clip_dist, -clip_near / clip_dist], [0, 0, 1, 0]], np.float64)
+
+
+def make_camera(position, look_at, up, fov, resolution, clip_near=0.01, clip_far=10000.0,
+                cam_to_world=None):
+    """camera.py:7-47 -> a delta_ray.Camera."""
+    c2w = look_at_matrix(position, look_at, up) if cam_to_world is None else np.asarray(cam_to_world, np.float64)
+    w2c = np.linalg.inv(c2w)
+    aspect = resolution[0] / resolution[1]
+    s = np.diag([-0.5, -0.5 * aspect, 1.0, 1.0])
+    t = np.eye(4)
+    t[:3, 3] = [-1.0, -1.0 / aspect, 0.0]
+    p = perspective_matrix(fov, clip_near, clip_far)
+    c2s = s @ t @ p
+    s2c = np.linalg.inv(c2s)
+    fov_factor = math.tan(math.radians(0.5 * fov))
+    return dr.Camera(c2w.astype(np.float32), w2c.astype(np.float32), s2c.astype(np.float32),
+                     c2s.astype(np.float32), np.float32(fov_factor), aspect, clip_near, False)
+
+
+def diffuse_material(albedo, specular=(0.0, 0.0, 0.0), roughness=(1.0,), two_sided=False):
+    one = np.ones(2, np.float32)
+    return dr.Material(np.asarray(albedo, np.float32), np.asarray(specular, np.float32),
+                       np.asarray(roughness, np.float32), one, one, one, two_sided)
+
+
+def quad(center, normal, size, up_hint=(0.0, 1.0, 0.0)):
+    """Two triangles of a square of side `size` facing `normal`; the winding
+    makes the geometric normal (cross(v1-v0, v2-v0)) equal `normal`."""
+    n = np.asarray(normal, np.float64)
+    n = n / np.linalg.norm(n)
+    a = np.asarray(up_hint, np.float64)
+    if abs(np.dot(a, n)) > 0.9:
+        a = np.array([1.0, 0.0, 0.0])
+    t = np.cross(a, n)
+    t /= np.linalg.norm(t)
+    b = np.cross(n, t)
+    h = 0.5 * size
+    c = np.asarray(center, np.float64)
+    v = np.array([c - h * t - h * b, c + h * t - h * b, c - h * t + h * b, c + h * t + h * b])
+    ind = np.array([[0, 1, 2], [1, 3, 2]], np.int32)
+    gn = np.cross(v[1] - v[0], v[2] - v[0])
+    if np.dot(gn, n) < 0:
+        ind = ind[:, [0, 2, 1]]
+    return v.astype(np.float32), ind
+
+
+def _orient(v, ind, want):
+    """Flip triangle windings so that geometric normals point along `want`."""
+    ind = ind.copy()
+    for k in range(ind.shape[0]):
+        a, b, c = v[ind[k]]
+        if np.dot(np.cross(b - a, c - a), want) < 0:
+            ind[k] = ind[k][[0, 2, 1]]
+    return ind
+
+
+# ------------------------------------------------------------- scenes -----
+def single_triangle(seed=1, resolution=(256, 256), num_samples=4, max_bounces=1):
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(-1.0, 1.0, size=(3, 2))
+    v = np.concatenate([xy, np.zeros((3, 1))], axis=1).astype(np.float32)
+    ind = _orient(v, np.array([[0, 1, 2]], np.int32), np.array([0, 0, 1.0]))
+    cam = make_camera([0, 0, 5], [0, 0, 0], [0, 1, 0], 45.0, resolution)
+    grey = diffuse_material([0.5, 0.5, 0.5])
+    black = diffuse_material([0.0, 0.0, 0.0])
+    lv = np.array([[-1, -1, 7], [-1, 1, 7], [1, -1, 7]], np.float32)
+    li = np.array([[0, 1, 2]], np.int32)
+    tri = dr.Shape(v, ind, None, None, grey, None)
+    light_shape = dr.Shape(lv, li, None, None, black, None)
+    light = dr.Light(light_shape, np.array([20, 20, 20], np.float32))
+    light_shape.light = light
+    return dict(name="single_triangle", camera=cam, shapes=[tri, light_shape], materials=[grey, black],
+                lights=[light], resolution=tuple(resolution), num_samples=num_samples,
+                max_bounces=max_bounces, params=[("shape", 0, "vertices")])
+
+
+def shadow(seed=2, resolution=(256, 256), num_samples=16, max_bounces=1, light_intensity=10.0):
+    rng = np.random.default_rng(seed)
+    gv = np.array([[-5, 0, -5], [-5, 0, 5], [5, 0, -5], [5, 0, 5]], np.float32)
+    gi = np.array([[0, 1, 2], [1, 3, 2]], np.int32)
+    xz = rng.uniform(-1.0, 1.0, size=(3, 2))
+    y = rng.uniform(1.0, 2.0, size=(3, 1))
+    bv = np.concatenate([xz[:, :1], y, xz[:, 1:]], axis=1).astype(np.float32)
+    bi = _orient(bv, np.array([[0, 1, 2]], np.int32), np.array([0, 1.0, 0]))
+    lv = np.array([[-0.1, 5, -0.1], [-0.1, 5, 0.1], [0.1, 5, -0.1], [0.1, 5, 0.1]], np.float32)
+    li = np.array([[0, 2, 1], [1, 2, 3]], np.int32)
+    cam = make_camera([0, 2, -5], [0, 0, 0], [0, 1, 0], 45.0, resolution)
+    grey = diffuse_material([0.5, 0.5, 0.5])
+    black = diffuse_material([0.0, 0.0, 0.0])
+    ground = dr.Shape(gv, gi, None, None, grey, None)
+    blocker = dr.Shape(bv, bi, None, None, grey, None)
+    light_shape = dr.Shape(lv, li, None, None, black, None)
+    light = dr.Light(light_shape, np.full(3, light_intensity, np.float32))
+    light_shape.light = light
+    return dict(name="shadow", camera=cam, shapes=[ground, blocker, light_shape], materials=[grey, black],
+                lights=[light], resolution=tuple(resolution), num_samples=num_samples,
+                max_bounces=max_bounces, params=[("shape", 1, "vertices")])
+
+
+def value_noise_fbm(n, rng, octaves=6, lacunarity=2.0, gain=0.5, base=4):
+    """fBm of bilinearly interpolated value noise on an n x n grid in [0,1]^2."""
+    xs = np.linspace(0.0, 1.0, n)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    out = np.zeros((n, n))
+    amp, freq, norm = 1.0, float(base), 0.0
+    for _ in range(octaves):
+        g = int(math.ceil(freq)) + 2
+        lattice = rng.uniform(-1.0, 1.0, size=(g, g))
+        fx, fy = X * freq, Y * freq
+        ix, iy = np.floor(fx).astype(int), np.floor(fy).astype(int)
+        tx, ty = fx - ix, fy - iy
+        sx, sy = tx * tx * (3 - 2 * tx), ty * ty * (3 - 2 * ty)
+        a = lattice[ix, iy]
+        b = lattice[ix + 1, iy]
+        c = lattice[ix, iy + 1]
+        d = lattice[ix + 1, iy + 1]
+        out += amp * ((a * (1 - sx) + b * sx) * (1 - sy) + (c * (1 - sx) + d * sx) * sy)
+        norm += amp
+        amp *= gain
+        freq *= lacunarity
+    return out / norm
+
+
+def vertex_normals(v, ind):
+    """Angle-weighted vertex normals (Thurmer & Wuthrich 1998), the weighting
+    of the reference's heightfield_gan.py:55-83."""
+    v = v.astype(np.float64)
+    nrm = np.zeros_like(v)
+    p = [v[ind[:, 0]], v[ind[:, 1]], v[ind[:, 2]]]
+    fn = np.cross(p[1] - p[0], p[2] - p[0])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    for i in range(3):
+        a = p[(i + 1) % 3] - p[i]
+        b = p[(i + 2) % 3] - p[i]
+        a /= np.linalg.norm(a, axis=1, keepdims=True)
+        b /= np.linalg.norm(b, axis=1, keepdims=True)
+        ang = np.arccos(np.clip(np.sum(a * b, axis=1), -1.0, 1.0))
+        np.add.at(nrm, ind[:, i], fn * ang[:, None])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    return nrm.astype(np.float32)
+
+
+def heightfield(seed=3, grid=128, resolution=(512, 512), num_samples=64, max_bounces=2,
+                amplitude=0.2, extent=5.0, light_intensity=50.0, light_size=1.0):
+    rng = np.random.default_rng(seed)
+    h = amplitude * value_noise_fbm(grid, rng)
+    xs = np.linspace(-extent, extent, grid)
+    X, Z = np.meshgrid(xs, xs, indexing="ij")
+    v = np.stack([X, h, Z], axis=-1).reshape(-1, 3).astype(np.float32)
+    uu = np.linspace(0.0, 1.0, grid)
+    U, V = np.meshgrid(uu, uu, indexing="ij")
+    uvs = np.stack([U, V], axis=-1).reshape(-1, 2).astype(np.float32)
+    i = np.arange(grid - 1)
+    I, J = np.meshgrid(i, i, indexing="ij")
+    a = (I * grid + J).reshape(-1)
+    b = a + 1
+    c = a + grid
+    d = c + 1
+    ind = np.concatenate([np.stack([a, b, c], 1), np.stack([b, d, c], 1)], 0).astype(np.int32)
+    ind = _orient_fast(v, ind, np.array([0.0, 1.0, 0.0]))
+    nrm = vertex_normals(v, ind)
+    f0 = rng.uniform(0.04, 1.0)
+    rough = rng.uniform(0.05, 0.5, size=(grid, grid, 1)).astype(np.float32)
+    one = np.ones(2, np.float32)
+    mat = dr.Material(np.full(3, 0.5 * (1.0 - f0), np.float32), np.full(3, f0, np.float32), rough,
+                      one, one, one, False)
+    black = diffuse_material([0.0, 0.0, 0.0])
+    hf = dr.Shape(v, ind, uvs, nrm, mat, None)
+    shapes = [hf]
+    lights = []
+    for sx in (-3.0, 3.0):
+        c = np.array([sx, 6.0, 0.0])
+        lv, li = quad(c, -c, light_size)
+        ls = dr.Shape(lv, li, None, None, black, None)
+        l = dr.Light(ls, np.full(3, light_intensity, np.float32))
+        ls.light = l
+        shapes.append(ls)
+        lights.append(l)
+    cam = make_camera([0, 3, -6], [0, 0, 0], [0, 1, 0], 45.0, resolution)
+    return dict(name="heightfield", camera=cam, shapes=shapes, materials=[mat, black], lights=lights,
+                resolution=tuple(resolution), num_samples=num_samples, max_bounces=max_bounces,
+                params=[("shape", 0, "vertices"), ("material", 0, "roughness")])
+
+
+def _orient_fast(v, ind, want):
+    p0, p1, p2 = v[ind[:, 0]], v[ind[:, 1]], v[ind[:, 2]]
+    flip = np.sum(np.cross(p1 - p0, p2 - p0) * want, axis=1) < 0
+    out = ind.copy()
+    out[flip] = out[flip][:, [0, 2, 1]]
+    return out
+
+
+WORKLOADS = {"single_triangle": single_triangle, "shadow": shadow, "heightfield": heightfield}

@@ -1,0 +1,280 @@
+/*
+ * delta_ray (MI355X) — C-ABI of the differentiable path tracer.
+ *
+ * This is the boundary the Python front end (delta_ray_amd/delta_ray.py,
+ * delta_ray_amd/render_pytorch.py) binds with ctypes.  It replaces the
+ * reference's pybind11 module:
+ *
+ *   reference delta_ray.cpp:1838-1920   PYBIND11_MODULE(delta_ray): Material,
+ *                                       Shape, Light, Camera, D* classes,
+ *                                       RenderResult, render, test_line_integral
+ *   reference delta_ray.cpp:1017-1026   render(camera, shapes, materials, lights,
+ *                                       resolution, d_image, num_samples,
+ *                                       max_bounces, seed, print_progress)
+ *
+ * The reference rebuilds its Scene (scene.cpp:19-136) inside every render()
+ * call.  Here the scene is a handle: dr_scene_create uploads geometry,
+ * materials, lights and camera once and builds the triangle BVH, the edge
+ * list, the two edge trees and the light distributions; dr_render and
+ * dr_render_adjoint then run on the device and may be called many times.
+ *
+ * Conventions
+ *   - Scene description pointers are HOST pointers (the reference takes numpy
+ *     buffers, shape.h:11-18, material.h:14-22).  The library copies them.
+ *   - Image, d_image, gradient and ray buffers are DEVICE pointers, owned by
+ *     the caller.  Work is ordered on the given HIP stream (a hipStream_t passed
+ *     as void*; NULL = default stream).  Nothing blocks the host except
+ *     dr_scene_create / dr_scene_set_* and the error query.
+ *   - Every entry point returns DR_OK or an error code; dr_last_error()
+ *     returns a thread-local message for the last failure.  No exception
+ *     crosses the boundary.
+ *   - Calls on one scene handle must not overlap (they share its scratch).
+ */
+#ifndef DELTA_RAY_H
+#define DELTA_RAY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dr_status {
+    DR_OK = 0,
+    DR_ERROR_INVALID_ARGUMENT = 1,
+    DR_ERROR_OUT_OF_MEMORY = 2,
+    DR_ERROR_DEVICE = 3,
+    DR_ERROR_UNSUPPORTED = 4
+} dr_status;
+
+/* Camera (reference camera.h:12-21).  Matrices are row-major 4x4.
+ * fov_factor = tan(fov/2).  Only the pinhole model is implemented; a scene
+ * with fisheye != 0 is rejected with DR_ERROR_UNSUPPORTED. */
+typedef struct dr_camera {
+    float cam_to_world[16];
+    float world_to_cam[16];
+    float sample_to_cam[16];
+    float cam_to_sample[16];
+    float fov_factor;
+    float aspect_ratio;
+    float clip_near;
+    int fisheye;
+} dr_camera;
+
+/* A material channel (reference material.h:14-22): either a constant
+ * (height == 0: `channels` floats) or a texture of shape [height, width,
+ * channels] sampled bilinearly with wrap-around (material.h:108-239). */
+typedef struct dr_texture {
+    const float *data;
+    int height;
+    int width;
+    int channels;
+} dr_texture;
+
+typedef struct dr_material {
+    dr_texture diffuse_reflectance;   /* 3 channels */
+    dr_texture specular_reflectance;  /* 3 channels */
+    dr_texture roughness;             /* 1 channel  */
+    float diffuse_uv_scale[2];
+    float specular_uv_scale[2];
+    float roughness_uv_scale[2];
+    int two_sided;
+} dr_material;
+
+/* Triangle mesh (reference shape.h:11-18).  uvs / normals may be NULL. */
+typedef struct dr_shape {
+    const float *vertices;   /* [num_vertices, 3] */
+    int num_vertices;
+    const int *indices;      /* [num_triangles, 3] */
+    int num_triangles;
+    const float *uvs;        /* [num_vertices, 2] or NULL */
+    const float *normals;    /* [num_vertices, 3] or NULL */
+    int material_id;
+    int light_id;            /* index into lights, -1 if not emissive */
+} dr_shape;
+
+/* Area light attached to a shape (reference light.h:15-18). */
+typedef struct dr_light {
+    int shape_id;
+    float intensity[3];
+} dr_light;
+
+typedef struct dr_scene_desc {
+    dr_camera camera;
+    const dr_shape *shapes;
+    int num_shapes;
+    const dr_material *materials;
+    int num_materials;
+    const dr_light *lights;
+    int num_lights;
+} dr_scene_desc;
+
+/* Render options.  The reference's render() arguments (delta_ray.cpp:1017-
+ * 1026) plus switches.  Quirk bits reproduce reference behaviours that are
+ * not the exact derivative of the rendered image; they are ON in the default
+ * (parity) mode, DR_FLAGS_PARITY. */
+enum {
+    /* sub-pixel jitter is U/spp instead of U (delta_ray.cpp:1236-1237) */
+    DR_QUIRK_JITTER_OVER_SPP = 1u << 0,
+    /* texel adjoints of corners (0,0) and (1,1) land on swapped texels
+     * (delta_ray.cpp:1364-1375, 1391-1402, 1417-1424) */
+    DR_QUIRK_TEXEL_SWAP = 1u << 1,
+    /* shading-normal adjoint of vertex 0 is scattered to all three vertices
+     * (delta_ray.cpp:1263-1269) */
+    DR_QUIRK_NORMAL_DN0 = 1u << 2,
+    /* of several Dirac position derivatives at one shading point, only the
+     * last one is propagated (set_adjoint overwrites, autodiff.h:340-342,
+     * delta_ray.cpp:977-982) */
+    DR_QUIRK_DP_LAST = 1u << 3,
+    /* terms (for tests): skip the interior adjoint / secondary edges /
+     * primary edges */
+    DR_SKIP_INTERIOR = 1u << 8,
+    DR_SKIP_SECONDARY_EDGES = 1u << 9,
+    DR_SKIP_PRIMARY_EDGES = 1u << 10,
+    /* write per-pixel secondary edge counts / 1000 into dx_image
+     * (delta_ray.cpp:1251-1254) */
+    DR_WRITE_DX_IMAGE = 1u << 11
+};
+#define DR_FLAGS_PARITY (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP | \
+                         DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST)
+
+typedef struct dr_render_options {
+    int width;
+    int height;
+    int num_samples;      /* samples per pixel */
+    int max_bounces;      /* >= 1 */
+    uint64_t seed;
+    uint32_t flags;
+    /* >0: number of primary-edge samples; 0: num_samples * width * height,
+     * as in delta_ray.cpp:1474-1475 */
+    int64_t num_primary_edge_samples;
+} dr_render_options;
+
+/* Gradient layout.  All parameter gradients of one render live in ONE float
+ * buffer; this computes the offsets (in floats).  The order matches the
+ * argument order of the reference's RenderFunction.backward
+ * (render_pytorch.py:283-333): camera, materials, shapes, lights. */
+#define DR_MAX_LAYOUT_ENTRIES 4096
+typedef struct dr_grad_layout {
+    size_t cam_to_world, world_to_cam, sample_to_cam, cam_to_sample; /* 16 each */
+    size_t total;
+} dr_grad_layout;
+
+static inline size_t dr_texture_size(const dr_texture *t) {
+    return t->height > 0 ? (size_t)t->height * (size_t)t->width * (size_t)t->channels
+                         : (size_t)t->channels;
+}
+
+/* Offsets of the per-object gradients.  Pass arrays of the scene's sizes (or
+ * NULL to skip); normals_off[i] is (size_t)-1 when shape i has no normals. */
+static inline size_t dr_grad_layout_compute(const dr_scene_desc *d,
+                                            dr_grad_layout *lay,
+                                            size_t *diffuse_off,
+                                            size_t *specular_off,
+                                            size_t *roughness_off,
+                                            size_t *vertices_off,
+                                            size_t *normals_off,
+                                            size_t *intensity_off) {
+    size_t o = 0;
+    int i;
+    if (lay) {
+        lay->cam_to_world = 0; lay->world_to_cam = 16;
+        lay->sample_to_cam = 32; lay->cam_to_sample = 48;
+    }
+    o = 64;
+    for (i = 0; i < d->num_materials; i++) {
+        if (diffuse_off) diffuse_off[i] = o;
+        o += dr_texture_size(&d->materials[i].diffuse_reflectance);
+        if (specular_off) specular_off[i] = o;
+        o += dr_texture_size(&d->materials[i].specular_reflectance);
+        if (roughness_off) roughness_off[i] = o;
+        o += dr_texture_size(&d->materials[i].roughness);
+    }
+    for (i = 0; i < d->num_shapes; i++) {
+        if (vertices_off) vertices_off[i] = o;
+        o += (size_t)d->shapes[i].num_vertices * 3;
+        if (d->shapes[i].normals) {
+            if (normals_off) normals_off[i] = o;
+            o += (size_t)d->shapes[i].num_vertices * 3;
+        } else if (normals_off) {
+            normals_off[i] = (size_t)-1;
+        }
+    }
+    for (i = 0; i < d->num_lights; i++) {
+        if (intensity_off) intensity_off[i] = o;
+        o += 3;
+    }
+    if (lay) lay->total = o;
+    return o;
+}
+
+typedef struct dr_scene dr_scene;
+
+/* Build a device-resident scene (replaces Scene::Scene, scene.cpp:19-136,
+ * and build_double_edge_tree, edge_tree.cpp:263-316). */
+dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out);
+void dr_scene_destroy(dr_scene *scene);
+
+/* Replace the camera (no rebuild of geometry; the primary-edge distribution,
+ * which depends on the camera, is recomputed on the device). */
+dr_status dr_scene_set_camera(dr_scene *scene, const dr_camera *camera);
+
+/* Number of floats of the gradient buffer for this scene. */
+size_t dr_scene_grad_size(const dr_scene *scene);
+
+/* Forward render (reference render() with a 0-d d_image: pass 1 with
+ * render_sample_fast, delta_ray.cpp:640-692).  image: device [H, W, 3]. */
+dr_status dr_render(dr_scene *scene, const dr_render_options *opts,
+                    float *image, void *stream);
+
+/* Forward + adjoint render (reference render() with d_image [H, W, 3]:
+ * pass 1 with render_sample, delta_ray.cpp:694-1005, then pass 2, the
+ * primary-edge samples, delta_ray.cpp:1468-1550).  d_image: device [H,W,3].
+ * image: device [H,W,3] (written).  grads: device, dr_scene_grad_size floats
+ * (overwritten: the call clears it first).  dx_image: device [H,W,3] or NULL
+ * (written when DR_WRITE_DX_IMAGE is set). */
+dr_status dr_render_adjoint(dr_scene *scene, const dr_render_options *opts,
+                            const float *d_image, float *image, float *grads,
+                            float *dx_image, void *stream);
+
+/* Ray queries on the scene BVH (closest hit: tnear 1e-3, tfar inf,
+ * scene.cpp:177-202; any hit: the caller's tmin/tmax, scene.cpp:147-175).
+ * rays: device [n, 8] = org.xyz, tmin, dir.xyz, tmax.
+ * hits: device [n, 4] int32 = shape id (-1 = miss), triangle id, u bits,
+ * v bits (float32 bit patterns of intersect.h:71-90's u, v).  For any_hit
+ * only hits[i*4] is written: 1 occluded, 0 free. */
+dr_status dr_intersect(dr_scene *scene, const float *rays, int64_t n,
+                       int32_t *hits, int any_hit, void *stream);
+
+/* Launch statistics of the last render on this scene (host, after a sync):
+ * counts of rays traced, BVH nodes and triangles visited, edges sampled. */
+typedef struct dr_stats {
+    uint64_t rays;
+    uint64_t nodes_visited;
+    uint64_t tris_visited;
+    uint64_t secondary_edges;
+    uint64_t primary_edge_samples;
+} dr_stats;
+dr_status dr_scene_stats(dr_scene *scene, int enable, dr_stats *out);
+
+/* Scene sizes (host). */
+typedef struct dr_scene_info {
+    int num_triangles;
+    int num_vertices;
+    int bvh_nodes;
+    int num_edges;
+    int num_primary_edges;
+    int silhouette_tree_nodes;
+    int normal_tree_nodes;
+    double build_ms;
+} dr_scene_info;
+dr_status dr_scene_get_info(const dr_scene *scene, dr_scene_info *out);
+
+const char *dr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

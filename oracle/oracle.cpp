@@ -1,0 +1,1483 @@
+// ORACLE — test infrastructure only.  Only tests/, __graft_entry__.smoke() and
+// bench.py's CPU-baseline leg load liboracle.so, and only as the checker / CPU
+// baseline; the product never links or calls it.
+//
+// A CPU restatement of the reference estimator (BachiLi/delta_ray), written
+// from its behaviour, scalar-templated so that the same code runs in float
+// (forward) and on the tape variable ADR (interior gradients):
+//   camera / projection      camera.h:48-324, transform.h:9-59
+//   surface point            intersect.h:71-148, scene.h:121-134
+//   material                 material.h:108-462 (textures, bsdf, sample, pdf)
+//   LTC lookup               material.h:464-489 (table: delta_ray_amd/data)
+//   light sampling           scene.h:49-108, sample.h:6-21
+//   path tracer              pathtrace.h:7-297
+//   per-sample fwd + AD      delta_ray.cpp:640-1005
+//   secondary edge sample    delta_ray.cpp:45-365
+//   primary edge sample      delta_ray.cpp:377-557
+//   edge-tree sampling       edge_tree.cpp:318-725, edge.h:59-79
+//   render driver            delta_ray.cpp:1017-1570 (8x8 tiles, work list
+//                            thread pool as parallel.cpp:13-228)
+// Random numbers come from counter-based streams (or_rng.h) instead of the
+// reference's per-tile mt19937, so the device renderer can draw the same ones.
+#include "or_rng.h"
+#include "or_scene.h"
+
+#include <atomic>
+#include <cstdio>
+#include <fstream>
+#include <mutex>
+#include <string>
+#include <thread>
+
+namespace oracle {
+
+thread_local Tape *g_tape = nullptr;
+
+static std::vector<float> g_ltc;  // 16384 x 9, row-major 3x3 per entry
+
+// --------------------------------------------------------------- types ----
+template <class T>
+struct SP {
+    V3<T> pos, ng;
+    Frame<T> fr;
+    V2<T> uv;
+};
+struct Isect {
+    int shape = -1, tri = -1;
+    bool valid() const { return shape >= 0; }
+};
+template <class TO, class TI>
+inline SP<TO> conv_sp(const SP<TI> &s) {
+    return SP<TO>{conv3<TO>(s.pos), conv3<TO>(s.ng), conv_frame<TO>(s.fr), conv2<TO>(s.uv)};
+}
+
+template <class T>
+struct TexP1 {
+    int xi = -1, yi = -1;
+    T t00, t01, t10, t11;
+};
+template <class T>
+struct TexP3 {
+    int xi = -1, yi = -1;
+    V3<T> t00, t01, t10, t11;
+};
+template <class T>
+struct MatP {
+    int mat = -1;
+    TexP3<T> d, s;
+    TexP1<T> r;
+};
+template <class T>
+struct IsP {
+    int shape = -1, tri = -1;
+    V3<T> v0, v1, v2, n0, n1, n2;
+};
+template <class T>
+struct LiP {
+    int light;
+    V3<T> I;
+};
+template <class T>
+struct Rec {
+    std::vector<IsP<T>> is;
+    std::vector<MatP<T>> mat;
+    std::vector<LiP<T>> li;
+};
+
+struct LightSample {
+    float uv[2];
+    float light_sel, tri_sel;
+};
+struct BSDFSample {
+    float uv[2];
+    float w;
+};
+struct SecondarySample {
+    LightSample ls;
+    BSDFSample bs;
+    float rr;
+};
+inline LightSample draw_light_sample(Philox &r) {
+    LightSample s;
+    s.uv[0] = r.next();
+    s.uv[1] = r.next();
+    s.light_sel = r.next();
+    s.tri_sel = r.next();
+    return s;
+}
+inline SecondarySample draw_secondary(Philox &r) {
+    SecondarySample s;
+    s.ls = draw_light_sample(r);
+    s.bs.uv[0] = r.next();
+    s.bs.uv[1] = r.next();
+    s.bs.w = r.next();
+    s.rr = r.next();
+    return s;
+}
+
+// ------------------------------------------------------------ textures ---
+inline int modulo(int a, int b) {
+    int r = a % b;
+    return r < 0 ? r + b : r;
+}
+
+template <class T>
+T tex1(const Tex &t, const V2<T> &uv, TexP1<T> *p) {
+    if (t.h == 0) {
+        T v = conv<T>(t.data[0]);
+        if (p) { p->xi = p->yi = -1; p->t00 = v; }
+        return v;
+    }
+    int w = t.w, h = t.h;
+    int xf = (int)std::floor(val(uv[0]) * w);
+    int yf = (int)std::floor(val(uv[1]) * h);
+    int xc = xf + 1, yc = yf + 1;
+    T u = uv[0] * (float)w - (float)xf;
+    T v = uv[1] * (float)h - (float)yf;
+    int xfi = modulo(xf, w), yfi = modulo(yf, h), xci = modulo(xc, w), yci = modulo(yc, h);
+    auto at = [&](int y, int x) { return t.data[((size_t)y * w + x) * t.c]; };
+    T ff = conv<T>(at(yfi, xfi)), fc = conv<T>(at(yci, xfi)), cf = conv<T>(at(yfi, xci)), cc = conv<T>(at(yci, xci));
+    if (p) { p->xi = xfi; p->yi = yfi; p->t00 = ff; p->t01 = fc; p->t10 = cf; p->t11 = cc; }
+    return ff * (1.f - u) * (1.f - v) + fc * (1.f - u) * v + cf * u * (1.f - v) + cc * u * v;
+}
+
+template <class T>
+V3<T> tex3(const Tex &t, const V2<T> &uv, TexP3<T> *p) {
+    if (t.h == 0) {
+        V3<T> v = v3<T>(conv<T>(t.data[0]), conv<T>(t.data[1]), conv<T>(t.data[2]));
+        if (p) { p->xi = p->yi = -1; p->t00 = v; }
+        return v;
+    }
+    int w = t.w, h = t.h;
+    int xf = (int)std::floor(val(uv[0]) * w);
+    int yf = (int)std::floor(val(uv[1]) * h);
+    int xc = xf + 1, yc = yf + 1;
+    T u = uv[0] * (float)w - (float)xf;
+    T v = uv[1] * (float)h - (float)yf;
+    int xfi = modulo(xf, w), yfi = modulo(yf, h), xci = modulo(xc, w), yci = modulo(yc, h);
+    auto at = [&](int y, int x) {
+        const float *q = &t.data[((size_t)y * w + x) * 3];
+        return v3<T>(conv<T>(q[0]), conv<T>(q[1]), conv<T>(q[2]));
+    };
+    V3<T> ff = at(yfi, xfi), fc = at(yci, xfi), cf = at(yfi, xci), cc = at(yci, xci);
+    V3<T> col = ff * (1.f - u) * (1.f - v) + fc * (1.f - u) * v + cf * u * (1.f - v) + cc * u * v;
+    if (p) { p->xi = xfi; p->yi = yfi; p->t00 = ff; p->t01 = fc; p->t10 = cf; p->t11 = cc; }
+    return col;
+}
+
+template <class T>
+V3<T> get_diffuse(const Material &m, const V2<T> &uv, MatP<T> *p) {
+    V2<T> sc = v2(conv<T>(m.duv[0]), conv<T>(m.duv[1]));
+    return tex3(m.diffuse, uv * sc, p ? &p->d : nullptr);
+}
+template <class T>
+V3<T> get_specular(const Material &m, const V2<T> &uv, MatP<T> *p) {
+    V2<T> sc = v2(conv<T>(m.suv[0]), conv<T>(m.suv[1]));
+    return tex3(m.specular, uv * sc, p ? &p->s : nullptr);
+}
+template <class T>
+T get_roughness(const Material &m, const V2<T> &uv, MatP<T> *p) {
+    V2<T> sc = v2(conv<T>(m.ruv[0]), conv<T>(m.ruv[1]));
+    return tex1(m.roughness, uv * sc, p ? &p->r : nullptr);
+}
+template <class T>
+inline MatP<T> *push_mat(Rec<T> *rec, int mat) {
+    if (!rec) return nullptr;
+    rec->mat.emplace_back();
+    rec->mat.back().mat = mat;
+    return &rec->mat.back();
+}
+
+// ------------------------------------------------------------- material ---
+template <class T>
+inline T roughness_to_phong(const T &r) { return max(2.f / r, T(0.f)); }
+inline float roughness_to_phong(float r) { return max(2.f / r, 0.f); }
+
+// material.h:257-351
+template <class T>
+V3<T> bsdf(const Scene &sc, int mid, const SP<T> &sp, const V3<T> &wi, const V3<T> &wo, Rec<T> *rec) {
+    const Material &m = sc.mats[mid];
+    V3<T> n = sp.fr[2];
+    if (m.two_sided && dot(wi, n) < 0.f) n = -n;
+    T bsdf_cos = dot(n, wo);
+    if (bsdf_cos <= 5e-3f) return v3<T>(T(0.f), T(0.f), T(0.f));
+    MatP<T> *p = push_mat(rec, mid);
+    V3<T> diffuse = get_diffuse(m, sp.uv, p);
+    V3<T> specular = get_specular(m, sp.uv, p);
+    T roughness = get_roughness(m, sp.uv, p);
+    V3<T> diffuse_contrib = diffuse * bsdf_cos / float(M_PI);
+    V3<T> specular_contrib = v3<T>(T(0.f), T(0.f), T(0.f));
+    if (sum(specular) > 0.f) {
+        if (dot(wi, n) > 0.f) {
+            V3<T> h = normalize(wi + wo);
+            V3<T> hl = to_local(sp.fr, h);
+            if (hl[2] > 0.f) {
+                T e = roughness_to_phong(roughness);
+                T D = pow(max(hl[2], 0.f), e) * (e + 2.f) / float(2 * M_PI);
+                auto project_roughness = [&](const V3<T> &v) {
+                    V3<T> vl = to_local(sp.fr, v);
+                    T c = vl[2];
+                    T s2 = 1.f - c * c;
+                    if (s2 <= 0.f) return sqrt(roughness);
+                    T inv = 1.f / s2;
+                    T cp2 = vl[0] * vl[0] * inv;
+                    T sp2 = vl[1] * vl[1] * inv;
+                    return sqrt(cp2 * roughness + sp2 * roughness);
+                };
+                auto smithG1 = [&](const V3<T> &v) {
+                    T c = dot(v, n);
+                    if (dot(v, h) * c <= 0.f) return T(0.f);
+                    T tan_t = sqrt(max(1.f / (c * c) - 1.f, 0.f));
+                    if (tan_t == 0.f) return T(1.f);
+                    T alpha = project_roughness(v);
+                    T a = 1.f / (alpha * tan_t);
+                    if (a >= 1.6f) return T(1.f);
+                    T a2 = a * a;
+                    return (3.535f * a + 2.181f * a2) / (1.0f + 2.276f * a + 2.577f * a2);
+                };
+                T G = smithG1(wi) * smithG1(wo);
+                T cos_d = dot(h, wo);
+                T schlick = pow(max(1.f - cos_d, 0.f), 5.f);
+                V3<T> F = v3<T>(specular[0] + (1.f - specular[0]) * schlick,
+                                specular[1] + (1.f - specular[1]) * schlick,
+                                specular[2] + (1.f - specular[2]) * schlick);
+                specular_contrib = F * D * G / (4.f * dot(wi, n));
+            }
+        }
+    }
+    return diffuse_contrib + specular_contrib;
+}
+
+// material.h:353-412.  Returns the direction; *is_specular set.
+template <class T>
+V3<T> sample_bsdf_dir(const Scene &sc, int mid, const SP<T> &sp, const V3<T> &wi, const BSDFSample &bs,
+                      Rec<T> *rec, bool *is_specular) {
+    const Material &m = sc.mats[mid];
+    *is_specular = false;
+    MatP<T> *p = push_mat(rec, mid);
+    V3<T> diffuse = get_diffuse(m, sp.uv, p);
+    V3<T> specular = get_specular(m, sp.uv, p);
+    T dw = luminance(diffuse), sw = luminance(specular);
+    T ws = dw + sw;
+    V3<T> zero = v3<T>(T(0.f), T(0.f), T(0.f));
+    if (ws <= 0.f) return zero;
+    T dpmf = dw / ws;
+    if (bs.w <= val(dpmf)) {
+        if (dpmf <= 0.f) return zero;
+        V3f l = cos_hemisphere(v2(bs.uv[0], bs.uv[1]));
+        return sp.fr[0] * l[0] + sp.fr[1] * l[1] + sp.fr[2] * l[2];
+    }
+    if (sw <= 0.f) return zero;
+    T roughness = get_roughness(m, sp.uv, p);
+    T e = roughness_to_phong(roughness);
+    float phi = 2.f * float(M_PI) * bs.uv[1];
+    float sin_phi = std::sin(phi), cos_phi = std::cos(phi);
+    T cos_t = pow(bs.uv[0], 1.0f / (e + 2.0f));
+    T sin_t = sqrt(max(1.f - cos_t * cos_t, 0.f));
+    V3<T> ml = v3<T>(sin_t * cos_phi, sin_t * sin_phi, cos_t);
+    V3<T> mw = to_world(sp.fr, ml);
+    V3<T> dir = 2.f * dot(wi, mw) * mw - wi;
+    *is_specular = roughness <= 0.01f;
+    return dir;
+}
+
+// material.h:414-462
+template <class T>
+T bsdf_pdf(const Scene &sc, int mid, const SP<T> &sp, const V3<T> &wi, const V3<T> &wo, Rec<T> *rec) {
+    const Material &m = sc.mats[mid];
+    MatP<T> *p = push_mat(rec, mid);
+    V3<T> diffuse = get_diffuse(m, sp.uv, p);
+    V3<T> specular = get_specular(m, sp.uv, p);
+    T dw = luminance(diffuse), sw = luminance(specular);
+    T ws = dw + sw;
+    if (ws <= 0.f) return T(0.f);
+    T dpmf = dw / ws, spmf = sw / ws;
+    T dpdf = T(0.f);
+    if (dpmf > 0.f) {
+        T c = dot(sp.fr[2], wo);
+        if (!m.two_sided) c = max(c, 0.f);
+        else c = fabs(c);
+        dpdf = dpmf * c / float(M_PI);
+    }
+    T spdf = T(0.f);
+    if (spmf > 0.f) {
+        V3<T> h = normalize(wi + wo);
+        V3<T> hl = to_local(sp.fr, h);
+        if (hl[2] > 0.f) {
+            T roughness = get_roughness(m, sp.uv, p);
+            T e = roughness_to_phong(roughness);
+            T D = pow(hl[2], e) * (e + 2.f) / float(2 * M_PI);
+            spdf = spmf * D / (4.f * fabs(dot(wo, h)));
+        }
+    }
+    return dpdf + spdf;
+}
+
+// material.h:464-476 (nearest entry of the 128x128 LTC table)
+M3 get_ltc_matrix(const Scene &sc, int mid, const SP<float> &sp, const V3f &wi) {
+    float r = get_roughness<float>(sc.mats[mid], sp.uv, nullptr);
+    float cos_t = dot(wi, sp.fr[2]);
+    float theta = std::acos(cos_t);
+    int rid = int(r * (128 - 1));
+    int tid = int((theta / (M_PI / 2.f)) * (128 - 1));
+    rid = std::min(std::max(rid, 0), 127);
+    tid = std::min(std::max(tid, 0), 127);
+    M3 out;
+    const float *e = &g_ltc[(size_t)(rid + tid * 128) * 9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out.m[i][j] = e[i * 3 + j];
+    return out;
+}
+
+// ----------------------------------------------------------- geometry ----
+template <class T>
+inline IsP<T> *push_is(Rec<T> *rec, int shape, int tri, const V3<T> &v0, const V3<T> &v1, const V3<T> &v2) {
+    if (!rec) return nullptr;
+    IsP<T> p;
+    p.shape = shape;
+    p.tri = tri;
+    p.v0 = v0; p.v1 = v1; p.v2 = v2;
+    rec->is.push_back(p);
+    return &rec->is.back();
+}
+
+// intersect.h:100-148
+template <class T>
+SP<T> surface_point(const Scene &sc, const Isect &is, const V3<T> &org, const V3<T> &dir, Rec<T> *rec) {
+    const Shape &s = sc.shapes[is.shape];
+    int i0 = s.idx(is.tri, 0), i1 = s.idx(is.tri, 1), i2 = s.idx(is.tri, 2);
+    V3<T> q0 = conv3<T>(s.vert(i0)), q1 = conv3<T>(s.vert(i1)), q2 = conv3<T>(s.vert(i2));
+    V2<T> uv0, uv1, uv2;
+    if (s.has_uv) {
+        uv0 = conv2<T>(s.uv(i0)); uv1 = conv2<T>(s.uv(i1)); uv2 = conv2<T>(s.uv(i2));
+    } else {
+        uv0 = v2(T(0.f), T(0.f)); uv1 = v2(T(1.f), T(0.f)); uv2 = v2(T(1.f), T(1.f));
+    }
+    IsP<T> *p = push_is(rec, is.shape, is.tri, q0, q1, q2);
+    T u, v, t;
+    mt_uvt(q0, q1, q2, org, dir, u, v, t);
+    T w = 1.f - (u + v);
+    V2<T> uv = w * uv0 + u * uv1 + v * uv2;
+    V3<T> pos = org + dir * t;
+    V3<T> gn = normalize(cross(q1 - q0, q2 - q0));
+    V3<T> sn = gn;
+    if (s.has_n) {
+        V3<T> n0 = conv3<T>(s.normal(i0)), n1 = conv3<T>(s.normal(i1)), n2 = conv3<T>(s.normal(i2));
+        if (p) { p->n0 = n0; p->n1 = n1; p->n2 = n2; }
+        sn = normalize(w * n0 + u * n1 + v * n2);
+        if (dot(gn, sn) < 0.f) gn = -gn;
+    }
+    return SP<T>{pos, gn, make_frame(sn), uv};
+}
+
+// scene.h:121-134 (closest hit, tnear 1e-3, tfar inf)
+template <class T>
+Isect intersect(const Scene &sc, const V3<T> &org, const V3<T> &dir, Rec<T> *rec, SP<T> &out) {
+    Hit h = nearest_hit(sc, conv3<float>(org), conv3<float>(dir), 1e-3f, std::numeric_limits<float>::infinity());
+    Isect is;
+    if (h.shape < 0) {
+        out = SP<T>{};
+        return is;
+    }
+    is.shape = h.shape;
+    is.tri = h.tri;
+    out = surface_point(sc, is, org, dir, rec);
+    return is;
+}
+
+// ------------------------------------------------------------- lights ----
+template <class T>
+V3<T> get_intensity(const Scene &sc, int lid, Rec<T> *rec) {
+    const Light &l = sc.lights[lid];
+    V3<T> r = v3<T>(conv<T>(l.I[0]), conv<T>(l.I[1]), conv<T>(l.I[2]));
+    if (rec) rec->li.push_back(LiP<T>{lid, r});
+    return r;
+}
+
+// scene.h:49-75 + sample.h:6-21
+template <class T>
+Isect sample_point_on_light(const Scene &sc, const LightSample &ls, Rec<T> *rec, SP<T> &out) {
+    int lid = sc.light_dist.sample_discrete(ls.light_sel);
+    const Light &l = sc.lights[lid];
+    int tri = sc.light_tri[lid].sample_discrete(ls.tri_sel);
+    const Shape &s = sc.shapes[l.shape];
+    V3<T> q0 = conv3<T>(s.vert(s.idx(tri, 0))), q1 = conv3<T>(s.vert(s.idx(tri, 1))),
+          q2 = conv3<T>(s.vert(s.idx(tri, 2)));
+    push_is(rec, l.shape, tri, q0, q1, q2);
+    float a = std::sqrt(ls.uv[0]);
+    float b1 = 1.f - a;
+    float b2 = a * ls.uv[1];
+    V3<T> e1 = q1 - q0, e2 = q2 - q0;
+    V3<T> n = normalize(cross(e1, e2));
+    out = SP<T>{q0 + e1 * b1 + e2 * b2, n, make_frame(n), v2(conv<T>(ls.uv[0]), conv<T>(ls.uv[1]))};
+    Isect is;
+    is.shape = l.shape;
+    is.tri = tri;
+    return is;
+}
+
+// scene.h:77-108 (d tri_pmf / dv ignored, as there)
+template <class T>
+T light_pdf(const Scene &sc, const Isect &is, Rec<T> *rec) {
+    if (!is.valid()) return T(0.f);
+    int lid = sc.shapes[is.shape].light;
+    if (lid < 0) return T(0.f);
+    V3<T> I = v3<T>(conv<T>(sc.lights[lid].I[0]), conv<T>(sc.lights[lid].I[1]), conv<T>(sc.lights[lid].I[2]));
+    T light_pmf = luminance(I) / sc.total_light_intensity;
+    const Shape &s = sc.shapes[is.shape];
+    V3<T> q0 = conv3<T>(s.vert(s.idx(is.tri, 0))), q1 = conv3<T>(s.vert(s.idx(is.tri, 1))),
+          q2 = conv3<T>(s.vert(s.idx(is.tri, 2)));
+    push_is(rec, is.shape, is.tri, q0, q1, q2);
+    T area = 0.5f * length(cross(q1 - q0, q2 - q0));
+    float tri_pmf = val(area) / sc.light_area[lid];
+    return light_pmf * tri_pmf / area;
+}
+
+// ------------------------------------------------------------ shading ----
+// pathtrace.h:7-57 (visibility is always tested: the test_visibility
+// argument is not read there)
+template <class T>
+V3<T> eval(const Scene &sc, const V3<T> &wi, const Isect &sis, const SP<T> &sp, const Isect &lis,
+           const SP<T> &lp, Rec<T> *rec) {
+    V3<T> zero = v3<T>(T(0.f), T(0.f), T(0.f));
+    if (!sis.valid() || !lis.valid()) return zero;
+    int lid = sc.shapes[lis.shape].light;
+    if (lid < 0) return zero;
+    V3<T> d = lp.pos - sp.pos;
+    T dist = length(d);
+    if (dist == 0.f) return zero;
+    V3<T> nd = d / dist;
+    if (occluded(sc, conv3<float>(sp.pos), conv3<float>(nd), val(dist))) return zero;
+    V3<T> b = bsdf(sc, sc.shapes[sis.shape].mat, sp, wi, nd, rec);
+    T lcos = dot(lp.fr[2], -nd);
+    if (lcos <= 0.f) return zero;
+    T dist2 = square(dist);
+    V3<T> lc = get_intensity(sc, lid, rec) * (lcos / dist2);
+    return b * lc;
+}
+
+// pathtrace.h:59-102 (MIS weight not differentiated)
+template <class T>
+V3<T> nee(const Scene &sc, const LightSample &ls, const V3<T> &wi, const Isect &sis, const SP<T> &sp, Rec<T> *rec) {
+    if (!sis.valid()) return v3<T>(T(0.f), T(0.f), T(0.f));
+    SP<T> lp;
+    Isect lis = sample_point_on_light(sc, ls, rec, lp);
+    V3<T> ldir = normalize(lp.pos - sp.pos);
+    T lcos = fabs(dot(ldir, lp.ng));
+    T lpdf = light_pdf(sc, lis, rec);
+    float bpdf = bsdf_pdf<float>(sc, sc.shapes[sis.shape].mat, conv_sp<float>(sp), conv3<float>(wi),
+                                 conv3<float>(ldir), nullptr) *
+                 val(lcos) / length_squared(conv3<float>(lp.pos) - conv3<float>(sp.pos));
+    float mis = val(square(lpdf) / (square(bpdf) + square(lpdf)));
+    return eval(sc, wi, sis, sp, lis, lp, rec) * (mis / lpdf);
+}
+
+struct SecondaryPointF {
+    int bounce;
+    V3f wi;
+    Isect is;
+    SP<float> spf;
+    int ad_slot;  // index into the AD shading points, -1 for float runs
+    V3f throughput;
+};
+
+// pathtrace.h:104-162 (scene-level BSDF sampling + next intersection)
+template <class T>
+Isect sample_bsdf_scene(const Scene &sc, const BSDFSample &bs, const V3<T> &wi, const Isect &is, const SP<T> &sp_in,
+                        Rec<T> *rec, SP<T> &out, bool &is_specular) {
+    is_specular = false;
+    if (!is.valid()) return Isect{};
+    int mid = sc.shapes[is.shape].mat;
+    SP<T> sp = sp_in;
+    if (sc.mats[mid].two_sided && dot(wi, sp.fr[2]) < 0.f) {
+        sp.fr[0] = -sp.fr[0];
+        sp.fr[1] = -sp.fr[1];
+        sp.fr[2] = -sp.fr[2];
+        sp.ng = -sp.ng;
+    }
+    V3<T> dir = sample_bsdf_dir(sc, mid, sp, wi, bs, rec, &is_specular);
+    if (length(dir) <= 0.f) {
+        is_specular = false;
+        return Isect{};
+    }
+    Isect nis = intersect(sc, sp.pos, dir, is_specular ? rec : (Rec<T> *)nullptr, out);
+    if (!is_specular) out = conv_sp<T>(conv_sp<float>(out));
+    return nis;
+}
+
+// pathtrace.h:182-297
+template <class T>
+V3<T> shade(const Scene &sc, const V3<T> &wi, const Isect &pis, const SP<T> &psp, const LightSample &ls,
+            const std::vector<SecondarySample> &ss, bool include_primary_light,
+            std::vector<SecondaryPointF> *spts, std::vector<SP<T>> *spts_ad, Rec<T> *rec) {
+    V3<T> color = v3<T>(T(0.f), T(0.f), T(0.f));
+    if (!pis.valid()) return color;
+    int plid = sc.shapes[pis.shape].light;
+    if (include_primary_light && plid >= 0) {
+        if (dot(psp.fr[2], wi) > 0.f) color += get_intensity(sc, plid, rec);
+    }
+    V3<T> thr = v3<T>(T(1.f), T(1.f), T(1.f));
+    V3<T> cur_wi = wi;
+    Isect cur = pis;
+    SP<T> csp = psp;
+    bool hit_diffuse = false;
+    for (int b = 0; b < (int)ss.size(); b++) {
+        color += thr * nee(sc, ls, wi, cur, csp, rec);
+        if (spts && !hit_diffuse) {
+            SecondaryPointF p;
+            p.bounce = b;
+            p.wi = conv3<float>(cur_wi);
+            p.is = cur;
+            p.spf = conv_sp<float>(csp);
+            p.ad_slot = -1;
+            if (spts_ad) {
+                p.ad_slot = (int)spts_ad->size();
+                spts_ad->push_back(csp);
+            }
+            p.throughput = conv3<float>(thr);
+            spts->push_back(p);
+        }
+        SP<T> nsp;
+        bool spec = false;
+        Isect nis = sample_bsdf_scene(sc, ss[b].bs, cur_wi, cur, csp, rec, nsp, spec);
+        hit_diffuse = hit_diffuse || !spec;
+        if (!nis.valid()) break;
+        V3<T> wo = normalize(nsp.pos - csp.pos);
+        if (hit_diffuse) wo = conv3<T>(conv3<float>(wo));
+        int mid = sc.shapes[cur.shape].mat;
+        T pdf = bsdf_pdf(sc, mid, csp, cur_wi, wo, rec);
+        if (pdf <= 0.f) break;
+        V3<T> be = bsdf(sc, mid, csp, cur_wi, wo, rec);
+        if (sum(be) <= 0.f) break;
+        int nlid = sc.shapes[nis.shape].light;
+        if (nlid >= 0) {
+            if (dot(nsp.fr[2], -wo) > 0.f) {
+                V3<T> contrib = be * get_intensity(sc, nlid, rec);
+                float lpdf = light_pdf<float>(sc, nis, nullptr) *
+                             length_squared(conv3<float>(nsp.pos) - conv3<float>(csp.pos)) /
+                             std::fabs(dot(conv3<float>(wo), conv3<float>(nsp.ng)));
+                float mis = val(square(pdf) / (square(lpdf) + square(pdf)));
+                color += thr * contrib * (mis / pdf);
+            }
+        }
+        thr = thr * (be / pdf);
+        if (b >= 5) {
+            float q = std::min(maxc(conv3<float>(thr)), 0.95f);
+            if (ss[b].rr >= q) break;
+            thr = thr / q;
+        }
+        cur_wi = -wo;
+        cur = nis;
+        csp = nsp;
+    }
+    return color;
+}
+
+// ------------------------------------------------------------- camera ----
+struct CamT {
+    M4T<ADR> c2w;
+    ADR fov, aspect;
+};
+
+template <class M, class T>
+void sample_primary(const M &c2w, const T &fov, const T &aspect, const V2<T> &s, V3<T> &org, V3<T> &dir) {
+    org = xfm_point(c2w, v3<T>(T(0.f), T(0.f), T(0.f)));
+    T nx = (s[0] - 0.5f) * -2.f;
+    T ny = (s[1] - 0.5f) * -2.f / aspect;
+    V3<T> d = v3<T>(fov * nx, fov * ny, T(1.f));
+    dir = xfm_vector(c2w, normalize(d));
+}
+
+inline V3f unproject(const Camera &c, const V2f &s) {
+    float nx = (s[0] - 0.5f) * -2.f;
+    float ny = (s[1] - 0.5f) * -2.f / c.aspect;
+    return v3(c.fov * nx, c.fov * ny, 1.f);
+}
+inline V2f project_local(const Camera &c, const V3f &l) {
+    float x = (-l[0] / (l[2] * c.fov) + 1.f) * 0.5f;
+    float y = (-l[1] / (l[2] * c.fov * c.aspect) + 1.f) * 0.5f;
+    return v2(x, y);
+}
+// camera.h:216-242
+bool project_edge(const Camera &c, V3f p0, V3f p1, V2f &s0, V2f &s1) {
+    V3f a = xfm_point(c.w2c, p0), b = xfm_point(c.w2c, p1);
+    if (a[2] < c.clip_near && b[2] < c.clip_near) return false;
+    if (a[2] < c.clip_near) {
+        V3f d = a - b;
+        float t = -(b[2] - c.clip_near) / d[2];
+        a = b + t * d;
+    } else if (b[2] < c.clip_near) {
+        V3f d = b - a;
+        float t = -(a[2] - c.clip_near) / d[2];
+        b = a + t * d;
+    }
+    s0 = project_local(c, a);
+    s1 = project_local(c, b);
+    return true;
+}
+inline V3f d_project_local(const Camera &c, const V3f &l, float dx, float dy) {
+    float zf = l[2] * c.fov;
+    float dxd0 = -1.f / zf;
+    float dxd2 = c.fov * l[0] / (zf * zf);
+    float zfy = zf * c.aspect;
+    float dyd1 = -1.f / zfy;
+    float dyd2 = c.fov * c.aspect * l[1] / (zfy * zfy);
+    return v3(dx * dxd0, dy * dyd1, dx * dxd2 + dy * dyd2);
+}
+// camera.h:244-313 (clip uses -(z + clip_near) here; d_cam_to_sample untouched)
+void d_project(const Camera &c, const V3f &p0, const V3f &p1, float d0x, float d0y, float d1x, float d1y,
+               float dw2c[16]) {
+    V3f a = xfm_point(c.w2c, p0), b = xfm_point(c.w2c, p1);
+    V3f ca = a, cb = b;
+    if (a[2] < c.clip_near) {
+        V3f d = a - b;
+        float t = -(b[2] + c.clip_near) / d[2];
+        ca = b + t * d;
+    } else if (b[2] < c.clip_near) {
+        V3f d = b - a;
+        float t = -(a[2] + c.clip_near) / d[2];
+        cb = a + t * d;
+    }
+    V3f dca = d_project_local(c, ca, d0x, d0y);
+    V3f dcb = d_project_local(c, cb, d1x, d1y);
+    V3f da = v3(0.f, 0.f, 0.f), db = v3(0.f, 0.f, 0.f);
+    if (a[2] < c.clip_near) {
+        V3f d = a - b;
+        float t = -(b[2] + c.clip_near) / d[2];
+        db += dca;
+        float dt = dot(d, dca);
+        V3f dd = t * dca;
+        db[2] += (-dt / d[2]);
+        dd[2] += dt * (b[2] / (d[2] * d[2]));
+        da += dd;
+        db = db - dd;
+        db += dcb;
+    } else if (b[2] < c.clip_near) {
+        V3f d = b - a;
+        float t = -(a[2] + c.clip_near) / d[2];
+        da += dcb;
+        float dt = dot(d, dcb);
+        V3f dd = t * dcb;
+        da[2] += (-dt / d[2]);
+        dd[2] += dt * (a[2] / (d[2] * d[2]));
+        db += dd;
+        da = da - dd;
+        da += dca;
+    } else {
+        da += dca;
+        db += dcb;
+    }
+    d_xfm_point(c.w2c, p0, da, dw2c);
+    d_xfm_point(c.w2c, p1, db, dw2c);
+}
+
+// ---------------------------------------------------------- silhouette ---
+// edge.h:59-79
+bool is_silhouette(const Scene &sc, const V3f &p, const Edge &e) {
+    const Shape &s = sc.shapes[e.shape];
+    if (!s.has_n) return true;
+    if (e.f0 == -1 || e.f1 == -1) return true;
+    V3f v0 = s.vert(e.v0), v1 = s.vert(e.v1);
+    auto ns = [&](int f) {
+        for (int k = 0; k < 3; k++) {
+            int id = s.idx(f, k);
+            if (id != e.v0 && id != e.v1) return s.vert(id);
+        }
+        return v3(0.f, 0.f, 0.f);
+    };
+    V3f a = ns(e.f0), b = ns(e.f1);
+    V3f n0 = normalize(cross(v0 - a, v1 - a));
+    V3f n1 = normalize(cross(v1 - b, v0 - b));
+    bool f0 = dot(p - v0, n0) > 0.f && dot(p - v1, n0) > 0.f && dot(p - a, n0) > 0.f;
+    bool f1 = dot(p - v0, n1) > 0.f && dot(p - v1, n1) > 0.f && dot(p - b, n1) > 0.f;
+    return (f0 && !f1) || (!f0 && f1);
+}
+
+// ------------------------------------------------------- edge sampling ---
+struct EdgeW {
+    Edge e;
+    float w;
+};
+
+static float min_sq_bound(float lo, float hi) {
+    if (lo <= 0.f && hi >= 0.f) return 0.f;
+    if (lo < 0.f && hi < 0.f) return hi * hi;
+    return lo * lo;
+}
+static float max_sq_bound(float lo, float hi) { return std::max(lo * lo, hi * hi); }
+
+// edge_tree.cpp:345-480
+float bsdf_bound(const Scene &sc, const EdgeNode &node, const V3f &wi, int mid, const SP<float> &sp,
+                 float dlum, float slum) {
+    const AABB &bx = node.bounds;
+    float cos_max = 0.f;
+    for (int i = 0; i < 8; i++)
+        if (dot(sp.fr[2], bx.corner(i) - sp.pos) > 0.f) { cos_max = 1.f; break; }
+    if (cos_max <= 0.f) return 0.f;
+    float phong_resp = 0.f;
+    if (slum > 0.f) {
+        V3f hy = normalize(cross(sp.fr[2], wi));
+        Frame<float> hf{{normalize(cross(wi, hy)), hy, wi}};
+        AABB hb;
+        for (int i = 0; i < 8; i++) hb = merge(hb, to_local(hf, bx.corner(i) - sp.pos));
+        float cos_phi_h_max = 0.f;
+        float hx = hb.hi[0];
+        if (hx > 0.f) {
+            float ysq = min_sq_bound(hb.lo[1], hb.hi[1]);
+            cos_phi_h_max = hx / std::sqrt(ysq + hx * hx);
+        } else if (hx == 0.f) {
+            cos_phi_h_max = max(cos_phi_h_max, 0.f);
+        } else {
+            float ysq = max_sq_bound(hb.lo[1], hb.hi[1]);
+            cos_phi_h_max = hx / std::sqrt(ysq + hx * hx);
+        }
+        float cto_min, cto_max;
+        if (hb.lo[2] > 0.f) {
+            float xs = max_sq_bound(hb.lo[0], hb.hi[0]), ys = max_sq_bound(hb.lo[1], hb.hi[1]);
+            cto_min = hb.lo[2] / (xs + ys + hb.lo[2] * hb.lo[2]);
+        } else {
+            float xs = min_sq_bound(hb.lo[0], hb.hi[0]), ys = min_sq_bound(hb.lo[1], hb.hi[1]);
+            cto_min = hb.lo[2] / (xs + ys + hb.lo[2] * hb.lo[2]);
+        }
+        if (hb.hi[2] > 0.f) {
+            float xs = min_sq_bound(hb.lo[0], hb.hi[0]), ys = min_sq_bound(hb.lo[1], hb.hi[1]);
+            cto_max = hb.hi[2] / (xs + ys + hb.hi[2] * hb.hi[2]);
+        } else {
+            float xs = max_sq_bound(hb.lo[0], hb.hi[0]), ys = max_sq_bound(hb.lo[1], hb.hi[1]);
+            cto_max = hb.hi[2] / (xs + ys + hb.hi[2] * hb.hi[2]);
+        }
+        float cth_min = std::sqrt((cto_min + 1.f) / 2.f);
+        float cth_max = std::sqrt((cto_max + 1.f) / 2.f);
+        float ch = 0.f;
+        V3f ln = to_local(hf, sp.fr[2]);
+        if (cos_phi_h_max >= 0.f) {
+            ch = std::sqrt(ln[2] * ln[2] / (ln[2] * ln[2] + ln[0] * ln[0] * cos_phi_h_max * cos_phi_h_max));
+        } else {
+            ch = 1.f;
+        }
+        if (ch > cth_max) ch = cth_max;
+        else if (ch < cth_min) ch = cth_min;
+        float sh = std::sqrt(max(1.f - ch * ch, 0.f));
+        float hn = min(std::fabs(ln[0] * sh * cos_phi_h_max) + ln[2] * ch, 1.f);
+        float e = roughness_to_phong(get_roughness<float>(sc.mats[mid], sp.uv, nullptr));
+        phong_resp = std::pow(hn, e) * (e + 2.f) / float(2 * M_PI);
+    }
+    return cos_max * (dlum / float(M_PI) + slum * phong_resp);
+}
+
+// edge_tree.cpp:482-497
+bool contains_silhouette(const EdgeNode &n, const V3f &p) {
+    V3f c;
+    float r;
+    n.bounds.sphere(c, r);
+    V3f d = c - p;
+    float len = length(d);
+    float tan_v = r / len;
+    if (tan_v >= 1.f) return true;
+    float va = std::atan(tan_v);
+    d = d / len;
+    float ang = std::acos(dot(-d, n.cone_axis));
+    float range = n.cone_angle + va;
+    return ang - range <= float(M_PI / 2.f) && ang + range >= float(M_PI / 2.f);
+}
+
+struct TraverseCtx {
+    const Scene *sc;
+    const EdgeTree *tree;
+    bool directional;
+    V3f wi;
+    int mid;
+    SP<float> sp;
+    float dlum, slum;
+    Cone cone;
+    std::vector<EdgeW> *out;
+    Philox *rng;
+};
+
+// edge_tree.cpp:511-652
+void sample_edge_rec(TraverseCtx &c, bool cone_mode, int node_id, float nb, float cone_pmf, float all_pmf) {
+    if (nb <= 0.f) return;
+    const EdgeTree &t = *c.tree;
+    const EdgeNode &node = t.nodes[node_id];
+    if (node.nprims > 0) {
+        float w = cone_mode ? cone_pmf / (cone_pmf * cone_pmf + all_pmf * all_pmf)
+                            : all_pmf / (cone_pmf * cone_pmf + all_pmf * all_pmf);
+        for (int i = 0; i < node.nprims; i++) {
+            const Edge &e = t.edges[node.offset + i];
+            if (is_silhouette(*c.sc, c.sp.pos, e)) c.out->push_back(EdgeW{e, w});
+        }
+        return;
+    }
+    int li = node_id + 1, ri = node.offset;
+    const EdgeNode &L = t.nodes[li], &R = t.nodes[ri];
+    bool all_split = nb > 1.f || node.bounds.inside(c.sp.pos);
+    bool lcs = true, rcs = true;
+    if (c.directional) {
+        lcs = contains_silhouette(L, c.sp.pos);
+        rcs = contains_silhouette(R, c.sp.pos);
+    }
+    float lb = lcs ? bsdf_bound(*c.sc, L, c.wi, c.mid, c.sp, c.dlum, c.slum) : 0.f;
+    float rb = rcs ? bsdf_bound(*c.sc, R, c.wi, c.mid, c.sp, c.dlum, c.slum) : 0.f;
+    float il = lcs ? lb * L.total_length / length_squared(L.bounds.center() - c.sp.pos) : 0.f;
+    float ir = rcs ? rb * R.total_length / length_squared(R.bounds.center() - c.sp.pos) : 0.f;
+    if (il + ir <= 1e-10f) return;
+    bool lhc = aabb_cone_intersect(L.bounds, c.cone);
+    bool rhc = aabb_cone_intersect(R.bounds, c.cone);
+    bool cone_split = all_split || (lhc && rhc && lcs && rcs && c.cone.cos_angle > 0.95f);
+    float ilc = lhc ? il : 0.f, irc = rhc ? ir : 0.f;
+    if (cone_mode && ilc + irc <= 1e-10f) return;
+    float plc = 0.f, prc = 0.f, pla = 0.f, pra = 0.f;
+    if (cone_split) {
+        plc = prc = 1.f;
+    } else if (ilc + irc > 1e-10f) {
+        plc = ilc / (ilc + irc);
+        prc = 1.f - plc;
+    }
+    if (all_split) {
+        pla = pra = 1.f;
+    } else {
+        pla = il / (il + ir);
+        pra = 1.f - pla;
+    }
+    bool split = cone_mode ? cone_split : all_split;
+    if (split) {
+        if (lcs) sample_edge_rec(c, cone_mode, li, lb, plc * cone_pmf, pla * all_pmf);
+        if (rcs) sample_edge_rec(c, cone_mode, ri, rb, prc * cone_pmf, pra * all_pmf);
+    } else {
+        float u = c.rng->next();
+        bool left = cone_mode ? u < plc : u < pla;
+        if (left) sample_edge_rec(c, cone_mode, li, lb, plc * cone_pmf, pla * all_pmf);
+        else sample_edge_rec(c, cone_mode, ri, rb, prc * cone_pmf, pra * all_pmf);
+    }
+}
+
+void sample_tree(const Scene &sc, const EdgeTree &tree, bool directional, const V3f &wi, const Isect &is,
+                 const SP<float> &sp, const Cone &cone, float pmf, std::vector<EdgeW> &out, Philox &rng) {
+    if (tree.nodes.empty()) return;
+    int mid = sc.shapes[is.shape].mat;
+    const Material &m = sc.mats[mid];
+    float dlum = luminance(get_diffuse<float>(m, sp.uv, nullptr));
+    float slum = luminance(get_specular<float>(m, sp.uv, nullptr));
+    TraverseCtx c{&sc, &tree, directional, wi, mid, sp, dlum, slum, cone, &out, &rng};
+    float b = bsdf_bound(sc, tree.nodes[0], wi, mid, sp, dlum, slum);
+    sample_edge_rec(c, true, 0, b, pmf, pmf);
+    sample_edge_rec(c, false, 0, b, pmf, pmf);
+}
+
+// edge_tree.cpp:685-725
+void sample_shading_edges(const Scene &sc, const V3f &wi, const Isect &is, const SP<float> &sp, const Cone &cone,
+                          std::vector<EdgeW> &out, Philox &rng) {
+    if (dot(wi, sp.fr[2]) < 0.f) return;
+    if (sc.nrm_tree.nodes.empty()) {
+        sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, 1.f, out, rng);
+        return;
+    }
+    if (sc.sil_tree.nodes.empty()) {
+        sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, 1.f, out, rng);
+        return;
+    }
+    float ni = sc.nrm_tree.nodes[0].total_length, si = sc.sil_tree.nodes[0].total_length;
+    float npmf = ni / (ni + si), spmf = si / (ni + si);
+    float u = rng.next();
+    if (u < npmf) sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, npmf, out, rng);
+    else sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, spmf, out, rng);
+}
+
+struct EdgeResult {
+    V3f dp, dv0, dv1;
+};
+
+// delta_ray.cpp:45-365
+EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, float eu, const Edge &edge,
+                                     const LightSample &ls, const std::vector<SecondarySample> &ss, const V3f &wi,
+                                     const Isect &is, const SP<float> &sp, const V3f &weight, const V3f &d_color,
+                                     bool direct_only) {
+    EdgeResult R{v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f)};
+    if (!is.valid()) return R;
+    if (!is_silhouette(sc, sp.pos, edge)) return R;
+    const Shape &es = sc.shapes[edge.shape];
+    V3f v0 = es.vert(edge.v0), v1 = es.vert(edge.v1);
+    if (abs_sum(v1 - v0) == 1e-10f) return R;
+    int mid = sc.shapes[is.shape].mat;
+    const Material &m = sc.mats[mid];
+    float dw = luminance(get_diffuse<float>(m, sp.uv, nullptr));
+    float sw = luminance(get_specular<float>(m, sp.uv, nullptr));
+    float ws = dw + sw;
+    if (ws <= 0.f) return R;
+    float dpmf = dw / ws, spmf = sw / ws;
+    V3f n = sp.fr[2];
+    V3f fx = normalize(wi - n * dot(wi, n));
+    V3f fy = cross(n, fx);
+    Frame<float> iso{{fx, fy, n}};
+    M3 M, Minv;
+    float m_pmf;
+    if (bsdf_component <= dpmf) {
+        Minv = m3_from_frame(iso);
+        M = inverse(Minv);
+        m_pmf = dpmf;
+    } else {
+        Minv = inverse(get_ltc_matrix(sc, mid, sp, wi)) * m3_from_frame(iso);
+        M = inverse(Minv);
+        m_pmf = spmf;
+    }
+    V3f a = Minv * (v0 - sp.pos);
+    V3f b = Minv * (v1 - sp.pos);
+    if (a[2] <= 0.f && b[2] <= 0.f) return R;
+    if (a[2] < 0.f) a = (a * b[2] - b * a[2]) / (b[2] - a[2]);
+    if (b[2] < 0.f) b = (a * b[2] - b * a[2]) / (b[2] - a[2]);
+    V3f vd = b - a;
+    if (abs_sum(vd) < 1e-10f) return R;
+    V3f wt = normalize(vd);
+    float l0 = dot(a, wt), l1 = dot(b, wt);
+    V3f vo = a - l0 * wt;
+    float d = length(vo);
+    auto I = [&](float l) {
+        return (l / (d * (d * d + l * l)) + std::atan(l / d) / (d * d)) * vo[2] + (l * l / (d * (d * d + l * l))) * wt[2];
+    };
+    float Il0 = I(l0), Il1 = I(l1);
+    float norm = Il1 - Il0;
+    if (norm <= 1e-10f) return R;
+    auto cdf = [&](float l) { return (I(l) - Il0) / norm; };
+    auto pdf = [&](float l) {
+        float dsq = d * d + l * l;
+        return 2.f * d * (vo + l * wt)[2] / (norm * dsq * dsq);
+    };
+    float lb = l0, ub = l1;
+    if (lb > ub) std::swap(lb, ub);
+    float l = 0.5f * (lb + ub);
+    for (int it = 0; it < 20; it++) {
+        if (!(l >= lb && l <= ub)) l = 0.5f * (lb + ub);
+        float value = cdf(l) - eu;
+        if (std::fabs(value) < 1e-5f || it == 19) break;
+        if (value > 0.f) ub = l;
+        else lb = l;
+        l -= value / pdf(l);
+    }
+    if (pdf(l) <= 0.f) return R;
+    V3f sample_p = M * (vo + l * wt);
+    V3f spn = normalize(cross(v0 - sp.pos, v1 - sp.pos));
+    float delta = 1e-3f * length(sample_p);
+    V3f sdir = normalize(sample_p);
+    V3f rdir = normalize(sdir + delta * spn);
+    V3f ldir = normalize(sdir - delta * spn);
+    SP<float> spr, spl;
+    Isect ir = intersect<float>(sc, sp.pos, rdir, nullptr, spr);
+    Isect il = intersect<float>(sc, sp.pos, ldir, nullptr, spl);
+    bool rc = ir.shape == edge.shape && (ir.tri == edge.f0 || ir.tri == edge.f1);
+    bool lc = il.shape == edge.shape && (il.tri == edge.f0 || il.tri == edge.f1);
+    if ((!rc && !lc) || (rc && lc) || (ir.shape == is.shape && ir.tri == is.tri) ||
+        (il.shape == is.shape && il.tri == is.tri))
+        return R;
+    V3f eb = bsdf<float>(sc, mid, sp, wi, sdir, nullptr);
+    if (sum(eb) < 1e-6f) return R;
+    V3f shr = v3(0.f, 0.f, 0.f), shl = v3(0.f, 0.f, 0.f);
+    if (!direct_only) {
+        shr = shade<float>(sc, -sdir, ir, spr, ls, ss, false, nullptr, nullptr, nullptr);
+        shl = shade<float>(sc, -sdir, il, spl, ls, ss, false, nullptr, nullptr, nullptr);
+        auto fix = [&](const Isect &h, const SP<float> &hp, V3f &sh) {
+            if (!h.valid()) return;
+            V3f dd = hp.pos - sp.pos;
+            float dsq = length_squared(dd);
+            if (dsq > 0.f) {
+                sh = sh / std::max(dsq, 1e-2f);
+                sh = sh * std::fabs(dot(normalize(dd), hp.ng));
+            } else {
+                sh = v3(0.f, 0.f, 0.f);
+            }
+        };
+        fix(ir, spr, shr);
+        fix(il, spl, shl);
+    }
+    V3f er = eval<float>(sc, wi, is, sp, ir, spr, nullptr) + eb * shr;
+    V3f el = eval<float>(sc, wi, is, sp, il, spl, nullptr) + eb * shl;
+    auto too_close = [&](const Isect &h, const SP<float> &hp) {
+        if (!h.valid()) return false;
+        V3f dd = hp.pos - sp.pos;
+        float dist = length(dd);
+        float ct = std::fabs(dot(normalize(dd), hp.ng));
+        return dist * ct < 1e-3f;
+    };
+    if (too_close(ir, spr) || too_close(il, spl)) return R;
+    auto accum = [&](const SP<float> &hp, const V3f &ev) {
+        M3 nj = normalize_jacobian(sample_p);
+        M3 ij = intersect_jacobian(sdir, hp.pos, hp.ng);
+        float line_j = length(((ij * nj) * M) * wt) / (m_pmf * pdf(l));
+        V3f p = sp.pos, x = hp.pos;
+        V3f d0 = v0 - p, d1 = v1 - p;
+        float dirac_j = length(cross(d0, d1));
+        float w = line_j / dirac_j;
+        V3f dp = w * (cross(d1, d0) + cross(x - p, d1) + cross(d0, x - p));
+        V3f dv0 = w * cross(d1, x - p);
+        V3f dv1 = w * cross(x - p, d0);
+        float we = dot(ev * weight, d_color);
+        R.dp += dp * we;
+        R.dv0 += dv0 * we;
+        R.dv1 += dv1 * we;
+    };
+    if (ir.valid() && abs_sum(er) > 0.f) accum(spr, er);
+    if (il.valid() && abs_sum(el) > 0.f) accum(spl, -el);
+    return R;
+}
+
+// ------------------------------------------------------ primary edges ----
+struct PrimaryResult {
+    bool ok = false;
+    float sx = -1.f, sy = -1.f;
+    V3f dv0, dv1;
+    float dw2c[16];
+};
+
+inline bool in_screen(const V2f &p) { return p[0] >= 0.f && p[0] < 1.f && p[1] >= 0.f && p[1] < 1.f; }
+
+// delta_ray.cpp:377-557
+PrimaryResult sample_primary_edge_point(const Scene &sc, const Edge &edge, float eu, const LightSample &ls,
+                                        const std::vector<SecondarySample> &ss, const float *d_image, int W, int H,
+                                        float weight) {
+    PrimaryResult R;
+    for (int i = 0; i < 16; i++) R.dw2c[i] = 0.f;
+    R.dv0 = R.dv1 = v3(0.f, 0.f, 0.f);
+    const Camera &cam = sc.cam;
+    const Shape &es = sc.shapes[edge.shape];
+    V3f v0 = es.vert(edge.v0), v1 = es.vert(edge.v1);
+    V2f v0s, v1s;
+    if (!project_edge(cam, v0, v1, v0s, v1s)) return R;
+    V3f v0d = unproject(cam, v0s), v1d = unproject(cam, v1s);
+    V3f vd3 = v1d - v0d;
+    float t = eu;
+    V3f v0l = xfm_point(cam.w2c, v0), v1l = xfm_point(cam.w2c, v1);
+    V3f ep3 = v0d + t * vd3;
+    V3f elocal = v0l + t * v1l;
+    V2f ep = project_local(cam, ep3);
+    if (!in_screen(ep)) return R;
+    int xi = int(ep[0] * W), yi = int(ep[1] * H);
+    const float *dc = &d_image[((size_t)yi * W + xi) * 3];
+    V3f d_color = v3(dc[0], dc[1], dc[2]);
+    float delta = 1e-4f * length(elocal);
+    V3f spn = normalize(cross(v0d, v1d));
+    V3f rdir = normalize(ep3 + delta * spn);
+    V2f rpt = project_local(cam, rdir);
+    V3f rorg, rd;
+    sample_primary(cam.c2w, cam.fov, cam.aspect, rpt, rorg, rd);
+    SP<float> spr, spl;
+    Isect ir = intersect<float>(sc, rorg, rd, nullptr, spr);
+    V3f ldir = normalize(ep3 - delta * spn);
+    V2f lpt = project_local(cam, ldir);
+    V3f lorg, ld;
+    sample_primary(cam.c2w, cam.fov, cam.aspect, lpt, lorg, ld);
+    Isect il = intersect<float>(sc, lorg, ld, nullptr, spl);
+    bool rc = ir.shape == edge.shape && (ir.tri == edge.f0 || ir.tri == edge.f1);
+    bool lc = il.shape == edge.shape && (il.tri == edge.f0 || il.tri == edge.f1);
+    if (!rc && !lc) return R;
+    V3f rcol = shade<float>(sc, -rd, ir, spr, ls, ss, true, nullptr, nullptr, nullptr);
+    V3f lcol = shade<float>(sc, -ld, il, spl, ls, ss, true, nullptr, nullptr, nullptr);
+    V3f diff = (rcol - lcol) * weight;
+    V3f xdiff = diff, ydiff = diff;
+    if (lpt[0] > rpt[0]) xdiff = -xdiff;
+    if (lpt[1] > rpt[1]) ydiff = -ydiff;
+    V3f edir = unproject(cam, ep);
+    V3f dx = v3(-2.f * cam.fov, 0.f, 0.f);
+    V3f dy = v3(0.f, -2.f * cam.fov / cam.aspect, 0.f);
+    V3f c01 = cross(v0d, v1d);
+    float dadx = dot(dx, c01), dady = dot(dy, c01);
+    float da0x = std::fabs(dot(cross(v1d, edir), dx));
+    float da0y = std::fabs(dot(cross(v1d, edir), dy));
+    float da1x = std::fabs(dot(cross(edir, v0d), dx));
+    float da1y = std::fabs(dot(cross(edir, v0d), dy));
+    float dirac_j = 1.f / std::sqrt(dadx * dadx + dady * dady);
+    V3f ep3d = v0d + (t + delta) * vd3;
+    V2f epd = project_local(cam, ep3d);
+    float line_j = length2((epd - ep) / delta);
+    float jac = line_j * dirac_j;
+    float d0x = dot(-xdiff * da0x * jac, d_color);
+    float d0y = dot(-ydiff * da0y * jac, d_color);
+    float d1x = dot(-xdiff * da1x * jac, d_color);
+    float d1y = dot(-ydiff * da1y * jac, d_color);
+    const float fd = 1e-3f;
+    V2f p[6][2];
+    V3f ex = v3(fd, 0.f, 0.f), ey = v3(0.f, fd, 0.f), ez = v3(0.f, 0.f, fd);
+    if (!project_edge(cam, v0 + ex, v1, p[0][0], p[0][1]) || !project_edge(cam, v0 + ey, v1, p[1][0], p[1][1]) ||
+        !project_edge(cam, v0 + ez, v1, p[2][0], p[2][1]) || !project_edge(cam, v0, v1 + ex, p[3][0], p[3][1]) ||
+        !project_edge(cam, v0, v1 + ey, p[4][0], p[4][1]) || !project_edge(cam, v0, v1 + ez, p[5][0], p[5][1]))
+        return R;
+    float g[6];
+    for (int k = 0; k < 6; k++) {
+        V2f a = (p[k][0] - v0s) / fd;
+        V2f b = (p[k][1] - v1s) / fd;
+        g[k] = d0x * a[0] + d0y * a[1] + d1x * b[0] + d1y * b[1];
+    }
+    R.dv0 = v3(g[0], g[1], g[2]);
+    R.dv1 = v3(g[3], g[4], g[5]);
+    d_project(cam, v0, v1, d0x, d0y, d1x, d1y, R.dw2c);
+    R.sx = ep[0];
+    R.sy = ep[1];
+    R.ok = true;
+    return R;
+}
+
+// ------------------------------------------------------------ driver -----
+struct GradLayout {
+    std::vector<size_t> diffuse, specular, roughness, vertices, normals, intensity;
+    size_t total;
+};
+
+struct OScene {
+    Scene sc;
+    GradLayout lay;
+};
+
+struct Accum {
+    std::vector<double> g;  // gradients (double accumulation per thread)
+    uint64_t rays = 0;
+};
+
+// scatter one material channel adjoint
+template <class TP, int C>
+void scatter_tex(const Tex &t, const TP &p, size_t off, bool swap, std::vector<double> &g,
+                 float (*get)(const TP &, int, int), bool roughness_const_finite_only) {
+    if (t.h == 0) {
+        for (int k = 0; k < C; k++) {
+            float d = get(p, 0, k);
+            if (roughness_const_finite_only && !std::isfinite(d)) continue;
+            g[off + k] += d;
+        }
+        return;
+    }
+    if (p.xi < 0) return;
+    int w = t.w, h = t.h;
+    int xf = p.xi, yf = p.yi, xc = modulo(xf + 1, w), yc = modulo(yf + 1, h);
+    // corner order t00 (yf,xf), t01 (yc,xf), t10 (yf,xc), t11 (yc,xc)
+    int ys[4] = {yf, yc, yf, yc}, xs[4] = {xf, xf, xc, xc};
+    if (swap) {
+        std::swap(ys[0], ys[3]);
+        std::swap(xs[0], xs[3]);
+    }
+    for (int q = 0; q < 4; q++)
+        for (int k = 0; k < C; k++) g[off + ((size_t)ys[q] * w + xs[q]) * C + k] += get(p, q, k);
+}
+float get3(const TexP3<ADR> &p, int q, int k) {
+    const V3<ADR> &v = q == 0 ? p.t00 : q == 1 ? p.t01 : q == 2 ? p.t10 : p.t11;
+    return get_adjoint(v[k]);
+}
+float get1(const TexP1<ADR> &p, int q, int) {
+    const ADR &v = q == 0 ? p.t00 : q == 1 ? p.t01 : q == 2 ? p.t10 : p.t11;
+    return get_adjoint(v);
+}
+
+struct RenderCtx {
+    const OScene *os;
+    const dr_render_options *o;
+    const float *d_image;
+    bool adjoint;
+};
+
+// One pixel sample.  Returns its colour contribution (already / spp).
+V3f render_pixel_sample(const RenderCtx &rc, int xi, int yi, int s, Accum &acc, Tape &tape, float *dx_count) {
+    const Scene &sc = rc.os->sc;
+    const dr_render_options &o = *rc.o;
+    const GradLayout &lay = rc.os->lay;
+    uint64_t sid = ((uint64_t)yi * o.width + xi) * (uint64_t)o.num_samples + s;
+    Philox rng(o.seed, 0, sid, STREAM_MAIN);
+    float ux = rng.next(), uy = rng.next();
+    float sub_x = (o.flags & DR_QUIRK_JITTER_OVER_SPP) ? ux / o.num_samples : ux;
+    float sub_y = (o.flags & DR_QUIRK_JITTER_OVER_SPP) ? uy / o.num_samples : uy;
+    float sample_weight = 1.f / o.num_samples;
+    LightSample ls = draw_light_sample(rng);
+    std::vector<SecondarySample> ss;
+    for (int i = 0; i < o.max_bounces; i++) ss.push_back(draw_secondary(rng));
+    V3f d_color = v3(0.f, 0.f, 0.f);
+    if (rc.adjoint) {
+        const float *q = &rc.d_image[((size_t)yi * o.width + xi) * 3];
+        d_color = v3(q[0], q[1], q[2]);
+    }
+    if (!rc.adjoint || is_zero(d_color)) {
+        float x = float((xi + sub_x) / o.width), y = float((yi + sub_y) / o.height);
+        V3f org, dir;
+        sample_primary(sc.cam.c2w, sc.cam.fov, sc.cam.aspect, v2(x, y), org, dir);
+        SP<float> sp;
+        Isect is = intersect<float>(sc, org, dir, nullptr, sp);
+        if (!is.valid()) return v3(0.f, 0.f, 0.f);
+        return shade<float>(sc, -dir, is, sp, ls, ss, true, nullptr, nullptr, nullptr) * sample_weight;
+    }
+    // ---- forward with tape (delta_ray.cpp:694-866)
+    g_tape = &tape;
+    tape.clear();
+    CamT cam;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) cam.c2w.m[i][j] = ADR(sc.cam.c2w.m[i][j]);
+    cam.fov = ADR(sc.cam.fov);
+    cam.aspect = ADR(sc.cam.aspect);
+    ADR x = ADR((xi + sub_x) / o.width), y = ADR((yi + sub_y) / o.height);
+    V3<ADR> org, dir;
+    sample_primary(cam.c2w, cam.fov, cam.aspect, v2(x, y), org, dir);
+    Rec<ADR> rec;
+    SP<ADR> sp;
+    Isect pis = intersect<ADR>(sc, org, dir, &rec, sp);
+    if (!pis.valid()) return v3(0.f, 0.f, 0.f);
+    std::vector<SecondaryPointF> spts;
+    std::vector<SP<ADR>> spts_ad;
+    V3<ADR> color = shade<ADR>(sc, -dir, pis, sp, ls, ss, true, &spts, &spts_ad, &rec);
+    color = color * sample_weight;
+    V3f result = conv3<float>(color);
+    std::vector<double> &g = acc.g;
+    const bool skip_int = (o.flags & DR_SKIP_INTERIOR) != 0;
+    auto scatter_camera_and_vertices = [&](bool with_normals) {
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) g[i * 4 + j] += get_adjoint(cam.c2w.m[i][j]);
+        for (const IsP<ADR> &p : rec.is) {
+            const Shape &s = sc.shapes[p.shape];
+            size_t vo = lay.vertices[p.shape];
+            const V3<ADR> *vv[3] = {&p.v0, &p.v1, &p.v2};
+            for (int k = 0; k < 3; k++) {
+                int vid = s.idx(p.tri, k);
+                for (int c = 0; c < 3; c++) g[vo + (size_t)vid * 3 + c] += get_adjoint((*vv[k])[c]);
+            }
+            if (with_normals && s.has_n) {
+                size_t no = lay.normals[p.shape];
+                const V3<ADR> *nn[3] = {&p.n0, &p.n1, &p.n2};
+                for (int k = 0; k < 3; k++) {
+                    int vid = s.idx(p.tri, k);
+                    const V3<ADR> &src = (o.flags & DR_QUIRK_NORMAL_DN0) ? *nn[0] : *nn[k];
+                    for (int c = 0; c < 3; c++) g[no + (size_t)vid * 3 + c] += get_adjoint(src[c]);
+                }
+            }
+        }
+    };
+    if (!skip_int) {
+        tape.zero_adjoints();
+        for (int k = 0; k < 3; k++) set_adjoint(color[k], d_color[k]);
+        tape.propagate();
+        scatter_camera_and_vertices(true);
+        bool sw = (o.flags & DR_QUIRK_TEXEL_SWAP) != 0;
+        for (const MatP<ADR> &p : rec.mat) {
+            const Material &m = sc.mats[p.mat];
+            scatter_tex<TexP3<ADR>, 3>(m.diffuse, p.d, lay.diffuse[p.mat], sw, g, get3, false);
+            scatter_tex<TexP3<ADR>, 3>(m.specular, p.s, lay.specular[p.mat], sw, g, get3, false);
+            scatter_tex<TexP1<ADR>, 1>(m.roughness, p.r, lay.roughness[p.mat], sw, g, get1, true);
+        }
+        for (const LiP<ADR> &p : rec.li)
+            for (int c = 0; c < 3; c++) g[lay.intensity[p.light] + c] += get_adjoint(p.I[c]);
+    }
+    // ---- secondary edges (delta_ray.cpp:868-966)
+    std::vector<std::pair<int, V3f>> dps;
+    int last_edge_count = 0;
+    if (!(o.flags & DR_SKIP_SECONDARY_EDGES)) {
+        std::vector<EdgeW> edges;
+        for (const SecondaryPointF &p : spts) {
+            Philox prng(o.seed, 0, sid, stream_point(p.bounce));
+            float bsdf_component = prng.next(), eu = prng.next();
+            int lid = sc.light_dist.sample_discrete(ls.light_sel);
+            V3f bc;
+            float br;
+            sc.light_bounds[lid].sphere(bc, br);
+            V3f dl = bc - p.spf.pos;
+            float tan_a = br / length(dl);
+            float cos_a = tan_a <= 1.f ? 1.f / std::sqrt(tan_a * tan_a + 1.f) : 0.f;
+            Cone cone{p.spf.pos, normalize(dl), cos_a};
+            edges.clear();
+            sample_shading_edges(sc, p.wi, p.is, p.spf, cone, edges, prng);
+            last_edge_count = (int)edges.size();
+            for (int j = 0; j < (int)edges.size(); j++) {
+                const EdgeW &ew = edges[j];
+                if (ew.w == 0.f) continue;
+                Philox erng(o.seed, 0, sid, stream_edge(p.bounce, j));
+                LightSample els = draw_light_sample(erng);
+                std::vector<SecondarySample> ess;
+                for (int i = 0; i < o.max_bounces - p.bounce - 1; i++) ess.push_back(draw_secondary(erng));
+                V3f weight = p.throughput * (sample_weight * ew.w);
+                EdgeResult er = sample_shading_edge_point(sc, bsdf_component, eu, ew.e, els, ess, p.wi, p.is, p.spf,
+                                                          weight, d_color, o.max_bounces == 1);
+                if (!is_zero(er.dp)) dps.push_back(std::make_pair(p.ad_slot, er.dp));
+                if (!is_zero(er.dv0) || !is_zero(er.dv1)) {
+                    size_t vo = lay.vertices[ew.e.shape];
+                    for (int c = 0; c < 3; c++) {
+                        g[vo + (size_t)ew.e.v0 * 3 + c] += er.dv0[c];
+                        g[vo + (size_t)ew.e.v1 * 3 + c] += er.dv1[c];
+                    }
+                }
+            }
+        }
+    }
+    if (dx_count) *dx_count += last_edge_count / 1000.f;
+    // ---- Dirac position propagation (delta_ray.cpp:968-1002)
+    if (!dps.empty()) {
+        tape.zero_adjoints();
+        for (const auto &d : dps) {
+            const V3<ADR> &pp = spts_ad[d.first].pos;
+            for (int k = 0; k < 3; k++) {
+                if (o.flags & DR_QUIRK_DP_LAST) set_adjoint(pp[k], d.second[k]);
+                else if (pp[k].id >= 0) tape.adj[pp[k].id] += d.second[k];
+            }
+        }
+        tape.propagate();
+        scatter_camera_and_vertices(false);
+    }
+    return result;
+}
+
+void render_primary_edge(const RenderCtx &rc, int64_t k, Accum &acc) {
+    const Scene &sc = rc.os->sc;
+    const dr_render_options &o = *rc.o;
+    if (sc.primary_edges.empty()) return;
+    Philox rng(o.seed, 1, (uint64_t)k, STREAM_MAIN);
+    float u = rng.next();
+    float pmf = 0.f;
+    int id = sc.primary_dist.sample_discrete(u, &pmf);
+    float w = pmf > 0.f ? 1.f / pmf : 0.f;
+    if (w <= 0.f) return;
+    float bsdf_component = rng.next();
+    (void)bsdf_component;
+    float eu = rng.next();
+    LightSample ls = draw_light_sample(rng);
+    std::vector<SecondarySample> ss;
+    for (int i = 0; i < o.max_bounces; i++) ss.push_back(draw_secondary(rng));
+    const Edge &e = sc.primary_edges[id];
+    PrimaryResult R = sample_primary_edge_point(sc, e, eu, ls, ss, rc.d_image, o.width, o.height, w / o.num_samples);
+    if (!R.ok) return;
+    if (R.sx < 0.f || R.sy < 0.f || R.sx > 1.f || R.sy > 1.f) return;
+    const GradLayout &lay = rc.os->lay;
+    std::vector<double> &g = acc.g;
+    size_t vo = lay.vertices[e.shape];
+    for (int c = 0; c < 3; c++) {
+        g[vo + (size_t)e.v0 * 3 + c] += R.dv0[c];
+        g[vo + (size_t)e.v1 * 3 + c] += R.dv1[c];
+    }
+    for (int i = 0; i < 16; i++) g[16 + i] += R.dw2c[i];
+}
+
+}  // namespace oracle
+
+// ============================================================== C-ABI ====
+using namespace oracle;
+
+extern "C" {
+
+int or_load_ltc(const char *path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return 1;
+    g_ltc.resize(16384 * 9);
+    f.read((char *)g_ltc.data(), g_ltc.size() * sizeof(float));
+    return f ? 0 : 2;
+}
+
+void *or_scene_create(const dr_scene_desc *d) {
+    OScene *os = new OScene();
+    build_scene(os->sc, d);
+    GradLayout &L = os->lay;
+    L.diffuse.resize(d->num_materials);
+    L.specular.resize(d->num_materials);
+    L.roughness.resize(d->num_materials);
+    L.vertices.resize(d->num_shapes);
+    L.normals.resize(d->num_shapes);
+    L.intensity.resize(d->num_lights);
+    L.total = dr_grad_layout_compute(d, nullptr, L.diffuse.data(), L.specular.data(), L.roughness.data(),
+                                     L.vertices.data(), L.normals.data(), L.intensity.data());
+    return os;
+}
+
+void or_scene_destroy(void *s) { delete (OScene *)s; }
+
+void or_scene_use_bvh(void *s, int use) { ((OScene *)s)->sc.use_bvh = use != 0; }
+
+// out[0..5] = edges, primary edges, silhouette nodes, normal nodes, bvh nodes, triangles
+void or_scene_info(void *s, int *out) {
+    const Scene &sc = ((OScene *)s)->sc;
+    out[0] = (int)sc.edges.size();
+    out[1] = (int)sc.primary_edges.size();
+    out[2] = (int)sc.sil_tree.nodes.size();
+    out[3] = (int)sc.nrm_tree.nodes.size();
+    out[4] = (int)sc.bvh.size();
+    out[5] = (int)sc.tri_shape.size();
+}
+
+size_t or_grad_size(void *s) { return ((OScene *)s)->lay.total; }
+
+// Forward (d_image == NULL) or forward+adjoint render on nthreads CPU threads.
+// Pixel work is split in 8x8 tiles taken from a shared counter, primary-edge
+// samples in chunks of 4096, as the reference's work-list pool does.
+int or_render(void *s, const dr_render_options *o, const float *d_image, float *image, float *grads,
+              float *dx_image, int nthreads) {
+    OScene *os = (OScene *)s;
+    if (g_ltc.empty()) return 3;
+    RenderCtx rc{os, o, d_image, d_image != nullptr};
+    int W = o->width, H = o->height;
+    int ntx = (W + 7) / 8, nty = (H + 7) / 8;
+    if (nthreads <= 0) nthreads = (int)std::thread::hardware_concurrency();
+    std::vector<Accum> accs(nthreads);
+    for (auto &a : accs) a.g.assign(rc.adjoint ? os->lay.total : 0, 0.0);
+    std::atomic<int> next_tile(0);
+    auto tile_worker = [&](int tid) {
+        Tape tape;
+        Accum &acc = accs[tid];
+        while (true) {
+            int t = next_tile.fetch_add(1);
+            if (t >= ntx * nty) break;
+            int tx = t % ntx, ty = t / ntx;
+            for (int yi = ty * 8; yi < std::min(ty * 8 + 8, H); yi++)
+                for (int xi = tx * 8; xi < std::min(tx * 8 + 8, W); xi++) {
+                    V3f c = v3(0.f, 0.f, 0.f);
+                    float dx = 0.f;
+                    for (int smp = 0; smp < o->num_samples; smp++)
+                        c += render_pixel_sample(rc, xi, yi, smp, acc, tape, &dx);
+                    float *px = &image[((size_t)yi * W + xi) * 3];
+                    px[0] = c[0]; px[1] = c[1]; px[2] = c[2];
+                    if (dx_image) {
+                        float *q = &dx_image[((size_t)yi * W + xi) * 3];
+                        q[0] = q[1] = q[2] = dx;
+                    }
+                }
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (int i = 0; i < nthreads; i++) th.emplace_back(tile_worker, i);
+        for (auto &t : th) t.join();
+    }
+    if (rc.adjoint && !(o->flags & DR_SKIP_PRIMARY_EDGES)) {
+        int64_t N = o->num_primary_edge_samples > 0 ? o->num_primary_edge_samples
+                                                    : (int64_t)o->num_samples * W * H;
+        std::atomic<int64_t> next(0);
+        const int64_t chunk = 4096;
+        auto edge_worker = [&](int tid) {
+            Accum &acc = accs[tid];
+            while (true) {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= N) break;
+                int64_t e = std::min(b + chunk, N);
+                for (int64_t k = b; k < e; k++) render_primary_edge(rc, k, acc);
+            }
+        };
+        std::vector<std::thread> th;
+        for (int i = 0; i < nthreads; i++) th.emplace_back(edge_worker, i);
+        for (auto &t : th) t.join();
+    }
+    if (rc.adjoint && grads) {
+        for (size_t i = 0; i < os->lay.total; i++) {
+            double v = 0.0;
+            for (auto &a : accs) v += a.g[i];
+            grads[i] = (float)v;
+        }
+    }
+    return 0;
+}
+
+// Ray queries with the oracle's hit rule.  rays [n, 8]: org, tmin, dir, tmax.
+// hits [n, 4]: shape, tri, u bits, v bits (closest) or occluded flag (any).
+void or_intersect(void *s, const float *rays, int64_t n, int32_t *hits, int any) {
+    const Scene &sc = ((OScene *)s)->sc;
+    for (int64_t i = 0; i < n; i++) {
+        const float *r = &rays[i * 8];
+        V3f org = v3(r[0], r[1], r[2]), dir = v3(r[4], r[5], r[6]);
+        int32_t *h = &hits[i * 4];
+        if (any) {
+            h[0] = any_hit(sc, org, dir, r[3], r[7]) ? 1 : 0;
+            continue;
+        }
+        Hit hh = nearest_hit(sc, org, dir, r[3], r[7]);
+        h[0] = hh.shape;
+        h[1] = hh.tri;
+        std::memcpy(&h[2], &hh.u, 4);
+        std::memcpy(&h[3], &hh.v, 4);
+    }
+}
+
+}  // extern "C"
