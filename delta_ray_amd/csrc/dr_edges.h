@@ -1,0 +1,535 @@
+// Boundary (visibility) terms: silhouette test, edge-tree importance
+// sampling, the secondary (shading) edge sample with LTC line sampling, and the
+// primary (screen) edge sample.
+#pragma once
+
+#include "dr_device.h"
+
+// edge.h:59-79
+DR_INLINE bool is_silhouette(const DrSceneView &S, F3 p, const DrEdge &e) {
+    const DrShape &sh = S.shapes[e.shape];
+    if (!sh.has_n) return true;
+    if (e.f0 == -1 || e.f1 == -1) return true;
+    F3 v0 = vert(S, sh.vert_off + e.v0), v1 = vert(S, sh.vert_off + e.v1);
+    const int *i0 = S.tri_vidx + 3 * (size_t)(sh.tri_off + e.f0);
+    const int *i1 = S.tri_vidx + 3 * (size_t)(sh.tri_off + e.f1);
+    int g0 = sh.vert_off + e.v0, g1 = sh.vert_off + e.v1;
+    int a = i0[0] != g0 && i0[0] != g1 ? i0[0] : (i0[1] != g0 && i0[1] != g1 ? i0[1] : i0[2]);
+    int b = i1[0] != g0 && i1[0] != g1 ? i1[0] : (i1[1] != g0 && i1[1] != g1 ? i1[1] : i1[2]);
+    F3 va = vert(S, a), vb = vert(S, b);
+    F3 n0 = normalize(cross(v0 - va, v1 - va));
+    F3 n1 = normalize(cross(v1 - vb, v0 - vb));
+    bool f0 = dot(p - v0, n0) > 0.f && dot(p - v1, n0) > 0.f && dot(p - va, n0) > 0.f;
+    bool f1 = dot(p - v0, n1) > 0.f && dot(p - v1, n1) > 0.f && dot(p - vb, n1) > 0.f;
+    return f0 != f1;
+}
+
+struct Cone {
+    F3 org, dir;
+    float cos_angle;
+};
+
+// Box-cone overlap of Eberly's GeometricTools (IntrAlignedBox3Cone3), as the
+// reference uses it (aabb.cpp:98-311).  Box corners in CCW order for each
+// of the 27 apex regions; geometric data that must match the reference.
+__constant__ static const signed char kConePolyN[27] = {6, 6, 6, 6, 4, 6, 6, 6, 6, 6, 4, 6, 4, 0,
+                                                         4, 6, 4, 6, 6, 6, 6, 6, 4, 6, 6, 6, 6};
+__constant__ static const signed char kConePolyIdx[27][6] = {
+    {1, 5, 4, 6, 2, 3}, {0, 2, 3, 1, 5, 4}, {0, 2, 3, 7, 5, 4}, {0, 4, 6, 2, 3, 1}, {0, 2, 3, 1, 0, 0},
+    {0, 2, 3, 7, 5, 1}, {0, 4, 6, 7, 3, 1}, {0, 2, 6, 7, 3, 1}, {0, 2, 6, 7, 5, 1}, {0, 1, 5, 4, 6, 2},
+    {0, 1, 5, 4, 0, 0}, {0, 1, 3, 7, 5, 4}, {0, 4, 6, 2, 0, 0}, {0, 0, 0, 0, 0, 0}, {1, 3, 7, 5, 0, 0},
+    {0, 4, 6, 7, 3, 2}, {2, 6, 7, 3, 0, 0}, {1, 3, 2, 6, 7, 5}, {0, 1, 5, 7, 6, 2}, {0, 1, 5, 7, 6, 4},
+    {0, 1, 3, 7, 6, 4}, {0, 4, 5, 7, 6, 2}, {4, 5, 7, 6, 0, 0}, {1, 3, 7, 6, 4, 5}, {0, 4, 5, 7, 3, 2},
+    {2, 6, 4, 5, 7, 3}, {1, 3, 2, 6, 4, 5}};
+
+DR_INLINE bool cone_edge_test(F3 D, float c2, F3 maxX, F3 maxPmV, float maxSqrLen, float maxDd, F3 CmV, int jA,
+                              int jB, float flip, bool &decided) {
+    int jDiff = jB - jA;
+    float s = jDiff > 0 ? 1.f : -1.f;
+    int k0 = abs(jDiff) >> 1;
+    float fder = flip * s * (D[k0] * maxSqrLen - maxDd * maxPmV[k0]);
+    decided = fder > 0;
+    if (!decided) return false;
+    int k1 = (k0 + 1) % 3, k2 = (k1 + 1) % 3;
+    float numer = maxPmV[k1] * maxPmV[k1] + maxPmV[k2] * maxPmV[k2];
+    float denom = D[k1] * maxPmV[k1] + D[k2] * maxPmV[k2];
+    F3 M;
+    M[k0] = numer * D[k0];
+    M[k1] = denom * (maxX[k1] + CmV[k1]);
+    M[k2] = denom * (maxX[k2] + CmV[k2]);
+    float DdM = dot(D, M);
+    if (DdM * DdM - c2 * dot(M, M) > 0) return true;
+    return s * (D[k1] * maxPmV[k2] - D[k2] * maxPmV[k1]) <= 0;
+}
+
+DR_INLINE bool aabb_cone_intersect(const float *lo, const float *hi, const Cone &cone) {
+    if (cone.cos_angle <= 0.f) return true;
+    F3 center = 0.5f * (F3{lo[0], lo[1], lo[2]} + F3{hi[0], hi[1], hi[2]});
+    F3 ext = 0.5f * (F3{hi[0], hi[1], hi[2]} - F3{lo[0], lo[1], lo[2]});
+    F3 CmV = center - cone.org;
+    F3 D = cone.dir;
+    float DdCmV = dot(D, CmV);
+    float radius = ext.x * fabsf(D.x) + ext.y * fabsf(D.y) + ext.z * fabsf(D.z);
+    if (DdCmV + radius <= 0) return false;
+    int ix = CmV.x < -ext.x ? 2 : (CmV.x > ext.x ? 0 : 1);
+    int iy = CmV.y < -ext.y ? 2 : (CmV.y > ext.y ? 0 : 1);
+    int iz = CmV.z < -ext.z ? 2 : (CmV.z > ext.z ? 0 : 1);
+    int lookup = ix + 3 * iy + 9 * iz;
+    if (lookup == 13) return true;
+    int np = kConePolyN[lookup];
+    float c2 = cone.cos_angle * cone.cos_angle;
+    int iMax = -1, jMax = -1;
+    float maxSqrDd = 0.f, maxSqrLen = 0.f, maxDd = 0.f;
+    F3 maxX = F3{0.f, 0.f, 0.f}, maxPmV = F3{0.f, 0.f, 0.f};
+    for (int i = 0; i < np; i++) {
+        int j = kConePolyIdx[lookup][i];
+        F3 X = F3{(j & 1) ? ext.x : -ext.x, (j & 2) ? ext.y : -ext.y, (j & 4) ? ext.z : -ext.z};
+        float Dd = dot(D, X) + DdCmV;
+        if (Dd > 0) {
+            F3 PmV = X + CmV;
+            float sqrDd = Dd * Dd;
+            float sqrLen = dot(PmV, PmV);
+            if (sqrDd - c2 * sqrLen > 0) return true;
+            if (iMax == -1 || sqrDd * maxSqrLen > maxSqrDd * sqrLen) {
+                iMax = i; jMax = j;
+                maxSqrDd = sqrDd; maxSqrLen = sqrLen; maxDd = Dd; maxX = X; maxPmV = PmV;
+            }
+        }
+    }
+    if (iMax == -1) return false;
+    int jNext = kConePolyIdx[lookup][iMax < np - 1 ? iMax + 1 : 0];
+    bool decided;
+    bool r = cone_edge_test(D, c2, maxX, maxPmV, maxSqrLen, maxDd, CmV, jMax, jNext, 1.f, decided);
+    if (decided) return r;
+    int jPrev = kConePolyIdx[lookup][iMax > 0 ? iMax - 1 : np - 1];
+    r = cone_edge_test(D, c2, maxX, maxPmV, maxSqrLen, maxDd, CmV, jPrev, jMax, -1.f, decided);
+    if (decided) return r;
+    return false;
+}
+
+DR_INLINE float min_sq_bound(float lo, float hi) {
+    if (lo <= 0.f && hi >= 0.f) return 0.f;
+    if (lo < 0.f && hi < 0.f) return hi * hi;
+    return lo * lo;
+}
+DR_INLINE float max_sq_bound(float lo, float hi) { return fmaxf(lo * lo, hi * hi); }
+
+// edge_tree.cpp:345-480
+DR_INLINE float bsdf_bound(const DrSceneView &S, const DrEdgeNode &node, F3 wi, const SPoint &sp, float roughness,
+                           float dlum, float slum) {
+    float cos_max = 0.f;
+    for (int i = 0; i < 8; i++) {
+        F3 c = F3{(i & 1) ? node.hi[0] : node.lo[0], (i & 2) ? node.hi[1] : node.lo[1],
+                  (i & 4) ? node.hi[2] : node.lo[2]};
+        if (dot(sp.fr.n, c - sp.pos) > 0.f) { cos_max = 1.f; break; }
+    }
+    if (cos_max <= 0.f) return 0.f;
+    float phong_resp = 0.f;
+    if (slum > 0.f) {
+        F3 hy = normalize(cross(sp.fr.n, wi));
+        Frame hf = Frame{normalize(cross(wi, hy)), hy, wi};
+        float blo[3] = {DR_INF, DR_INF, DR_INF}, bhi[3] = {-DR_INF, -DR_INF, -DR_INF};
+        for (int i = 0; i < 8; i++) {
+            F3 c = F3{(i & 1) ? node.hi[0] : node.lo[0], (i & 2) ? node.hi[1] : node.lo[1],
+                      (i & 4) ? node.hi[2] : node.lo[2]};
+            F3 l = to_local(hf, c - sp.pos);
+            blo[0] = fminf(blo[0], l.x); blo[1] = fminf(blo[1], l.y); blo[2] = fminf(blo[2], l.z);
+            bhi[0] = fmaxf(bhi[0], l.x); bhi[1] = fmaxf(bhi[1], l.y); bhi[2] = fmaxf(bhi[2], l.z);
+        }
+        float cphm = 0.f;
+        float hx = bhi[0];
+        if (hx > 0.f) {
+            cphm = hx / sqrtf(min_sq_bound(blo[1], bhi[1]) + hx * hx);
+        } else if (hx == 0.f) {
+            cphm = fmaxf(cphm, 0.f);
+        } else {
+            cphm = hx / sqrtf(max_sq_bound(blo[1], bhi[1]) + hx * hx);
+        }
+        float cto_min, cto_max;
+        if (blo[2] > 0.f)
+            cto_min = blo[2] / (max_sq_bound(blo[0], bhi[0]) + max_sq_bound(blo[1], bhi[1]) + blo[2] * blo[2]);
+        else
+            cto_min = blo[2] / (min_sq_bound(blo[0], bhi[0]) + min_sq_bound(blo[1], bhi[1]) + blo[2] * blo[2]);
+        if (bhi[2] > 0.f)
+            cto_max = bhi[2] / (min_sq_bound(blo[0], bhi[0]) + min_sq_bound(blo[1], bhi[1]) + bhi[2] * bhi[2]);
+        else
+            cto_max = bhi[2] / (max_sq_bound(blo[0], bhi[0]) + max_sq_bound(blo[1], bhi[1]) + bhi[2] * bhi[2]);
+        float cth_min = sqrtf((cto_min + 1.f) / 2.f);
+        float cth_max = sqrtf((cto_max + 1.f) / 2.f);
+        F3 ln = to_local(hf, sp.fr.n);
+        float ch = cphm >= 0.f ? sqrtf(ln.z * ln.z / (ln.z * ln.z + ln.x * ln.x * cphm * cphm)) : 1.f;
+        if (ch > cth_max) ch = cth_max;
+        else if (ch < cth_min) ch = cth_min;
+        float sh = sqrtf(fmaxf(1.f - ch * ch, 0.f));
+        float hn = fminf(fabsf(ln.x * sh * cphm) + ln.z * ch, 1.f);
+        float e = roughness_to_phong(roughness);
+        phong_resp = powf(hn, e) * (e + 2.f) / (2.f * DR_PI);
+    }
+    return cos_max * (dlum / DR_PI + slum * phong_resp);
+}
+
+// edge_tree.cpp:482-497
+DR_INLINE bool contains_silhouette(const DrEdgeNode &n, F3 p) {
+    F3 lo = F3{n.lo[0], n.lo[1], n.lo[2]}, hi = F3{n.hi[0], n.hi[1], n.hi[2]};
+    F3 c = (lo + hi) / 2.f;
+    float r = length(hi - c);
+    F3 d = c - p;
+    float len = length(d);
+    float tv = r / len;
+    if (tv >= 1.f) return true;
+    float va = atanf(tv);
+    d = d / len;
+    float ang = acosf(dot(-d, F3{n.axis[0], n.axis[1], n.axis[2]}));
+    float range = n.angle + va;
+    return ang - range <= DR_PI / 2.f && ang + range >= DR_PI / 2.f;
+}
+
+DR_INLINE bool node_inside(const DrEdgeNode &n, F3 p) {
+    return p.x >= n.lo[0] && p.x <= n.hi[0] && p.y >= n.lo[1] && p.y <= n.hi[1] && p.z >= n.lo[2] && p.z <= n.hi[2];
+}
+DR_INLINE float node_importance(const DrEdgeNode &n, float bound, F3 p) {
+    F3 c = 0.5f * (F3{n.lo[0], n.lo[1], n.lo[2]} + F3{n.hi[0], n.hi[1], n.hi[2]});
+    return bound * n.total_length / length_sq(c - p);
+}
+
+// Edge-tree traversal (edge_tree.cpp:511-683), iterative: a stack of
+// pending right children reproduces the recursion's depth-first order and
+// therefore its order of random draws.  `emit(edge, weight)` is called for
+// every selected silhouette edge, in order.
+struct TreeEntry {
+    int node;
+    float bound, cone_pmf, all_pmf;
+};
+
+template <class Emit>
+DR_INLINE void traverse_edge_tree(const DrSceneView &S, const DrEdgeNode *nodes, int num_nodes, const DrEdge *edges,
+                                  bool directional,
+                                  bool cone_mode, float root_bound, float pmf, F3 wi, const SPoint &sp,
+                                  float roughness, float dlum, float slum, const Cone &cone, Philox &rng, Emit &emit) {
+    TreeEntry stack[DR_STACK];
+    int sp_ = 0;
+    stack[sp_++] = TreeEntry{0, root_bound, pmf, pmf};
+    // each node is popped at most once per pass; the cap guards a malformed tree
+    int pops = 0;
+    const int cap = 2 * num_nodes + 8;
+    while (sp_ > 0 && pops++ < cap) {
+        TreeEntry e = stack[--sp_];
+        if (e.bound <= 0.f) continue;
+        const DrEdgeNode &node = nodes[e.node];
+        if (node.nprims > 0) {
+            float den = e.cone_pmf * e.cone_pmf + e.all_pmf * e.all_pmf;
+            float w = (cone_mode ? e.cone_pmf : e.all_pmf) / den;
+            for (int i = 0; i < node.nprims; i++) {
+                const DrEdge &ed = edges[node.offset + i];
+                if (is_silhouette(S, sp.pos, ed)) emit(ed, w);
+            }
+            continue;
+        }
+        int li = e.node + 1, ri = node.offset;
+        const DrEdgeNode &L = nodes[li], &R = nodes[ri];
+        bool all_split = e.bound > 1.f || node_inside(node, sp.pos);
+        bool lcs = true, rcs = true;
+        if (directional) {
+            lcs = contains_silhouette(L, sp.pos);
+            rcs = contains_silhouette(R, sp.pos);
+        }
+        float lb = lcs ? bsdf_bound(S, L, wi, sp, roughness, dlum, slum) : 0.f;
+        float rb = rcs ? bsdf_bound(S, R, wi, sp, roughness, dlum, slum) : 0.f;
+        float il = lcs ? node_importance(L, lb, sp.pos) : 0.f;
+        float ir = rcs ? node_importance(R, rb, sp.pos) : 0.f;
+        if (il + ir <= 1e-10f) continue;
+        bool lhc = aabb_cone_intersect(L.lo, L.hi, cone);
+        bool rhc = aabb_cone_intersect(R.lo, R.hi, cone);
+        bool cone_split = all_split || (lhc && rhc && lcs && rcs && cone.cos_angle > 0.95f);
+        float ilc = lhc ? il : 0.f, irc = rhc ? ir : 0.f;
+        if (cone_mode && ilc + irc <= 1e-10f) continue;
+        float plc = 0.f, prc = 0.f, pla, pra;
+        if (cone_split) {
+            plc = prc = 1.f;
+        } else if (ilc + irc > 1e-10f) {
+            plc = ilc / (ilc + irc);
+            prc = 1.f - plc;
+        }
+        if (all_split) {
+            pla = pra = 1.f;
+        } else {
+            pla = il / (il + ir);
+            pra = 1.f - pla;
+        }
+        bool split = cone_mode ? cone_split : all_split;
+        TreeEntry le = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf};
+        TreeEntry re = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf};
+        if (split) {
+            if (rcs && sp_ < DR_STACK) stack[sp_++] = re;
+            if (lcs && sp_ < DR_STACK) stack[sp_++] = le;
+        } else {
+            float u = philox_next(rng);
+            bool left = cone_mode ? u < plc : u < pla;
+            if (sp_ < DR_STACK) stack[sp_++] = left ? le : re;
+        }
+    }
+}
+
+// edge_tree.cpp:654-725: choose the tree, then a cone pass and an "all" pass
+template <class Emit>
+DR_INLINE void sample_shading_edges(const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp, const Cone &cone,
+                                    Philox &rng, Emit &emit) {
+    if (dot(wi, sp.fr.n) < 0.f) return;
+    bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;
+    if (!has_n && !has_s) return;
+    const DrEdgeNode *nodes;
+    const DrEdge *edges;
+    int nn;
+    bool dir;
+    float pmf = 1.f;
+    if (!has_n) {
+        nodes = S.sil_nodes; edges = S.sil_edges; nn = S.num_sil_nodes; dir = false;
+    } else if (!has_s) {
+        nodes = S.nrm_nodes; edges = S.nrm_edges; nn = S.num_nrm_nodes; dir = true;
+    } else {
+        float ni = S.nrm_nodes[0].total_length, si = S.sil_nodes[0].total_length;
+        float npmf = ni / (ni + si), spmf = si / (ni + si);
+        float u = philox_next(rng);
+        if (u < npmf) { nodes = S.nrm_nodes; edges = S.nrm_edges; nn = S.num_nrm_nodes; dir = true; pmf = npmf; }
+        else { nodes = S.sil_nodes; edges = S.sil_edges; nn = S.num_sil_nodes; dir = false; pmf = spmf; }
+    }
+    const DrMaterial &m = S.materials[S.shapes[is.shape].mat];
+    float dlum = luminance(get_diffuse(S, m, sp.uv));
+    float slum = luminance(get_specular(S, m, sp.uv));
+    float roughness = get_roughness(S, m, sp.uv);
+    float b = bsdf_bound(S, nodes[0], wi, sp, roughness, dlum, slum);
+    traverse_edge_tree(S, nodes, nn, edges, dir, true, b, pmf, wi, sp, roughness, dlum, slum, cone, rng, emit);
+    traverse_edge_tree(S, nodes, nn, edges, dir, false, b, pmf, wi, sp, roughness, dlum, slum, cone, rng, emit);
+}
+
+struct EdgeResult {
+    F3 dp, dv0, dv1;
+};
+
+// delta_ray.cpp:45-365: one point on one silhouette edge, seen from a
+// shading point; LTC line sampling, two rays across the edge, Dirac
+// Jacobian.  ss / nss: bounces after the edge hit (empty when direct only).
+DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_component, float eu, const DrEdge &edge,
+                                               const LightSample &ls, const SecondarySample *ss, int nss, F3 wi,
+                                               const Isect &is, const SPoint &sp, F3 weight, F3 d_color,
+                                               bool direct_only) {
+    EdgeResult R = EdgeResult{F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
+    if (!valid(is)) return R;
+    if (!is_silhouette(S, sp.pos, edge)) return R;
+    const DrShape &es = S.shapes[edge.shape];
+    F3 v0 = vert(S, es.vert_off + edge.v0), v1 = vert(S, es.vert_off + edge.v1);
+    if (abs_sum(v1 - v0) == 1e-10f) return R;
+    int mid = S.shapes[is.shape].mat;
+    const DrMaterial &m = S.materials[mid];
+    float dw = luminance(get_diffuse(S, m, sp.uv));
+    float sw = luminance(get_specular(S, m, sp.uv));
+    float ws = dw + sw;
+    if (ws <= 0.f) return R;
+    float dpmf = dw / ws, spmf = sw / ws;
+    F3 n = sp.fr.n;
+    F3 fx = normalize(wi - n * dot(wi, n));
+    F3 fy = cross(n, fx);
+    M3 iso = m3_rows(fx, fy, n);
+    M3 M, Minv;
+    float m_pmf;
+    if (bsdf_component <= dpmf) {
+        Minv = iso;
+        M = inverse(Minv);
+        m_pmf = dpmf;
+    } else {
+        Minv = mul(inverse(get_ltc_matrix(S, mid, sp, wi)), iso);
+        M = inverse(Minv);
+        m_pmf = spmf;
+    }
+    F3 a = mul(Minv, v0 - sp.pos);
+    F3 b = mul(Minv, v1 - sp.pos);
+    if (a.z <= 0.f && b.z <= 0.f) return R;
+    if (a.z < 0.f) a = (a * b.z - b * a.z) / (b.z - a.z);
+    if (b.z < 0.f) b = (a * b.z - b * a.z) / (b.z - a.z);
+    F3 vd = b - a;
+    if (abs_sum(vd) < 1e-10f) return R;
+    F3 wt = normalize(vd);
+    float l0 = dot(a, wt), l1 = dot(b, wt);
+    F3 vo = a - l0 * wt;
+    float d = length(vo);
+    auto I = [&](float l) {
+        return (l / (d * (d * d + l * l)) + atanf(l / d) / (d * d)) * vo.z + (l * l / (d * (d * d + l * l))) * wt.z;
+    };
+    float Il0 = I(l0), Il1 = I(l1);
+    float norm = Il1 - Il0;
+    if (norm <= 1e-10f) return R;
+    auto pdf = [&](float l) {
+        float dsq = d * d + l * l;
+        return 2.f * d * (vo + l * wt).z / (norm * dsq * dsq);
+    };
+    float lb = l0, ub = l1;
+    if (lb > ub) { float t = lb; lb = ub; ub = t; }
+    float l = 0.5f * (lb + ub);
+    for (int it = 0; it < 20; it++) {
+        if (!(l >= lb && l <= ub)) l = 0.5f * (lb + ub);
+        float value = (I(l) - Il0) / norm - eu;
+        if (fabsf(value) < 1e-5f || it == 19) break;
+        if (value > 0.f) ub = l;
+        else lb = l;
+        l -= value / pdf(l);
+    }
+    float pl = pdf(l);
+    if (pl <= 0.f) return R;
+    F3 sample_p = mul(M, vo + l * wt);
+    F3 spn = normalize(cross(v0 - sp.pos, v1 - sp.pos));
+    float delta = 1e-3f * length(sample_p);
+    F3 sdir = normalize(sample_p);
+    F3 rdir = normalize(sdir + delta * spn);
+    F3 ldir = normalize(sdir - delta * spn);
+    SPoint spr, spl;
+    Isect ir = intersect(S, sp.pos, rdir, spr);
+    Isect il = intersect(S, sp.pos, ldir, spl);
+    bool rc = ir.shape == edge.shape && (ir.tri == edge.f0 || ir.tri == edge.f1);
+    bool lc = il.shape == edge.shape && (il.tri == edge.f0 || il.tri == edge.f1);
+    if (rc == lc || (ir.shape == is.shape && ir.tri == is.tri) || (il.shape == is.shape && il.tri == is.tri))
+        return R;
+    F3 eb = bsdf(S, mid, sp, wi, sdir);
+    if (sum3(eb) < 1e-6f) return R;
+    F3 shr = F3{0.f, 0.f, 0.f}, shl = F3{0.f, 0.f, 0.f};
+    if (!direct_only) {
+        shr = shade(S, -sdir, ir, spr, ls, ss, nss, false);
+        shl = shade(S, -sdir, il, spl, ls, ss, nss, false);
+        if (valid(ir)) {
+            F3 dd = spr.pos - sp.pos;
+            float dsq = length_sq(dd);
+            if (dsq > 0.f) {
+                shr = shr / fmaxf(dsq, 1e-2f);
+                shr = shr * fabsf(dot(normalize(dd), spr.ng));
+            } else {
+                shr = F3{0.f, 0.f, 0.f};
+            }
+        }
+        if (valid(il)) {
+            F3 dd = spl.pos - sp.pos;
+            float dsq = length_sq(dd);
+            if (dsq > 0.f) {
+                shl = shl / fmaxf(dsq, 1e-2f);
+                shl = shl * fabsf(dot(normalize(dd), spl.ng));
+            } else {
+                shl = F3{0.f, 0.f, 0.f};
+            }
+        }
+    }
+    int rl = valid(ir) ? S.shapes[ir.shape].light : -1;
+    int ll = valid(il) ? S.shapes[il.shape].light : -1;
+    F3 er = eval_light(S, wi, is, sp, rl, spr.pos, spr.fr.n) + eb * shr;
+    F3 el = eval_light(S, wi, is, sp, ll, spl.pos, spl.fr.n) + eb * shl;
+    if (valid(ir)) {
+        F3 dd = spr.pos - sp.pos;
+        if (length(dd) * fabsf(dot(normalize(dd), spr.ng)) < 1e-3f) return R;
+    }
+    if (valid(il)) {
+        F3 dd = spl.pos - sp.pos;
+        if (length(dd) * fabsf(dot(normalize(dd), spl.ng)) < 1e-3f) return R;
+    }
+    F3 d0 = v0 - sp.pos, d1 = v1 - sp.pos;
+    float dirac_j = length(cross(d0, d1));
+    auto accum = [&](const SPoint &hp, F3 ev) {
+        M3 nj = normalize_jacobian(sample_p);
+        M3 ij = intersect_jacobian(sdir, hp.pos, hp.ng);
+        float line_j = length(mul(mul(mul(ij, nj), M), wt)) / (m_pmf * pl);
+        float w = line_j / dirac_j;
+        F3 x = hp.pos - sp.pos;
+        F3 dp = w * (cross(d1, d0) + cross(x, d1) + cross(d0, x));
+        F3 dv0 = w * cross(d1, x);
+        F3 dv1 = w * cross(x, d0);
+        float we = dot(ev * weight, d_color);
+        R.dp += dp * we;
+        R.dv0 += dv0 * we;
+        R.dv1 += dv1 * we;
+    };
+    if (valid(ir) && abs_sum(er) > 0.f) accum(spr, er);
+    if (valid(il) && abs_sum(el) > 0.f) accum(spl, -el);
+    return R;
+}
+
+struct PrimaryResult {
+    bool ok;
+    F3 dv0, dv1;
+    float dw2c[16];
+};
+
+// delta_ray.cpp:377-557 (world Jacobian by 12-point finite differences, as there)
+DR_INLINE void sample_primary_edge_point(const DrSceneView &S, const DrEdge &edge, float eu, const LightSample &ls,
+                                         const SecondarySample *ss, int nss, const float *d_image, int W, int H,
+                                         float weight, PrimaryResult &R) {
+    R.ok = false;
+    const DrCamera &cam = S.cam;
+    const DrShape &es = S.shapes[edge.shape];
+    F3 v0 = vert(S, es.vert_off + edge.v0), v1 = vert(S, es.vert_off + edge.v1);
+    F2 v0s, v1s;
+    if (!project_edge(cam, v0, v1, v0s, v1s)) return;
+    F3 v0d = unproject(cam, v0s), v1d = unproject(cam, v1s);
+    F3 vd3 = v1d - v0d;
+    float t = eu;
+    F3 v0l = xfm_point(cam.w2c, v0), v1l = xfm_point(cam.w2c, v1);
+    F3 ep3 = v0d + t * vd3;
+    F3 elocal = v0l + t * v1l;
+    F2 ep = project_local(cam, ep3);
+    if (!(ep.x >= 0.f && ep.x < 1.f && ep.y >= 0.f && ep.y < 1.f)) return;
+    int xi = min((int)(ep.x * W), W - 1), yi = min((int)(ep.y * H), H - 1);
+    F3 d_color = load3(d_image + ((size_t)yi * W + xi) * 3);
+    float delta = 1e-4f * length(elocal);
+    F3 spn = normalize(cross(v0d, v1d));
+    F3 rdir = normalize(ep3 + delta * spn);
+    F2 rpt = project_local(cam, rdir);
+    F3 rorg, rd;
+    sample_primary(cam, rpt, rorg, rd);
+    SPoint spr, spl;
+    Isect ir = intersect(S, rorg, rd, spr);
+    F3 ldir = normalize(ep3 - delta * spn);
+    F2 lpt = project_local(cam, ldir);
+    F3 lorg, ld;
+    sample_primary(cam, lpt, lorg, ld);
+    Isect il = intersect(S, lorg, ld, spl);
+    bool rc = ir.shape == edge.shape && (ir.tri == edge.f0 || ir.tri == edge.f1);
+    bool lc = il.shape == edge.shape && (il.tri == edge.f0 || il.tri == edge.f1);
+    if (!rc && !lc) return;
+    F3 rcol = shade(S, -rd, ir, spr, ls, ss, nss, true);
+    F3 lcol = shade(S, -ld, il, spl, ls, ss, nss, true);
+    F3 diff = (rcol - lcol) * weight;
+    F3 xdiff = lpt.x > rpt.x ? -diff : diff;
+    F3 ydiff = lpt.y > rpt.y ? -diff : diff;
+    F3 edir = unproject(cam, ep);
+    F3 dx = F3{-2.f * cam.fov, 0.f, 0.f};
+    F3 dy = F3{0.f, -2.f * cam.fov / cam.aspect, 0.f};
+    F3 c01 = cross(v0d, v1d);
+    float dadx = dot(dx, c01), dady = dot(dy, c01);
+    float da0x = fabsf(dot(cross(v1d, edir), dx));
+    float da0y = fabsf(dot(cross(v1d, edir), dy));
+    float da1x = fabsf(dot(cross(edir, v0d), dx));
+    float da1y = fabsf(dot(cross(edir, v0d), dy));
+    float dirac_j = 1.f / sqrtf(dadx * dadx + dady * dady);
+    F3 ep3d = v0d + (t + delta) * vd3;
+    F2 epd = project_local(cam, ep3d);
+    F2 dlt = epd - ep;
+    float line_j = length2(F2{dlt.x / delta, dlt.y / delta});
+    float jac = line_j * dirac_j;
+    float d0x = dot(-xdiff * da0x * jac, d_color);
+    float d0y = dot(-ydiff * da0y * jac, d_color);
+    float d1x = dot(-xdiff * da1x * jac, d_color);
+    float d1y = dot(-ydiff * da1y * jac, d_color);
+    const float fd = 1e-3f;
+    F3 ex = F3{fd, 0.f, 0.f}, ey = F3{0.f, fd, 0.f}, ez = F3{0.f, 0.f, fd};
+    float g[6];
+    for (int k = 0; k < 6; k++) {
+        F3 e = (k % 3 == 0) ? ex : ((k % 3 == 1) ? ey : ez);
+        F2 p0, p1;
+        bool ok = k < 3 ? project_edge(cam, v0 + e, v1, p0, p1) : project_edge(cam, v0, v1 + e, p0, p1);
+        if (!ok) return;
+        F2 a = p0 - v0s, b = p1 - v1s;
+        a = F2{a.x / fd, a.y / fd};
+        b = F2{b.x / fd, b.y / fd};
+        g[k] = d0x * a.x + d0y * a.y + d1x * b.x + d1y * b.y;
+    }
+    R.dv0 = F3{g[0], g[1], g[2]};
+    R.dv1 = F3{g[3], g[4], g[5]};
+    for (int i = 0; i < 16; i++) R.dw2c[i] = 0.f;
+    d_project(cam, v0, v1, d0x, d0y, d1x, d1y, R.dw2c);
+    R.ok = true;
+}

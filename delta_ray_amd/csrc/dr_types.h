@@ -1,0 +1,116 @@
+// Device-resident scene layout shared by the host builder (scene_build.cpp)
+// and the kernels (render.hip).  Plain data only.
+//
+// HBM layout (all arrays 16-byte aligned, one allocation per array):
+//   nodes      BVH2 nodes, 64 B: both child boxes in the parent so one node
+//              fetch tests two boxes (one 64 B cache-line-friendly load)
+//   tris       48 B per triangle in leaf order: v0, e1 = v1-v0, e2 = v2-v0
+//              (float4 each; v0.w carries the global triangle id)
+//   verts      float3 per global vertex (all shapes concatenated)
+//   tri_vidx   int3 global vertex ids per global triangle
+//   tri_shape  shape id per global triangle
+//   uvs/normals per global vertex, present for shapes that have them
+//   texels     every material texture, concatenated
+//   edges      mesh edges (sorted by shape, v0, v1), edge-tree nodes and the
+//              primary-edge distribution
+//   ltc        16384 x 9 floats (the fitted LTC matrices)
+#pragma once
+
+#include <stdint.h>
+
+#define DR_MAX_SHAPES 65536
+
+struct DrNode {
+    // child 0 box: a.xyz lo, a.w hi.x, b.xy hi.yz; child 1 box: b.zw lo.xy,
+    // c.x lo.z, c.yzw hi.xyz; d = (child0, child1, count0, count1) where a
+    // count > 0 marks a leaf whose child field is its first triangle slot
+    float a[4], b[4], c[4];
+    int d[4];
+};
+
+struct DrTri {
+    float v0[4];  // w: global triangle id (int bits)
+    float e1[4];
+    float e2[4];
+};
+
+struct DrShape {
+    int vert_off, tri_off, nv, nt;
+    int has_uv, has_n, mat, light;
+    long long vgrad_off, ngrad_off;  // gradient-buffer offsets (ngrad -1: none)
+};
+
+struct DrTex {
+    int off, h, w, c;  // h == 0: constant of c floats at off
+    long long grad_off;
+};
+
+struct DrMaterial {
+    DrTex diffuse, specular, roughness;
+    float duv[2], suv[2], ruv[2];
+    int two_sided, pad;
+};
+
+struct DrLight {
+    int shape, ntri, func_off, cdf_off;  // triangle-area distribution
+    float I[3], lum;
+    float area;          // total area of the light's triangles
+    float func_int;      // integral of the triangle distribution
+    float sphere_c[3];   // bounding sphere of the light's triangles
+    float sphere_r;
+    long long grad_off;
+};
+
+struct DrEdge {
+    int shape, v0, v1, f0, f1;  // shape-local vertex / face ids (f1 -1: boundary)
+};
+
+struct DrEdgeNode {
+    float lo[3], hi[3];
+    float axis[3];
+    float angle;
+    int offset;   // leaf: first edge in the tree's edge list; interior: second child
+    int nprims;   // 0: interior
+    float total_length;
+    float pad;
+};
+
+struct DrCamera {
+    float c2w[16], w2c[16];
+    float fov, aspect, clip_near, pad;
+};
+
+// Everything a kernel needs, passed by value.
+struct DrSceneView {
+    const DrNode *nodes;
+    const DrTri *tris;
+    const float *verts;
+    const int *tri_vidx;
+    const int *tri_shape;
+    const float *uvs;
+    const float *normals;
+    const float *texels;
+    const DrShape *shapes;
+    const DrMaterial *materials;
+    const DrLight *lights;
+    const float *light_func;  // per light: triangle areas, then cdf (n+1)
+    const float *light_cdf;
+    const float *lights_func;  // light selection by luminance
+    const float *lights_cdf;
+    float lights_func_int, light_total_intensity;
+    int num_lights, num_shapes;
+    int num_tris, num_nodes;
+    const DrEdge *sil_edges;
+    const DrEdgeNode *sil_nodes;
+    const DrEdge *nrm_edges;
+    const DrEdgeNode *nrm_nodes;
+    int num_sil_nodes, num_nrm_nodes;
+    const DrEdge *prim_edges;
+    const float *prim_func;
+    const float *prim_cdf;
+    float prim_func_int;
+    int num_prim_edges;
+    const float *ltc;
+    DrCamera cam;
+    unsigned long long *stats;  // NULL or 5 counters
+};

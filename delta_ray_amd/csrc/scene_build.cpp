@@ -1,0 +1,671 @@
+// Host-side scene preparation for the device renderer.
+//   light distributions          reference scene.cpp:24-52, distribution.h:7-59
+//   edge extraction + filtering  reference scene.cpp:54-97 (sorted by
+//                                (shape, v0, v1), as the reference's std::map)
+//   edge trees                   reference edge_tree.cpp:49-316 (median
+//                                splits; the same build as the CPU oracle, so
+//                                both sample the same edges)
+//   primary-edge distribution    reference edge_tree.cpp:266-294
+//   triangle BVH                 binned SAH, BVH2 with both child boxes in
+//                                the parent node (replaces Embree)
+#include "scene_build.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <tuple>
+
+namespace drh {
+
+namespace {
+
+struct V3 {
+    float x, y, z;
+    float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+inline V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+inline V3 operator/(V3 a, float s) { float inv = 1.f / s; return a * inv; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float length(V3 a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+inline V3 normalize(V3 a) { return a / length(a); }
+inline float luminance(V3 a) { return 0.212671f * a.x + 0.715160f * a.y + 0.072169f * a.z; }
+
+struct Box {
+    V3 lo, hi;
+    Box() {
+        float inf = std::numeric_limits<float>::infinity();
+        lo = V3{inf, inf, inf};
+        hi = V3{-inf, -inf, -inf};
+    }
+    Box(V3 a, V3 b) {
+        lo = V3{std::min(a.x, b.x), std::min(a.y, b.y), std::min(a.z, b.z)};
+        hi = V3{std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z)};
+    }
+    void grow(const Box &b) {
+        lo = V3{std::min(lo.x, b.lo.x), std::min(lo.y, b.lo.y), std::min(lo.z, b.lo.z)};
+        hi = V3{std::max(hi.x, b.hi.x), std::max(hi.y, b.hi.y), std::max(hi.z, b.hi.z)};
+    }
+    void grow(V3 p) { grow(Box(p, p)); }
+    float area() const {
+        if (lo.x > hi.x) return 0.f;
+        V3 d = hi - lo;
+        return 2.f * (d.x * d.y + d.x * d.z + d.y * d.z);
+    }
+};
+
+// pbrt-style 1-D distribution with a float CDF accumulated in order
+Dist make_dist(const std::vector<float> &f) {
+    Dist d;
+    int n = (int)f.size();
+    d.func = f;
+    d.cdf.assign(n + 1, 0.f);
+    for (int i = 1; i < n + 1; i++) d.cdf[i] = d.cdf[i - 1] + d.func[i - 1] / n;
+    d.func_int = d.cdf[n];
+    if (d.func_int == 0.f) {
+        for (int i = 1; i < n + 1; i++) d.cdf[i] = float(i) / float(n);
+    } else {
+        for (int i = 1; i < n + 1; i++) d.cdf[i] /= d.func_int;
+    }
+    return d;
+}
+
+V3 getv(const HostScene &hs, int shape, int v) {
+    const float *p = &hs.verts[3 * (size_t)(hs.shapes[shape].vert_off + v)];
+    return V3{p[0], p[1], p[2]};
+}
+V3 getn(const HostScene &hs, int shape, int v) {
+    const float *p = &hs.normals[3 * (size_t)(hs.shapes[shape].vert_off + v)];
+    return V3{p[0], p[1], p[2]};
+}
+int tri_index(const HostScene &hs, int shape, int t, int k) {
+    return hs.tri_vidx[3 * (size_t)(hs.shapes[shape].tri_off + t) + k] - hs.shapes[shape].vert_off;
+}
+V3 face_normal(const HostScene &hs, int shape, int t) {
+    V3 a = getv(hs, shape, tri_index(hs, shape, t, 0)), b = getv(hs, shape, tri_index(hs, shape, t, 1)),
+       c = getv(hs, shape, tri_index(hs, shape, t, 2));
+    return normalize(cross(b - a, c - a));
+}
+
+// --------------------------------------------------------- triangle BVH --
+struct BPrim {
+    Box box;
+    V3 c;
+    int id;
+};
+struct BNode {
+    Box box;
+    int left = -1, right = -1, first = 0, count = 0;
+};
+
+int bvh_rec(std::vector<BNode> &nodes, std::vector<int> &order, std::vector<BPrim> &p, int b, int e) {
+    int id = (int)nodes.size();
+    nodes.emplace_back();
+    Box box, cb;
+    for (int i = b; i < e; i++) {
+        box.grow(p[i].box);
+        cb.grow(p[i].c);
+    }
+    nodes[id].box = box;
+    int n = e - b;
+    auto leaf = [&]() {
+        nodes[id].first = (int)order.size();
+        nodes[id].count = n;
+        for (int i = b; i < e; i++) order.push_back(p[i].id);
+    };
+    if (n <= 4) {
+        leaf();
+        return id;
+    }
+    V3 ext = cb.hi - cb.lo;
+    int axis = 0;
+    if (ext.y > ext[axis]) axis = 1;
+    if (ext.z > ext[axis]) axis = 2;
+    int mid = (b + e) / 2;
+    if (ext[axis] > 0.f) {
+        const int NB = 32;
+        Box bb[NB];
+        int bc[NB] = {0};
+        float scale = NB / ext[axis];
+        auto bin = [&](const BPrim &q) { return std::min(std::max((int)((q.c[axis] - cb.lo[axis]) * scale), 0), NB - 1); };
+        for (int i = b; i < e; i++) {
+            int k = bin(p[i]);
+            bb[k].grow(p[i].box);
+            bc[k]++;
+        }
+        Box rb[NB];
+        int rc[NB];
+        Box acc;
+        int cnt = 0;
+        for (int k = NB - 1; k >= 0; k--) {
+            acc.grow(bb[k]);
+            cnt += bc[k];
+            rb[k] = acc;
+            rc[k] = cnt;
+        }
+        float best = std::numeric_limits<float>::infinity();
+        int best_k = -1;
+        Box lb;
+        int lc = 0;
+        for (int k = 1; k < NB; k++) {
+            lb.grow(bb[k - 1]);
+            lc += bc[k - 1];
+            if (lc == 0 || rc[k] == 0) continue;
+            float cost = lb.area() * lc + rb[k].area() * rc[k];
+            if (cost < best) { best = cost; best_k = k; }
+        }
+        if (best_k > 0) {
+            auto it = std::partition(p.begin() + b, p.begin() + e, [&](const BPrim &q) { return bin(q) < best_k; });
+            mid = (int)(it - p.begin());
+            if (mid == b || mid == e) mid = (b + e) / 2;
+        }
+    }
+    int l = bvh_rec(nodes, order, p, b, mid);
+    int r = bvh_rec(nodes, order, p, mid, e);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    return id;
+}
+
+void build_bvh(HostScene &hs) {
+    int nt = (int)hs.tri_shape.size();
+    hs.nodes.clear();
+    hs.tris.clear();
+    if (nt == 0) return;
+    std::vector<BPrim> p(nt);
+    for (int g = 0; g < nt; g++) {
+        const int *vi = &hs.tri_vidx[3 * (size_t)g];
+        V3 a = V3{hs.verts[3 * (size_t)vi[0]], hs.verts[3 * (size_t)vi[0] + 1], hs.verts[3 * (size_t)vi[0] + 2]};
+        V3 b = V3{hs.verts[3 * (size_t)vi[1]], hs.verts[3 * (size_t)vi[1] + 1], hs.verts[3 * (size_t)vi[1] + 2]};
+        V3 c = V3{hs.verts[3 * (size_t)vi[2]], hs.verts[3 * (size_t)vi[2] + 1], hs.verts[3 * (size_t)vi[2] + 2]};
+        Box bx(a, b);
+        bx.grow(c);
+        p[g].box = bx;
+        p[g].c = (bx.lo + bx.hi) * 0.5f;
+        p[g].id = g;
+    }
+    std::vector<BNode> bn;
+    std::vector<int> order;
+    bvh_rec(bn, order, p, 0, nt);
+    // flatten into 2-wide nodes holding both child boxes; leaves are folded
+    // into their parent.  A root that is itself a leaf gets an empty sibling.
+    hs.tris.resize(nt);
+    for (int i = 0; i < nt; i++) {
+        int g = order[i];
+        const int *vi = &hs.tri_vidx[3 * (size_t)g];
+        float v[3][3];
+        for (int k = 0; k < 3; k++)
+            for (int c = 0; c < 3; c++) v[k][c] = hs.verts[3 * (size_t)vi[k] + c];
+        DrTri &T = hs.tris[i];
+        for (int c = 0; c < 3; c++) {
+            T.v0[c] = v[0][c];
+            T.e1[c] = v[1][c] - v[0][c];
+            T.e2[c] = v[2][c] - v[0][c];
+        }
+        std::memcpy(&T.v0[3], &g, 4);
+        T.e1[3] = T.e2[3] = 0.f;
+    }
+    std::vector<int> map(bn.size(), -1);
+    // assign output indices to interior nodes in preorder
+    std::vector<int> stack = {0};
+    std::vector<int> interior;
+    while (!stack.empty()) {
+        int n = stack.back();
+        stack.pop_back();
+        if (bn[n].left < 0) continue;
+        map[n] = (int)interior.size();
+        interior.push_back(n);
+        stack.push_back(bn[n].right);
+        stack.push_back(bn[n].left);
+    }
+    auto put_child = [&](DrNode &o, int slot, int c) {
+        const BNode &ch = bn[c];
+        float *lo, *hi;
+        float box[6] = {ch.box.lo.x, ch.box.lo.y, ch.box.lo.z, ch.box.hi.x, ch.box.hi.y, ch.box.hi.z};
+        (void)lo;
+        (void)hi;
+        if (slot == 0) {
+            o.a[0] = box[0]; o.a[1] = box[1]; o.a[2] = box[2]; o.a[3] = box[3];
+            o.b[0] = box[4]; o.b[1] = box[5];
+        } else {
+            o.b[2] = box[0]; o.b[3] = box[1];
+            o.c[0] = box[2]; o.c[1] = box[3]; o.c[2] = box[4]; o.c[3] = box[5];
+        }
+        if (ch.left < 0) {
+            o.d[slot] = ch.first;
+            o.d[2 + slot] = ch.count;
+        } else {
+            o.d[slot] = map[c];
+            o.d[2 + slot] = 0;
+        }
+    };
+    if (interior.empty()) {
+        DrNode o;
+        std::memset(&o, 0, sizeof(o));
+        put_child(o, 0, 0);
+        float inf = std::numeric_limits<float>::infinity();
+        o.b[2] = o.b[3] = o.c[0] = inf;
+        o.c[1] = o.c[2] = o.c[3] = -inf;
+        o.d[1] = 0;
+        o.d[3] = 0;
+        hs.nodes.push_back(o);
+        return;
+    }
+    hs.nodes.resize(interior.size());
+    for (size_t i = 0; i < interior.size(); i++) {
+        DrNode o;
+        std::memset(&o, 0, sizeof(o));
+        put_child(o, 0, bn[interior[i]].left);
+        put_child(o, 1, bn[interior[i]].right);
+        hs.nodes[i] = o;
+    }
+}
+
+// ------------------------------------------------------------ edge trees --
+struct EPrim {
+    int id;
+    Box box;
+    V3 c;
+    V3 n0, n1;
+    Box dbox;
+    V3 dc;
+};
+
+int edge_rec(const std::vector<DrEdge> &edges, std::vector<EPrim> &p, int b, int e, bool directional,
+             std::vector<DrEdgeNode> &nodes, std::vector<DrEdge> &ordered, std::vector<int> &child1) {
+    int id = (int)nodes.size();
+    nodes.emplace_back();
+    child1.push_back(-1);
+    Box box, dbox;
+    for (int i = b; i < e; i++) {
+        box.grow(p[i].box);
+        if (directional) dbox.grow(p[i].dbox);
+    }
+    V3 nsum = V3{0.f, 0.f, 0.f};
+    for (int i = b; i < e; i++) {
+        nsum = nsum + p[i].n0;
+        nsum = nsum + p[i].n1;
+    }
+    V3 axis = normalize(nsum);
+    float cmin = 1.f;
+    for (int i = b; i < e; i++) {
+        cmin = std::min(cmin, dot(p[i].n0, axis));
+        cmin = std::min(cmin, dot(p[i].n1, axis));
+    }
+    DrEdgeNode &N = nodes[id];
+    N.lo[0] = box.lo.x; N.lo[1] = box.lo.y; N.lo[2] = box.lo.z;
+    N.hi[0] = box.hi.x; N.hi[1] = box.hi.y; N.hi[2] = box.hi.z;
+    N.axis[0] = axis.x; N.axis[1] = axis.y; N.axis[2] = axis.z;
+    N.angle = std::acos(cmin);
+    N.nprims = 0;
+    N.offset = 0;
+    N.total_length = 0.f;
+    N.pad = 0.f;
+    int n = e - b;
+    auto leaf = [&]() {
+        nodes[id].offset = (int)ordered.size();
+        nodes[id].nprims = n;
+        for (int i = b; i < e; i++) ordered.push_back(edges[p[i].id]);
+    };
+    if (n == 1) {
+        leaf();
+        return id;
+    }
+    Box cb, db;
+    for (int i = b; i < e; i++) {
+        cb.grow(p[i].c);
+        if (directional) db.grow(p[i].dc);
+    }
+    int dim = -1;
+    float mext = -std::numeric_limits<float>::infinity();
+    for (int i = 0; i < 3; i++) {
+        float x = cb.hi[i] - cb.lo[i];
+        if (x > mext) { dim = i; mext = x; }
+    }
+    if (directional) {
+        for (int i = 0; i < 3; i++) {
+            float x = db.hi[i] - db.lo[i];
+            if (x > mext) { dim = i + 3; mext = x; }
+        }
+    }
+    bool spatial = dim < 3;
+    const Box &used = spatial ? cb : db;
+    if (dim >= 3) dim -= 3;
+    if (used.hi[dim] == used.lo[dim]) {
+        leaf();
+        return id;
+    }
+    int mid = (b + e) / 2;
+    std::nth_element(p.begin() + b, p.begin() + mid, p.begin() + e, [dim, spatial](const EPrim &x, const EPrim &y) {
+        return spatial ? x.c[dim] < y.c[dim] : x.dc[dim] < y.dc[dim];
+    });
+    edge_rec(edges, p, b, mid, directional, nodes, ordered, child1);
+    int r = edge_rec(edges, p, mid, e, directional, nodes, ordered, child1);
+    child1[id] = r;
+    return id;
+}
+
+void build_edge_tree(const HostScene &hs, const std::vector<DrEdge> &edges, bool directional,
+                     std::vector<DrEdgeNode> &nodes, std::vector<DrEdge> &ordered) {
+    nodes.clear();
+    ordered.clear();
+    if (edges.empty()) return;
+    std::vector<EPrim> p(edges.size());
+    Box world;
+    for (size_t i = 0; i < edges.size(); i++) {
+        p[i].id = (int)i;
+        p[i].box = Box(getv(hs, edges[i].shape, edges[i].v0), getv(hs, edges[i].shape, edges[i].v1));
+        p[i].c = .5f * p[i].box.lo + .5f * p[i].box.hi;
+        p[i].n0 = V3{0.f, 0.f, 0.f};
+        p[i].n1 = V3{0.f, 0.f, 0.f};
+        world.grow(p[i].box);
+    }
+    if (directional) {
+        float factor = length(world.hi - world.lo) / 8.f;
+        for (size_t i = 0; i < edges.size(); i++) {
+            const DrEdge &ed = edges[i];
+            V3 sn = normalize(getn(hs, ed.shape, ed.v0) + getn(hs, ed.shape, ed.v1));
+            V3 n0 = face_normal(hs, ed.shape, ed.f0);
+            V3 n1 = face_normal(hs, ed.shape, ed.f1);
+            if (dot(n0, sn) < 0.f) n0 = -n0;
+            if (dot(n1, sn) < 0.f) n1 = -n1;
+            p[i].n0 = n0;
+            p[i].n1 = n1;
+            p[i].dbox = Box(n0 * factor, n1 * factor);
+            p[i].dc = 0.5f * (p[i].dbox.lo + p[i].dbox.hi);
+        }
+    }
+    std::vector<int> child1;
+    edge_rec(edges, p, 0, (int)edges.size(), directional, nodes, ordered, child1);
+    for (int i = (int)nodes.size() - 1; i >= 0; i--) {
+        DrEdgeNode &n = nodes[i];
+        if (n.nprims > 0) {
+            n.total_length = 0.f;
+            for (int k = 0; k < n.nprims; k++) {
+                const DrEdge &ed = ordered[n.offset + k];
+                n.total_length += length(getv(hs, ed.shape, ed.v1) - getv(hs, ed.shape, ed.v0));
+            }
+        } else {
+            n.offset = child1[i];
+            n.total_length = nodes[i + 1].total_length + nodes[n.offset].total_length;
+        }
+    }
+}
+
+// ------------------------------------------------------- camera helpers --
+V3 xfm_point(const float *a, V3 p) {
+    float x = a[0] * p.x + a[1] * p.y + a[2] * p.z + a[3];
+    float y = a[4] * p.x + a[5] * p.y + a[6] * p.z + a[7];
+    float z = a[8] * p.x + a[9] * p.y + a[10] * p.z + a[11];
+    float w = a[12] * p.x + a[13] * p.y + a[14] * p.z + a[15];
+    float inv = 1.f / w;
+    return V3{x * inv, y * inv, z * inv};
+}
+struct V2 {
+    float x, y;
+};
+V2 project_local(const DrCamera &c, V3 l) {
+    return V2{(-l.x / (l.z * c.fov) + 1.f) * 0.5f, (-l.y / (l.z * c.fov * c.aspect) + 1.f) * 0.5f};
+}
+bool project_edge(const DrCamera &c, V3 p0, V3 p1, V2 &s0, V2 &s1) {
+    V3 a = xfm_point(c.w2c, p0), b = xfm_point(c.w2c, p1);
+    if (a.z < c.clip_near && b.z < c.clip_near) return false;
+    if (a.z < c.clip_near) {
+        V3 d = a - b;
+        float t = -(b.z - c.clip_near) / d.z;
+        a = b + t * d;
+    } else if (b.z < c.clip_near) {
+        V3 d = b - a;
+        float t = -(a.z - c.clip_near) / d.z;
+        b = a + t * d;
+    }
+    s0 = project_local(c, a);
+    s1 = project_local(c, b);
+    return true;
+}
+// Cohen-Sutherland clip to [0,1]^2 (reference line_clip.h:13-102)
+bool clip_line(V2 &a, V2 &b) {
+    auto code = [](const V2 &v) {
+        int c = 0;
+        if (v.x < 0.f) c |= 1; else if (v.x > 1.f) c |= 2;
+        if (v.y < 0.f) c |= 4; else if (v.y > 1.f) c |= 8;
+        return c;
+    };
+    int c0 = code(a), c1 = code(b);
+    while (true) {
+        if (!(c0 | c1)) return true;
+        if (c0 & c1) return false;
+        int out = c0 ? c0 : c1;
+        V2 v = V2{0.f, 0.f};
+        if (out & 8) { v.x = a.x + (b.x - a.x) * (1.f - a.y) / (b.y - a.y); v.y = 1.f; }
+        else if (out & 4) { v.x = a.x + (b.x - a.x) * (0.f - a.y) / (b.y - a.y); v.y = 0.f; }
+        else if (out & 2) { v.y = a.y + (b.y - a.y) * (1.f - a.x) / (b.x - a.x); v.x = 1.f; }
+        else if (out & 1) { v.y = a.y + (b.y - a.y) * (0.f - a.x) / (b.x - a.x); v.x = 0.f; }
+        if (out == c0) { a = v; c0 = code(a); } else { b = v; c1 = code(b); }
+    }
+}
+// reference edge.h:59-79
+bool is_silhouette(const HostScene &hs, V3 p, const DrEdge &e) {
+    const DrShape &sh = hs.shapes[e.shape];
+    if (!sh.has_n) return true;
+    if (e.f0 == -1 || e.f1 == -1) return true;
+    V3 v0 = getv(hs, e.shape, e.v0), v1 = getv(hs, e.shape, e.v1);
+    auto ns = [&](int f) {
+        for (int k = 0; k < 3; k++) {
+            int id = tri_index(hs, e.shape, f, k);
+            if (id != e.v0 && id != e.v1) return getv(hs, e.shape, id);
+        }
+        return V3{0.f, 0.f, 0.f};
+    };
+    V3 a = ns(e.f0), b = ns(e.f1);
+    V3 n0 = normalize(cross(v0 - a, v1 - a));
+    V3 n1 = normalize(cross(v1 - b, v0 - b));
+    bool f0 = dot(p - v0, n0) > 0.f && dot(p - v1, n0) > 0.f && dot(p - a, n0) > 0.f;
+    bool f1 = dot(p - v0, n1) > 0.f && dot(p - v1, n1) > 0.f && dot(p - b, n1) > 0.f;
+    return f0 != f1;
+}
+
+}  // namespace
+
+void build_primary_edges(HostScene &hs) {
+    hs.prim_edges.clear();
+    std::vector<float> lens;
+    V3 org = xfm_point(hs.cam.c2w, V3{0.f, 0.f, 0.f});
+    for (const DrEdge &e : hs.edges) {
+        float dist = 0.f;
+        V2 a, b;
+        if (project_edge(hs.cam, getv(hs, e.shape, e.v0), getv(hs, e.shape, e.v1), a, b)) {
+            if (clip_line(a, b)) {
+                float dx = b.x - a.x, dy = b.y - a.y;
+                dist = std::sqrt(dx * dx + dy * dy);
+            }
+        }
+        if (!is_silhouette(hs, org, e)) dist = 0.f;
+        if (dist > 0.f) {
+            hs.prim_edges.push_back(e);
+            lens.push_back(dist);
+        }
+    }
+    hs.prim = lens.empty() ? Dist() : make_dist(lens);
+}
+
+void set_camera(HostScene &hs, const dr_camera &c) {
+    std::memcpy(hs.cam.c2w, c.cam_to_world, sizeof(float) * 16);
+    std::memcpy(hs.cam.w2c, c.world_to_cam, sizeof(float) * 16);
+    hs.cam.fov = c.fov_factor;
+    hs.cam.aspect = c.aspect_ratio;
+    hs.cam.clip_near = c.clip_near;
+    hs.cam.pad = 0.f;
+}
+
+std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
+    if (!d) return "null scene description";
+    if (d->camera.fisheye) return "fisheye cameras are not supported";
+    if (d->num_shapes < 0 || d->num_materials < 0 || d->num_lights < 0) return "negative counts";
+    if (d->num_lights == 0) return "the scene has no light (the reference indexes an empty light list)";
+    set_camera(hs, d->camera);
+    // gradient layout
+    std::vector<size_t> doff(d->num_materials), soff(d->num_materials), roff(d->num_materials);
+    std::vector<size_t> voff(d->num_shapes), noff(d->num_shapes), ioff(d->num_lights);
+    hs.grad_size = dr_grad_layout_compute(d, nullptr, doff.data(), soff.data(), roff.data(), voff.data(),
+                                          noff.data(), ioff.data());
+    // materials
+    auto tex = [&](const dr_texture &t, int channels, size_t goff, DrTex &o) -> std::string {
+        if (t.channels != channels) return "material channel count mismatch";
+        if (t.height < 0 || t.width < 0 || (t.height > 0 && t.width <= 0)) return "bad texture size";
+        if (!t.data) return "null material data";
+        o.off = (int)hs.texels.size();
+        o.h = t.height;
+        o.w = t.width;
+        o.c = channels;
+        o.grad_off = (long long)goff;
+        size_t n = dr_texture_size(&t);
+        hs.texels.insert(hs.texels.end(), t.data, t.data + n);
+        return "";
+    };
+    for (int i = 0; i < d->num_materials; i++) {
+        const dr_material &m = d->materials[i];
+        DrMaterial o;
+        std::memset(&o, 0, sizeof(o));
+        std::string err;
+        if (!(err = tex(m.diffuse_reflectance, 3, doff[i], o.diffuse)).empty()) return err;
+        if (!(err = tex(m.specular_reflectance, 3, soff[i], o.specular)).empty()) return err;
+        if (!(err = tex(m.roughness, 1, roff[i], o.roughness)).empty()) return err;
+        for (int k = 0; k < 2; k++) {
+            o.duv[k] = m.diffuse_uv_scale[k];
+            o.suv[k] = m.specular_uv_scale[k];
+            o.ruv[k] = m.roughness_uv_scale[k];
+        }
+        o.two_sided = m.two_sided ? 1 : 0;
+        hs.materials.push_back(o);
+    }
+    // shapes
+    int vbase = 0, tbase = 0;
+    for (int i = 0; i < d->num_shapes; i++) {
+        const dr_shape &s = d->shapes[i];
+        if (!s.vertices || !s.indices || s.num_vertices < 0 || s.num_triangles < 0) return "bad shape arrays";
+        if (s.material_id < 0 || s.material_id >= d->num_materials) return "shape material id out of range";
+        DrShape o;
+        std::memset(&o, 0, sizeof(o));
+        o.vert_off = vbase;
+        o.tri_off = tbase;
+        o.nv = s.num_vertices;
+        o.nt = s.num_triangles;
+        o.has_uv = s.uvs != nullptr;
+        o.has_n = s.normals != nullptr;
+        o.mat = s.material_id;
+        o.light = -1;
+        o.vgrad_off = (long long)voff[i];
+        o.ngrad_off = s.normals ? (long long)noff[i] : -1;
+        hs.verts.insert(hs.verts.end(), s.vertices, s.vertices + 3 * (size_t)s.num_vertices);
+        if (s.uvs) hs.uvs.insert(hs.uvs.end(), s.uvs, s.uvs + 2 * (size_t)s.num_vertices);
+        else hs.uvs.insert(hs.uvs.end(), 2 * (size_t)s.num_vertices, 0.f);
+        if (s.normals) hs.normals.insert(hs.normals.end(), s.normals, s.normals + 3 * (size_t)s.num_vertices);
+        else hs.normals.insert(hs.normals.end(), 3 * (size_t)s.num_vertices, 0.f);
+        for (int t = 0; t < s.num_triangles; t++) {
+            for (int k = 0; k < 3; k++) {
+                int v = s.indices[3 * (size_t)t + k];
+                if (v < 0 || v >= s.num_vertices) return "triangle vertex index out of range";
+                hs.tri_vidx.push_back(vbase + v);
+            }
+            hs.tri_shape.push_back(i);
+        }
+        vbase += s.num_vertices;
+        tbase += s.num_triangles;
+        hs.shapes.push_back(o);
+    }
+    // lights (reference scene.cpp:24-52)
+    std::vector<float> imp;
+    hs.light_total_intensity = 0.f;
+    for (int i = 0; i < d->num_lights; i++) {
+        const dr_light &l = d->lights[i];
+        if (l.shape_id < 0 || l.shape_id >= d->num_shapes) return "light shape id out of range";
+        DrShape &sh = hs.shapes[l.shape_id];
+        if (sh.nt == 0) return "light shape has no triangles";
+        sh.light = i;
+        DrLight o;
+        std::memset(&o, 0, sizeof(o));
+        o.shape = l.shape_id;
+        for (int k = 0; k < 3; k++) o.I[k] = l.intensity[k];
+        float lum = luminance(V3{l.intensity[0], l.intensity[1], l.intensity[2]});
+        o.lum = lum;
+        imp.push_back(lum);
+        hs.light_total_intensity += lum;
+        std::vector<float> areas;
+        Box bounds;
+        float asum = 0.f;
+        for (int t = 0; t < sh.nt; t++) {
+            V3 a = getv(hs, l.shape_id, tri_index(hs, l.shape_id, t, 0));
+            V3 b = getv(hs, l.shape_id, tri_index(hs, l.shape_id, t, 1));
+            V3 c = getv(hs, l.shape_id, tri_index(hs, l.shape_id, t, 2));
+            float ar = 0.5f * length(cross(b - a, c - a));
+            asum += ar;
+            areas.push_back(ar);
+            Box tb(a, b);
+            tb.grow(c);
+            bounds.grow(tb);
+        }
+        Dist dd = make_dist(areas);
+        o.ntri = sh.nt;
+        o.func_off = (int)hs.light_func.size();
+        o.cdf_off = (int)hs.light_cdf.size();
+        o.func_int = dd.func_int;
+        hs.light_func.insert(hs.light_func.end(), dd.func.begin(), dd.func.end());
+        hs.light_cdf.insert(hs.light_cdf.end(), dd.cdf.begin(), dd.cdf.end());
+        o.area = asum;
+        V3 c = (bounds.lo + bounds.hi) / 2.f;
+        o.sphere_c[0] = c.x; o.sphere_c[1] = c.y; o.sphere_c[2] = c.z;
+        o.sphere_r = length(bounds.hi - c);
+        o.grad_off = (long long)ioff[i];
+        hs.lights.push_back(o);
+    }
+    hs.light_sel = make_dist(imp);
+    build_bvh(hs);
+    // edges (reference scene.cpp:54-97)
+    std::map<std::tuple<int, int, int>, DrEdge> emap;
+    for (int i = 0; i < (int)hs.shapes.size(); i++) {
+        const DrShape &s = hs.shapes[i];
+        for (int t = 0; t < s.nt; t++) {
+            int a = tri_index(hs, i, t, 0), b = tri_index(hs, i, t, 1), c = tri_index(hs, i, t, 2);
+            auto add = [&](int x, int y) {
+                if (x > y) std::swap(x, y);
+                auto key = std::make_tuple(i, x, y);
+                auto it = emap.find(key);
+                if (it == emap.end()) emap[key] = DrEdge{i, x, y, t, -1};
+                else it->second.f1 = t;
+            };
+            add(a, b);
+            add(b, c);
+            add(c, a);
+        }
+    }
+    hs.edges.clear();
+    for (auto &kv : emap) {
+        const DrEdge &e = kv.second;
+        V3 dv = getv(hs, e.shape, e.v0) - getv(hs, e.shape, e.v1);
+        if (std::fabs(dv.x) + std::fabs(dv.y) + std::fabs(dv.z) < 1e-10f) continue;
+        if (e.f1 != -1) {
+            float ang = std::fabs(dot(face_normal(hs, e.shape, e.f0), face_normal(hs, e.shape, e.f1)));
+            if (ang == 1.f || ang == -1.f) continue;
+        }
+        hs.edges.push_back(e);
+    }
+    build_primary_edges(hs);
+    std::vector<DrEdge> sil, nrm;
+    for (const DrEdge &e : hs.edges) {
+        if (e.f0 == -1 || e.f1 == -1 || !hs.shapes[e.shape].has_n) sil.push_back(e);
+        else nrm.push_back(e);
+    }
+    build_edge_tree(hs, sil, false, hs.sil_nodes, hs.sil_edges);
+    build_edge_tree(hs, nrm, true, hs.nrm_nodes, hs.nrm_edges);
+    return "";
+}
+
+}  // namespace drh

@@ -1,0 +1,45 @@
+// Host-side scene preparation for the device renderer: flattening of shapes
+// into global arrays, the triangle BVH, edge extraction, the two edge trees
+// and the light / primary-edge distributions.
+#pragma once
+
+#include "../../include/delta_ray.h"
+#include "dr_types.h"
+
+#include <string>
+#include <vector>
+
+namespace drh {
+
+struct Dist {
+    std::vector<float> func, cdf;
+    float func_int = 0.f;
+};
+
+struct HostScene {
+    DrCamera cam;
+    std::vector<DrShape> shapes;
+    std::vector<DrMaterial> materials;
+    std::vector<DrLight> lights;
+    std::vector<float> verts, uvs, normals, texels;
+    std::vector<int> tri_vidx, tri_shape;
+    std::vector<DrNode> nodes;
+    std::vector<DrTri> tris;
+    std::vector<float> light_func, light_cdf;  // per light, concatenated
+    Dist light_sel;
+    float light_total_intensity = 0.f;
+    std::vector<DrEdge> edges;  // all edges, sorted by (shape, v0, v1)
+    std::vector<DrEdge> sil_edges, nrm_edges;
+    std::vector<DrEdgeNode> sil_nodes, nrm_nodes;
+    std::vector<DrEdge> prim_edges;
+    Dist prim;
+    size_t grad_size = 0;
+};
+
+// Returns an empty string on success, else the reason for rejecting desc.
+std::string build_host_scene(const dr_scene_desc *desc, HostScene &hs);
+void set_camera(HostScene &hs, const dr_camera &c);
+// Primary-edge list and distribution for the current camera
+void build_primary_edges(HostScene &hs);
+
+}  // namespace drh

@@ -1391,6 +1391,34 @@ void or_scene_info(void *s, int *out) {
 
 size_t or_grad_size(void *s) { return ((OScene *)s)->lay.total; }
 
+// Edge tree dump for invariant tests.  which: 0 silhouette, 1 normal.
+// nodes_out (if not NULL): per node 13 floats = lo[3], hi[3], axis[3], angle,
+// offset, nprims, total_length.  edges_out (if not NULL): per leaf-ordered
+// edge 5 ints = shape, v0, v1, f0, f1.  Returns the node count.
+int or_edge_tree(void *s, int which, float *nodes_out, int *edges_out, int *num_edges) {
+    const Scene &sc = ((OScene *)s)->sc;
+    const EdgeTree &t = which ? sc.nrm_tree : sc.sil_tree;
+    if (nodes_out) {
+        for (size_t i = 0; i < t.nodes.size(); i++) {
+            const EdgeNode &n = t.nodes[i];
+            float *o = nodes_out + 13 * i;
+            for (int k = 0; k < 3; k++) { o[k] = n.bounds.lo[k]; o[3 + k] = n.bounds.hi[k]; o[6 + k] = n.cone_axis[k]; }
+            o[9] = n.cone_angle;
+            o[10] = (float)n.offset;
+            o[11] = (float)n.nprims;
+            o[12] = n.total_length;
+        }
+    }
+    if (edges_out) {
+        for (size_t i = 0; i < t.edges.size(); i++) {
+            int *o = edges_out + 5 * i;
+            o[0] = t.edges[i].shape; o[1] = t.edges[i].v0; o[2] = t.edges[i].v1; o[3] = t.edges[i].f0; o[4] = t.edges[i].f1;
+        }
+    }
+    if (num_edges) *num_edges = (int)t.edges.size();
+    return (int)t.nodes.size();
+}
+
 // Forward (d_image == NULL) or forward+adjoint render on nthreads CPU threads.
 // Pixel work is split in 8x8 tiles taken from a shared counter, primary-edge
 // samples in chunks of 4096, as the reference's work-list pool does.

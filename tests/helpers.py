@@ -1,0 +1,102 @@
+"""Shared helpers of the parity tests: small workloads, GPU renders through the
+C-ABI, and comparison metrics."""
+import numpy as np
+
+from delta_ray_amd import _abi, workloads
+from delta_ray_amd import delta_ray as dr
+
+
+def small_workload(name, res=48, **kw):
+    if name == "single_triangle":
+        return workloads.single_triangle(resolution=(res, res), **kw)
+    if name == "shadow":
+        return workloads.shadow(resolution=(res, res), light_intensity=1000.0, **kw)
+    if name == "heightfield":
+        return workloads.heightfield(grid=kw.pop("grid", 24), resolution=(res, res), **kw)
+    if name == "glossy_plane":
+        return glossy_plane(res, **kw)
+    raise KeyError(name)
+
+
+def glossy_plane(res=48, roughness=0.005, two_sided=False, textured=False, seed=5):
+    """A specular (roughness <= 0.01) mirror-like plane under a light, seen by a
+    camera: exercises the specular chain of the adjoint (the next surface
+    point stays differentiable) and, with textured=True, texture adjoints."""
+    rng = np.random.default_rng(seed)
+    cam = workloads.make_camera([0, 2.5, -4], [0, 0, 0], [0, 1, 0], 45.0, (res, res))
+    gv = np.array([[-3, 0, -3], [-3, 0, 3], [3, 0, -3], [3, 0, 3]], np.float32)
+    gi = np.array([[0, 1, 2], [1, 3, 2]], np.int32)
+    uvs = np.array([[0, 0], [0, 1], [1, 0], [1, 1]], np.float32)
+    one = np.ones(2, np.float32)
+    if textured:
+        diff = rng.uniform(0.1, 0.6, size=(8, 8, 3)).astype(np.float32)
+        rough = rng.uniform(0.05, 0.4, size=(8, 8, 1)).astype(np.float32)
+        spec = np.array([0.5, 0.5, 0.5], np.float32)
+    else:
+        diff = np.array([0.2, 0.2, 0.2], np.float32)
+        rough = np.array([roughness], np.float32)
+        spec = np.array([0.7, 0.7, 0.7], np.float32)
+    mat = dr.Material(diff, spec, rough, one, one, one, two_sided)
+    grey = workloads.diffuse_material([0.6, 0.5, 0.4])
+    black = workloads.diffuse_material([0.0, 0.0, 0.0])
+    ground = dr.Shape(gv, gi, uvs, None, mat, None)
+    # a small box-ish blocker floating above the plane
+    bv = np.array([[-0.5, 0.6, -0.3], [0.4, 0.9, -0.5], [0.0, 1.2, 0.6]], np.float32)
+    bi = np.array([[0, 1, 2]], np.int32)
+    blocker = dr.Shape(bv, bi, None, None, grey, None)
+    lv, li = workloads.quad([0.0, 4.0, 1.5], [0.0, -1.0, -0.4], 1.0)
+    ls = dr.Shape(lv, li, None, None, black, None)
+    light = dr.Light(ls, np.array([30, 30, 30], np.float32))
+    ls.light = light
+    return dict(name="glossy_plane", camera=cam, shapes=[ground, blocker, ls], materials=[mat, grey, black],
+                lights=[light], resolution=(res, res), num_samples=1, max_bounces=2, params=[])
+
+
+def gpu_scene(w):
+    return dr.DeviceScene(w["camera"], w["shapes"], w["materials"], w["lights"])
+
+
+def gpu_render(scene, w, spp, nb, seed, flags=_abi.DR_FLAGS_PARITY, d_image=None, nprim=0, want_dx=False):
+    import torch
+    W, H = w["resolution"]
+    o = dr.DeviceScene.options(W, H, spp, nb, seed, flags, nprim)
+    dev = torch.device("cuda")
+    img = torch.zeros((H, W, 3), dtype=torch.float32, device=dev)
+    if d_image is None:
+        scene.render(o, img)
+        torch.cuda.synchronize()
+        return img.cpu().numpy(), None, None
+    if want_dx:
+        o.flags |= _abi.DR_WRITE_DX_IMAGE
+    d = torch.from_numpy(np.ascontiguousarray(d_image, np.float32)).to(dev)
+    g = torch.zeros((scene.grad_size,), dtype=torch.float32, device=dev)
+    dx = torch.zeros_like(img) if want_dx else None
+    scene.render_adjoint(o, d, img, g, dx)
+    torch.cuda.synchronize()
+    return img.cpu().numpy(), g.cpu().numpy(), (dx.cpu().numpy() if dx is not None else None)
+
+
+def rel_err(a, b):
+    """||a - b|| / max(||b||, tiny)"""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def random_rays(w, n, rng):
+    """Rays through the scene's bounding box from random outside points plus
+    camera rays, in the dr_intersect layout [org, tmin, dir, tmax]."""
+    pts = np.concatenate([np.asarray(s.vertices) for s in w["shapes"]], 0)
+    lo, hi = pts.min(0), pts.max(0)
+    c = 0.5 * (lo + hi)
+    ext = max(float(np.max(hi - lo)), 1e-3)
+    org = c + rng.normal(size=(n, 3)) * ext
+    tgt = lo + rng.uniform(size=(n, 3)) * (hi - lo)
+    d = tgt - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = org
+    rays[:, 3] = 1e-3
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    return rays
