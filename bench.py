@@ -1,0 +1,189 @@
+"""Benchmark: forward + adjoint samples/s of the differentiable path tracer.
+
+Workload (default): the procedural glossy heightfield, 128 x 128 grid
+(32 258 triangles + 2 light quads), 512 x 512 pixels, 64 samples per pixel,
+2 bounces, fp32.  One step = one forward+adjoint render (the reference's
+RenderFunction.backward call: pixel samples with interior adjoint and
+secondary edges, then spp*W*H primary-edge samples), with scene, d_image and
+buffers resident on the GPU before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--workload heightfield|shadow|single_triangle]
+
+Prints one JSON line.  `roofline` is for the dominant kernel (k_adjoint):
+algorithmic bytes per launch (BVH traversal + edge sampling + image/d_image
+I/O, from the kernel's own counters in a separate untimed run) divided by its
+average launch time measured with HIP events on its launch stream during the
+timed steps.  `cpu_baseline` times the CPU restatement of the reference
+(oracle/) on the host cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fwd+adjoint samples/s; time to gradient rel-RMSE 1e-2, learned vs plain"
+
+# algorithmic bytes (measurement section)
+B_RAY_IO = 32 + 16      # ray in (org, dir, tmin, tmax) + hit out (ids, u, v)
+B_NODE = 64             # one BVH2 node (two child boxes)
+B_TRI = 48              # one triangle (v0, e1, e2)
+B_PIXEL_IO = 12 + 12    # d_image read + image write per pixel sample
+B_EDGE = 24 + 24 + 24 + 24 + 36  # edge v0/v1, faces, non-shared verts, normals, LTC matrix
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="heightfield")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-spp", type=int, default=1, help="spp of the bounded CPU sample")
+    return ap.parse_args()
+
+
+def make_workload(name):
+    from delta_ray_amd import workloads
+    return workloads.WORKLOADS[name]()
+
+
+def d_image_for(w, scene, dev):
+    """dLoss/dImage of an L2 loss against a target rendered from perturbed
+    parameters (as the reference's optimisation scripts do)."""
+    import numpy as np
+    import torch
+    from delta_ray_amd import delta_ray as dr
+    W, H = w["resolution"]
+    spp, nb = w["num_samples"], w["max_bounces"]
+    rng = np.random.default_rng(101)
+    shapes = []
+    pert = {i for kind, i, _ in w["params"] if kind == "shape"}
+    for i, s in enumerate(w["shapes"]):
+        v = np.asarray(s.vertices).copy()
+        if i in pert:
+            v += rng.normal(0.0, 0.05, size=v.shape).astype(np.float32)
+        ns = dr.Shape(v, s.indices, s.uvs, s.normals, s.material, None)
+        shapes.append(ns)
+    lights = []
+    for l in w["lights"]:
+        k = w["shapes"].index(l.shape)
+        nl = dr.Light(shapes[k], l.intensity)
+        shapes[k].light = nl
+        lights.append(nl)
+    tscene = dr.DeviceScene(w["camera"], shapes, w["materials"], lights)
+    o = dr.DeviceScene.options(W, H, spp, nb, 12345)
+    target = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    tscene.render(o, target)
+    img = torch.empty_like(target)
+    scene.render(dr.DeviceScene.options(W, H, spp, nb, 777), img)
+    torch.cuda.synchronize()
+    return (2.0 * (img - target)).contiguous()
+
+
+def cpu_baseline(w, d_img, spp, threads):
+    from tests import oracle_lib
+    osc = oracle_lib.scene_from_workload(w)
+    W, H = w["resolution"]
+    t0 = time.perf_counter()
+    osc.render(W, H, spp, w["max_bounces"], 3, d_image=d_img, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return spp * W * H / dt, dt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: multi-GPU runs are not part of this build "
+              "(the path would shard pixel samples per GPU and all-reduce the gradient buffer)")
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from delta_ray_amd import _abi
+    from delta_ray_amd import delta_ray as dr
+
+    dev = torch.device("cuda:0")
+    w = make_workload(args.workload)
+    W, H = w["resolution"]
+    spp, nb = w["num_samples"], w["max_bounces"]
+    t0 = time.perf_counter()
+    scene = dr.DeviceScene(w["camera"], w["shapes"], w["materials"], w["lights"])
+    build_s = time.perf_counter() - t0
+    info = scene.info()
+    d_img = d_image_for(w, scene, dev)
+    image = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    grads = torch.empty((scene.grad_size,), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step(i):
+        o = dr.DeviceScene.options(W, H, spp, nb, 1000 + i)
+        scene.render_adjoint(o, d_img, image, grads, None, stream)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    scene.stats(_abi.DR_STATS_TIMING)  # clear, start kernel timing
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = scene.stats(0)
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    samples = spp * W * H
+    value = samples / (ms_per_step / 1000.0)
+
+    # counters of the dominant kernel (untimed run, adjoint kernel only)
+    scene.stats(_abi.DR_STATS_COUNTERS)
+    o = dr.DeviceScene.options(W, H, spp, nb, 999, _abi.DR_FLAGS_PARITY | _abi.DR_SKIP_PRIMARY_EDGES)
+    scene.render_adjoint(o, d_img, image, grads, None, stream)
+    ca = scene.stats(0)
+    scene.stats(_abi.DR_STATS_COUNTERS)
+    o = dr.DeviceScene.options(W, H, spp, nb, 999, _abi.DR_FLAGS_PARITY | _abi.DR_SKIP_INTERIOR |
+                               _abi.DR_SKIP_SECONDARY_EDGES)
+    scene.render_adjoint(o, d_img, image, grads, None, stream)
+    cp = scene.stats(0)
+    adj_ms = kt["adjoint_ms"] / max(kt["adjoint_launches"], 1)
+    prim_ms = kt["primary_ms"] / max(kt["primary_launches"], 1)
+    bytes_adj = (B_RAY_IO * ca["rays"] + B_NODE * ca["nodes_visited"] + B_TRI * ca["tris_visited"] +
+                 B_PIXEL_IO * samples + B_EDGE * ca["secondary_edges"])
+    achieved = bytes_adj / (adj_ms / 1000.0) / 1e9
+    rays_total = ca["rays"] + cp["rays"]
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_adjoint", "kernel_ms": round(adj_ms, 3),
+                "bytes_per_launch": int(bytes_adj),
+                "per_ray": {"nodes": round(ca["nodes_visited"] / max(ca["rays"], 1), 2),
+                            "tris": round(ca["tris_visited"] / max(ca["rays"], 1), 2)}}
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded procedural scene)",
+           "config": {"workload": "%s %dx%d %dspp %d-bounce, %d triangles" %
+                      (w["name"], W, H, spp, nb, info["num_triangles"]),
+                      "samples_per_step": samples, "primary_edge_samples_per_step": samples,
+                      "parallelism": "single GPU"},
+           "roofline": roofline,
+           "extra": {"rays_per_s": round(rays_total / (ms_per_step / 1000.0), 1),
+                     "rays_per_step": int(rays_total),
+                     "secondary_edges_per_step": int(ca["secondary_edges"]),
+                     "k_adjoint_ms": round(adj_ms, 3), "k_primary_ms": round(prim_ms, 3),
+                     "scene_build_s": round(build_s, 3), "scene": info}}
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cspp = args.cpu_spp
+        cval, cdt = cpu_baseline(w, d_img.cpu().numpy(), cspp, threads)
+        out["cpu_baseline"] = {"value": round(cval, 1), "unit": "samples/s", "cores": threads, "kind": "port",
+                               "sample": "same scene and resolution at %d spp (%d pixel samples + as many "
+                                         "primary-edge samples), fwd+adjoint, %.1f s" % (cspp, cspp * W * H, cdt)}
+        out["extra"]["gpu_over_cpu"] = round(value / cval, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -81,10 +81,18 @@ class dr_render_options(ctypes.Structure):
                 ("num_primary_edge_samples", ctypes.c_int64)]
 
 
+DR_STATS_COUNTERS = 1
+DR_STATS_TIMING = 2
+
+
 class dr_stats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_uint64), ("nodes_visited", ctypes.c_uint64),
                 ("tris_visited", ctypes.c_uint64), ("secondary_edges", ctypes.c_uint64),
-                ("primary_edge_samples", ctypes.c_uint64)]
+                ("primary_edge_samples", ctypes.c_uint64),
+                ("forward_ms", ctypes.c_double), ("adjoint_ms", ctypes.c_double),
+                ("primary_ms", ctypes.c_double),
+                ("forward_launches", ctypes.c_uint64), ("adjoint_launches", ctypes.c_uint64),
+                ("primary_launches", ctypes.c_uint64)]
 
 
 class dr_scene_info(ctypes.Structure):

@@ -163,12 +163,22 @@ static dr_status fail(dr_status s, const std::string &msg) {
     return s;
 }
 
+// kernel timing: event pairs recorded around launches when enabled
+enum { TK_FORWARD = 0, TK_ADJOINT = 1, TK_PRIMARY = 2 };
+struct TimedLaunch {
+    int kind;
+    hipEvent_t a, b;
+};
+
 struct dr_scene {
     drh::HostScene hs;
     DrSceneView view;
     std::vector<void *> allocs;
     unsigned long long *stats_dev = nullptr;
     bool stats_on = false;
+    bool timing_on = false;
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t> event_pool;
     double build_ms = 0.0;
     void *prim_bufs[3] = {nullptr, nullptr, nullptr};
 };
@@ -212,7 +222,40 @@ static bool upload_primary(dr_scene *s) {
     return true;
 }
 
+static hipEvent_t get_event(dr_scene *s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// record an event before (begin) / after (end) a launch when timing is on
+struct LaunchTimer {
+    dr_scene *s;
+    int kind;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    LaunchTimer(dr_scene *s_, int k, hipStream_t st_) : s(s_), kind(k), st(st_) {
+        if (s->timing_on) {
+            a = get_event(s);
+            if (a) (void)hipEventRecord(a, st);
+        }
+    }
+    ~LaunchTimer() {
+        if (!a) return;
+        hipEvent_t b = get_event(s);
+        if (b && hipEventRecord(b, st) == hipSuccess) s->timed.push_back(TimedLaunch{kind, a, b});
+        else s->event_pool.push_back(a);
+    }
+};
+
 static void free_scene(dr_scene *s) {
+    for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (auto e : s->event_pool) (void)hipEventDestroy(e);
     for (void *p : s->allocs) (void)hipFree(p);
     for (int i = 0; i < 3; i++)
         if (s->prim_bufs[i]) (void)hipFree(s->prim_bufs[i]);
@@ -252,9 +295,13 @@ static void launch_adjoint(dr_scene *s, const dr_render_options *o, const float 
     }
     long long total = (long long)o->width * o->height * chunks;
     SampleCtx c = make_ctx(o);
-    k_adjoint<MAXB><<<(unsigned)((total + DR_BLOCK - 1) / DR_BLOCK), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks,
+    {
+        LaunchTimer timer(s, TK_ADJOINT, hs);
+        k_adjoint<MAXB><<<(unsigned)((total + DR_BLOCK - 1) / DR_BLOCK), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks,
                                                                                        d_image, image, grads, dx);
+    }
     if (!(o->flags & DR_SKIP_PRIMARY_EDGES)) {
+        LaunchTimer timer(s, TK_PRIMARY, hs);
         long long N = o->num_primary_edge_samples > 0 ? o->num_primary_edge_samples
                                                       : (long long)o->num_samples * o->width * o->height;
         k_primary<MAXB><<<(unsigned)((N + DR_BLOCK - 1) / DR_BLOCK), DR_BLOCK, 0, hs>>>(s->view, c, N, d_image,
@@ -357,19 +404,33 @@ dr_status dr_scene_stats(dr_scene *s, int enable, dr_stats *out) {
             return fail(DR_ERROR_OUT_OF_MEMORY, "stats allocation failed");
         (void)hipMemset(s->stats_dev, 0, 8 * sizeof(unsigned long long));
     }
+    if (hipDeviceSynchronize() != hipSuccess) return fail(DR_ERROR_DEVICE, "device synchronize failed");
     if (out) {
+        std::memset(out, 0, sizeof(*out));
         unsigned long long h[8];
-        if (hipDeviceSynchronize() != hipSuccess ||
-            hipMemcpy(h, s->stats_dev, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipMemcpy(h, s->stats_dev, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
             return fail(DR_ERROR_DEVICE, "stats copy failed");
         out->rays = h[0];
         out->nodes_visited = h[1];
         out->tris_visited = h[2];
         out->secondary_edges = h[3];
         out->primary_edge_samples = h[4];
+        for (auto &t : s->timed) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) ms = 0.f;
+            if (t.kind == TK_FORWARD) { out->forward_ms += ms; out->forward_launches++; }
+            if (t.kind == TK_ADJOINT) { out->adjoint_ms += ms; out->adjoint_launches++; }
+            if (t.kind == TK_PRIMARY) { out->primary_ms += ms; out->primary_launches++; }
+        }
     }
+    for (auto &t : s->timed) {
+        s->event_pool.push_back(t.a);
+        s->event_pool.push_back(t.b);
+    }
+    s->timed.clear();
     (void)hipMemset(s->stats_dev, 0, 8 * sizeof(unsigned long long));
-    s->stats_on = enable != 0;
+    s->timing_on = (enable & DR_STATS_TIMING) != 0;
+    s->stats_on = (enable & DR_STATS_COUNTERS) != 0;
     s->view.stats = s->stats_on ? s->stats_dev : nullptr;
     return DR_OK;
 }
@@ -400,6 +461,7 @@ dr_status dr_render(dr_scene *s, const dr_render_options *o, float *image, void 
     size_t n = 3ull * o->width * o->height;
     if (chunks > 1) k_fill<<<1024, 256, 0, hs>>>(image, n, 0.f);
     long long total = (long long)o->width * o->height * chunks;
+    LaunchTimer timer(s, TK_FORWARD, hs);
     k_forward<<<(unsigned)((total + DR_BLOCK - 1) / DR_BLOCK), DR_BLOCK, 0, hs>>>(s->view, make_ctx(o), per, chunks,
                                                                                 image);
     return launch_check("k_forward");

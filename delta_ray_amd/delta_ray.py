@@ -319,8 +319,11 @@ class DeviceScene:
                                                    1 if any_hit else 0, s), "dr_intersect")
 
     def stats(self, enable):
+        """enable: bit mask of _abi.DR_STATS_COUNTERS / DR_STATS_TIMING (True =
+        counters).  Returns the statistics gathered since the previous call."""
         st = _abi.dr_stats()
-        _abi.check(self.lib, self.lib.dr_scene_stats(self.handle, 1 if enable else 0, ctypes.byref(st)),
+        mask = _abi.DR_STATS_COUNTERS if enable is True else int(enable)
+        _abi.check(self.lib, self.lib.dr_scene_stats(self.handle, mask, ctypes.byref(st)),
                    "dr_scene_stats")
         return {k: getattr(st, k) for k, _ in _abi.dr_stats._fields_}
 

@@ -247,14 +247,22 @@ dr_status dr_render_adjoint(dr_scene *scene, const dr_render_options *opts,
 dr_status dr_intersect(dr_scene *scene, const float *rays, int64_t n,
                        int32_t *hits, int any_hit, void *stream);
 
-/* Launch statistics of the last render on this scene (host, after a sync):
- * counts of rays traced, BVH nodes and triangles visited, edges sampled. */
+/* Statistics of the renders on this scene since the previous call (host;
+ * the call synchronises the device).  enable: DR_STATS_COUNTERS counts rays
+ * traced, BVH nodes and triangles visited, secondary edges and primary-edge
+ * samples (device atomics: slows the kernels, for measurement runs only);
+ * DR_STATS_TIMING records HIP events around every kernel on its launch
+ * stream and reports the summed device time per kernel.  The counters and
+ * times are cleared by every call. */
+enum { DR_STATS_COUNTERS = 1, DR_STATS_TIMING = 2 };
 typedef struct dr_stats {
     uint64_t rays;
     uint64_t nodes_visited;
     uint64_t tris_visited;
     uint64_t secondary_edges;
     uint64_t primary_edge_samples;
+    double forward_ms, adjoint_ms, primary_ms;   /* summed kernel times */
+    uint64_t forward_launches, adjoint_launches, primary_launches;
 } dr_stats;
 dr_status dr_scene_stats(dr_scene *scene, int enable, dr_stats *out);
 
