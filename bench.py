@@ -9,11 +9,11 @@ buffers resident on the GPU before the timed region.
 
     python bench.py [--steps K] [--warmup W] [--workload heightfield|shadow|single_triangle]
 
-Prints one JSON line.  `roofline` is for the dominant kernel (k_adjoint):
-algorithmic bytes per launch (BVH traversal + edge sampling + image/d_image
-I/O, from the kernel's own counters in a separate untimed run) divided by its
-average launch time measured with HIP events on its launch stream during the
-timed steps.  `cpu_baseline` times the CPU restatement of the reference
+Prints one JSON line.  `roofline` is for the dominant kernel (the one with
+the most device time per step): algorithmic bytes per launch (BVH traversal +
+edge sampling + image/d_image I/O, from the kernels' own counters in separate
+untimed runs) divided by its average launch time measured with HIP events on
+its launch stream during the timed steps.  `cpu_baseline` times the CPU restatement of the reference
 (oracle/) on the host cores on a bounded sample of the same workload.
 """
 import argparse
@@ -138,28 +138,40 @@ def main():
     samples = spp * W * H
     value = samples / (ms_per_step / 1000.0)
 
-    # counters of the dominant kernel (untimed run, adjoint kernel only)
-    scene.stats(_abi.DR_STATS_COUNTERS)
-    o = dr.DeviceScene.options(W, H, spp, nb, 999, _abi.DR_FLAGS_PARITY | _abi.DR_SKIP_PRIMARY_EDGES)
-    scene.render_adjoint(o, d_img, image, grads, None, stream)
-    ca = scene.stats(0)
-    scene.stats(_abi.DR_STATS_COUNTERS)
-    o = dr.DeviceScene.options(W, H, spp, nb, 999, _abi.DR_FLAGS_PARITY | _abi.DR_SKIP_INTERIOR |
-                               _abi.DR_SKIP_SECONDARY_EDGES)
-    scene.render_adjoint(o, d_img, image, grads, None, stream)
-    cp = scene.stats(0)
-    adj_ms = kt["adjoint_ms"] / max(kt["adjoint_launches"], 1)
-    prim_ms = kt["primary_ms"] / max(kt["primary_launches"], 1)
-    bytes_adj = (B_RAY_IO * ca["rays"] + B_NODE * ca["nodes_visited"] + B_TRI * ca["tris_visited"] +
-                 B_PIXEL_IO * samples + B_EDGE * ca["secondary_edges"])
-    achieved = bytes_adj / (adj_ms / 1000.0) / 1e9
-    rays_total = ca["rays"] + cp["rays"]
+    # per-kernel counters from untimed runs: path only, path + secondary
+    # edges, path + primary edges; differences attribute rays to kernels
+    def counters(extra_flags):
+        scene.stats(_abi.DR_STATS_COUNTERS)
+        o = dr.DeviceScene.options(W, H, spp, nb, 999, _abi.DR_FLAGS_PARITY | extra_flags)
+        scene.render_adjoint(o, d_img, image, grads, None, stream)
+        return scene.stats(0)
+    ca = counters(_abi.DR_SKIP_SECONDARY_EDGES | _abi.DR_SKIP_PRIMARY_EDGES)
+    cb = counters(_abi.DR_SKIP_PRIMARY_EDGES)
+    cc = counters(_abi.DR_SKIP_SECONDARY_EDGES)
+
+    def diff(x, y):
+        return {k: x[k] - y[k] for k in ("rays", "nodes_visited", "tris_visited")}
+    per_kernel = {"path": dict(ca), "edge": diff(cb, ca), "primary": diff(cc, ca)}
+    per_kernel["edge"]["edges"] = cb["secondary_edges"]
+    bytes_k = {}
+    for k, c in per_kernel.items():
+        bytes_k[k] = B_RAY_IO * c["rays"] + B_NODE * c["nodes_visited"] + B_TRI * c["tris_visited"]
+    bytes_k["path"] += B_PIXEL_IO * samples
+    bytes_k["edge"] += B_EDGE * cb["secondary_edges"]
+    kms = {k: kt[k + "_ms"] / max(kt[k + "_launches"], 1) for k in ("path", "select", "edge", "dp", "primary")}
+    kms_step = {k: kt[k + "_ms"] / args.steps for k in ("path", "select", "edge", "dp", "primary")}
+    dom = max(("path", "edge", "primary"), key=lambda k: kms_step[k])
+    launches_per_step = max(kt[dom + "_launches"] // args.steps, 1)
+    bytes_launch = bytes_k[dom] / launches_per_step
+    achieved = bytes_launch / (kms[dom] / 1000.0) / 1e9
+    rays_total = cb["rays"] + per_kernel["primary"]["rays"]
+    c = per_kernel[dom]
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_adjoint", "kernel_ms": round(adj_ms, 3),
-                "bytes_per_launch": int(bytes_adj),
-                "per_ray": {"nodes": round(ca["nodes_visited"] / max(ca["rays"], 1), 2),
-                            "tris": round(ca["tris_visited"] / max(ca["rays"], 1), 2)}}
+                "kernel": "k_" + dom, "kernel_ms_per_launch": round(kms[dom], 3),
+                "bytes_per_launch": int(bytes_launch),
+                "per_ray": {"nodes": round(c["nodes_visited"] / max(c["rays"], 1), 2),
+                            "tris": round(c["tris_visited"] / max(c["rays"], 1), 2)}}
     out = {"metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -171,8 +183,8 @@ def main():
            "roofline": roofline,
            "extra": {"rays_per_s": round(rays_total / (ms_per_step / 1000.0), 1),
                      "rays_per_step": int(rays_total),
-                     "secondary_edges_per_step": int(ca["secondary_edges"]),
-                     "k_adjoint_ms": round(adj_ms, 3), "k_primary_ms": round(prim_ms, 3),
+                     "secondary_edges_per_step": int(cb["secondary_edges"]),
+                     "kernel_ms_per_step": {k: round(v, 3) for k, v in kms_step.items()},
                      "scene_build_s": round(build_s, 3), "scene": info}}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -85,14 +85,14 @@ DR_STATS_COUNTERS = 1
 DR_STATS_TIMING = 2
 
 
+KERNELS = ["forward", "path", "primary", "select", "edge", "dp"]
+
+
 class dr_stats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_uint64), ("nodes_visited", ctypes.c_uint64),
                 ("tris_visited", ctypes.c_uint64), ("secondary_edges", ctypes.c_uint64),
                 ("primary_edge_samples", ctypes.c_uint64),
-                ("forward_ms", ctypes.c_double), ("adjoint_ms", ctypes.c_double),
-                ("primary_ms", ctypes.c_double),
-                ("forward_launches", ctypes.c_uint64), ("adjoint_launches", ctypes.c_uint64),
-                ("primary_launches", ctypes.c_uint64)]
+                ("kernel_ms", ctypes.c_double * 6), ("kernel_launches", ctypes.c_uint64 * 6)]
 
 
 class dr_scene_info(ctypes.Structure):

@@ -6,6 +6,7 @@
 #include "dr_types.h"
 
 #define DR_STACK 64
+#define DR_TREE_STACK 32
 #define DR_INF __builtin_huge_valf()
 
 struct Isect {

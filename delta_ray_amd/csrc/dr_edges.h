@@ -194,7 +194,7 @@ DR_INLINE float node_importance(const DrEdgeNode &n, float bound, F3 p) {
 
 // Edge-tree traversal (edge_tree.cpp:511-683), iterative: a stack of
 // pending right children reproduces the recursion's depth-first order and
-// therefore its order of random draws.  `emit(edge, weight)` is called for
+// therefore its order of random draws.  `emit(edge, weight, tree, index)` is called for
 // every selected silhouette edge, in order.
 struct TreeEntry {
     int node;
@@ -206,7 +206,9 @@ DR_INLINE void traverse_edge_tree(const DrSceneView &S, const DrEdgeNode *nodes,
                                   bool directional,
                                   bool cone_mode, float root_bound, float pmf, F3 wi, const SPoint &sp,
                                   float roughness, float dlum, float slum, const Cone &cone, Philox &rng, Emit &emit) {
-    TreeEntry stack[DR_STACK];
+    // depth-first with pending right children: at most depth + 1 entries
+    // (median-split trees are balanced: depth ~ log2(edges))
+    TreeEntry stack[DR_TREE_STACK];
     int sp_ = 0;
     stack[sp_++] = TreeEntry{0, root_bound, pmf, pmf};
     // each node is popped at most once per pass; the cap guards a malformed tree
@@ -221,7 +223,7 @@ DR_INLINE void traverse_edge_tree(const DrSceneView &S, const DrEdgeNode *nodes,
             float w = (cone_mode ? e.cone_pmf : e.all_pmf) / den;
             for (int i = 0; i < node.nprims; i++) {
                 const DrEdge &ed = edges[node.offset + i];
-                if (is_silhouette(S, sp.pos, ed)) emit(ed, w);
+                if (is_silhouette(S, sp.pos, ed)) emit(ed, w, directional, node.offset + i);
             }
             continue;
         }
@@ -260,19 +262,20 @@ DR_INLINE void traverse_edge_tree(const DrSceneView &S, const DrEdgeNode *nodes,
         TreeEntry le = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf};
         TreeEntry re = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf};
         if (split) {
-            if (rcs && sp_ < DR_STACK) stack[sp_++] = re;
-            if (lcs && sp_ < DR_STACK) stack[sp_++] = le;
+            if (rcs && sp_ < DR_TREE_STACK) stack[sp_++] = re;
+            if (lcs && sp_ < DR_TREE_STACK) stack[sp_++] = le;
         } else {
             float u = philox_next(rng);
             bool left = cone_mode ? u < plc : u < pla;
-            if (sp_ < DR_STACK) stack[sp_++] = left ? le : re;
+            if (sp_ < DR_TREE_STACK) stack[sp_++] = left ? le : re;
         }
     }
 }
 
-// edge_tree.cpp:654-725: choose the tree, then a cone pass and an "all" pass
+// edge_tree.cpp:654-725: choose the tree, then a cone pass and an "all" pass.
+// emit(edge, weight, in_normal_tree, index_in_tree_edge_list)
 template <class Emit>
-DR_INLINE void sample_shading_edges(const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp, const Cone &cone,
+DR_INLINE void sample_shading_edges_idx(const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp, const Cone &cone,
                                     Philox &rng, Emit &emit) {
     if (dot(wi, sp.fr.n) < 0.f) return;
     bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;

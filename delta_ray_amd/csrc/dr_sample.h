@@ -87,6 +87,28 @@ struct BRec {
     bool spec, flipped;
 };
 
+// adjoint of the primary surface point and of the camera ray
+// (reference: adjoints of cam_to_world read after the sweep,
+// delta_ray.cpp:770-781)
+DR_INLINE void primary_adjoint(const DrSceneView &S, const GradSink &G, const Isect &is0, F3 cam_org, F3 cam_dir,
+                               F3 cam_nd, const AdjSP &dsp0, F3 d_wi) {
+    TriData td = load_tri(S, is0);
+    F3 d_org = F3{0.f, 0.f, 0.f}, d_dir = F3{0.f, 0.f, 0.f};
+    F3 dv[3] = {F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
+    F3 dn[3] = {F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
+    surface_point_adj(td, cam_org, cam_dir, dsp0, d_org, d_dir, dv, dn);
+    scatter_tri(S, G, td, is0.shape, dv, dn);
+    d_dir -= d_wi;
+    const float *m = S.cam.c2w;
+    float iw = 1.f / m[15];
+    G.cam_add(3, d_org.x * iw);
+    G.cam_add(7, d_org.y * iw);
+    G.cam_add(11, d_org.z * iw);
+    G.cam_add(15, -(d_org.x * m[3] + d_org.y * m[7] + d_org.z * m[11]) * iw * iw);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) G.cam_add(i * 4 + j, d_dir[i] * cam_nd[j]);
+}
+
 // Reverse sweep over the recorded path.  color_mode: seeds d_c on the colour
 // (everything scattered); otherwise seeds dp at the shading-point positions
 // (vertices and camera only, reference delta_ray.cpp:968-1002).
@@ -170,72 +192,63 @@ DR_INLINE void reverse_path(const DrSceneView &S, const GradSink &G, const Light
         }
     }
     d_wi += d_cw[0];
-    // primary surface point
-    TriData td = load_tri(S, is[0]);
-    F3 d_org = F3{0.f, 0.f, 0.f}, d_dir = F3{0.f, 0.f, 0.f};
-    F3 dv[3] = {F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
-    F3 dn[3] = {F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
-    surface_point_adj(td, cam_org, cam_dir, dsp[0], d_org, d_dir, dv, dn);
-    scatter_tri(S, G, td, is[0].shape, dv, dn);
-    d_dir -= d_wi;
+    primary_adjoint(S, G, is[0], cam_org, cam_dir, cam_nd, dsp[0], d_wi);
     if (cm && primary_light >= 0) G.add3(S.lights[primary_light].grad_off, d_c);
-    // camera (reference: adjoints of cam_to_world read after the sweep)
-    const float *m = S.cam.c2w;
-    float w = m[15];
-    float iw = 1.f / w;
-    G.cam_add(3, d_org.x * iw);
-    G.cam_add(7, d_org.y * iw);
-    G.cam_add(11, d_org.z * iw);
-    G.cam_add(15, -(d_org.x * m[3] + d_org.y * m[7] + d_org.z * m[11]) * iw * iw);
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) G.cam_add(i * 4 + j, d_dir[i] * cam_nd[j]);
 }
 
+// Forward path of one pixel sample with the bounded per-bounce record the
+// reverse sweep needs (reference render_sample, delta_ray.cpp:694-766).
 template <int MAXB>
-DR_INLINE F3 adjoint_sample(const DrSceneView &S, const SampleCtx &c, GradSink &G, int xi, int yi, int s, F3 d_color,
-                            float &dx_count) {
-    Philox rng;
-    F2 scr;
+struct PathRec {
     LightSample ls;
-    pixel_sample_draws(c, xi, yi, s, rng, scr, ls);
-    SecondarySample ss[MAXB];
-    for (int b = 0; b < c.nb; b++) ss[b] = draw_secondary(rng);
     F3 org, dir, nd;
-    sample_primary(S.cam, scr, org, dir, &nd);
     Isect is[MAXB + 1];
     SPoint sp[MAXB + 1];
     BRec R[MAXB];
     bool chain[MAXB + 1];
+    int nused;
+    int primary_light;
+    F3 color;  // not yet divided by spp
+};
+
+template <int MAXB>
+DR_INLINE bool trace_path(const DrSceneView &S, const SampleCtx &c, int xi, int yi, int s, PathRec<MAXB> &P) {
+    Philox rng;
+    F2 scr;
+    pixel_sample_draws(c, xi, yi, s, rng, scr, P.ls);
+    SecondarySample ss[MAXB];
+    for (int b = 0; b < c.nb; b++) ss[b] = draw_secondary(rng);
+    sample_primary(S.cam, scr, P.org, P.dir, &P.nd);
+    P.nused = 0;
+    P.primary_light = -1;
+    P.color = F3{0.f, 0.f, 0.f};
     // the differentiated path computes surface points as the tape does
-    is[0] = intersect(S, org, dir, sp[0], true);
-    if (!valid(is[0])) return F3{0.f, 0.f, 0.f};
-    F3 wi = -dir;
-    F3 color = F3{0.f, 0.f, 0.f};
-    int plid = S.shapes[is[0].shape].light;
-    int primary_light = -1;
-    if (plid >= 0 && dot(sp[0].fr.n, wi) > 0.f) {
-        color += F3{S.lights[plid].I[0], S.lights[plid].I[1], S.lights[plid].I[2]};
-        primary_light = plid;
+    P.is[0] = intersect(S, P.org, P.dir, P.sp[0], true);
+    if (!valid(P.is[0])) return false;
+    F3 wi = -P.dir;
+    int plid = S.shapes[P.is[0].shape].light;
+    if (plid >= 0 && dot(P.sp[0].fr.n, wi) > 0.f) {
+        P.color += F3{S.lights[plid].I[0], S.lights[plid].I[1], S.lights[plid].I[2]};
+        P.primary_light = plid;
     }
     F3 thr = F3{1.f, 1.f, 1.f};
     F3 cur_wi = wi;
-    chain[0] = true;
-    int nused = 0;
+    P.chain[0] = true;
     for (int b = 0; b < c.nb; b++) {
-        BRec &r = R[b];
-        nused = b + 1;
+        BRec &r = P.R[b];
+        P.nused = b + 1;
         r.cur_wi = cur_wi;
         r.thr = thr;
-        r.nee = nee(S, ls, wi, is[b], sp[b]);
-        color += thr * r.nee;
+        r.nee = nee(S, P.ls, wi, P.is[b], P.sp[b]);
+        P.color += thr * r.nee;
         r.stage = 0;
         r.emis_light = -1;
         r.q = 1.f;
         r.b0 = ss[b].b0;
         r.b1 = ss[b].b1;
-        chain[b + 1] = false;
-        int mid = S.shapes[is[b].shape].mat;
-        SPoint spf = sp[b];
+        P.chain[b + 1] = false;
+        int mid = S.shapes[P.is[b].shape].mat;
+        SPoint spf = P.sp[b];
         r.flipped = S.materials[mid].two_sided && dot(cur_wi, spf.fr.n) < 0.f;
         if (r.flipped) {
             spf.fr = neg_frame(spf.fr);
@@ -248,24 +261,25 @@ DR_INLINE F3 adjoint_sample(const DrSceneView &S, const SampleCtx &c, GradSink &
             r.spec = false;
             break;
         }
-        is[b + 1] = intersect(S, sp[b].pos, r.sdir, sp[b + 1], true);
-        chain[b + 1] = chain[b] && r.spec;
-        if (!valid(is[b + 1])) break;
-        F3 wo = normalize(sp[b + 1].pos - sp[b].pos);
-        float pdf = bsdf_pdf(S, mid, sp[b], cur_wi, wo);
+        P.is[b + 1] = intersect(S, P.sp[b].pos, r.sdir, P.sp[b + 1], true);
+        P.chain[b + 1] = P.chain[b] && r.spec;
+        if (!valid(P.is[b + 1])) break;
+        F3 wo = normalize(P.sp[b + 1].pos - P.sp[b].pos);
+        float pdf = bsdf_pdf(S, mid, P.sp[b], cur_wi, wo);
         if (pdf <= 0.f) break;
-        F3 be = bsdf(S, mid, sp[b], cur_wi, wo);
+        F3 be = bsdf(S, mid, P.sp[b], cur_wi, wo);
         if (sum3(be) <= 0.f) break;
         r.stage = 1;
         r.wo = wo;
         r.pdf = pdf;
         r.be = be;
-        int nlid = S.shapes[is[b + 1].shape].light;
-        if (nlid >= 0 && dot(sp[b + 1].fr.n, -wo) > 0.f) {
+        int nlid = S.shapes[P.is[b + 1].shape].light;
+        if (nlid >= 0 && dot(P.sp[b + 1].fr.n, -wo) > 0.f) {
             F3 contrib = be * F3{S.lights[nlid].I[0], S.lights[nlid].I[1], S.lights[nlid].I[2]};
-            float lpdf = light_pdf(S, is[b + 1]) * length_sq(sp[b + 1].pos - sp[b].pos) / fabsf(dot(wo, sp[b + 1].ng));
+            float lpdf =
+                light_pdf(S, P.is[b + 1]) * length_sq(P.sp[b + 1].pos - P.sp[b].pos) / fabsf(dot(wo, P.sp[b + 1].ng));
             float mis = sq(pdf) / (sq(lpdf) + sq(pdf));
-            color += thr * contrib * (mis / pdf);
+            P.color += thr * contrib * (mis / pdf);
             r.emis_light = nlid;
             r.mis = mis;
         }
@@ -278,70 +292,148 @@ DR_INLINE F3 adjoint_sample(const DrSceneView &S, const SampleCtx &c, GradSink &
         }
         cur_wi = -wo;
     }
-    float sw = 1.f / c.spp;
-    color = color * sw;
-    if (!(c.flags & DR_SKIP_INTERIOR)) {
-        G.color_mode = true;
-        reverse_path<MAXB>(S, G, ls, org, dir, nd, is, sp, R, nused, chain, d_color * sw, nullptr,
-                           primary_light);
-    }
-    // secondary edges (reference delta_ray.cpp:868-966)
-    F3 dps[MAXB];
-    bool any_dp = false;
-    for (int b = 0; b < MAXB; b++) dps[b] = F3{0.f, 0.f, 0.f};
-    int last_count = 0;
-    if (!(c.flags & DR_SKIP_SECONDARY_EDGES)) {
-        uint64_t sid = ((uint64_t)yi * c.W + xi) * (uint64_t)c.spp + s;
-        for (int b = 0; b < nused; b++) {
-            if (!chain[b]) continue;
-            Philox prng;
-            philox_init(prng, c.seed, 0, sid, dr_stream_point(b));
-            float bcomp = philox_next(prng), eu = philox_next(prng);
-            int lid = sample_discrete(S.lights_cdf, S.num_lights, ls.light_sel);
-            const DrLight &L = S.lights[lid];
-            F3 bc = F3{L.sphere_c[0], L.sphere_c[1], L.sphere_c[2]};
-            F3 dl = bc - sp[b].pos;
-            float tan_a = L.sphere_r / length(dl);
-            float cos_a = tan_a <= 1.f ? 1.f / sqrtf(tan_a * tan_a + 1.f) : 0.f;
-            Cone cone = Cone{sp[b].pos, normalize(dl), cos_a};
-            int j = 0;
-            const BRec &r = R[b];
-            const SPoint &spb = sp[b];
-            const Isect &isb = is[b];
-            int nss = c.nb - b - 1;
-            auto emit = [&](const DrEdge &edge, float w) {
-                int jj = j++;
-                if (w == 0.f) return;
-                Philox erng;
-                philox_init(erng, c.seed, 0, sid, dr_stream_edge(b, jj));
-                LightSample els = draw_light_sample(erng);
-                SecondarySample ess[MAXB];
-                for (int i = 0; i < nss; i++) ess[i] = draw_secondary(erng);
-                F3 weight = r.thr * (sw * w);
-                EdgeResult er = sample_shading_edge_point(S, bcomp, eu, edge, els, ess, nss, r.cur_wi, isb, spb,
-                                                          weight, d_color, c.nb == 1);
-                if (!is_zero(er.dp)) {
-                    if (c.flags & DR_QUIRK_DP_LAST) dps[b] = er.dp;
-                    else dps[b] += er.dp;
-                    any_dp = true;
-                }
-                if (!is_zero(er.dv0) || !is_zero(er.dv1)) {
-                    const DrShape &es = S.shapes[edge.shape];
-                    G.add3(es.vgrad_off + 3ll * edge.v0, er.dv0);
-                    G.add3(es.vgrad_off + 3ll * edge.v1, er.dv1);
-                }
-            };
-            sample_shading_edges(S, r.cur_wi, isb, spb, cone, prng, emit);
-            last_count = j;
-            if (S.stats) atomicAdd(&S.stats[3], (unsigned long long)j);
-        }
-    }
-    dx_count += last_count / 1000.f;
-    if (any_dp) {
-        G.color_mode = false;
-        reverse_path<MAXB>(S, G, ls, org, dir, nd, is, sp, R, nused, chain, F3{0.f, 0.f, 0.f}, dps,
-                           -1);
-    }
-    return color;
+    return true;
 }
 
+// A secondary shading point (reference SecondaryPoint, pathtrace.h:170-176):
+// recorded for every bounce reached without a diffuse sample; the edge
+// stages read it.  valid = 0 marks an empty slot.
+struct PointRec {
+    SPoint sp;
+    F3 wi, thr;
+    int shape, tri, gtri, valid;
+};
+
+// Stage A: forward path + interior adjoint of one pixel sample; fills the
+// MAXB point slots of the sample.  Returns the colour / spp.
+template <int MAXB>
+DR_INLINE F3 stage_path(const DrSceneView &S, const SampleCtx &c, GradSink &G, int xi, int yi, int s, F3 d_color,
+                        PointRec *pts) {
+    PathRec<MAXB> P;
+    for (int b = 0; b < MAXB; b++) pts[b].valid = 0;
+    if (!trace_path<MAXB>(S, c, xi, yi, s, P)) return F3{0.f, 0.f, 0.f};
+    float sw = 1.f / c.spp;
+    if (!(c.flags & DR_SKIP_INTERIOR)) {
+        G.color_mode = true;
+        reverse_path<MAXB>(S, G, P.ls, P.org, P.dir, P.nd, P.is, P.sp, P.R, P.nused, P.chain, d_color * sw, nullptr,
+                           P.primary_light);
+    }
+    if (!(c.flags & DR_SKIP_SECONDARY_EDGES)) {
+        for (int b = 0; b < P.nused; b++) {
+            if (!P.chain[b]) continue;
+            PointRec &q = pts[b];
+            q.sp = P.sp[b];
+            q.wi = P.R[b].cur_wi;
+            q.thr = P.R[b].thr;
+            q.shape = P.is[b].shape;
+            q.tri = P.is[b].tri;
+            q.gtri = P.is[b].gtri;
+            q.valid = 1;
+        }
+    }
+    return P.color * sw;
+}
+
+// Edge-tree cone toward the light chosen by the primary light sample
+// (reference delta_ray.cpp:876-890)
+DR_INLINE Cone light_cone(const DrSceneView &S, float light_sel, F3 p) {
+    int lid = sample_discrete(S.lights_cdf, S.num_lights, light_sel);
+    const DrLight &L = S.lights[lid];
+    F3 dl = F3{L.sphere_c[0], L.sphere_c[1], L.sphere_c[2]} - p;
+    float tan_a = L.sphere_r / length(dl);
+    float cos_a = tan_a <= 1.f ? 1.f / sqrtf(tan_a * tan_a + 1.f) : 0.f;
+    return Cone{p, normalize(dl), cos_a};
+}
+
+DR_INLINE uint64_t sample_id(const SampleCtx &c, int xi, int yi, int s) {
+    return ((uint64_t)yi * c.W + xi) * (uint64_t)c.spp + s;
+}
+
+// Stage B: edge selection at one secondary point (edge_tree.cpp:685-725);
+// emit(tree, edge_index, weight, ordinal) per selected edge.  Returns the
+// number of selected edges.
+template <class Emit>
+DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, uint64_t sid, int b, const PointRec &p,
+                           Emit &emit) {
+    Philox prng;
+    philox_init(prng, c.seed, 0, sid, dr_stream_point(b));
+    philox_next(prng);  // bsdf component of the point's edge sample
+    philox_next(prng);  // position on the edge
+    // the primary light sample of this pixel sample: its light_sel is the
+    // third draw after the jitter of the main stream
+    Philox mr;
+    philox_init(mr, c.seed, 0, sid, DR_STREAM_MAIN);
+    philox_next(mr);
+    philox_next(mr);
+    LightSample ls = draw_light_sample(mr);
+    Cone cone = light_cone(S, ls.light_sel, p.sp.pos);
+    Isect is = Isect{p.shape, p.tri, p.gtri};
+    int j = 0;
+    auto em = [&](const DrEdge &e, float w, bool nrm, int idx) {
+        emit(nrm, idx, w, j);
+        j++;
+    };
+    sample_shading_edges_idx(S, p.wi, is, p.sp, cone, prng, em);
+    return j;
+}
+
+// Stage C: one selected edge (reference delta_ray.cpp:904-966).  Scatters
+// the vertex derivatives of the edge and returns dp (the Dirac derivative at
+// the shading point).
+template <int MAXB>
+DR_INLINE F3 stage_edge(const DrSceneView &S, const SampleCtx &c, GradSink &G, uint64_t sid, int b,
+                        const PointRec &p, const DrEdge &edge, float w, int j, F3 d_color) {
+    Philox prng;
+    philox_init(prng, c.seed, 0, sid, dr_stream_point(b));
+    float bcomp = philox_next(prng), eu = philox_next(prng);
+    Philox erng;
+    philox_init(erng, c.seed, 0, sid, dr_stream_edge(b, j));
+    LightSample els = draw_light_sample(erng);
+    SecondarySample ess[MAXB];
+    int nss = c.nb - b - 1;
+    for (int i = 0; i < nss; i++) ess[i] = draw_secondary(erng);
+    F3 weight = p.thr * ((1.f / c.spp) * w);
+    Isect is = Isect{p.shape, p.tri, p.gtri};
+    EdgeResult er = sample_shading_edge_point(S, bcomp, eu, edge, els, ess, nss, p.wi, is, p.sp, weight, d_color,
+                                              c.nb == 1);
+    if (!is_zero(er.dv0) || !is_zero(er.dv1)) {
+        const DrShape &es = S.shapes[edge.shape];
+        G.add3(es.vgrad_off + 3ll * edge.v0, er.dv0);
+        G.add3(es.vgrad_off + 3ll * edge.v1, er.dv1);
+    }
+    return er.dp;
+}
+
+// Stage D: propagate the Dirac position derivatives of a pixel sample's
+// secondary points to vertices and camera (reference delta_ray.cpp:968-1002).
+// A bounce-0 point depends only on the primary ray, rebuilt from the
+// sample's draws; deeper (specular-chain) points replay the path.
+template <int MAXB>
+DR_INLINE void stage_dp(const DrSceneView &S, const SampleCtx &c, GradSink &G, int xi, int yi, int s,
+                        const PointRec *pts, const F3 *dps) {
+    bool deep = false, any = false;
+    for (int b = 0; b < MAXB; b++) {
+        if (!is_zero(dps[b])) {
+            any = true;
+            if (b > 0) deep = true;
+        }
+    }
+    if (!any) return;
+    G.color_mode = false;
+    if (!deep) {
+        Philox rng;
+        F2 scr;
+        LightSample ls;
+        pixel_sample_draws(c, xi, yi, s, rng, scr, ls);
+        F3 org, dir, nd;
+        sample_primary(S.cam, scr, org, dir, &nd);
+        AdjSP a = adj_zero();
+        a.pos = dps[0];
+        primary_adjoint(S, G, Isect{pts[0].shape, pts[0].tri, pts[0].gtri}, org, dir, nd, a, F3{0.f, 0.f, 0.f});
+        return;
+    }
+    PathRec<MAXB> P;
+    if (!trace_path<MAXB>(S, c, xi, yi, s, P)) return;
+    reverse_path<MAXB>(S, G, P.ls, P.org, P.dir, P.nd, P.is, P.sp, P.R, P.nused, P.chain, F3{0.f, 0.f, 0.f}, dps,
+                       -1);
+}

@@ -51,16 +51,43 @@ __global__ void __launch_bounds__(DR_BLOCK) k_forward(DrSceneView S, SampleCtx c
     }
 }
 
+// ---- wavefront adjoint: path (A) -> edge selection (B) -> edge samples (C)
+// -> Dirac position sweep (D).  A thread index t of A and D enumerates
+// (pixel, chunk of `per` samples); a point slot is ((t - t0) * per + k) * MAXB + b.
+struct EdgeEntry {
+    int slot;
+    int edge;  // index into the tree's edge list; bit 31 set: normal tree
+    float w;
+    int j;     // ordinal of the edge in the point's selection
+};
+
+struct SlotDecode {
+    int pix, xi, yi, s, b;
+    uint64_t sid;
+};
+DR_INLINE SlotDecode decode_slot(const SampleCtx &c, int per, int chunks, long long t0, long long slot, int maxb) {
+    SlotDecode d;
+    d.b = (int)(slot % maxb);
+    long long rest = slot / maxb;
+    int k = (int)(rest % per);
+    long long t = t0 + rest / per;
+    d.pix = (int)(t / chunks);
+    d.s = (int)(t % chunks) * per + k;
+    d.xi = d.pix % c.W;
+    d.yi = d.pix / c.W;
+    d.sid = sample_id(c, d.xi, d.yi, d.s);
+    return d;
+}
+
 template <int MAXB>
-__global__ void __launch_bounds__(DR_BLOCK) k_adjoint(DrSceneView S, SampleCtx c, int per, int chunks,
-                                                      const float *d_image, float *image, float *grads,
-                                                      float *dx_image) {
+__global__ void __launch_bounds__(DR_BLOCK) k_path(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
+                                                   long long t1, const float *d_image, float *image, float *grads,
+                                                   PointRec *pts) {
     __shared__ float s_cam[64];
     if (threadIdx.x < 64) s_cam[threadIdx.x] = 0.f;
     __syncthreads();
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long total = (long long)c.W * c.H * chunks;
-    if (t < total) {
+    long long t = t0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < t1) {
         int pix = (int)(t / chunks), ch = (int)(t % chunks);
         int xi = pix % c.W, yi = pix / c.W;
         F3 d_color = load3(d_image + 3 * (size_t)pix);
@@ -70,27 +97,157 @@ __global__ void __launch_bounds__(DR_BLOCK) k_adjoint(DrSceneView S, SampleCtx c
         G.flags = c.flags;
         G.color_mode = true;
         F3 acc = F3{0.f, 0.f, 0.f};
-        float dx = 0.f;
-        int s0 = ch * per, s1 = min(s0 + per, c.spp);
         bool zero = is_zero(d_color);
-        for (int s = s0; s < s1; s++) {
-            if (zero) acc += forward_sample(S, c, xi, yi, s);
-            else acc += adjoint_sample<MAXB>(S, c, G, xi, yi, s, d_color, dx);
+        for (int k = 0; k < per; k++) {
+            int s = ch * per + k;
+            PointRec *P = pts + ((t - t0) * per + k) * MAXB;
+            if (s >= c.spp || zero) {
+                for (int b = 0; b < MAXB; b++) P[b].valid = 0;
+                if (s < c.spp) acc += forward_sample(S, c, xi, yi, s);
+                continue;
+            }
+            acc += stage_path<MAXB>(S, c, G, xi, yi, s, d_color, P);
         }
         float *px = image + 3 * (size_t)pix;
         if (chunks == 1) {
             px[0] = acc.x; px[1] = acc.y; px[2] = acc.z;
-            if (dx_image) { float *q = dx_image + 3 * (size_t)pix; q[0] = q[1] = q[2] = dx; }
         } else {
             atomicAdd(px, acc.x); atomicAdd(px + 1, acc.y); atomicAdd(px + 2, acc.z);
-            if (dx_image) {
-                float *q = dx_image + 3 * (size_t)pix;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && s_cam[threadIdx.x] != 0.f) atomicAdd(grads + threadIdx.x, s_cam[threadIdx.x]);
+}
+
+// B: one thread per point slot; selected edges are appended to the queue in
+// groups of 8 per thread (one queue atomic per group)
+__global__ void __launch_bounds__(DR_BLOCK) k_select(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
+                                                     long long nslots, int maxb, const PointRec *pts, EdgeEntry *q,
+                                                     unsigned long long *counter, long long cap, int *counts) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    const PointRec &p = pts[i];
+    if (!p.valid) {
+        counts[i] = 0;
+        return;
+    }
+    SlotDecode d = decode_slot(c, per, chunks, t0, i, maxb);
+    EdgeEntry buf[8];
+    int nbuf = 0;
+    auto flush = [&]() {
+        unsigned long long base = atomicAdd(counter, (unsigned long long)nbuf);
+        for (int e = 0; e < nbuf; e++)
+            if ((long long)(base + e) < cap) q[base + e] = buf[e];
+        nbuf = 0;
+    };
+    auto emit = [&](bool nrm, int idx, float w, int j) {
+        // zero-weight edges carry no work (reference delta_ray.cpp:906-908)
+        if (w == 0.f) return;
+        buf[nbuf++] = EdgeEntry{(int)i, idx | (nrm ? (int)0x80000000 : 0), w, j};
+        if (nbuf == 8) flush();
+    };
+    int n = stage_select(S, c, d.sid, d.b, p, emit);
+    if (nbuf > 0) flush();
+    counts[i] = n;
+    if (S.stats) atomicAdd(&S.stats[3], (unsigned long long)n);
+}
+
+// C: one thread per selected edge
+template <int MAXB>
+__global__ void __launch_bounds__(DR_BLOCK) k_edge(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
+                                                   const EdgeEntry *q, long long n, const PointRec *pts,
+                                                   const float *d_image, float *grads, float *pt_dp, int *pt_lastj,
+                                                   float4 *entry_dp) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    EdgeEntry E = q[e];
+    SlotDecode d = decode_slot(c, per, chunks, t0, E.slot, MAXB);
+    const PointRec &p = pts[E.slot];
+    int idx = E.edge & 0x7fffffff;
+    const DrEdge &edge = (E.edge & 0x80000000) ? S.nrm_edges[idx] : S.sil_edges[idx];
+    GradSink G;
+    G.grads = grads;
+    G.cam = nullptr;
+    G.flags = c.flags;
+    G.color_mode = true;
+    F3 d_color = load3(d_image + 3 * (size_t)d.pix);
+    F3 dp = stage_edge<MAXB>(S, c, G, d.sid, d.b, p, edge, E.w, E.j, d_color);
+    bool nz = !is_zero(dp);
+    if (c.flags & DR_QUIRK_DP_LAST) {
+        entry_dp[e] = make_float4(dp.x, dp.y, dp.z, nz ? __int_as_float(E.j) : __int_as_float(-1));
+        if (nz) atomicMax(pt_lastj + E.slot, E.j);
+    } else if (nz) {
+        atomicAdd(pt_dp + 3ll * E.slot, dp.x);
+        atomicAdd(pt_dp + 3ll * E.slot + 1, dp.y);
+        atomicAdd(pt_dp + 3ll * E.slot + 2, dp.z);
+    }
+}
+
+// reference set_adjoint overwrite: of several dp at one point the last
+// (largest ordinal) is the one propagated
+__global__ void k_dp_last(const EdgeEntry *q, long long n, const float4 *entry_dp, const int *pt_lastj,
+                          float *pt_dp) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float4 v = entry_dp[e];
+    int j = __float_as_int(v.w);
+    int slot = q[e].slot;
+    if (j >= 0 && j == pt_lastj[slot]) {
+        pt_dp[3ll * slot] = v.x;
+        pt_dp[3ll * slot + 1] = v.y;
+        pt_dp[3ll * slot + 2] = v.z;
+    }
+}
+
+// D: Dirac position sweep per pixel sample, and the dx_image edge counts
+template <int MAXB>
+__global__ void __launch_bounds__(DR_BLOCK) k_dp(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
+                                                 long long t1, const PointRec *pts, const float *pt_dp,
+                                                 const int *counts, float *grads, float *dx_image) {
+    __shared__ float s_cam[64];
+    if (threadIdx.x < 64) s_cam[threadIdx.x] = 0.f;
+    __syncthreads();
+    long long t = t0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < t1) {
+        int pix = (int)(t / chunks), ch = (int)(t % chunks);
+        int xi = pix % c.W, yi = pix / c.W;
+        GradSink G;
+        G.grads = grads;
+        G.cam = s_cam;
+        G.flags = c.flags;
+        G.color_mode = false;
+        float dx = 0.f;
+        for (int k = 0; k < per; k++) {
+            int s = ch * per + k;
+            if (s >= c.spp) break;
+            long long base = ((t - t0) * per + k) * MAXB;
+            const PointRec *P = pts + base;
+            F3 dps[MAXB];
+            int last = -1;
+            for (int b = 0; b < MAXB; b++) {
+                dps[b] = load3(pt_dp + 3 * (base + b));
+                if (P[b].valid) last = b;
+            }
+            if (last >= 0) dx += counts[base + last] / 1000.f;
+            stage_dp<MAXB>(S, c, G, xi, yi, s, P, dps);
+        }
+        if (dx_image) {
+            float *q = dx_image + 3 * (size_t)pix;
+            if (chunks == 1) {
+                q[0] = q[1] = q[2] = dx;
+            } else {
                 atomicAdd(q, dx); atomicAdd(q + 1, dx); atomicAdd(q + 2, dx);
             }
         }
     }
     __syncthreads();
     if (threadIdx.x < 64 && s_cam[threadIdx.x] != 0.f) atomicAdd(grads + threadIdx.x, s_cam[threadIdx.x]);
+}
+
+__global__ void k_fill_i(int *p, size_t n, int v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
 }
 
 template <int MAXB>
@@ -164,7 +321,21 @@ static dr_status fail(dr_status s, const std::string &msg) {
 }
 
 // kernel timing: event pairs recorded around launches when enabled
-enum { TK_FORWARD = 0, TK_ADJOINT = 1, TK_PRIMARY = 2 };
+enum { TK_FORWARD = 0, TK_PATH = 1, TK_PRIMARY = 2, TK_SELECT = 3, TK_EDGE = 4, TK_DP = 5, TK_COUNT = 6 };
+
+// scratch of the wavefront adjoint, grown on demand, owned by the scene
+struct AdjBuffers {
+    PointRec *pts = nullptr;
+    float *pt_dp = nullptr;
+    int *pt_lastj = nullptr;
+    int *counts = nullptr;
+    long long slot_cap = 0;
+    EdgeEntry *q = nullptr;
+    float4 *entry_dp = nullptr;
+    long long q_cap = 0;
+    unsigned long long *counter = nullptr;
+    unsigned long long *counter_host = nullptr;
+};
 struct TimedLaunch {
     int kind;
     hipEvent_t a, b;
@@ -179,6 +350,7 @@ struct dr_scene {
     bool timing_on = false;
     std::vector<TimedLaunch> timed;
     std::vector<hipEvent_t> event_pool;
+    AdjBuffers adj;
     double build_ms = 0.0;
     void *prim_bufs[3] = {nullptr, nullptr, nullptr};
 };
@@ -253,7 +425,16 @@ struct LaunchTimer {
     }
 };
 
+static void free_adj(AdjBuffers &a) {
+    void *ps[] = {a.pts, a.pt_dp, a.pt_lastj, a.counts, a.q, a.entry_dp, a.counter};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (a.counter_host) (void)hipHostFree(a.counter_host);
+    a = AdjBuffers();
+}
+
 static void free_scene(dr_scene *s) {
+    free_adj(s->adj);
     for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : s->event_pool) (void)hipEventDestroy(e);
     for (void *p : s->allocs) (void)hipFree(p);
@@ -283,9 +464,47 @@ static SampleCtx make_ctx(const dr_render_options *o) {
     return c;
 }
 
+static bool grow(void **p, size_t bytes) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    return hipMalloc(p, bytes) == hipSuccess;
+}
+
+static dr_status ensure_slots(AdjBuffers &a, long long n) {
+    if (n <= a.slot_cap) return DR_OK;
+    if (!grow((void **)&a.pts, n * sizeof(PointRec)) || !grow((void **)&a.pt_dp, n * 3 * sizeof(float)) ||
+        !grow((void **)&a.pt_lastj, n * sizeof(int)) || !grow((void **)&a.counts, n * sizeof(int))) {
+        a.slot_cap = 0;
+        return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint point buffers");
+    }
+    a.slot_cap = n;
+    return DR_OK;
+}
+
+static dr_status ensure_queue(AdjBuffers &a, long long n) {
+    if (!a.counter) {
+        if (hipMalloc((void **)&a.counter, sizeof(unsigned long long)) != hipSuccess ||
+            hipHostMalloc((void **)&a.counter_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+            return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint queue counter");
+    }
+    if (n <= a.q_cap) return DR_OK;
+    if (!grow((void **)&a.q, n * sizeof(EdgeEntry)) || !grow((void **)&a.entry_dp, n * sizeof(float4))) {
+        a.q_cap = 0;
+        return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint edge queue");
+    }
+    a.q_cap = n;
+    return DR_OK;
+}
+
+static unsigned grid_of(long long n) { return (unsigned)((n + DR_BLOCK - 1) / DR_BLOCK); }
+
+// samples per batch of the wavefront adjoint (bounds the point and edge
+// buffers; one host read of the edge count per batch)
+#define DR_BATCH_SAMPLES (1ll << 22)
+
 template <int MAXB>
-static void launch_adjoint(dr_scene *s, const dr_render_options *o, const float *d_image, float *image, float *grads,
-                           float *dx, hipStream_t hs) {
+static dr_status launch_adjoint(dr_scene *s, const dr_render_options *o, const float *d_image, float *image,
+                                float *grads, float *dx, hipStream_t hs) {
     int per, chunks;
     plan_chunks(o, per, chunks);
     size_t n = 3ull * o->width * o->height;
@@ -294,21 +513,65 @@ static void launch_adjoint(dr_scene *s, const dr_render_options *o, const float 
         if (dx) k_fill<<<1024, 256, 0, hs>>>(dx, n, 0.f);
     }
     long long total = (long long)o->width * o->height * chunks;
+    long long bt = std::max(1ll, std::min(total, DR_BATCH_SAMPLES / per));
     SampleCtx c = make_ctx(o);
-    {
-        LaunchTimer timer(s, TK_ADJOINT, hs);
-        k_adjoint<MAXB><<<(unsigned)((total + DR_BLOCK - 1) / DR_BLOCK), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks,
-                                                                                       d_image, image, grads, dx);
+    AdjBuffers &A = s->adj;
+    dr_status st = ensure_slots(A, bt * per * MAXB);
+    if (st != DR_OK) return st;
+    st = ensure_queue(A, std::max(A.q_cap, bt * per * 4));
+    if (st != DR_OK) return st;
+    const bool edges = !(o->flags & DR_SKIP_SECONDARY_EDGES);
+    for (long long t0 = 0; t0 < total; t0 += bt) {
+        long long t1 = std::min(total, t0 + bt);
+        long long nslots = (t1 - t0) * per * MAXB;
+        {
+            LaunchTimer timer(s, TK_PATH, hs);
+            k_path<MAXB><<<grid_of(t1 - t0), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, t1, d_image, image,
+                                                                 grads, A.pts);
+        }
+        if (!edges) continue;
+        k_fill<<<1024, 256, 0, hs>>>(A.pt_dp, (size_t)nslots * 3, 0.f);
+        k_fill_i<<<1024, 256, 0, hs>>>(A.pt_lastj, (size_t)nslots, -1);
+        long long nedges = 0;
+        for (int attempt = 0; attempt < 2; attempt++) {
+            k_fill<<<1, 64, 0, hs>>>((float *)A.counter, 2, 0.f);  // 8-byte counter = two zero floats
+            {
+                LaunchTimer timer(s, TK_SELECT, hs);
+                k_select<<<grid_of(nslots), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, nslots, MAXB, A.pts, A.q,
+                                                               A.counter, A.q_cap, A.counts);
+            }
+            if (hipMemcpyAsync(A.counter_host, A.counter, sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) !=
+                    hipSuccess ||
+                hipStreamSynchronize(hs) != hipSuccess)
+                return fail(DR_ERROR_DEVICE, "reading the edge queue size failed");
+            nedges = (long long)*A.counter_host;
+            if (nedges <= A.q_cap) break;
+            // the selection is deterministic: grow the queue and select again
+            st = ensure_queue(A, nedges + nedges / 4);
+            if (st != DR_OK) return st;
+        }
+        if (nedges > 0) {
+            LaunchTimer timer(s, TK_EDGE, hs);
+            k_edge<MAXB><<<grid_of(nedges), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, A.q, nedges, A.pts,
+                                                                d_image, grads, A.pt_dp, A.pt_lastj, A.entry_dp);
+        }
+        if (nedges > 0 && (o->flags & DR_QUIRK_DP_LAST))
+            k_dp_last<<<grid_of(nedges), DR_BLOCK, 0, hs>>>(A.q, nedges, A.entry_dp, A.pt_lastj, A.pt_dp);
+        {
+            LaunchTimer timer(s, TK_DP, hs);
+            k_dp<MAXB><<<grid_of(t1 - t0), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, t1, A.pts, A.pt_dp,
+                                                               A.counts, grads, dx);
+        }
     }
+    if (dx && !edges) k_fill<<<1024, 256, 0, hs>>>(dx, n, 0.f);
     if (!(o->flags & DR_SKIP_PRIMARY_EDGES)) {
         LaunchTimer timer(s, TK_PRIMARY, hs);
         long long N = o->num_primary_edge_samples > 0 ? o->num_primary_edge_samples
                                                       : (long long)o->num_samples * o->width * o->height;
-        k_primary<MAXB><<<(unsigned)((N + DR_BLOCK - 1) / DR_BLOCK), DR_BLOCK, 0, hs>>>(s->view, c, N, d_image,
-                                                                                       grads);
+        k_primary<MAXB><<<grid_of(N), DR_BLOCK, 0, hs>>>(s->view, c, N, d_image, grads);
     }
+    return DR_OK;
 }
-
 
 extern "C" {
 
@@ -418,9 +681,9 @@ dr_status dr_scene_stats(dr_scene *s, int enable, dr_stats *out) {
         for (auto &t : s->timed) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) ms = 0.f;
-            if (t.kind == TK_FORWARD) { out->forward_ms += ms; out->forward_launches++; }
-            if (t.kind == TK_ADJOINT) { out->adjoint_ms += ms; out->adjoint_launches++; }
-            if (t.kind == TK_PRIMARY) { out->primary_ms += ms; out->primary_launches++; }
+            if (t.kind < 0 || t.kind >= TK_COUNT) continue;
+            out->kernel_ms[t.kind] += ms;
+            out->kernel_launches[t.kind]++;
         }
     }
     for (auto &t : s->timed) {
@@ -476,11 +739,12 @@ dr_status dr_render_adjoint(dr_scene *s, const dr_render_options *o, const float
     hipStream_t hs = (hipStream_t)stream;
     k_fill<<<256, 256, 0, hs>>>(grads, s->hs.grad_size, 0.f);
     float *dx = (o->flags & DR_WRITE_DX_IMAGE) ? dx_image : nullptr;
-    if (o->max_bounces <= 1) launch_adjoint<1>(s, o, d_image, image, grads, dx, hs);
-    else if (o->max_bounces <= 2) launch_adjoint<2>(s, o, d_image, image, grads, dx, hs);
-    else if (o->max_bounces <= 4) launch_adjoint<4>(s, o, d_image, image, grads, dx, hs);
-    else launch_adjoint<8>(s, o, d_image, image, grads, dx, hs);
-    return launch_check("k_adjoint");
+    if (o->max_bounces <= 1) st = launch_adjoint<1>(s, o, d_image, image, grads, dx, hs);
+    else if (o->max_bounces <= 2) st = launch_adjoint<2>(s, o, d_image, image, grads, dx, hs);
+    else if (o->max_bounces <= 4) st = launch_adjoint<4>(s, o, d_image, image, grads, dx, hs);
+    else st = launch_adjoint<8>(s, o, d_image, image, grads, dx, hs);
+    if (st != DR_OK) return st;
+    return launch_check("adjoint kernels");
 }
 
 dr_status dr_intersect(dr_scene *s, const float *rays, int64_t n, int32_t *hits, int any_hit, void *stream) {

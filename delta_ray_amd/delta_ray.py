@@ -325,7 +325,12 @@ class DeviceScene:
         mask = _abi.DR_STATS_COUNTERS if enable is True else int(enable)
         _abi.check(self.lib, self.lib.dr_scene_stats(self.handle, mask, ctypes.byref(st)),
                    "dr_scene_stats")
-        return {k: getattr(st, k) for k, _ in _abi.dr_stats._fields_}
+        out = {k: getattr(st, k) for k in ("rays", "nodes_visited", "tris_visited", "secondary_edges",
+                                           "primary_edge_samples")}
+        for i, k in enumerate(_abi.KERNELS):
+            out[k + "_ms"] = st.kernel_ms[i]
+            out[k + "_launches"] = st.kernel_launches[i]
+        return out
 
 
 def _stream_ptr(stream):

@@ -229,6 +229,10 @@ dr_status dr_render(dr_scene *scene, const dr_render_options *opts,
                     float *image, void *stream);
 
 /* Forward + adjoint render (reference render() with d_image [H, W, 3]:
+ * The secondary-edge stage appends the selected edges of a batch of pixel
+ * samples to a queue whose length the host reads (one stream synchronisation
+ * per batch of 2^22 samples); a queue that overflows is grown and the
+ * deterministic selection is repeated.
  * pass 1 with render_sample, delta_ray.cpp:694-1005, then pass 2, the
  * primary-edge samples, delta_ray.cpp:1468-1550).  d_image: device [H,W,3].
  * image: device [H,W,3] (written).  grads: device, dr_scene_grad_size floats
@@ -255,14 +259,18 @@ dr_status dr_intersect(dr_scene *scene, const float *rays, int64_t n,
  * stream and reports the summed device time per kernel.  The counters and
  * times are cleared by every call. */
 enum { DR_STATS_COUNTERS = 1, DR_STATS_TIMING = 2 };
+/* kernel_ms / kernel_launches index: forward, path (radiance + interior
+ * adjoint), primary edges, edge selection, edge samples, Dirac sweep */
+enum { DR_K_FORWARD = 0, DR_K_PATH = 1, DR_K_PRIMARY = 2, DR_K_SELECT = 3, DR_K_EDGE = 4, DR_K_DP = 5,
+       DR_K_COUNT = 6 };
 typedef struct dr_stats {
     uint64_t rays;
     uint64_t nodes_visited;
     uint64_t tris_visited;
     uint64_t secondary_edges;
     uint64_t primary_edge_samples;
-    double forward_ms, adjoint_ms, primary_ms;   /* summed kernel times */
-    uint64_t forward_launches, adjoint_launches, primary_launches;
+    double kernel_ms[DR_K_COUNT];          /* summed kernel times */
+    uint64_t kernel_launches[DR_K_COUNT];
 } dr_stats;
 dr_status dr_scene_stats(dr_scene *scene, int enable, dr_stats *out);
 
