@@ -460,7 +460,7 @@ struct PrimaryResult {
 // delta_ray.cpp:377-557 (world Jacobian by 12-point finite differences, as there)
 DR_INLINE void sample_primary_edge_point(const DrSceneView &S, const DrEdge &edge, float eu, const LightSample &ls,
                                          const SecondarySample *ss, int nss, const float *d_image, int W, int H,
-                                         float weight, PrimaryResult &R) {
+                                         float weight, unsigned flags, PrimaryResult &R) {
     R.ok = false;
     const DrCamera &cam = S.cam;
     const DrShape &es = S.shapes[edge.shape];
@@ -533,6 +533,8 @@ DR_INLINE void sample_primary_edge_point(const DrSceneView &S, const DrEdge &edg
     R.dv0 = F3{g[0], g[1], g[2]};
     R.dv1 = F3{g[3], g[4], g[5]};
     for (int i = 0; i < 16; i++) R.dw2c[i] = 0.f;
-    d_project(cam, v0, v1, d0x, d0y, d1x, d1y, R.dw2c);
+    // the exact camera derivative carries project_local's 0.5
+    const float cs = (flags & DR_QUIRK_PROJECT_SCALE) ? 1.f : 0.5f;
+    d_project(cam, v0, v1, cs * d0x, cs * d0y, cs * d1x, cs * d1y, R.dw2c);
     R.ok = true;
 }

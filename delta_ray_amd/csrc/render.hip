@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_primary(DrSceneView S, SampleCtx c
             for (int b = 0; b < c.nb; b++) ss[b] = draw_secondary(rng);
             const DrEdge &e = S.prim_edges[id];
             PrimaryResult R;
-            sample_primary_edge_point(S, e, eu, ls, ss, c.nb, d_image, c.W, c.H, w / c.spp, R);
+            sample_primary_edge_point(S, e, eu, ls, ss, c.nb, d_image, c.W, c.H, w / c.spp, c.flags, R);
             if (R.ok) {
                 const DrShape &es = S.shapes[e.shape];
                 if (R.dv0.x != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0, R.dv0.x);

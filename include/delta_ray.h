@@ -127,6 +127,10 @@ enum {
      * last one is propagated (set_adjoint overwrites, autodiff.h:340-342,
      * delta_ray.cpp:977-982) */
     DR_QUIRK_DP_LAST = 1u << 3,
+    /* the world_to_cam gradient of primary edges is twice the derivative:
+     * d_project_local (camera.h:204-213) drops the 0.5 of project_local
+     * (camera.h:170-171) */
+    DR_QUIRK_PROJECT_SCALE = 1u << 4,
     /* terms (for tests): skip the interior adjoint / secondary edges /
      * primary edges */
     DR_SKIP_INTERIOR = 1u << 8,
@@ -137,7 +141,7 @@ enum {
     DR_WRITE_DX_IMAGE = 1u << 11
 };
 #define DR_FLAGS_PARITY (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP | \
-                         DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST)
+                         DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE)
 
 typedef struct dr_render_options {
     int width;

@@ -887,11 +887,25 @@ struct EdgeResult {
     V3f dp, dv0, dv1;
 };
 
+// Oracle-only knobs (render option flag bits the public header leaves
+// unused) that replace two approximations of the reference's secondary-edge
+// estimator by their exact forms, so that closed-form tests can pin the rest:
+// - OR_EXACT_DIRAC: divide by the gradient of the edge-plane equation
+//   tangential to the hit surface; the reference (delta_ray.cpp:340) divides
+//   by its full 3-D length |d0 x d1|, which scales each edge's expected
+//   contribution by sqrt(1 - (n_plane . n_hit)^2) <= 1;
+// - OR_FINE_OFFSET: cast the two side rays 1e-5 instead of 1e-3
+//   (delta_ray.cpp:191) radians off the edge, so that no light-side hit is
+//   lost near the corners of an occluded region.
+// Neither is part of the reference algorithm or of the device path.
+constexpr uint32_t OR_EXACT_DIRAC = 1u << 16;
+constexpr uint32_t OR_FINE_OFFSET = 1u << 17;
+
 // delta_ray.cpp:45-365
 EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, float eu, const Edge &edge,
                                      const LightSample &ls, const std::vector<SecondarySample> &ss, const V3f &wi,
                                      const Isect &is, const SP<float> &sp, const V3f &weight, const V3f &d_color,
-                                     bool direct_only) {
+                                     bool direct_only, uint32_t flags = 0) {
     EdgeResult R{v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f)};
     if (!is.valid()) return R;
     if (!is_silhouette(sc, sp.pos, edge)) return R;
@@ -956,7 +970,7 @@ EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, floa
     if (pdf(l) <= 0.f) return R;
     V3f sample_p = M * (vo + l * wt);
     V3f spn = normalize(cross(v0 - sp.pos, v1 - sp.pos));
-    float delta = 1e-3f * length(sample_p);
+    float delta = ((flags & OR_FINE_OFFSET) ? 1e-5f : 1e-3f) * length(sample_p);
     V3f sdir = normalize(sample_p);
     V3f rdir = normalize(sdir + delta * spn);
     V3f ldir = normalize(sdir - delta * spn);
@@ -1004,7 +1018,8 @@ EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, floa
         float line_j = length(((ij * nj) * M) * wt) / (m_pmf * pdf(l));
         V3f p = sp.pos, x = hp.pos;
         V3f d0 = v0 - p, d1 = v1 - p;
-        float dirac_j = length(cross(d0, d1));
+        V3f c01 = cross(d0, d1);
+        float dirac_j = (flags & OR_EXACT_DIRAC) ? length(c01 - hp.ng * dot(c01, hp.ng)) : length(c01);
         float w = line_j / dirac_j;
         V3f dp = w * (cross(d1, d0) + cross(x - p, d1) + cross(d0, x - p));
         V3f dv0 = w * cross(d1, x - p);
@@ -1032,7 +1047,7 @@ inline bool in_screen(const V2f &p) { return p[0] >= 0.f && p[0] < 1.f && p[1] >
 // delta_ray.cpp:377-557
 PrimaryResult sample_primary_edge_point(const Scene &sc, const Edge &edge, float eu, const LightSample &ls,
                                         const std::vector<SecondarySample> &ss, const float *d_image, int W, int H,
-                                        float weight) {
+                                        float weight, uint32_t flags) {
     PrimaryResult R;
     for (int i = 0; i < 16; i++) R.dw2c[i] = 0.f;
     R.dv0 = R.dv1 = v3(0.f, 0.f, 0.f);
@@ -1107,7 +1122,9 @@ PrimaryResult sample_primary_edge_point(const Scene &sc, const Edge &edge, float
     }
     R.dv0 = v3(g[0], g[1], g[2]);
     R.dv1 = v3(g[3], g[4], g[5]);
-    d_project(cam, v0, v1, d0x, d0y, d1x, d1y, R.dw2c);
+    // the exact camera derivative carries project_local's 0.5
+    float cs = (flags & DR_QUIRK_PROJECT_SCALE) ? 1.f : 0.5f;
+    d_project(cam, v0, v1, cs * d0x, cs * d0y, cs * d1x, cs * d1y, R.dw2c);
     R.sx = ep[0];
     R.sy = ep[1];
     R.ok = true;
@@ -1285,7 +1302,7 @@ V3f render_pixel_sample(const RenderCtx &rc, int xi, int yi, int s, Accum &acc, 
                 for (int i = 0; i < o.max_bounces - p.bounce - 1; i++) ess.push_back(draw_secondary(erng));
                 V3f weight = p.throughput * (sample_weight * ew.w);
                 EdgeResult er = sample_shading_edge_point(sc, bsdf_component, eu, ew.e, els, ess, p.wi, p.is, p.spf,
-                                                          weight, d_color, o.max_bounces == 1);
+                                                          weight, d_color, o.max_bounces == 1, o.flags);
                 if (!is_zero(er.dp)) dps.push_back(std::make_pair(p.ad_slot, er.dp));
                 if (!is_zero(er.dv0) || !is_zero(er.dv1)) {
                     size_t vo = lay.vertices[ew.e.shape];
@@ -1331,7 +1348,8 @@ void render_primary_edge(const RenderCtx &rc, int64_t k, Accum &acc) {
     std::vector<SecondarySample> ss;
     for (int i = 0; i < o.max_bounces; i++) ss.push_back(draw_secondary(rng));
     const Edge &e = sc.primary_edges[id];
-    PrimaryResult R = sample_primary_edge_point(sc, e, eu, ls, ss, rc.d_image, o.width, o.height, w / o.num_samples);
+    PrimaryResult R = sample_primary_edge_point(sc, e, eu, ls, ss, rc.d_image, o.width, o.height, w / o.num_samples,
+                                                  o.flags);
     if (!R.ok) return;
     if (R.sx < 0.f || R.sy < 0.f || R.sx > 1.f || R.sy > 1.f) return;
     const GradLayout &lay = rc.os->lay;
