@@ -43,7 +43,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="heightfield")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-spp", type=int, default=1, help="spp of the bounded CPU sample")
+    ap.add_argument("--cpu-spp", type=int, default=16, help="spp of the bounded CPU sample")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (a kernel of known HBM traffic: "
+                         "256 MiB read + 256 MiB written) to scale hardware byte counters")
     return ap.parse_args()
 
 
@@ -186,6 +189,13 @@ def main():
                      "secondary_edges_per_step": int(cb["secondary_edges"]),
                      "kernel_ms_per_step": {k: round(v, 3) for k, v in kms_step.items()},
                      "scene_build_s": round(build_s, 3), "scene": info}}
+    if args.calibrate:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["extra"]["calibration_copy_bytes"] = 2 * src.numel() * 4
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cspp = args.cpu_spp

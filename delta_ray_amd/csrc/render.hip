@@ -1,16 +1,22 @@
 // Kernels and C-ABI of the MI355X path tracer (include/delta_ray.h).
 //
-// Kernels (one thread = one pixel sample-chunk or one edge sample; 256-thread
-// workgroups = 4 wave64):
-//   k_forward   radiance only (reference render_sample_fast,
-//               delta_ray.cpp:640-692)
-//   k_adjoint   radiance + interior adjoint + secondary edges + Dirac
-//               position sweep (reference render_sample, delta_ray.cpp:694-1002)
+// Kernels (256-thread workgroups = 4 wave64):
+//   k_forward   radiance only, thread = (pixel, chunk of samples)
+//               (reference render_sample_fast, delta_ray.cpp:640-692)
+//   adjoint (reference render_sample, delta_ray.cpp:694-1002) as a wavefront
+//   pipeline over batches of pixel samples:
+//     k_path    camera path + radiance + interior adjoint; stores one
+//               PointRec per (sample, bounce) shading point
+//     k_select  edge-tree traversal per shading point -> queue of edges
+//     k_edge    thread = one selected edge: secondary edge sample, vertex
+//               gradients, Dirac position derivative dp of its point
+//     k_dp_last keeps the last dp per point (set_adjoint overwrite)
+//     k_dp      replays the path to propagate dp (2nd adjoint sweep)
 //   k_primary   primary-edge samples (reference pass 2, delta_ray.cpp:1468-1550)
 //   k_intersect ray queries for the parity tests
-// Gradients: camera matrices are reduced in LDS per workgroup (64 floats),
-// everything else goes through the workgroup's LDS hash (GradCache) so that
-// a vertex touched by many lanes of a workgroup reaches HBM as one atomic.
+// Gradients: camera matrices are reduced in LDS per workgroup (64 floats);
+// the other parameters are scattered with float atomics (zero adjoints are
+// skipped).
 #include "../../include/delta_ray.h"
 #include "dr_sample.h"
 #include "scene_build.h"

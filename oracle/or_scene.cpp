@@ -383,6 +383,8 @@ static bool clip_line(V2f &a, V2f &b) {
     }
 }
 
+bool probe_clip_line(V2f &a, V2f &b) { return clip_line(a, b); }
+
 bool project_edge(const Camera &cam, V3f p0, V3f p1, V2f &s0, V2f &s1);
 bool is_silhouette(const Scene &sc, const V3f &p, const Edge &e);
 

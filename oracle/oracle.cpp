@@ -1364,6 +1364,10 @@ void render_primary_edge(const RenderCtx &rc, int64_t k, Accum &acc) {
 
 }  // namespace oracle
 
+namespace oracle {
+bool probe_clip_line(V2f &a, V2f &b);
+}
+
 // ============================================================== C-ABI ====
 using namespace oracle;
 
@@ -1504,6 +1508,140 @@ int or_render(void *s, const dr_render_options *o, const float *d_image, float *
         }
     }
     return 0;
+}
+
+// ------------------------------------------------------------- probes ----
+// Single functions of the restatement, for comparison with the reference's
+// own headers (oracle/ref_probe.cpp) and with closed forms.
+// sp[17] = pos[3], geom_normal[3], frame rows s, t, n [9], uv[2].
+static SP<float> probe_sp(const float *sp) {
+    SP<float> p;
+    p.pos = v3(sp[0], sp[1], sp[2]);
+    p.ng = v3(sp[3], sp[4], sp[5]);
+    for (int i = 0; i < 3; i++) p.fr[i] = v3(sp[6 + 3 * i], sp[7 + 3 * i], sp[8 + 3 * i]);
+    p.uv = v2(sp[15], sp[16]);
+    return p;
+}
+
+// out[0..2] bsdf, out[3] pdf
+void or_probe_bsdf(void *s, int mid, const float *sp, const float *wi, const float *wo, float *out) {
+    const Scene &sc = ((OScene *)s)->sc;
+    SP<float> p = probe_sp(sp);
+    V3f a = v3(wi[0], wi[1], wi[2]), b = v3(wo[0], wo[1], wo[2]);
+    V3f f = bsdf<float>(sc, mid, p, a, b, nullptr);
+    for (int i = 0; i < 3; i++) out[i] = f[i];
+    out[3] = bsdf_pdf<float>(sc, mid, p, a, b, nullptr);
+}
+
+// bs = (u0, u1, w); out[0..2] direction, out[3] is_specular
+void or_probe_sample_bsdf(void *s, int mid, const float *sp, const float *wi, const float *bs, float *out) {
+    const Scene &sc = ((OScene *)s)->sc;
+    BSDFSample b{{bs[0], bs[1]}, bs[2]};
+    bool spec = false;
+    V3f d = sample_bsdf_dir<float>(sc, mid, probe_sp(sp), v3(wi[0], wi[1], wi[2]), b, nullptr, &spec);
+    for (int i = 0; i < 3; i++) out[i] = d[i];
+    out[3] = spec ? 1.f : 0.f;
+}
+
+void or_probe_ltc(void *s, int mid, const float *sp, const float *wi, float *out) {
+    const Scene &sc = ((OScene *)s)->sc;
+    M3 M = get_ltc_matrix(sc, mid, probe_sp(sp), v3(wi[0], wi[1], wi[2]));
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[i * 3 + j] = M.m[i][j];
+}
+
+// camera of the scene: out = org[3], dir[3]
+void or_probe_sample_primary(void *s, const float *screen, float *out) {
+    const Camera &c = ((OScene *)s)->sc.cam;
+    V3f org, dir;
+    sample_primary(c.c2w, c.fov, c.aspect, v2(screen[0], screen[1]), org, dir);
+    for (int i = 0; i < 3; i++) {
+        out[i] = org[i];
+        out[3 + i] = dir[i];
+    }
+}
+
+// returns 0 when the segment is behind the near plane; out = s0[2], s1[2]
+int or_probe_project(void *s, const float *p0, const float *p1, float *out) {
+    const Camera &c = ((OScene *)s)->sc.cam;
+    V2f a, b;
+    if (!project_edge(c, v3(p0[0], p0[1], p0[2]), v3(p1[0], p1[1], p1[2]), a, b)) return 0;
+    out[0] = a[0]; out[1] = a[1]; out[2] = b[0]; out[3] = b[1];
+    return 1;
+}
+
+// dxy = d s0.x, d s0.y, d s1.x, d s1.y; out = d world_to_cam [16] (accumulated
+// from zero, as the reference's fresh Matrix4x4f)
+void or_probe_d_project(void *s, const float *p0, const float *p1, const float *dxy, float *out) {
+    const Camera &c = ((OScene *)s)->sc.cam;
+    for (int i = 0; i < 16; i++) out[i] = 0.f;
+    d_project(c, v3(p0[0], p0[1], p0[2]), v3(p1[0], p1[1], p1[2]), dxy[0], dxy[1], dxy[2], dxy[3], out);
+}
+
+int or_probe_cone_box(const float *lo, const float *hi, const float *org, const float *dir, float cos_angle) {
+    AABB box(v3(lo[0], lo[1], lo[2]), v3(hi[0], hi[1], hi[2]));
+    Cone cone{v3(org[0], org[1], org[2]), v3(dir[0], dir[1], dir[2]), cos_angle};
+    return aabb_cone_intersect(box, cone) ? 1 : 0;
+}
+
+int or_probe_clip_line(const float *a, const float *b, float *out) {
+    V2f x = v2(a[0], a[1]), y = v2(b[0], b[1]);
+    if (!probe_clip_line(x, y)) return 0;
+    out[0] = x[0]; out[1] = x[1]; out[2] = y[0]; out[3] = y[1];
+    return 1;
+}
+
+// out[0..8] normalize_jacobian(x), out[9..17] intersect_jacobian(dir, p, n)
+void or_probe_jacobians(const float *x, const float *dir, const float *p, const float *n, float *out) {
+    M3 a = normalize_jacobian(v3(x[0], x[1], x[2]));
+    M3 b = intersect_jacobian(v3(dir[0], dir[1], dir[2]), v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]));
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            out[i * 3 + j] = a.m[i][j];
+            out[9 + i * 3 + j] = b.m[i][j];
+        }
+}
+
+// Moller-Trumbore values (u, v, t) of the oracle's hit test
+void or_probe_mt(const float *v0, const float *v1, const float *v2_, const float *org, const float *dir,
+                 float *out) {
+    float u, v, t;
+    mt_uvt(v3(v0[0], v0[1], v0[2]), v3(v1[0], v1[1], v1[2]), v3(v2_[0], v2_[1], v2_[2]),
+           v3(org[0], org[1], org[2]), v3(dir[0], dir[1], dir[2]), u, v, t);
+    out[0] = u; out[1] = v; out[2] = t;
+}
+
+int or_probe_sample_discrete(const float *f, int n, float u, float *pmf) {
+    Dist1D d(f, n);
+    return d.sample_discrete(u, pmf);
+}
+
+// point on a triangle from (u0, u1) as the light sampler draws it: out =
+// position[3], normal[3]
+void or_probe_sample_triangle(const float *v0, const float *v1, const float *v2_, const float *u, float *out) {
+    float a = std::sqrt(u[0]);
+    float b1 = 1.f - a, b2 = a * u[1];
+    V3f q0 = v3(v0[0], v0[1], v0[2]), q1 = v3(v1[0], v1[1], v1[2]), q2 = v3(v2_[0], v2_[1], v2_[2]);
+    V3f e1 = q1 - q0, e2 = q2 - q0;
+    V3f n = normalize(cross(e1, e2));
+    V3f p = q0 + e1 * b1 + e2 * b2;
+    for (int i = 0; i < 3; i++) {
+        out[i] = p[i];
+        out[3 + i] = n[i];
+    }
+}
+
+// Raw Philox4x32-10 block: out = philox(key, ctr), for known-answer tests.
+void or_philox4x32(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]) {
+    Philox p(0, 0, 0, 0);
+    p.k0 = key[0];
+    p.k1 = key[1];
+    p.c0 = ctr[0];
+    p.c1 = ctr[1];
+    p.c2 = ctr[2];
+    p.c3 = ctr[3];
+    p.refill();
+    for (int i = 0; i < 4; i++) out[i] = p.buf[i];
 }
 
 // Ray queries with the oracle's hit rule.  rays [n, 8]: org, tmin, dir, tmax.
