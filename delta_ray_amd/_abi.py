@@ -119,6 +119,8 @@ ENTRY_POINTS = [
                                     ctypes.c_int, ctypes.c_void_p]),
     ("dr_scene_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(dr_stats)]),
     ("dr_scene_get_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_scene_info)]),
+    ("dr_test_line_integral", ctypes.c_int, [ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.c_void_p]),
     ("dr_last_error", ctypes.c_char_p, []),
 ]
 

@@ -312,6 +312,52 @@ struct EdgeResult {
 // delta_ray.cpp:45-365: one point on one silhouette edge, seen from a
 // shading point; LTC line sampling, two rays across the edge, Dirac
 // Jacobian.  ss / nss: bounces after the edge hit (empty when direct only).
+// A point on the segment a-b (above z = 0) drawn with density proportional to
+// the clamped cosine to +z over the solid angle (the line integral of a
+// cosine lobe): closed-form CDF I(l), inverted by bisection + Newton, at most
+// 20 iterations, |CDF - u| < 1e-5 (delta_ray.cpp:108-180).  The point is
+// vo + l wt; pdf is the density in l.
+struct LineSample {
+    F3 vo, wt;
+    float l, pdf;
+};
+DR_INLINE bool ltc_line_sample(F3 a, F3 b, float eu, LineSample &L) {
+    F3 vd = b - a;
+    if (abs_sum(vd) < 1e-10f) return false;
+    F3 wt = normalize(vd);
+    float l0 = dot(a, wt), l1 = dot(b, wt);
+    F3 vo = a - l0 * wt;
+    float d = length(vo);
+    auto I = [&](float l) {
+        return (l / (d * (d * d + l * l)) + atanf(l / d) / (d * d)) * vo.z + (l * l / (d * (d * d + l * l))) * wt.z;
+    };
+    float Il0 = I(l0), Il1 = I(l1);
+    float norm = Il1 - Il0;
+    if (norm <= 1e-10f) return false;
+    auto pdf = [&](float l) {
+        float dsq = d * d + l * l;
+        return 2.f * d * (vo + l * wt).z / (norm * dsq * dsq);
+    };
+    float lb = l0, ub = l1;
+    if (lb > ub) { float t = lb; lb = ub; ub = t; }
+    float l = 0.5f * (lb + ub);
+    for (int it = 0; it < 20; it++) {
+        if (!(l >= lb && l <= ub)) l = 0.5f * (lb + ub);
+        float value = (I(l) - Il0) / norm - eu;
+        if (fabsf(value) < 1e-5f || it == 19) break;
+        if (value > 0.f) ub = l;
+        else lb = l;
+        l -= value / pdf(l);
+    }
+    float pl = pdf(l);
+    if (pl <= 0.f) return false;
+    L.vo = vo;
+    L.wt = wt;
+    L.l = l;
+    L.pdf = pl;
+    return true;
+}
+
 DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_component, float eu, const DrEdge &edge,
                                                const LightSample &ls, const SecondarySample *ss, int nss, F3 wi,
                                                const Isect &is, const SPoint &sp, F3 weight, F3 d_color,
@@ -349,35 +395,10 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
     if (a.z <= 0.f && b.z <= 0.f) return R;
     if (a.z < 0.f) a = (a * b.z - b * a.z) / (b.z - a.z);
     if (b.z < 0.f) b = (a * b.z - b * a.z) / (b.z - a.z);
-    F3 vd = b - a;
-    if (abs_sum(vd) < 1e-10f) return R;
-    F3 wt = normalize(vd);
-    float l0 = dot(a, wt), l1 = dot(b, wt);
-    F3 vo = a - l0 * wt;
-    float d = length(vo);
-    auto I = [&](float l) {
-        return (l / (d * (d * d + l * l)) + atanf(l / d) / (d * d)) * vo.z + (l * l / (d * (d * d + l * l))) * wt.z;
-    };
-    float Il0 = I(l0), Il1 = I(l1);
-    float norm = Il1 - Il0;
-    if (norm <= 1e-10f) return R;
-    auto pdf = [&](float l) {
-        float dsq = d * d + l * l;
-        return 2.f * d * (vo + l * wt).z / (norm * dsq * dsq);
-    };
-    float lb = l0, ub = l1;
-    if (lb > ub) { float t = lb; lb = ub; ub = t; }
-    float l = 0.5f * (lb + ub);
-    for (int it = 0; it < 20; it++) {
-        if (!(l >= lb && l <= ub)) l = 0.5f * (lb + ub);
-        float value = (I(l) - Il0) / norm - eu;
-        if (fabsf(value) < 1e-5f || it == 19) break;
-        if (value > 0.f) ub = l;
-        else lb = l;
-        l -= value / pdf(l);
-    }
-    float pl = pdf(l);
-    if (pl <= 0.f) return R;
+    LineSample L;
+    if (!ltc_line_sample(a, b, eu, L)) return R;
+    float l = L.l, pl = L.pdf;
+    F3 vo = L.vo, wt = L.wt;
     F3 sample_p = mul(M, vo + l * wt);
     F3 spn = normalize(cross(v0 - sp.pos, v1 - sp.pos));
     float delta = 1e-3f * length(sample_p);

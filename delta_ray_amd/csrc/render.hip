@@ -315,6 +315,77 @@ __global__ void k_intersect(DrSceneView S, const float *rays, long long n, int *
     h[3] = __float_as_int(v);
 }
 
+// The four line-integral estimators of the reference's test_line_integral
+// (delta_ray.cpp:1572-1836): a constant emitter on the plane y = 3 seen along
+// the segment (-10, 3, 0)-(10, 3, 0) from the origin, weighted by a cosine lobe
+// about +y and by the glossy LTC lobe Minv = diag(3, 3, 1) * frame(+y);
+// (0) uniform position on the segment, cosine lobe; (1) the LTC line sampler,
+// cosine lobe; (2) uniform, glossy; (3) the LTC line sampler, glossy.
+// out: per estimator the sum and the sum of squares (double).
+__global__ void __launch_bounds__(256) k_line_integral(long long n, uint64_t seed, double *out) {
+    __shared__ double red[8][256];
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        const F3 p1 = F3{-10.f, 3.f, 0.f}, p2 = F3{10.f, 3.f, 0.f};
+        const float y = 3.f;
+        const F3 down = F3{0.f, -1.f, 0.f};
+        Frame fr = make_frame(F3{0.f, 1.f, 0.f});
+        M3 Fm = m3_rows(fr.x, fr.y, fr.n);
+        M3 Ft = m3_rows(F3{fr.x.x, fr.y.x, fr.n.x}, F3{fr.x.y, fr.y.y, fr.n.y}, F3{fr.x.z, fr.y.z, fr.n.z});
+        M3 S = m3_rows(F3{3.f, 0.f, 0.f}, F3{0.f, 3.f, 0.f}, F3{0.f, 0.f, 1.f});
+        M3 Si = m3_rows(F3{1.f / 3.f, 0.f, 0.f}, F3{0.f, 1.f / 3.f, 0.f}, F3{0.f, 0.f, 1.f});
+        M3 Minv_g = mul(S, Fm), M_g = mul(Ft, Si);
+        const float det_g = 9.f;
+        Philox rng;
+        for (int k = 0; k < 4; k++) {
+            philox_init(rng, seed, 2, (uint64_t)i, (uint32_t)k);
+            float u = philox_next(rng);
+            bool glossy = k >= 2;
+            F3 p, wline;
+            float inv_pdf = 1.f;
+            if ((k & 1) == 0) {
+                p = p1 + u * (p2 - p1);
+                wline = p2 - p1;
+            } else {
+                const M3 &Minv = glossy ? Minv_g : Fm;
+                LineSample L;
+                if (!ltc_line_sample(mul(Minv, p1), mul(Minv, p2), u, L)) continue;
+                F3 lp = L.vo + L.l * L.wt;
+                const M3 &M = glossy ? M_g : Ft;
+                p = mul(M, lp);
+                wline = mul(M, L.wt);
+                inv_pdf = 1.f / L.pdf;
+            }
+            F3 dir = normalize(p);
+            F3 hit = dir * (y / dir.y);
+            M3 J = mul(intersect_jacobian(dir, hit, down), normalize_jacobian(p));
+            float jac = length(mul(J, wline)) * inv_pdf;
+            float geom = dir.y / length_sq(hit);
+            float brdf;
+            if (!glossy) {
+                brdf = dir.y / DR_PI;
+            } else {
+                F3 td = mul(Minv_g, dir);
+                float tl = length(td);
+                brdf = (td.z / tl) / DR_PI * (det_g / (tl * tl * tl));
+            }
+            v[k] = (double)(brdf * geom * jac);
+        }
+    }
+    for (int k = 0; k < 4; k++) {
+        red[2 * k][threadIdx.x] = v[k];
+        red[2 * k + 1][threadIdx.x] = v[k] * v[k];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int j = 0; j < 8; j++) red[j][threadIdx.x] += red[j][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) atomicAdd(out + threadIdx.x, red[threadIdx.x][0]);
+}
+
 // ------------------------------------------------------------------- host --
 extern "C" const float dr_ltc_blob[];
 extern "C" const float dr_ltc_blob_end[];
@@ -759,6 +830,26 @@ dr_status dr_intersect(dr_scene *s, const float *rays, int64_t n, int32_t *hits,
     hipStream_t hs = (hipStream_t)stream;
     k_intersect<<<(unsigned)((n + 255) / 256), 256, 0, hs>>>(s->view, rays, n, hits, any_hit);
     return launch_check("k_intersect");
+}
+
+dr_status dr_test_line_integral(int64_t num_samples, uint64_t seed, double *out, void *stream) {
+    if (num_samples <= 0 || !out) return fail(DR_ERROR_INVALID_ARGUMENT, "bad arguments");
+    hipStream_t hs = (hipStream_t)stream;
+    double *dev = nullptr;
+    if (hipMalloc(&dev, 8 * sizeof(double)) != hipSuccess) return fail(DR_ERROR_OUT_OF_MEMORY, "hipMalloc");
+    k_fill<<<1, 64, 0, hs>>>((float *)dev, 16, 0.f);
+    k_line_integral<<<(unsigned)((num_samples + 255) / 256), 256, 0, hs>>>(num_samples, seed, dev);
+    double sums[8];
+    hipError_t e = hipMemcpyAsync(sums, dev, sizeof(sums), hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    hipFree(dev);
+    if (e != hipSuccess) return fail(DR_ERROR_DEVICE, hipGetErrorString(e));
+    for (int k = 0; k < 4; k++) {
+        double mean = sums[2 * k] / (double)num_samples;
+        out[k] = mean;
+        out[4 + k] = sums[2 * k + 1] / (double)num_samples - mean * mean;
+    }
+    return launch_check("k_line_integral");
 }
 
 }  // extern "C"

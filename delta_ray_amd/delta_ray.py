@@ -376,14 +376,25 @@ def render(camera, shapes, materials, lights, resolution, d_image, num_samples, 
     return res
 
 
-def test_line_integral():
-    """reference delta_ray.cpp:1572-1836 (run by unit_tests.py): compare
-    uniform and LTC-importance-sampled line integrals of a cosine and a
-    glossy lobe over the segment (-10,3,0)-(10,3,0); prints the four means and
-    variances.  Computed on the host in float32 with numpy (it is a
-    known-answer check of the line-sampling maths, not a hot path)."""
-    from . import line_integral
-    res = line_integral.run()
-    for k, v in res.items():
-        print("%s:%g" % (k, v))
+def test_line_integral(num_samples=65536, seed=0, stream=None):
+    """reference delta_ray.cpp:1572-1836 (run by unit_tests.py): uniform vs
+    LTC-importance-sampled estimates of the line integral of a cosine and of a
+    glossy lobe over the segment (-10,3,0)-(10,3,0), num_samples each, on the
+    GPU with the device's line sampler (the one secondary edges use).  Prints
+    the four means and variances with the reference's labels and returns them
+    as a dict."""
+    lib = _abi.load_library()
+    out = (ctypes.c_double * 8)()
+    st = _stream_ptr(stream)
+    _abi.check(lib, lib.dr_test_line_integral(int(num_samples), int(seed), out, st), "dr_test_line_integral")
+    labels = ["Uniform line", "Cosine line", "Uniform line glossy", "Cosine line glossy"]
+    vlabels = ["Uniform variance", "Cosine variance", "Uniform line glossy variance",
+               "Cosine line glossy variance"]
+    res = {}
+    for k in range(4):
+        res[labels[k]] = out[k]
+        res[vlabels[k]] = out[4 + k]
+    for k in range(4):
+        print("%s:%g" % (labels[k], out[k]))
+        print("%s:%g" % (vlabels[k], out[4 + k]))
     return res

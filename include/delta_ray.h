@@ -291,6 +291,14 @@ typedef struct dr_scene_info {
 } dr_scene_info;
 dr_status dr_scene_get_info(const dr_scene *scene, dr_scene_info *out);
 
+/* The reference's test_line_integral (delta_ray.cpp:1572-1836) on the GPU
+ * with the device's LTC line sampler: four estimates of one line integral
+ * (uniform / LTC-sampled position, cosine / glossy lobe) from num_samples
+ * samples each.  out (host, 8 doubles): the four means, then the four
+ * (population) variances, in the reference's print order.  Synchronises the
+ * stream. */
+dr_status dr_test_line_integral(int64_t num_samples, uint64_t seed, double *out, void *stream);
+
 const char *dr_last_error(void);
 
 #ifdef __cplusplus

@@ -34,3 +34,136 @@ def test_philox_known_answers(oracle, ctr, key, expect):
     out = (ctypes.c_uint32 * 4)()
     lib.or_philox4x32(k, c, out)
     assert tuple(out) == expect
+
+
+def _declared_functions():
+    """Exported (non-static) function names declared in include/*.h."""
+    names = []
+    inc = os.path.join(ROOT, "include")
+    for fn in sorted(os.listdir(inc)):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^(?!static)[A-Za-z_][\w \*]*?\b(dr_\w+)\s*\(", text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    """The C-ABI library loads (no GPU needed to load it) and exports every
+    function of include/delta_ray.h; the ctypes mirror covers them all."""
+    names = _declared_functions()
+    assert "dr_render_adjoint" in names and "dr_scene_create" in names
+    lib = ctypes.CDLL(_abi.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    mirrored = {name for name, _, _ in _abi.ENTRY_POINTS}
+    assert set(names) == mirrored, set(names) ^ mirrored
+
+
+def test_ltc_blob_is_the_reference_table():
+    """The embedded LTC table is ltc.inc's tabM converted to float32 by
+    delta_ray_amd/data/make_ltc_table.py; its checksum is recorded beside it."""
+    data = os.path.join(ROOT, "delta_ray_amd", "data")
+    blob = open(os.path.join(data, "ltc_tabM.f32"), "rb").read()
+    assert len(blob) == 128 * 128 * 9 * 4
+    want = open(os.path.join(data, "ltc_tabM.sha256")).read().split()[0]
+    assert hashlib.sha256(blob).hexdigest() == want
+    tab = np.frombuffer(blob, np.float32).reshape(128 * 128, 3, 3)
+    # roughness index 0 / theta index 0 and the opposite corner are finite,
+    # invertible 3x3 matrices (the sampler inverts them)
+    for i in (0, 127, 127 * 128, 128 * 128 - 1):
+        assert np.all(np.isfinite(tab[i])) and abs(np.linalg.det(tab[i].astype(np.float64))) > 0
+
+
+def test_ltc_blob_equals_reference_text():
+    """Where the reference sources are mounted, the blob equals every value of
+    ltc.inc's tabM (the table text is the reference's own fixture)."""
+    src = "/root/reference/ltc.inc"
+    if not os.path.exists(src):
+        pytest.skip("reference sources not mounted")
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_ltc_table", os.path.join(ROOT, "delta_ray_amd", "data", "make_ltc_table.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    tab = mod.parse_tabM(open(src).read())
+    blob = np.fromfile(os.path.join(ROOT, "delta_ray_amd", "data", "ltc_tabM.f32"), dtype="<f4")
+    np.testing.assert_array_equal(blob.reshape(-1, 9), tab)
+
+
+def _tree(osc, which):
+    lib = osc.lib
+    lib.or_edge_tree.restype = ctypes.c_int
+    lib.or_edge_tree.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.POINTER(ctypes.c_int)]
+    ne = ctypes.c_int(0)
+    nn = lib.or_edge_tree(osc.h, which, None, None, ctypes.byref(ne))
+    nodes = np.zeros((max(nn, 1), 13), np.float32)
+    edges = np.zeros((max(ne.value, 1), 5), np.int32)
+    lib.or_edge_tree(osc.h, which, nodes.ctypes.data, edges.ctypes.data, ctypes.byref(ne))
+    return nodes[:nn], edges[:ne.value]
+
+
+@pytest.mark.parametrize("name", ["shadow", "heightfield", "glossy_plane"])
+def test_edge_tree_invariants(oracle, name):
+    """edge_tree.cpp:49-316 build: every edge in exactly one leaf, boxes
+    contain their children and their edges, node lengths are sums of edge
+    lengths, normal cones contain the adjacent face normals, and the two trees
+    partition the scene's edges (silhouette tree: boundary edges or shapes
+    without shading normals)."""
+    w = helpers.small_workload(name, res=16)
+    osc = oracle.scene_from_workload(w)
+    info = osc.info()
+    total = 0
+    for which in (0, 1):
+        nodes, edges = _tree(osc, which)
+        total += len(edges)
+        if len(nodes) == 0:
+            assert len(edges) == 0
+            continue
+        seen = np.zeros(len(edges), np.int32)
+
+        def ends(e):
+            s = w["shapes"][e[0]]
+            V = np.asarray(s.vertices, np.float64)
+            return V[e[1]], V[e[2]]
+
+        def face_normal(shape, f):
+            s = w["shapes"][shape]
+            V = np.asarray(s.vertices, np.float64)
+            i = np.asarray(s.indices)[f]
+            n = np.cross(V[i[1]] - V[i[0]], V[i[2]] - V[i[0]])
+            return n / np.linalg.norm(n)
+
+        def visit(i):
+            lo, hi = nodes[i, 0:3], nodes[i, 3:6]
+            off, npr, length = int(nodes[i, 10]), int(nodes[i, 11]), float(nodes[i, 12])
+            if npr > 0:
+                s = 0.0
+                for k in range(off, off + npr):
+                    seen[k] += 1
+                    a, b = ends(edges[k])
+                    for p in (a, b):
+                        assert np.all(p >= lo - 1e-5) and np.all(p <= hi + 1e-5)
+                    s += np.linalg.norm(b - a)
+                    if which == 1:
+                        axis, ang = nodes[i, 6:9].astype(np.float64), float(nodes[i, 9])
+                        for f in (edges[k][3], edges[k][4]):
+                            n = face_normal(edges[k][0], f)
+                            c = np.clip(np.dot(axis / np.linalg.norm(axis), n), -1.0, 1.0)
+                            assert np.arccos(c) <= ang + 1e-3
+                    else:
+                        assert edges[k][4] == -1 or w["shapes"][edges[k][0]].normals is None
+                assert abs(s - length) <= 1e-4 * max(1.0, s)
+                return length
+            l_len = visit(i + 1)
+            r_len = visit(off)
+            for c in (i + 1, off):
+                assert np.all(nodes[c, 0:3] >= lo - 1e-6) and np.all(nodes[c, 3:6] <= hi + 1e-6)
+            assert abs(l_len + r_len - length) <= 1e-4 * max(1.0, length)
+            return length
+        visit(0)
+        assert np.all(seen == 1)
+    assert total == info["edges"]
