@@ -1,0 +1,132 @@
+"""torch.autograd front end with the reference's calling convention.
+
+Mirrors render_pytorch.py of the reference: ``RenderFunction.serialize_scene``
+builds the same flat argument list (render_pytorch.py:13-55),
+``RenderFunction.apply(seed, *args)`` returns the image [H, W, 3], and the
+backward returns one gradient per argument in the same order
+(render_pytorch.py:283-333): None for counts, ids, flags and sizes; tensors for
+the four camera matrices, material parameters, vertices, 2-D normals and light
+intensities.
+
+Differences that follow from running on the GPU:
+* the scene is uploaded once in forward and kept for backward (the reference
+  builds it twice per step);
+* the image is returned on the GPU when the vertices live there, otherwise as a
+  CPU tensor like the reference's;
+* the backward render uses the same random numbers as the forward render, so
+  its image equals the forward image (the reference's backward pass re-renders
+  with a differently consumed generator, render_pytorch.py:247-257);
+* uv-scale gradients are None (the reference returns empty arrays for them,
+  delta_ray.cpp:1127-1128).
+"""
+import numpy as np
+import torch
+
+from . import _abi
+from . import delta_ray as dr
+
+
+def _np(t):
+    if t is None:
+        return None
+    if isinstance(t, torch.Tensor):
+        return t.detach().to("cpu").numpy()
+    return np.asarray(t)
+
+
+class RenderFunction(torch.autograd.Function):
+    """Render a scene (reference render_pytorch.py:7-333)."""
+
+    flags = _abi.DR_FLAGS_PARITY
+
+    @staticmethod
+    def serialize_scene(cam, materials, shapes, lights, resolution, num_samples, max_bounces):
+        args = [len(materials), len(shapes), len(lights),
+                cam.cam_to_world, cam.world_to_cam, cam.sample_to_cam, cam.cam_to_sample,
+                float(cam.fov_factor.item() if isinstance(cam.fov_factor, torch.Tensor) else cam.fov_factor),
+                cam.aspect_ratio,
+                float(cam.clip_near.item() if isinstance(cam.clip_near, torch.Tensor) else cam.clip_near),
+                cam.fisheye]
+        for m in materials:
+            args += [m.diffuse_reflectance, m.specular_reflectance, m.roughness, m.diffuse_uv_scale,
+                     m.specular_uv_scale, m.roughness_uv_scale, m.two_sided]
+        for s in shapes:
+            args += [s.vertices, s.indices, s.uvs, s.normals, s.mat_id]
+        for l in lights:
+            args += [l.shape_id, l.intensity]
+        args += [resolution, num_samples, max_bounces]
+        return args
+
+    @staticmethod
+    def _unpack(args):
+        it = iter(args)
+        n_mat, n_shape, n_light = next(it), next(it), next(it)
+        cam = [next(it) for _ in range(4)]
+        fov, aspect, clip_near, fisheye = next(it), next(it), next(it), next(it)
+        mats = [[next(it) for _ in range(7)] for _ in range(n_mat)]
+        shapes = [[next(it) for _ in range(5)] for _ in range(n_shape)]
+        lights = [[next(it) for _ in range(2)] for _ in range(n_light)]
+        resolution, num_samples, max_bounces = next(it), next(it), next(it)
+        return cam, (fov, aspect, clip_near, fisheye), mats, shapes, lights, resolution, num_samples, max_bounces
+
+    @staticmethod
+    def forward(ctx, seed, *args):
+        cam, (fov, aspect, clip_near, fisheye), mats, shapes, lights, resolution, spp, nb = \
+            RenderFunction._unpack(args)
+        camera = dr.Camera(*[_np(m).astype(np.float32) for m in cam], float(fov), float(aspect),
+                           float(clip_near), bool(fisheye))
+        materials = [dr.Material(*[_np(a).astype(np.float32) for a in m[:6]], bool(m[6])) for m in mats]
+        shape_objs = []
+        for v, ind, uv, nrm, mid in shapes:
+            shape_objs.append(dr.Shape(_np(v).astype(np.float32), _np(ind).astype(np.int32),
+                                       None if uv is None else _np(uv).astype(np.float32),
+                                       None if nrm is None else _np(nrm).astype(np.float32),
+                                       materials[mid], None))
+        light_objs = []
+        for sid, inten in lights:
+            l = dr.Light(shape_objs[sid], _np(inten).astype(np.float32))
+            shape_objs[sid].light = l
+            light_objs.append(l)
+        scene = dr.DeviceScene(camera, shape_objs, materials, light_objs)
+        W, H = int(resolution[0]), int(resolution[1])
+        dev = torch.device("cuda")
+        image = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+        scene.render(dr.DeviceScene.options(W, H, spp, nb, seed, RenderFunction.flags), image)
+        ctx.scene, ctx.seed, ctx.size, ctx.spp, ctx.nb = scene, seed, (W, H), spp, nb
+        ctx.args = args
+        on_gpu = len(shapes) > 0 and isinstance(shapes[0][0], torch.Tensor) and shapes[0][0].is_cuda
+        ctx.on_gpu = on_gpu
+        return image if on_gpu else image.cpu()
+
+    @staticmethod
+    def backward(ctx, grad_img):
+        scene = ctx.scene
+        W, H = ctx.size
+        dev = torch.device("cuda")
+        d_image = grad_img.detach().to(dev, torch.float32).contiguous()
+        image = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+        grads = torch.empty((scene.grad_size,), dtype=torch.float32, device=dev)
+        scene.render_adjoint(dr.DeviceScene.options(W, H, ctx.spp, ctx.nb, ctx.seed, RenderFunction.flags),
+                             d_image, image, grads)
+        d_camera, d_shapes, d_materials, d_lights = dr.split_grads(grads, scene.sd.layout)
+        cam, _, mats, shapes, lights, _, _, _ = RenderFunction._unpack(ctx.args)
+
+        def like(g, ref):
+            if not isinstance(ref, torch.Tensor):
+                return None
+            return g.reshape(ref.shape).to(ref.device, ref.dtype)
+
+        ret = [None, None, None, None]   # seed, num_materials, num_shapes, num_lights
+        ret += [like(d_camera.d_cam_to_world, cam[0]), like(d_camera.d_world_to_cam, cam[1]),
+                like(d_camera.d_sample_to_cam, cam[2]), like(d_camera.d_cam_to_sample, cam[3])]
+        ret += [None, None, None, None]  # fov_factor, aspect_ratio, clip_near, fisheye
+        for dm, m in zip(d_materials, mats):
+            ret += [like(dm.diffuse_reflectance, m[0]), like(dm.specular_reflectance, m[1]),
+                    like(dm.roughness, m[2]), None, None, None, None]
+        for ds, s in zip(d_shapes, shapes):
+            d_n = like(ds.normals, s[3]) if (s[3] is not None and np.ndim(_np(s[3])) == 2) else None
+            ret += [like(ds.vertices, s[0]), None, None, d_n, None]
+        for dl, l in zip(d_lights, lights):
+            ret += [None, like(dl.intensity, l[1])]
+        ret += [None, None, None]        # resolution, num_samples, max_bounces
+        return tuple(ret)

@@ -1,0 +1,74 @@
+"""The torch front end with the reference's calling convention
+(render_pytorch.py): argument list on the CPU, forward/backward on the GPU
+against the C-ABI renderer directly."""
+import numpy as np
+import pytest
+import torch
+
+from delta_ray_amd import _abi, workloads
+from delta_ray_amd import delta_ray as dr
+from delta_ray_amd import scene as sc
+from delta_ray_amd.render_pytorch import RenderFunction
+
+
+def _scene(device="cpu"):
+    cam = sc.Camera(np.array([0, 0, 5], np.float32), np.array([0, 0, 0], np.float32),
+                    np.array([0, 1, 0], np.float32), None, 45.0, 0.01, 10000.0, (32, 32))
+    w = workloads.single_triangle(resolution=(32, 32))
+    tri, light_shape = w["shapes"]
+    grey = sc.Material(torch.tensor([0.5, 0.5, 0.5]))
+    black = sc.Material(torch.tensor([0.0, 0.0, 0.0]))
+    v = torch.tensor(tri.vertices, device=device, requires_grad=True)
+    shapes = [sc.Shape(v, torch.tensor(tri.indices, device=device), None, None, 0),
+              sc.Shape(torch.tensor(light_shape.vertices, device=device),
+                       torch.tensor(light_shape.indices, device=device), None, None, 1)]
+    inten = torch.tensor([20.0, 20.0, 20.0], requires_grad=True)
+    lights = [sc.Light(1, inten)]
+    return cam, [grey, black], shapes, lights, v, inten
+
+
+def test_camera_matches_workload_camera():
+    """scene.Camera (camera.py semantics) == workloads.make_camera."""
+    cam = sc.Camera(np.array([0.3, 1, 4], np.float32), np.array([0, 0.2, 0], np.float32),
+                    np.array([0, 1, 0], np.float32), None, 50.0, 0.05, 100.0, (64, 48))
+    ref = workloads.make_camera([0.3, 1, 4], [0, 0.2, 0], [0, 1, 0], 50.0, (64, 48), clip_near=0.05,
+                                clip_far=100.0)
+    for name in ("cam_to_world", "world_to_cam", "sample_to_cam", "cam_to_sample"):
+        np.testing.assert_allclose(getattr(cam, name).numpy(), getattr(ref, name), rtol=1e-5, atol=1e-5)
+    assert abs(cam.fov_factor.item() - ref.fov_factor) < 1e-6
+
+
+def test_serialize_scene_layout():
+    cam, mats, shapes, lights, _, _ = _scene()
+    args = RenderFunction.serialize_scene(cam, mats, shapes, lights, (32, 32), 4, 1)
+    # counts, 4 matrices, 4 camera scalars, 7 per material, 5 per shape,
+    # 2 per light, resolution / spp / bounces (render_pytorch.py:13-55)
+    assert len(args) == 3 + 4 + 4 + 7 * 2 + 5 * 2 + 2 + 3
+    assert args[:3] == [2, 2, 1]
+    assert args[-3:] == [(32, 32), 4, 1]
+    assert args[11 + 14] is shapes[0].vertices
+
+
+@pytest.mark.gpu
+def test_render_function_matches_c_abi(gpu):
+    cam, mats, shapes, lights, v, inten = _scene()
+    args = RenderFunction.serialize_scene(cam, mats, shapes, lights, (32, 32), 4, 1)
+    img = RenderFunction.apply(7, *args)
+    assert img.shape == (32, 32, 3) and img.device.type == "cpu"
+    d = torch.from_numpy(np.random.default_rng(0).normal(size=(32, 32, 3)).astype(np.float32))
+    (img * d).sum().backward()
+    # the same render through the C-ABI directly
+    w = workloads.single_triangle(resolution=(32, 32))
+    camera = dr.Camera(*[getattr(cam, k).numpy() for k in ("cam_to_world", "world_to_cam", "sample_to_cam",
+                                                          "cam_to_sample")],
+                       float(cam.fov_factor.item()), cam.aspect_ratio, float(cam.clip_near.item()), False)
+    scene = dr.DeviceScene(camera, w["shapes"], w["materials"], w["lights"])
+    o = dr.DeviceScene.options(32, 32, 4, 1, 7)
+    ref_img = torch.empty((32, 32, 3), device=gpu)
+    g = torch.empty((scene.grad_size,), device=gpu)
+    scene.render_adjoint(o, d.to(gpu), ref_img, g)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(img.numpy(), ref_img.cpu().numpy(), rtol=1e-5, atol=1e-6)
+    _, d_shapes, _, d_lights = dr.split_grads(g.cpu().numpy(), scene.sd.layout)
+    np.testing.assert_allclose(v.grad.numpy(), d_shapes[0].vertices, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(inten.grad.numpy(), d_lights[0].intensity, rtol=1e-4, atol=1e-6)
