@@ -34,6 +34,10 @@ B_NODE = 64             # one BVH2 node (two child boxes)
 B_TRI = 48              # one triangle (v0, e1, e2)
 B_PIXEL_IO = 12 + 12    # d_image read + image write per pixel sample
 B_EDGE = 24 + 24 + 24 + 24 + 36  # edge v0/v1, faces, non-shared verts, normals, LTC matrix
+B_POINT = 108                   # one shading-point record (read by selection / edge samples)
+B_TREE_EXPAND = 2 * 48          # an expanded edge-tree node reads its two children
+B_EDGE_TEST = 20 + 24 + 24 + 24  # edge record, its endpoints, faces' third vertices + indices
+B_QUEUE = 16                    # one selected-edge queue entry written
 
 
 def parse():
@@ -156,14 +160,19 @@ def main():
         return {k: x[k] - y[k] for k in ("rays", "nodes_visited", "tris_visited")}
     per_kernel = {"path": dict(ca), "edge": diff(cb, ca), "primary": diff(cc, ca)}
     per_kernel["edge"]["edges"] = cb["secondary_edges"]
+    per_kernel["select"] = {"rays": 0, "nodes_visited": 0, "tris_visited": 0,
+                            "edge_nodes_visited": cb["edge_nodes_visited"], "edges_tested": cb["edges_tested"]}
     bytes_k = {}
     for k, c in per_kernel.items():
         bytes_k[k] = B_RAY_IO * c["rays"] + B_NODE * c["nodes_visited"] + B_TRI * c["tris_visited"]
     bytes_k["path"] += B_PIXEL_IO * samples
-    bytes_k["edge"] += B_EDGE * cb["secondary_edges"]
+    bytes_k["edge"] += (B_EDGE + B_POINT) * cb["secondary_edges"]
+    n_points = samples * nb
+    bytes_k["select"] += (B_POINT * n_points + B_TREE_EXPAND * cb["edge_nodes_visited"] +
+                          B_EDGE_TEST * cb["edges_tested"] + B_QUEUE * cb["secondary_edges"])
     kms = {k: kt[k + "_ms"] / max(kt[k + "_launches"], 1) for k in ("path", "select", "edge", "dp", "primary")}
     kms_step = {k: kt[k + "_ms"] / args.steps for k in ("path", "select", "edge", "dp", "primary")}
-    dom = max(("path", "edge", "primary"), key=lambda k: kms_step[k])
+    dom = max(("path", "select", "edge", "primary"), key=lambda k: kms_step[k])
     launches_per_step = max(kt[dom + "_launches"] // args.steps, 1)
     bytes_launch = bytes_k[dom] / launches_per_step
     achieved = bytes_launch / (kms[dom] / 1000.0) / 1e9
@@ -174,7 +183,9 @@ def main():
                 "kernel": "k_" + dom, "kernel_ms_per_launch": round(kms[dom], 3),
                 "bytes_per_launch": int(bytes_launch),
                 "per_ray": {"nodes": round(c["nodes_visited"] / max(c["rays"], 1), 2),
-                            "tris": round(c["tris_visited"] / max(c["rays"], 1), 2)}}
+                            "tris": round(c["tris_visited"] / max(c["rays"], 1), 2)},
+                "per_kernel_gbs": {k: round(bytes_k[k] / max(kt[k + "_launches"] // args.steps, 1)
+                                            / max(kms[k] / 1000.0, 1e-12) / 1e9, 2) for k in bytes_k}}
     out = {"metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -187,6 +198,8 @@ def main():
            "extra": {"rays_per_s": round(rays_total / (ms_per_step / 1000.0), 1),
                      "rays_per_step": int(rays_total),
                      "secondary_edges_per_step": int(cb["secondary_edges"]),
+                     "edge_tree_nodes_per_point": round(cb["edge_nodes_visited"] / max(n_points, 1), 2),
+                     "edges_tested_per_point": round(cb["edges_tested"] / max(n_points, 1), 2),
                      "kernel_ms_per_step": {k: round(v, 3) for k, v in kms_step.items()},
                      "scene_build_s": round(build_s, 3), "scene": info}}
     if args.calibrate:

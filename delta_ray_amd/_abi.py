@@ -93,6 +93,7 @@ class dr_stats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_uint64), ("nodes_visited", ctypes.c_uint64),
                 ("tris_visited", ctypes.c_uint64), ("secondary_edges", ctypes.c_uint64),
                 ("primary_edge_samples", ctypes.c_uint64),
+                ("edge_nodes_visited", ctypes.c_uint64), ("edges_tested", ctypes.c_uint64),
                 ("kernel_ms", ctypes.c_double * 6), ("kernel_launches", ctypes.c_uint64 * 6)]
 
 

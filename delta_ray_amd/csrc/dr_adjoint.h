@@ -14,14 +14,55 @@
 
 #include "dr_edges.h"
 
+// Workgroup gradient cache in LDS.  Neighbouring pixel samples of one
+// workgroup hit the same triangles and texels, so their adjoints go to the
+// same few hundred gradient entries: they are summed here with LDS atomics
+// (open addressing on the flat gradient index, linear probing) and reach HBM
+// as one global atomic per distinct entry when the workgroup ends.  An entry
+// that finds no free slot within DR_GC_PROBES goes straight to HBM.
+#define DR_GC_LOG2 12
+#define DR_GC_SLOTS (1 << DR_GC_LOG2)
+#define DR_GC_PROBES 8
+struct GradCache {
+    int key[DR_GC_SLOTS];
+    float val[DR_GC_SLOTS];
+};
+// every thread of the workgroup calls these, followed / preceded by a barrier
+DR_INLINE void gc_init(GradCache *gc) {
+    for (int i = threadIdx.x; i < DR_GC_SLOTS; i += blockDim.x) {
+        gc->key[i] = -1;
+        gc->val[i] = 0.f;
+    }
+}
+DR_INLINE void gc_flush(GradCache *gc, float *grads) {
+    for (int i = threadIdx.x; i < DR_GC_SLOTS; i += blockDim.x) {
+        int k = gc->key[i];
+        float v = gc->val[i];
+        if (k >= 0 && v != 0.f) atomicAdd(grads + k, v);
+    }
+}
+
 // Gradient sink: where adjoints of scene parameters go.
 struct GradSink {
     float *grads;      // flat buffer, include/delta_ray.h layout
     float *cam;        // 64 floats (LDS) for camera matrices
     uint32_t flags;
     bool color_mode;   // false: Dirac position sweep (vertices + camera only)
+    GradCache *gc = nullptr;  // LDS cache of the workgroup, or none
     DR_INLINE void add(long long idx, float v) const {
-        if (v != 0.f) atomicAdd(grads + idx, v);
+        if (v == 0.f) return;
+        if (gc && idx < 0x7fffffffll) {
+            uint32_t h = ((uint32_t)idx * 2654435761u) >> (32 - DR_GC_LOG2);
+            for (int p = 0; p < DR_GC_PROBES; p++) {
+                int slot = (int)((h + p) & (DR_GC_SLOTS - 1));
+                int old = atomicCAS(&gc->key[slot], -1, (int)idx);
+                if (old == -1 || old == (int)idx) {
+                    atomicAdd(&gc->val[slot], v);
+                    return;
+                }
+            }
+        }
+        atomicAdd(grads + idx, v);
     }
     DR_INLINE void add3(long long idx, F3 v) const {
         add(idx, v.x); add(idx + 1, v.y); add(idx + 2, v.z);

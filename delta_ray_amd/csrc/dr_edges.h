@@ -194,115 +194,157 @@ DR_INLINE float node_importance(const DrEdgeNode &n, float bound, F3 p) {
 
 // Edge-tree traversal (edge_tree.cpp:511-683), iterative: a stack of
 // pending right children reproduces the recursion's depth-first order and
-// therefore its order of random draws.  `emit(edge, weight, tree, index)` is called for
-// every selected silhouette edge, in order.
+// therefore its order of random draws.  The traversal of one shading point is
+// a cone pass then an "all" pass over the chosen tree; it is kept as explicit
+// state advanced one node per et_step() so that a persistent kernel can give a
+// lane a new point as soon as its traversal ends.  et_step() calls
+// `emit(edge, weight, in_normal_tree, index)` for every selected silhouette
+// edge, in order.
 struct TreeEntry {
     int node;
     float bound, cone_pmf, all_pmf;
 };
 
-template <class Emit>
-DR_INLINE void traverse_edge_tree(const DrSceneView &S, const DrEdgeNode *nodes, int num_nodes, const DrEdge *edges,
-                                  bool directional,
-                                  bool cone_mode, float root_bound, float pmf, F3 wi, const SPoint &sp,
-                                  float roughness, float dlum, float slum, const Cone &cone, Philox &rng, Emit &emit) {
-    // depth-first with pending right children: at most depth + 1 entries
-    // (median-split trees are balanced: depth ~ log2(edges))
-    TreeEntry stack[DR_TREE_STACK];
-    int sp_ = 0;
-    stack[sp_++] = TreeEntry{0, root_bound, pmf, pmf};
-    // each node is popped at most once per pass; the cap guards a malformed tree
-    int pops = 0;
-    const int cap = 2 * num_nodes + 8;
-    while (sp_ > 0 && pops++ < cap) {
-        TreeEntry e = stack[--sp_];
-        if (e.bound <= 0.f) continue;
-        const DrEdgeNode &node = nodes[e.node];
-        if (node.nprims > 0) {
-            float den = e.cone_pmf * e.cone_pmf + e.all_pmf * e.all_pmf;
-            float w = (cone_mode ? e.cone_pmf : e.all_pmf) / den;
-            for (int i = 0; i < node.nprims; i++) {
-                const DrEdge &ed = edges[node.offset + i];
-                if (is_silhouette(S, sp.pos, ed)) emit(ed, w, directional, node.offset + i);
-            }
-            continue;
-        }
-        int li = e.node + 1, ri = node.offset;
-        const DrEdgeNode &L = nodes[li], &R = nodes[ri];
-        bool all_split = e.bound > 1.f || node_inside(node, sp.pos);
-        bool lcs = true, rcs = true;
-        if (directional) {
-            lcs = contains_silhouette(L, sp.pos);
-            rcs = contains_silhouette(R, sp.pos);
-        }
-        float lb = lcs ? bsdf_bound(S, L, wi, sp, roughness, dlum, slum) : 0.f;
-        float rb = rcs ? bsdf_bound(S, R, wi, sp, roughness, dlum, slum) : 0.f;
-        float il = lcs ? node_importance(L, lb, sp.pos) : 0.f;
-        float ir = rcs ? node_importance(R, rb, sp.pos) : 0.f;
-        if (il + ir <= 1e-10f) continue;
-        bool lhc = aabb_cone_intersect(L.lo, L.hi, cone);
-        bool rhc = aabb_cone_intersect(R.lo, R.hi, cone);
-        bool cone_split = all_split || (lhc && rhc && lcs && rcs && cone.cos_angle > 0.95f);
-        float ilc = lhc ? il : 0.f, irc = rhc ? ir : 0.f;
-        if (cone_mode && ilc + irc <= 1e-10f) continue;
-        float plc = 0.f, prc = 0.f, pla, pra;
-        if (cone_split) {
-            plc = prc = 1.f;
-        } else if (ilc + irc > 1e-10f) {
-            plc = ilc / (ilc + irc);
-            prc = 1.f - plc;
-        }
-        if (all_split) {
-            pla = pra = 1.f;
-        } else {
-            pla = il / (il + ir);
-            pra = 1.f - pla;
-        }
-        bool split = cone_mode ? cone_split : all_split;
-        TreeEntry le = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf};
-        TreeEntry re = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf};
-        if (split) {
-            if (rcs && sp_ < DR_TREE_STACK) stack[sp_++] = re;
-            if (lcs && sp_ < DR_TREE_STACK) stack[sp_++] = le;
-        } else {
-            float u = philox_next(rng);
-            bool left = cone_mode ? u < plc : u < pla;
-            if (sp_ < DR_TREE_STACK) stack[sp_++] = left ? le : re;
-        }
-    }
-}
-
-// edge_tree.cpp:654-725: choose the tree, then a cone pass and an "all" pass.
-// emit(edge, weight, in_normal_tree, index_in_tree_edge_list)
-template <class Emit>
-DR_INLINE void sample_shading_edges_idx(const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp, const Cone &cone,
-                                    Philox &rng, Emit &emit) {
-    if (dot(wi, sp.fr.n) < 0.f) return;
-    bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;
-    if (!has_n && !has_s) return;
+struct EdgeTraversal {
     const DrEdgeNode *nodes;
     const DrEdge *edges;
-    int nn;
-    bool dir;
-    float pmf = 1.f;
+    int num_nodes;
+    bool directional, cone_mode;
+    float root_bound, pmf;
+    float roughness, dlum, slum;
+    Cone cone;
+    int sp_, pops;
+    uint32_t n_inner, n_edges;
+    TreeEntry stack[DR_TREE_STACK];
+};
+
+DR_INLINE void et_start_pass(EdgeTraversal &T, bool cone_mode) {
+    T.cone_mode = cone_mode;
+    T.sp_ = 0;
+    T.pops = 0;
+    T.stack[T.sp_++] = TreeEntry{0, T.root_bound, T.pmf, T.pmf};
+}
+
+// edge_tree.cpp:685-725: choose the tree (one draw when both exist) and set
+// up the cone pass.  Returns false when there is nothing to traverse.
+DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp,
+                       const Cone &cone, Philox &rng) {
+    if (dot(wi, sp.fr.n) < 0.f) return false;
+    bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;
+    if (!has_n && !has_s) return false;
+    T.pmf = 1.f;
     if (!has_n) {
-        nodes = S.sil_nodes; edges = S.sil_edges; nn = S.num_sil_nodes; dir = false;
+        T.nodes = S.sil_nodes; T.edges = S.sil_edges; T.num_nodes = S.num_sil_nodes; T.directional = false;
     } else if (!has_s) {
-        nodes = S.nrm_nodes; edges = S.nrm_edges; nn = S.num_nrm_nodes; dir = true;
+        T.nodes = S.nrm_nodes; T.edges = S.nrm_edges; T.num_nodes = S.num_nrm_nodes; T.directional = true;
     } else {
         float ni = S.nrm_nodes[0].total_length, si = S.sil_nodes[0].total_length;
         float npmf = ni / (ni + si), spmf = si / (ni + si);
         float u = philox_next(rng);
-        if (u < npmf) { nodes = S.nrm_nodes; edges = S.nrm_edges; nn = S.num_nrm_nodes; dir = true; pmf = npmf; }
-        else { nodes = S.sil_nodes; edges = S.sil_edges; nn = S.num_sil_nodes; dir = false; pmf = spmf; }
+        if (u < npmf) {
+            T.nodes = S.nrm_nodes; T.edges = S.nrm_edges; T.num_nodes = S.num_nrm_nodes; T.directional = true;
+            T.pmf = npmf;
+        } else {
+            T.nodes = S.sil_nodes; T.edges = S.sil_edges; T.num_nodes = S.num_sil_nodes; T.directional = false;
+            T.pmf = spmf;
+        }
     }
     const DrMaterial &m = S.materials[S.shapes[is.shape].mat];
-    float dlum = luminance(get_diffuse(S, m, sp.uv));
-    float slum = luminance(get_specular(S, m, sp.uv));
-    float roughness = get_roughness(S, m, sp.uv);
-    float b = bsdf_bound(S, nodes[0], wi, sp, roughness, dlum, slum);
-    traverse_edge_tree(S, nodes, nn, edges, dir, true, b, pmf, wi, sp, roughness, dlum, slum, cone, rng, emit);
-    traverse_edge_tree(S, nodes, nn, edges, dir, false, b, pmf, wi, sp, roughness, dlum, slum, cone, rng, emit);
+    T.dlum = luminance(get_diffuse(S, m, sp.uv));
+    T.slum = luminance(get_specular(S, m, sp.uv));
+    T.roughness = get_roughness(S, m, sp.uv);
+    T.cone = cone;
+    T.root_bound = bsdf_bound(S, T.nodes[0], wi, sp, T.roughness, T.dlum, T.slum);
+    T.n_inner = 0;
+    T.n_edges = 0;
+    et_start_pass(T, true);
+    return true;
+}
+
+// One node of the current pass; at the end of the cone pass the "all" pass
+// starts.  Returns false when both passes are done.
+template <class Emit>
+DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoint &sp, Philox &rng, Emit &emit) {
+    // each node is popped at most once per pass; the cap guards a malformed tree
+    if (T.sp_ == 0 || T.pops >= 2 * T.num_nodes + 8) {
+        if (!T.cone_mode) {
+            if (S.stats) {
+                atomicAdd(&S.stats[5], (unsigned long long)T.n_inner);
+                atomicAdd(&S.stats[6], (unsigned long long)T.n_edges);
+            }
+            return false;
+        }
+        et_start_pass(T, false);
+        return true;
+    }
+    T.pops++;
+    TreeEntry e = T.stack[--T.sp_];
+    if (e.bound <= 0.f) return true;
+    const DrEdgeNode &node = T.nodes[e.node];
+    if (node.nprims > 0) {
+        float den = e.cone_pmf * e.cone_pmf + e.all_pmf * e.all_pmf;
+        float w = (T.cone_mode ? e.cone_pmf : e.all_pmf) / den;
+        T.n_edges += node.nprims;
+        for (int i = 0; i < node.nprims; i++) {
+            const DrEdge &ed = T.edges[node.offset + i];
+            if (is_silhouette(S, sp.pos, ed)) emit(ed, w, T.directional, node.offset + i);
+        }
+        return true;
+    }
+    int li = e.node + 1, ri = node.offset;
+    const DrEdgeNode &L = T.nodes[li], &R = T.nodes[ri];
+    T.n_inner++;
+    bool all_split = e.bound > 1.f || node_inside(node, sp.pos);
+    bool lcs = true, rcs = true;
+    if (T.directional) {
+        lcs = contains_silhouette(L, sp.pos);
+        rcs = contains_silhouette(R, sp.pos);
+    }
+    float lb = lcs ? bsdf_bound(S, L, wi, sp, T.roughness, T.dlum, T.slum) : 0.f;
+    float rb = rcs ? bsdf_bound(S, R, wi, sp, T.roughness, T.dlum, T.slum) : 0.f;
+    float il = lcs ? node_importance(L, lb, sp.pos) : 0.f;
+    float ir = rcs ? node_importance(R, rb, sp.pos) : 0.f;
+    if (il + ir <= 1e-10f) return true;
+    bool lhc = aabb_cone_intersect(L.lo, L.hi, T.cone);
+    bool rhc = aabb_cone_intersect(R.lo, R.hi, T.cone);
+    bool cone_split = all_split || (lhc && rhc && lcs && rcs && T.cone.cos_angle > 0.95f);
+    float ilc = lhc ? il : 0.f, irc = rhc ? ir : 0.f;
+    if (T.cone_mode && ilc + irc <= 1e-10f) return true;
+    float plc = 0.f, prc = 0.f, pla, pra;
+    if (cone_split) {
+        plc = prc = 1.f;
+    } else if (ilc + irc > 1e-10f) {
+        plc = ilc / (ilc + irc);
+        prc = 1.f - plc;
+    }
+    if (all_split) {
+        pla = pra = 1.f;
+    } else {
+        pla = il / (il + ir);
+        pra = 1.f - pla;
+    }
+    bool split = T.cone_mode ? cone_split : all_split;
+    TreeEntry le = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf};
+    TreeEntry re = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf};
+    if (split) {
+        if (rcs && T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = re;
+        if (lcs && T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = le;
+    } else {
+        float u = philox_next(rng);
+        bool left = T.cone_mode ? u < plc : u < pla;
+        if (T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = left ? le : re;
+    }
+    return true;
+}
+
+// Both passes at one shading point.
+template <class Emit>
+DR_INLINE void sample_shading_edges_idx(const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp, const Cone &cone,
+                                        Philox &rng, Emit &emit) {
+    EdgeTraversal T;
+    if (!et_init(T, S, wi, is, sp, cone, rng)) return;
+    while (et_step(T, S, wi, sp, rng, emit)) {
+    }
 }
 
 struct EdgeResult {

@@ -90,7 +90,9 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(DrSceneView S, SampleCtx c, i
                                                    long long t1, const float *d_image, float *image, float *grads,
                                                    PointRec *pts) {
     __shared__ float s_cam[64];
+    __shared__ GradCache gc;
     if (threadIdx.x < 64) s_cam[threadIdx.x] = 0.f;
+    gc_init(&gc);
     __syncthreads();
     long long t = t0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < t1) {
@@ -102,6 +104,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(DrSceneView S, SampleCtx c, i
         G.cam = s_cam;
         G.flags = c.flags;
         G.color_mode = true;
+        G.gc = &gc;
         F3 acc = F3{0.f, 0.f, 0.f};
         bool zero = is_zero(d_color);
         for (int k = 0; k < per; k++) {
@@ -123,21 +126,28 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(DrSceneView S, SampleCtx c, i
     }
     __syncthreads();
     if (threadIdx.x < 64 && s_cam[threadIdx.x] != 0.f) atomicAdd(grads + threadIdx.x, s_cam[threadIdx.x]);
+    gc_flush(&gc, grads);
 }
 
-// B: one thread per point slot; selected edges are appended to the queue in
-// groups of 8 per thread (one queue atomic per group)
+// B: edge selection, persistent.  Per-point traversal lengths vary by two
+// orders of magnitude (a point inside many node boxes splits at every level),
+// so a lane that has finished its point fetches the next one: idle lanes of a
+// wave take a batch of point slots from `work` with one atomic, and every
+// iteration advances each busy lane's traversal by one node.  Selected edges
+// are appended to the queue in groups of up to 8 per lane (one queue atomic
+// per group).  Every lane leaves the loop once `work` passes nslots.
 __global__ void __launch_bounds__(DR_BLOCK) k_select(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
                                                      long long nslots, int maxb, const PointRec *pts, EdgeEntry *q,
-                                                     unsigned long long *counter, long long cap, int *counts) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots) return;
-    const PointRec &p = pts[i];
-    if (!p.valid) {
-        counts[i] = 0;
-        return;
-    }
-    SlotDecode d = decode_slot(c, per, chunks, t0, i, maxb);
+                                                     unsigned long long *counter, long long cap, int *counts,
+                                                     unsigned long long *work) {
+    const int lane = (int)__lane_id();
+    EdgeTraversal T;
+    Philox prng;
+    SPoint sp;
+    F3 wi = F3{0.f, 0.f, 0.f};
+    long long slot = -1;
+    int j = 0;
+    bool more = true;
     EdgeEntry buf[8];
     int nbuf = 0;
     auto flush = [&]() {
@@ -146,26 +156,63 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(DrSceneView S, SampleCtx c,
             if ((long long)(base + e) < cap) q[base + e] = buf[e];
         nbuf = 0;
     };
-    auto emit = [&](bool nrm, int idx, float w, int j) {
+    auto emit = [&](const DrEdge &ed, float w, bool nrm, int idx) {
         // zero-weight edges carry no work (reference delta_ray.cpp:906-908)
-        if (w == 0.f) return;
-        buf[nbuf++] = EdgeEntry{(int)i, idx | (nrm ? (int)0x80000000 : 0), w, j};
-        if (nbuf == 8) flush();
+        if (w != 0.f) {
+            buf[nbuf++] = EdgeEntry{(int)slot, idx | (nrm ? (int)0x80000000 : 0), w, j};
+            if (nbuf == 8) flush();
+        }
+        j++;
     };
-    int n = stage_select(S, c, d.sid, d.b, p, emit);
-    if (nbuf > 0) flush();
-    counts[i] = n;
-    if (S.stats) atomicAdd(&S.stats[3], (unsigned long long)n);
+    while (true) {
+        bool want = slot < 0 && more;
+        unsigned long long idle = __ballot(want);
+        if (idle) {
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(idle));
+            base = __shfl(base, leader);
+            if (want) {
+                unsigned long long below = lane == 0 ? 0ull : (idle & (~0ull >> (64 - lane)));
+                long long s = (long long)(base + __popcll(below));
+                if (s >= nslots) {
+                    more = false;
+                } else {
+                    const PointRec &p = pts[s];
+                    bool busy = false;
+                    if (p.valid) {
+                        SlotDecode d = decode_slot(c, per, chunks, t0, s, maxb);
+                        busy = select_begin(S, c, d.sid, d.b, p, T, prng);
+                    }
+                    if (busy) {
+                        slot = s;
+                        sp = p.sp;
+                        wi = p.wi;
+                        j = 0;
+                    } else {
+                        counts[s] = 0;
+                    }
+                }
+            }
+        }
+        if (slot < 0) {
+            if (!more) break;
+            continue;
+        }
+        if (!et_step(T, S, wi, sp, prng, emit)) {
+            if (nbuf > 0) flush();
+            counts[slot] = j;
+            if (S.stats) atomicAdd(&S.stats[3], (unsigned long long)j);
+            slot = -1;
+        }
+    }
 }
 
 // C: one thread per selected edge
 template <int MAXB>
-__global__ void __launch_bounds__(DR_BLOCK) k_edge(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
-                                                   const EdgeEntry *q, long long n, const PointRec *pts,
-                                                   const float *d_image, float *grads, float *pt_dp, int *pt_lastj,
-                                                   float4 *entry_dp) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+DR_INLINE void edge_entry(DrSceneView S, SampleCtx c, int per, int chunks, long long t0, const EdgeEntry *q,
+                          long long e, const PointRec *pts, const float *d_image, float *grads, float *pt_dp,
+                          int *pt_lastj, float4 *entry_dp, GradCache *gcp) {
     EdgeEntry E = q[e];
     SlotDecode d = decode_slot(c, per, chunks, t0, E.slot, MAXB);
     const PointRec &p = pts[E.slot];
@@ -176,6 +223,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_edge(DrSceneView S, SampleCtx c, i
     G.cam = nullptr;
     G.flags = c.flags;
     G.color_mode = true;
+    G.gc = gcp;
     F3 d_color = load3(d_image + 3 * (size_t)d.pix);
     F3 dp = stage_edge<MAXB>(S, c, G, d.sid, d.b, p, edge, E.w, E.j, d_color);
     bool nz = !is_zero(dp);
@@ -187,6 +235,20 @@ __global__ void __launch_bounds__(DR_BLOCK) k_edge(DrSceneView S, SampleCtx c, i
         atomicAdd(pt_dp + 3ll * E.slot + 1, dp.y);
         atomicAdd(pt_dp + 3ll * E.slot + 2, dp.z);
     }
+}
+
+template <int MAXB>
+__global__ void __launch_bounds__(DR_BLOCK) k_edge(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
+                                                   const EdgeEntry *q, long long n, const PointRec *pts,
+                                                   const float *d_image, float *grads, float *pt_dp, int *pt_lastj,
+                                                   float4 *entry_dp) {
+    __shared__ GradCache gc;
+    gc_init(&gc);
+    __syncthreads();
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) edge_entry<MAXB>(S, c, per, chunks, t0, q, e, pts, d_image, grads, pt_dp, pt_lastj, entry_dp, &gc);
+    __syncthreads();
+    gc_flush(&gc, grads);
 }
 
 // reference set_adjoint overwrite: of several dp at one point the last
@@ -211,7 +273,9 @@ __global__ void __launch_bounds__(DR_BLOCK) k_dp(DrSceneView S, SampleCtx c, int
                                                  long long t1, const PointRec *pts, const float *pt_dp,
                                                  const int *counts, float *grads, float *dx_image) {
     __shared__ float s_cam[64];
+    __shared__ GradCache gc;
     if (threadIdx.x < 64) s_cam[threadIdx.x] = 0.f;
+    gc_init(&gc);
     __syncthreads();
     long long t = t0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < t1) {
@@ -222,6 +286,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_dp(DrSceneView S, SampleCtx c, int
         G.cam = s_cam;
         G.flags = c.flags;
         G.color_mode = false;
+        G.gc = &gc;
         float dx = 0.f;
         for (int k = 0; k < per; k++) {
             int s = ch * per + k;
@@ -248,6 +313,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_dp(DrSceneView S, SampleCtx c, int
     }
     __syncthreads();
     if (threadIdx.x < 64 && s_cam[threadIdx.x] != 0.f) atomicAdd(grads + threadIdx.x, s_cam[threadIdx.x]);
+    gc_flush(&gc, grads);
 }
 
 __global__ void k_fill_i(int *p, size_t n, int v) {
@@ -560,7 +626,7 @@ static dr_status ensure_slots(AdjBuffers &a, long long n) {
 
 static dr_status ensure_queue(AdjBuffers &a, long long n) {
     if (!a.counter) {
-        if (hipMalloc((void **)&a.counter, sizeof(unsigned long long)) != hipSuccess ||
+        if (hipMalloc((void **)&a.counter, 2 * sizeof(unsigned long long)) != hipSuccess ||
             hipHostMalloc((void **)&a.counter_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
             return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint queue counter");
     }
@@ -578,6 +644,23 @@ static unsigned grid_of(long long n) { return (unsigned)((n + DR_BLOCK - 1) / DR
 // samples per batch of the wavefront adjoint (bounds the point and edge
 // buffers; one host read of the edge count per batch)
 #define DR_BATCH_SAMPLES (1ll << 22)
+
+// Resident workgroups of the persistent selection kernel on this device.
+static long long select_blocks() {
+    static long long cached = 0;
+    if (cached == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_select, DR_BLOCK, 0) != hipSuccess ||
+            cus <= 0 || per_cu <= 0) {
+            cus = 256;
+            per_cu = 2;
+        }
+        cached = (long long)cus * per_cu;
+    }
+    return cached;
+}
 
 template <int MAXB>
 static dr_status launch_adjoint(dr_scene *s, const dr_render_options *o, const float *d_image, float *image,
@@ -611,11 +694,13 @@ static dr_status launch_adjoint(dr_scene *s, const dr_render_options *o, const f
         k_fill_i<<<1024, 256, 0, hs>>>(A.pt_lastj, (size_t)nslots, -1);
         long long nedges = 0;
         for (int attempt = 0; attempt < 2; attempt++) {
-            k_fill<<<1, 64, 0, hs>>>((float *)A.counter, 2, 0.f);  // 8-byte counter = two zero floats
+            // queue counter and work counter: two 8-byte zeros
+            k_fill<<<1, 64, 0, hs>>>((float *)A.counter, 4, 0.f);
             {
                 LaunchTimer timer(s, TK_SELECT, hs);
-                k_select<<<grid_of(nslots), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, nslots, MAXB, A.pts, A.q,
-                                                               A.counter, A.q_cap, A.counts);
+                long long blocks = std::min<long long>(grid_of(nslots), select_blocks());
+                k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, nslots, MAXB, A.pts,
+                                                                A.q, A.counter, A.q_cap, A.counts, A.counter + 1);
             }
             if (hipMemcpyAsync(A.counter_host, A.counter, sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) !=
                     hipSuccess ||
@@ -755,6 +840,8 @@ dr_status dr_scene_stats(dr_scene *s, int enable, dr_stats *out) {
         out->tris_visited = h[2];
         out->secondary_edges = h[3];
         out->primary_edge_samples = h[4];
+        out->edge_nodes_visited = h[5];
+        out->edges_tested = h[6];
         for (auto &t : s->timed) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) ms = 0.f;

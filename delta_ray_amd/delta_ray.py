@@ -326,7 +326,7 @@ class DeviceScene:
         _abi.check(self.lib, self.lib.dr_scene_stats(self.handle, mask, ctypes.byref(st)),
                    "dr_scene_stats")
         out = {k: getattr(st, k) for k in ("rays", "nodes_visited", "tris_visited", "secondary_edges",
-                                           "primary_edge_samples")}
+                                           "primary_edge_samples", "edge_nodes_visited", "edges_tested")}
         for i, k in enumerate(_abi.KERNELS):
             out[k + "_ms"] = st.kernel_ms[i]
             out[k + "_launches"] = st.kernel_launches[i]

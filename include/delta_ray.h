@@ -273,6 +273,8 @@ typedef struct dr_stats {
     uint64_t tris_visited;
     uint64_t secondary_edges;
     uint64_t primary_edge_samples;
+    uint64_t edge_nodes_visited;           /* interior edge-tree nodes expanded */
+    uint64_t edges_tested;                 /* leaf edges given the silhouette test */
     double kernel_ms[DR_K_COUNT];          /* summed kernel times */
     uint64_t kernel_launches[DR_K_COUNT];
 } dr_stats;
