@@ -7,7 +7,11 @@ RenderFunction.backward call: pixel samples with interior adjoint and
 secondary edges, then spp*W*H primary-edge samples), with scene, d_image and
 buffers resident on the GPU before the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--workload heightfield|shadow|single_triangle]
+    python bench.py [--steps K] [--warmup W] [--workload heightfield|shadow|single_triangle|batched]
+
+`--workload batched` times the uneven batch instead (64 scenes of 1k-200k
+triangles x 4 views, forward + adjoint each, one graph-captured batch per
+step, pass sizes from the autotuner).
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the one with
 the most device time per step): algorithmic bytes per launch (BVH traversal +
@@ -47,6 +51,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="heightfield")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched workload")
+    ap.add_argument("--no-autotune", action="store_true", help="batched workload: default pass sizes")
     ap.add_argument("--cpu-spp", type=int, default=16, help="spp of the bounded CPU sample")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known HBM traffic: "
@@ -102,12 +108,73 @@ def cpu_baseline(w, d_img, spp, threads):
     return spp * W * H / dt, dt
 
 
+def bench_batched(args):
+    """Uneven batch: scenes x 4 views, each view forward + adjoint, all in one
+    graph-captured launch sequence per step (pass sizes from the autotuner)."""
+    import numpy as np
+    import torch
+    from delta_ray_amd import delta_ray as dr
+    from delta_ray_amd import workloads
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    scenes = workloads.batched(num_scenes=args.batch_scenes)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    keep, jobs = [], []
+    rng = np.random.default_rng(5)
+    samples = 0
+    for sc in scenes:
+        ds = dr.DeviceScene(sc["views"][0][0], sc["shapes"], sc["materials"], sc["lights"])
+        grads = torch.zeros((ds.grad_size,), dtype=torch.float32, device=dev)
+        keep.append(ds)
+        for vi, (cam, (W, H), spp) in enumerate(sc["views"]):
+            view = None if vi == 0 else dr.View(ds, cam)
+            keep.append(view)
+            d_img = torch.from_numpy(rng.normal(size=(H, W, 3)).astype(np.float32)).to(dev)
+            img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+            jobs.append(dict(scene=ds, view=view, options=dr.DeviceScene.options(W, H, spp, sc["max_bounces"], 0),
+                             image=img, d_image=d_img, grads=grads))
+            samples += W * H * spp
+    build_s = time.perf_counter() - t0
+    batch = dr.Batch()
+    # the autotuner times 12 candidates: run it on every 8th job (a
+    # representative eighth of the batch) and keep its choice for all jobs
+    tune = None if args.no_autotune else batch.autotune(jobs[::8])
+
+    def step(i):
+        for k, j in enumerate(jobs):
+            j["options"].seed = 1000003 * i + k
+        batch.render(jobs, graph=True)
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    tris = [sc["num_triangles"] for sc in scenes]
+    out = {"metric": METRIC, "value": round(samples / dt, 1), "unit": "samples/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1000.0, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded procedural scenes, random d_image)",
+           "config": {"workload": "batched: %d scenes (%d-%d triangles, %d total) x %d views, 128-512 px, "
+                                  "16-64 spp, 2 bounces, forward + adjoint per view, one hipGraph per step" %
+                                  (len(scenes), min(tris), max(tris), sum(tris), len(scenes[0]["views"])),
+                      "samples_per_step": samples, "jobs": len(jobs), "parallelism": "single GPU"},
+           "extra": {"tuning": tune, "scene_generation_s": round(gen_s, 2), "scene_build_s": round(build_s, 2)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: multi-GPU runs are not part of this build "
               "(the path would shard pixel samples per GPU and all-reduce the gradient buffer)")
         sys.exit(2)
+    if args.workload == "batched":
+        bench_batched(args)
+        return
     import numpy as np
     import torch
     from delta_ray_amd import _abi

@@ -82,6 +82,19 @@ class dr_render_options(ctypes.Structure):
                 ("num_primary_edge_samples", ctypes.c_int64)]
 
 
+class dr_job(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("view", ctypes.c_void_p), ("opts", dr_render_options),
+                ("d_image", ctypes.c_void_p), ("image", ctypes.c_void_p), ("grads", ctypes.c_void_p),
+                ("dx_image", ctypes.c_void_p)]
+
+
+class dr_tuning(ctypes.Structure):
+    _fields_ = [("pass_samples", ctypes.c_int64), ("min_threads", ctypes.c_int64), ("best_ms", ctypes.c_double),
+                ("candidates", ctypes.c_int)]
+
+
+DR_BATCH_GRAPH = 1
+
 DR_STATS_COUNTERS = 1
 DR_STATS_TIMING = 2
 
@@ -120,6 +133,15 @@ ENTRY_POINTS = [
                                     ctypes.c_int, ctypes.c_void_p]),
     ("dr_scene_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(dr_stats)]),
     ("dr_scene_get_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_scene_info)]),
+    ("dr_view_create", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_camera), ctypes.POINTER(ctypes.c_void_p)]),
+    ("dr_view_destroy", None, [ctypes.c_void_p]),
+    ("dr_batch_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    ("dr_batch_destroy", None, [ctypes.c_void_p]),
+    ("dr_render_batch", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_job), ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p]),
+    ("dr_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_job), ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.POINTER(dr_tuning)]),
+    ("dr_batch_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_tuning)]),
     ("dr_test_line_integral", ctypes.c_int, [ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
                                              ctypes.c_void_p]),
     ("dr_last_error", ctypes.c_char_p, []),

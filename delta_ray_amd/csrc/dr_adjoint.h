@@ -17,28 +17,26 @@
 // Workgroup gradient cache in LDS.  Neighbouring pixel samples of one
 // workgroup hit the same triangles and texels, so their adjoints go to the
 // same few hundred gradient entries: they are summed here with LDS atomics
-// (open addressing on the flat gradient index, linear probing) and reach HBM
-// as one global atomic per distinct entry when the workgroup ends.  An entry
-// that finds no free slot within DR_GC_PROBES goes straight to HBM.
-#define DR_GC_LOG2 12
-#define DR_GC_SLOTS (1 << DR_GC_LOG2)
+// (open addressing, linear probing) and reach HBM as one global atomic per
+// distinct entry when the workgroup ends.  Keys are (job, flat index), so one
+// workgroup may serve several renders of a batch.  An adjoint that finds no
+// free slot within DR_GC_PROBES goes straight to HBM.
 #define DR_GC_PROBES 8
-struct GradCache {
-    int key[DR_GC_SLOTS];
-    float val[DR_GC_SLOTS];
-};
+#define DR_GC_EMPTY 0xffffffffffffffffull
 // every thread of the workgroup calls these, followed / preceded by a barrier
-DR_INLINE void gc_init(GradCache *gc) {
-    for (int i = threadIdx.x; i < DR_GC_SLOTS; i += blockDim.x) {
-        gc->key[i] = -1;
-        gc->val[i] = 0.f;
+DR_INLINE void gc_init(unsigned long long *key, float *val, int slots) {
+    for (int i = threadIdx.x; i < slots; i += blockDim.x) {
+        key[i] = DR_GC_EMPTY;
+        val[i] = 0.f;
     }
 }
-DR_INLINE void gc_flush(GradCache *gc, float *grads) {
-    for (int i = threadIdx.x; i < DR_GC_SLOTS; i += blockDim.x) {
-        int k = gc->key[i];
-        float v = gc->val[i];
-        if (k >= 0 && v != 0.f) atomicAdd(grads + k, v);
+// grads_of(job) -> that job's gradient buffer
+template <class GradsOf>
+DR_INLINE void gc_flush(const unsigned long long *key, const float *val, int slots, GradsOf grads_of) {
+    for (int i = threadIdx.x; i < slots; i += blockDim.x) {
+        unsigned long long k = key[i];
+        float v = val[i];
+        if (k != DR_GC_EMPTY && v != 0.f) atomicAdd(grads_of((int)(k >> 32)) + (k & 0xffffffffull), v);
     }
 }
 
@@ -48,16 +46,20 @@ struct GradSink {
     float *cam;        // 64 floats (LDS) for camera matrices
     uint32_t flags;
     bool color_mode;   // false: Dirac position sweep (vertices + camera only)
-    GradCache *gc = nullptr;  // LDS cache of the workgroup, or none
+    unsigned long long *gkey = nullptr;  // LDS cache of the workgroup, or none
+    float *gval = nullptr;
+    int gmask = 0;
+    unsigned job = 0;
     DR_INLINE void add(long long idx, float v) const {
         if (v == 0.f) return;
-        if (gc && idx < 0x7fffffffll) {
-            uint32_t h = ((uint32_t)idx * 2654435761u) >> (32 - DR_GC_LOG2);
+        if (gkey && idx < 0x100000000ll) {
+            unsigned long long key = ((unsigned long long)job << 32) | (unsigned long long)idx;
+            uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40);
             for (int p = 0; p < DR_GC_PROBES; p++) {
-                int slot = (int)((h + p) & (DR_GC_SLOTS - 1));
-                int old = atomicCAS(&gc->key[slot], -1, (int)idx);
-                if (old == -1 || old == (int)idx) {
-                    atomicAdd(&gc->val[slot], v);
+                int slot = (int)((h + p) & (uint32_t)gmask);
+                unsigned long long old = atomicCAS(&gkey[slot], DR_GC_EMPTY, key);
+                if (old == DR_GC_EMPTY || old == key) {
+                    atomicAdd(&gval[slot], v);
                     return;
                 }
             }

@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -39,18 +40,64 @@ __global__ void k_fill(float *p, size_t n, float v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-// thread -> (pixel, chunk of `per` samples)
-__global__ void __launch_bounds__(DR_BLOCK) k_forward(DrSceneView S, SampleCtx c, int per, int chunks, float *image) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long total = (long long)c.W * c.H * chunks;
-    if (t >= total) return;
-    int pix = (int)(t / chunks), ch = (int)(t % chunks);
+// ------------------------------------------------------- jobs, ranges ----
+// A job is one render: a scene seen by one camera with its options and its
+// buffers.  A launch covers a list of ranges, each a contiguous piece of one
+// job's threads; ranges are ordered and carry prefix offsets of their
+// workgroups and point slots, so any mix of jobs (scenes, views, sample
+// counts of uneven sizes) runs as one launch per stage.  Workgroups never
+// straddle two ranges; a workgroup finds its range by binary search.
+struct DrJob {
+    DrSceneView S;          // the scene, with this job's camera and primary edges
+    SampleCtx c;
+    int per, chunks;        // samples per thread, threads per pixel
+    long long prim_n;       // primary-edge samples
+    const float *d_image;   // NULL: forward only
+    float *image, *grads, *dx;
+};
+struct DrRange {
+    int job, pad;
+    long long t0, t1;       // (pixel, chunk) threads of the job, or primary-edge samples
+    long long slot0;        // first point slot of the range in its pass
+    long long block0;       // first workgroup of the range in its launch
+};
+DR_INLINE int range_of_block(const DrRange *r, int n, long long b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (r[mid].block0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+DR_INLINE int range_of_slot(const DrRange *r, int n, long long s) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (r[mid].slot0 <= s) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define DR_GC_SLOTS_BIG 4096
+#define DR_GC_SLOTS_SMALL 2048
+
+// thread -> (pixel, chunk of `per` samples) of a forward-only job
+__global__ void __launch_bounds__(DR_BLOCK) k_forward(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
+                                                      int nr) {
+    const DrRange rg = R[range_of_block(R, nr, blockIdx.x)];
+    const DrJob &J = jobs[rg.job];
+    long long t = rg.t0 + (long long)(blockIdx.x - rg.block0) * blockDim.x + threadIdx.x;
+    if (t >= rg.t1) return;
+    const SampleCtx &c = J.c;
+    int pix = (int)(t / J.chunks), ch = (int)(t % J.chunks);
     int xi = pix % c.W, yi = pix / c.W;
     F3 acc = F3{0.f, 0.f, 0.f};
-    int s0 = ch * per, s1 = min(s0 + per, c.spp);
-    for (int s = s0; s < s1; s++) acc += forward_sample(S, c, xi, yi, s);
-    float *px = image + 3 * (size_t)pix;
-    if (chunks == 1) {
+    int s0 = ch * J.per, s1 = min(s0 + J.per, c.spp);
+    for (int s = s0; s < s1; s++) acc += forward_sample(J.S, c, xi, yi, s);
+    float *px = J.image + 3 * (size_t)pix;
+    if (J.chunks == 1) {
         px[0] = acc.x; px[1] = acc.y; px[2] = acc.z;
     } else {
         atomicAdd(px, acc.x); atomicAdd(px + 1, acc.y); atomicAdd(px + 2, acc.z);
@@ -58,66 +105,78 @@ __global__ void __launch_bounds__(DR_BLOCK) k_forward(DrSceneView S, SampleCtx c
 }
 
 // ---- wavefront adjoint: path (A) -> edge selection (B) -> edge samples (C)
-// -> Dirac position sweep (D).  A thread index t of A and D enumerates
-// (pixel, chunk of `per` samples); a point slot is ((t - t0) * per + k) * MAXB + b.
+// -> Dirac position sweep (D), over the ranges of one pass.  A thread of A
+// and D is one (pixel, chunk) of a job; its point slots are
+// slot0 + ((t - t0) * per + k) * MAXB + b.
 struct EdgeEntry {
-    int slot;
+    int slot;  // point slot in the pass
     int edge;  // index into the tree's edge list; bit 31 set: normal tree
     float w;
     int j;     // ordinal of the edge in the point's selection
 };
 
 struct SlotDecode {
-    int pix, xi, yi, s, b;
+    int job, pix, xi, yi, s, b;
     uint64_t sid;
 };
-DR_INLINE SlotDecode decode_slot(const SampleCtx &c, int per, int chunks, long long t0, long long slot, int maxb) {
+DR_INLINE SlotDecode decode_slot(const DrJob *jobs, const DrRange *R, int nr, long long slot, int maxb) {
     SlotDecode d;
-    d.b = (int)(slot % maxb);
-    long long rest = slot / maxb;
-    int k = (int)(rest % per);
-    long long t = t0 + rest / per;
-    d.pix = (int)(t / chunks);
-    d.s = (int)(t % chunks) * per + k;
-    d.xi = d.pix % c.W;
-    d.yi = d.pix / c.W;
-    d.sid = sample_id(c, d.xi, d.yi, d.s);
+    const DrRange &rg = R[range_of_slot(R, nr, slot)];
+    const DrJob &J = jobs[rg.job];
+    long long local = slot - rg.slot0;
+    d.job = rg.job;
+    d.b = (int)(local % maxb);
+    long long rest = local / maxb;
+    int k = (int)(rest % J.per);
+    long long t = rg.t0 + rest / J.per;
+    d.pix = (int)(t / J.chunks);
+    d.s = (int)(t % J.chunks) * J.per + k;
+    d.xi = d.pix % J.c.W;
+    d.yi = d.pix / J.c.W;
+    d.sid = sample_id(J.c, d.xi, d.yi, d.s);
     return d;
 }
 
 template <int MAXB>
-__global__ void __launch_bounds__(DR_BLOCK) k_path(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
-                                                   long long t1, const float *d_image, float *image, float *grads,
-                                                   PointRec *pts) {
+__global__ void __launch_bounds__(DR_BLOCK) k_path(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
+                                                   int nr, PointRec *pts) {
     __shared__ float s_cam[64];
-    __shared__ GradCache gc;
+    __shared__ unsigned long long gkey[DR_GC_SLOTS_BIG];
+    __shared__ float gval[DR_GC_SLOTS_BIG];
     if (threadIdx.x < 64) s_cam[threadIdx.x] = 0.f;
-    gc_init(&gc);
+    gc_init(gkey, gval, DR_GC_SLOTS_BIG);
     __syncthreads();
-    long long t = t0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < t1) {
+    const DrRange rg = R[range_of_block(R, nr, blockIdx.x)];
+    const DrJob &J = jobs[rg.job];
+    long long t = rg.t0 + (long long)(blockIdx.x - rg.block0) * blockDim.x + threadIdx.x;
+    if (t < rg.t1) {
+        const SampleCtx &c = J.c;
+        const int per = J.per, chunks = J.chunks;
         int pix = (int)(t / chunks), ch = (int)(t % chunks);
         int xi = pix % c.W, yi = pix / c.W;
-        F3 d_color = load3(d_image + 3 * (size_t)pix);
+        F3 d_color = load3(J.d_image + 3 * (size_t)pix);
         GradSink G;
-        G.grads = grads;
+        G.grads = J.grads;
         G.cam = s_cam;
         G.flags = c.flags;
         G.color_mode = true;
-        G.gc = &gc;
+        G.gkey = gkey;
+        G.gval = gval;
+        G.gmask = DR_GC_SLOTS_BIG - 1;
+        G.job = (unsigned)rg.job;
         F3 acc = F3{0.f, 0.f, 0.f};
         bool zero = is_zero(d_color);
         for (int k = 0; k < per; k++) {
             int s = ch * per + k;
-            PointRec *P = pts + ((t - t0) * per + k) * MAXB;
+            PointRec *P = pts + rg.slot0 + ((t - rg.t0) * per + k) * MAXB;
             if (s >= c.spp || zero) {
                 for (int b = 0; b < MAXB; b++) P[b].valid = 0;
-                if (s < c.spp) acc += forward_sample(S, c, xi, yi, s);
+                if (s < c.spp) acc += forward_sample(J.S, c, xi, yi, s);
                 continue;
             }
-            acc += stage_path<MAXB>(S, c, G, xi, yi, s, d_color, P);
+            acc += stage_path<MAXB>(J.S, c, G, xi, yi, s, d_color, P);
         }
-        float *px = image + 3 * (size_t)pix;
+        float *px = J.image + 3 * (size_t)pix;
         if (chunks == 1) {
             px[0] = acc.x; px[1] = acc.y; px[2] = acc.z;
         } else {
@@ -125,8 +184,8 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(DrSceneView S, SampleCtx c, i
         }
     }
     __syncthreads();
-    if (threadIdx.x < 64 && s_cam[threadIdx.x] != 0.f) atomicAdd(grads + threadIdx.x, s_cam[threadIdx.x]);
-    gc_flush(&gc, grads);
+    if (threadIdx.x < 64 && s_cam[threadIdx.x] != 0.f) atomicAdd(J.grads + threadIdx.x, s_cam[threadIdx.x]);
+    gc_flush(gkey, gval, DR_GC_SLOTS_BIG, [&](int j) { return jobs[j].grads; });
 }
 
 // B: edge selection, persistent.  Per-point traversal lengths vary by two
@@ -136,22 +195,25 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(DrSceneView S, SampleCtx c, i
 // iteration advances each busy lane's traversal by one node.  Selected edges
 // are appended to the queue in groups of up to 8 per lane (one queue atomic
 // per group).  Every lane leaves the loop once `work` passes nslots.
-__global__ void __launch_bounds__(DR_BLOCK) k_select(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
-                                                     long long nslots, int maxb, const PointRec *pts, EdgeEntry *q,
-                                                     unsigned long long *counter, long long cap, int *counts,
-                                                     unsigned long long *work) {
+// cnt[0]: queue length (may exceed cap: the host grows the queue and renders
+// again), cnt[1]: work counter.
+__global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
+                                                     int nr, long long nslots, int maxb, const PointRec *pts,
+                                                     EdgeEntry *q, unsigned long long *cnt, long long cap,
+                                                     int *counts) {
     const int lane = (int)__lane_id();
     EdgeTraversal T;
     Philox prng;
     SPoint sp;
     F3 wi = F3{0.f, 0.f, 0.f};
+    const DrSceneView *Sp = &jobs[0].S;
     long long slot = -1;
     int j = 0;
     bool more = true;
     EdgeEntry buf[8];
     int nbuf = 0;
     auto flush = [&]() {
-        unsigned long long base = atomicAdd(counter, (unsigned long long)nbuf);
+        unsigned long long base = atomicAdd(&cnt[0], (unsigned long long)nbuf);
         for (int e = 0; e < nbuf; e++)
             if ((long long)(base + e) < cap) q[base + e] = buf[e];
         nbuf = 0;
@@ -170,7 +232,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(DrSceneView S, SampleCtx c,
         if (idle) {
             int leader = __ffsll((long long)idle) - 1;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(idle));
+            if (lane == leader) base = atomicAdd(&cnt[1], (unsigned long long)__popcll(idle));
             base = __shfl(base, leader);
             if (want) {
                 unsigned long long below = lane == 0 ? 0ull : (idle & (~0ull >> (64 - lane)));
@@ -181,8 +243,11 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(DrSceneView S, SampleCtx c,
                     const PointRec &p = pts[s];
                     bool busy = false;
                     if (p.valid) {
-                        SlotDecode d = decode_slot(c, per, chunks, t0, s, maxb);
-                        busy = select_begin(S, c, d.sid, d.b, p, T, prng);
+                        SlotDecode d = decode_slot(jobs, R, nr, s, maxb);
+                        const DrJob &J = jobs[d.job];
+                        Sp = &J.S;
+                        if (!(J.c.flags & DR_SKIP_SECONDARY_EDGES))
+                            busy = select_begin(*Sp, J.c, d.sid, d.b, p, T, prng);
                     }
                     if (busy) {
                         slot = s;
@@ -199,99 +264,123 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(DrSceneView S, SampleCtx c,
             if (!more) break;
             continue;
         }
-        if (!et_step(T, S, wi, sp, prng, emit)) {
+        if (!et_step(T, *Sp, wi, sp, prng, emit)) {
             if (nbuf > 0) flush();
             counts[slot] = j;
-            if (S.stats) atomicAdd(&S.stats[3], (unsigned long long)j);
+            if (Sp->stats) atomicAdd(&Sp->stats[3], (unsigned long long)j);
             slot = -1;
         }
     }
 }
 
-// C: one thread per selected edge
+// C: one selected edge per iteration of a grid-stride loop over the queue
+// (its length is read on the device, so no host round trip is needed).
 template <int MAXB>
-DR_INLINE void edge_entry(DrSceneView S, SampleCtx c, int per, int chunks, long long t0, const EdgeEntry *q,
-                          long long e, const PointRec *pts, const float *d_image, float *grads, float *pt_dp,
-                          int *pt_lastj, float4 *entry_dp, GradCache *gcp) {
-    EdgeEntry E = q[e];
-    SlotDecode d = decode_slot(c, per, chunks, t0, E.slot, MAXB);
+DR_INLINE void edge_entry(const DrJob *jobs, const DrRange *R, int nr, const EdgeEntry &E, long long e,
+                          const PointRec *pts, float *pt_dp, int *pt_lastj, float4 *entry_dp,
+                          unsigned long long *gkey, float *gval) {
+    SlotDecode d = decode_slot(jobs, R, nr, E.slot, MAXB);
+    const DrJob &J = jobs[d.job];
     const PointRec &p = pts[E.slot];
     int idx = E.edge & 0x7fffffff;
-    const DrEdge &edge = (E.edge & 0x80000000) ? S.nrm_edges[idx] : S.sil_edges[idx];
+    const DrEdge &edge = (E.edge & 0x80000000) ? J.S.nrm_edges[idx] : J.S.sil_edges[idx];
     GradSink G;
-    G.grads = grads;
+    G.grads = J.grads;
     G.cam = nullptr;
-    G.flags = c.flags;
+    G.flags = J.c.flags;
     G.color_mode = true;
-    G.gc = gcp;
-    F3 d_color = load3(d_image + 3 * (size_t)d.pix);
-    F3 dp = stage_edge<MAXB>(S, c, G, d.sid, d.b, p, edge, E.w, E.j, d_color);
+    G.gkey = gkey;
+    G.gval = gval;
+    G.gmask = DR_GC_SLOTS_BIG - 1;
+    G.job = (unsigned)d.job;
+    F3 d_color = load3(J.d_image + 3 * (size_t)d.pix);
+    F3 dp = stage_edge<MAXB>(J.S, J.c, G, d.sid, d.b, p, edge, E.w, E.j, d_color);
     bool nz = !is_zero(dp);
-    if (c.flags & DR_QUIRK_DP_LAST) {
+    if (J.c.flags & DR_QUIRK_DP_LAST) {
         entry_dp[e] = make_float4(dp.x, dp.y, dp.z, nz ? __int_as_float(E.j) : __int_as_float(-1));
         if (nz) atomicMax(pt_lastj + E.slot, E.j);
-    } else if (nz) {
-        atomicAdd(pt_dp + 3ll * E.slot, dp.x);
-        atomicAdd(pt_dp + 3ll * E.slot + 1, dp.y);
-        atomicAdd(pt_dp + 3ll * E.slot + 2, dp.z);
+    } else {
+        // a pass may mix jobs with and without the last-dp rule: mark the
+        // entry as not taking part in it
+        entry_dp[e] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+        if (nz) {
+            atomicAdd(pt_dp + 3ll * E.slot, dp.x);
+            atomicAdd(pt_dp + 3ll * E.slot + 1, dp.y);
+            atomicAdd(pt_dp + 3ll * E.slot + 2, dp.z);
+        }
     }
 }
 
+// cnt[2] keeps the largest queue length seen, for the host's overflow check
 template <int MAXB>
-__global__ void __launch_bounds__(DR_BLOCK) k_edge(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
-                                                   const EdgeEntry *q, long long n, const PointRec *pts,
-                                                   const float *d_image, float *grads, float *pt_dp, int *pt_lastj,
+__global__ void __launch_bounds__(DR_BLOCK) k_edge(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
+                                                   int nr, const EdgeEntry *q, unsigned long long *cnt, long long cap,
+                                                   const PointRec *pts, float *pt_dp, int *pt_lastj,
                                                    float4 *entry_dp) {
-    __shared__ GradCache gc;
-    gc_init(&gc);
+    __shared__ unsigned long long gkey[DR_GC_SLOTS_BIG];
+    __shared__ float gval[DR_GC_SLOTS_BIG];
+    gc_init(gkey, gval, DR_GC_SLOTS_BIG);
     __syncthreads();
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) edge_entry<MAXB>(S, c, per, chunks, t0, q, e, pts, d_image, grads, pt_dp, pt_lastj, entry_dp, &gc);
+    unsigned long long len = cnt[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt[2], len);
+    long long n = (long long)len < cap ? (long long)len : cap;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
+        edge_entry<MAXB>(jobs, R, nr, q[e], e, pts, pt_dp, pt_lastj, entry_dp, gkey, gval);
     __syncthreads();
-    gc_flush(&gc, grads);
+    gc_flush(gkey, gval, DR_GC_SLOTS_BIG, [&](int j) { return jobs[j].grads; });
 }
 
 // reference set_adjoint overwrite: of several dp at one point the last
 // (largest ordinal) is the one propagated
-__global__ void k_dp_last(const EdgeEntry *q, long long n, const float4 *entry_dp, const int *pt_lastj,
-                          float *pt_dp) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    float4 v = entry_dp[e];
-    int j = __float_as_int(v.w);
-    int slot = q[e].slot;
-    if (j >= 0 && j == pt_lastj[slot]) {
-        pt_dp[3ll * slot] = v.x;
-        pt_dp[3ll * slot + 1] = v.y;
-        pt_dp[3ll * slot + 2] = v.z;
+__global__ void k_dp_last(const EdgeEntry *q, const unsigned long long *cnt, long long cap, const float4 *entry_dp,
+                          const int *pt_lastj, float *pt_dp) {
+    long long n = (long long)cnt[0] < cap ? (long long)cnt[0] : cap;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        float4 v = entry_dp[e];
+        int j = __float_as_int(v.w);
+        int slot = q[e].slot;
+        if (j >= 0 && j == pt_lastj[slot]) {
+            pt_dp[3ll * slot] = v.x;
+            pt_dp[3ll * slot + 1] = v.y;
+            pt_dp[3ll * slot + 2] = v.z;
+        }
     }
 }
 
 // D: Dirac position sweep per pixel sample, and the dx_image edge counts
 template <int MAXB>
-__global__ void __launch_bounds__(DR_BLOCK) k_dp(DrSceneView S, SampleCtx c, int per, int chunks, long long t0,
-                                                 long long t1, const PointRec *pts, const float *pt_dp,
-                                                 const int *counts, float *grads, float *dx_image) {
+__global__ void __launch_bounds__(DR_BLOCK) k_dp(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R, int nr,
+                                                 const PointRec *pts, const float *pt_dp, const int *counts) {
     __shared__ float s_cam[64];
-    __shared__ GradCache gc;
+    __shared__ unsigned long long gkey[DR_GC_SLOTS_SMALL];
+    __shared__ float gval[DR_GC_SLOTS_SMALL];
     if (threadIdx.x < 64) s_cam[threadIdx.x] = 0.f;
-    gc_init(&gc);
+    gc_init(gkey, gval, DR_GC_SLOTS_SMALL);
     __syncthreads();
-    long long t = t0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < t1) {
+    const DrRange rg = R[range_of_block(R, nr, blockIdx.x)];
+    const DrJob &J = jobs[rg.job];
+    long long t = rg.t0 + (long long)(blockIdx.x - rg.block0) * blockDim.x + threadIdx.x;
+    if (t < rg.t1) {
+        const SampleCtx &c = J.c;
+        const int per = J.per, chunks = J.chunks;
         int pix = (int)(t / chunks), ch = (int)(t % chunks);
         int xi = pix % c.W, yi = pix / c.W;
         GradSink G;
-        G.grads = grads;
+        G.grads = J.grads;
         G.cam = s_cam;
         G.flags = c.flags;
         G.color_mode = false;
-        G.gc = &gc;
+        G.gkey = gkey;
+        G.gval = gval;
+        G.gmask = DR_GC_SLOTS_SMALL - 1;
+        G.job = (unsigned)rg.job;
         float dx = 0.f;
         for (int k = 0; k < per; k++) {
             int s = ch * per + k;
             if (s >= c.spp) break;
-            long long base = ((t - t0) * per + k) * MAXB;
+            long long base = rg.slot0 + ((t - rg.t0) * per + k) * MAXB;
             const PointRec *P = pts + base;
             F3 dps[MAXB];
             int last = -1;
@@ -300,20 +389,20 @@ __global__ void __launch_bounds__(DR_BLOCK) k_dp(DrSceneView S, SampleCtx c, int
                 if (P[b].valid) last = b;
             }
             if (last >= 0) dx += counts[base + last] / 1000.f;
-            stage_dp<MAXB>(S, c, G, xi, yi, s, P, dps);
+            stage_dp<MAXB>(J.S, c, G, xi, yi, s, P, dps);
         }
-        if (dx_image) {
-            float *q = dx_image + 3 * (size_t)pix;
+        if (J.dx) {
+            float *qx = J.dx + 3 * (size_t)pix;
             if (chunks == 1) {
-                q[0] = q[1] = q[2] = dx;
+                qx[0] = qx[1] = qx[2] = dx;
             } else {
-                atomicAdd(q, dx); atomicAdd(q + 1, dx); atomicAdd(q + 2, dx);
+                atomicAdd(qx, dx); atomicAdd(qx + 1, dx); atomicAdd(qx + 2, dx);
             }
         }
     }
     __syncthreads();
-    if (threadIdx.x < 64 && s_cam[threadIdx.x] != 0.f) atomicAdd(grads + threadIdx.x, s_cam[threadIdx.x]);
-    gc_flush(&gc, grads);
+    if (threadIdx.x < 64 && s_cam[threadIdx.x] != 0.f) atomicAdd(J.grads + threadIdx.x, s_cam[threadIdx.x]);
+    gc_flush(gkey, gval, DR_GC_SLOTS_SMALL, [&](int j) { return jobs[j].grads; });
 }
 
 __global__ void k_fill_i(int *p, size_t n, int v) {
@@ -322,14 +411,19 @@ __global__ void k_fill_i(int *p, size_t n, int v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// primary-edge samples; a range is a piece [t0, t1) of one job's samples
 template <int MAXB>
-__global__ void __launch_bounds__(DR_BLOCK) k_primary(DrSceneView S, SampleCtx c, long long N, const float *d_image,
-                                                      float *grads) {
+__global__ void __launch_bounds__(DR_BLOCK) k_primary(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
+                                                      int nr) {
     __shared__ float s_w2c[16];
     if (threadIdx.x < 16) s_w2c[threadIdx.x] = 0.f;
     __syncthreads();
-    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < N && S.num_prim_edges > 0) {
+    const DrRange rg = R[range_of_block(R, nr, blockIdx.x)];
+    const DrJob &J = jobs[rg.job];
+    const DrSceneView &S = J.S;
+    const SampleCtx &c = J.c;
+    long long k = rg.t0 + (long long)(blockIdx.x - rg.block0) * blockDim.x + threadIdx.x;
+    if (k < rg.t1 && S.num_prim_edges > 0) {
         Philox rng;
         philox_init(rng, c.seed, 1, (uint64_t)k, DR_STREAM_MAIN);
         float u = philox_next(rng);
@@ -343,24 +437,25 @@ __global__ void __launch_bounds__(DR_BLOCK) k_primary(DrSceneView S, SampleCtx c
             SecondarySample ss[MAXB];
             for (int b = 0; b < c.nb; b++) ss[b] = draw_secondary(rng);
             const DrEdge &e = S.prim_edges[id];
-            PrimaryResult R;
-            sample_primary_edge_point(S, e, eu, ls, ss, c.nb, d_image, c.W, c.H, w / c.spp, c.flags, R);
-            if (R.ok) {
+            PrimaryResult Rs;
+            sample_primary_edge_point(S, e, eu, ls, ss, c.nb, J.d_image, c.W, c.H, w / c.spp, c.flags, Rs);
+            if (Rs.ok) {
+                float *grads = J.grads;
                 const DrShape &es = S.shapes[e.shape];
-                if (R.dv0.x != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0, R.dv0.x);
-                if (R.dv0.y != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0 + 1, R.dv0.y);
-                if (R.dv0.z != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0 + 2, R.dv0.z);
-                if (R.dv1.x != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1, R.dv1.x);
-                if (R.dv1.y != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1 + 1, R.dv1.y);
-                if (R.dv1.z != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1 + 2, R.dv1.z);
+                if (Rs.dv0.x != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0, Rs.dv0.x);
+                if (Rs.dv0.y != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0 + 1, Rs.dv0.y);
+                if (Rs.dv0.z != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0 + 2, Rs.dv0.z);
+                if (Rs.dv1.x != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1, Rs.dv1.x);
+                if (Rs.dv1.y != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1 + 1, Rs.dv1.y);
+                if (Rs.dv1.z != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1 + 2, Rs.dv1.z);
                 for (int i = 0; i < 16; i++)
-                    if (R.dw2c[i] != 0.f) atomicAdd(&s_w2c[i], R.dw2c[i]);
+                    if (Rs.dw2c[i] != 0.f) atomicAdd(&s_w2c[i], Rs.dw2c[i]);
             }
         }
         if (S.stats) atomicAdd(&S.stats[4], 1ull);
     }
     __syncthreads();
-    if (threadIdx.x < 16 && s_w2c[threadIdx.x] != 0.f) atomicAdd(grads + 16 + threadIdx.x, s_w2c[threadIdx.x]);
+    if (threadIdx.x < 16 && s_w2c[threadIdx.x] != 0.f) atomicAdd(J.grads + 16 + threadIdx.x, s_w2c[threadIdx.x]);
 }
 
 __global__ void k_intersect(DrSceneView S, const float *rays, long long n, int *hits, int any) {
@@ -465,9 +560,61 @@ static dr_status fail(dr_status s, const std::string &msg) {
 
 // kernel timing: event pairs recorded around launches when enabled
 enum { TK_FORWARD = 0, TK_PATH = 1, TK_PRIMARY = 2, TK_SELECT = 3, TK_EDGE = 4, TK_DP = 5, TK_COUNT = 6 };
+struct TimedLaunch {
+    int kind;
+    hipEvent_t a, b;
+};
+struct Timing {
+    bool on = false;
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t> pool;
+};
+static hipEvent_t get_event(Timing &t) {
+    if (!t.pool.empty()) {
+        hipEvent_t e = t.pool.back();
+        t.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+static void free_timing(Timing &t) {
+    for (auto &x : t.timed) { (void)hipEventDestroy(x.a); (void)hipEventDestroy(x.b); }
+    for (auto e : t.pool) (void)hipEventDestroy(e);
+    t = Timing();
+}
+// record an event before (begin) / after (end) a launch when timing is on
+struct LaunchTimer {
+    Timing *t;
+    int kind;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    LaunchTimer(Timing *t_, int k, hipStream_t st_) : t(t_), kind(k), st(st_) {
+        if (t && t->on) {
+            a = get_event(*t);
+            if (a) (void)hipEventRecord(a, st);
+        }
+    }
+    ~LaunchTimer() {
+        if (!a) return;
+        hipEvent_t b = get_event(*t);
+        if (b && hipEventRecord(b, st) == hipSuccess) t->timed.push_back(TimedLaunch{kind, a, b});
+        else t->pool.push_back(a);
+    }
+};
 
-// scratch of the wavefront adjoint, grown on demand, owned by the scene
-struct AdjBuffers {
+// Pass sizes: pixel samples per pass (bounds the point and queue buffers)
+// and the least number of threads a job should offer (sets samples per
+// thread).  Chosen by dr_batch_autotune, or these defaults.
+struct Tuning {
+    long long pass_samples = 1ll << 22;
+    long long min_threads = 1ll << 20;
+};
+
+// Device working set of the pipeline, grown on demand, and the hipGraph of
+// the last captured job list.
+struct Workspace {
     PointRec *pts = nullptr;
     float *pt_dp = nullptr;
     int *pt_lastj = nullptr;
@@ -476,13 +623,26 @@ struct AdjBuffers {
     EdgeEntry *q = nullptr;
     float4 *entry_dp = nullptr;
     long long q_cap = 0;
-    unsigned long long *counter = nullptr;
-    unsigned long long *counter_host = nullptr;
+    unsigned long long *cnt = nullptr;       // queue length, work counter, largest queue length
+    unsigned long long *cnt_host = nullptr;  // pinned
+    DrJob *jobs = nullptr;
+    int jobs_cap = 0;
+    DrRange *ranges = nullptr;
+    int ranges_cap = 0;
+    hipStream_t cap_stream = nullptr;        // stream graphs are captured on
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned long long> graph_key;
 };
-struct TimedLaunch {
-    int kind;
-    hipEvent_t a, b;
-};
+
+static void free_workspace(Workspace &w) {
+    void *ps[] = {w.pts, w.pt_dp, w.pt_lastj, w.counts, w.q, w.entry_dp, w.cnt, w.jobs, w.ranges};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (w.cnt_host) (void)hipHostFree(w.cnt_host);
+    if (w.exec) (void)hipGraphExecDestroy(w.exec);
+    if (w.cap_stream) (void)hipStreamDestroy(w.cap_stream);
+    w = Workspace();
+}
 
 struct dr_scene {
     drh::HostScene hs;
@@ -490,12 +650,25 @@ struct dr_scene {
     std::vector<void *> allocs;
     unsigned long long *stats_dev = nullptr;
     bool stats_on = false;
-    bool timing_on = false;
-    std::vector<TimedLaunch> timed;
-    std::vector<hipEvent_t> event_pool;
-    AdjBuffers adj;
+    Timing tm;
+    Workspace ws;
     double build_ms = 0.0;
     void *prim_bufs[3] = {nullptr, nullptr, nullptr};
+};
+
+// a scene seen by another camera: its own primary-edge list
+struct dr_view {
+    dr_scene *scene;
+    DrCamera cam;
+    int num_prim_edges = 0;
+    float prim_func_int = 0.f;
+    void *bufs[3] = {nullptr, nullptr, nullptr};
+};
+
+struct dr_batch {
+    Workspace ws;
+    Timing tm;
+    Tuning tune;
 };
 
 template <class T>
@@ -512,88 +685,41 @@ static bool upload(dr_scene *s, const std::vector<T> &v, const T **out) {
     return true;
 }
 
-static bool upload_primary(dr_scene *s) {
+// primary edges and their distribution to the device (three buffers)
+static bool upload_prim(const std::vector<DrEdge> &edges, const drh::Dist &dist, void *bufs[3]) {
     for (int i = 0; i < 3; i++) {
-        if (s->prim_bufs[i]) (void)hipFree(s->prim_bufs[i]);
-        s->prim_bufs[i] = nullptr;
+        if (bufs[i]) (void)hipFree(bufs[i]);
+        bufs[i] = nullptr;
     }
+    if (edges.empty()) return true;
+    size_t sz[3] = {edges.size() * sizeof(DrEdge), dist.func.size() * sizeof(float), dist.cdf.size() * sizeof(float)};
+    const void *src[3] = {edges.data(), dist.func.data(), dist.cdf.data()};
+    for (int i = 0; i < 3; i++) {
+        if (hipMalloc(&bufs[i], sz[i]) != hipSuccess) return false;
+        if (hipMemcpy(bufs[i], src[i], sz[i], hipMemcpyHostToDevice) != hipSuccess) return false;
+    }
+    return true;
+}
+
+static bool upload_primary(dr_scene *s) {
     const drh::HostScene &hs = s->hs;
+    if (!upload_prim(hs.prim_edges, hs.prim, s->prim_bufs)) return false;
     s->view.num_prim_edges = (int)hs.prim_edges.size();
     s->view.prim_func_int = hs.prim.func_int;
-    s->view.prim_edges = nullptr;
-    s->view.prim_func = nullptr;
-    s->view.prim_cdf = nullptr;
-    if (hs.prim_edges.empty()) return true;
-    size_t sz[3] = {hs.prim_edges.size() * sizeof(DrEdge), hs.prim.func.size() * sizeof(float),
-                    hs.prim.cdf.size() * sizeof(float)};
-    const void *src[3] = {hs.prim_edges.data(), hs.prim.func.data(), hs.prim.cdf.data()};
-    for (int i = 0; i < 3; i++) {
-        if (hipMalloc(&s->prim_bufs[i], sz[i]) != hipSuccess) return false;
-        if (hipMemcpy(s->prim_bufs[i], src[i], sz[i], hipMemcpyHostToDevice) != hipSuccess) return false;
-    }
     s->view.prim_edges = (const DrEdge *)s->prim_bufs[0];
     s->view.prim_func = (const float *)s->prim_bufs[1];
     s->view.prim_cdf = (const float *)s->prim_bufs[2];
     return true;
 }
 
-static hipEvent_t get_event(dr_scene *s) {
-    if (!s->event_pool.empty()) {
-        hipEvent_t e = s->event_pool.back();
-        s->event_pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
-}
-
-// record an event before (begin) / after (end) a launch when timing is on
-struct LaunchTimer {
-    dr_scene *s;
-    int kind;
-    hipStream_t st;
-    hipEvent_t a = nullptr;
-    LaunchTimer(dr_scene *s_, int k, hipStream_t st_) : s(s_), kind(k), st(st_) {
-        if (s->timing_on) {
-            a = get_event(s);
-            if (a) (void)hipEventRecord(a, st);
-        }
-    }
-    ~LaunchTimer() {
-        if (!a) return;
-        hipEvent_t b = get_event(s);
-        if (b && hipEventRecord(b, st) == hipSuccess) s->timed.push_back(TimedLaunch{kind, a, b});
-        else s->event_pool.push_back(a);
-    }
-};
-
-static void free_adj(AdjBuffers &a) {
-    void *ps[] = {a.pts, a.pt_dp, a.pt_lastj, a.counts, a.q, a.entry_dp, a.counter};
-    for (void *p : ps)
-        if (p) (void)hipFree(p);
-    if (a.counter_host) (void)hipHostFree(a.counter_host);
-    a = AdjBuffers();
-}
-
 static void free_scene(dr_scene *s) {
-    free_adj(s->adj);
-    for (auto &t : s->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
-    for (auto e : s->event_pool) (void)hipEventDestroy(e);
+    free_workspace(s->ws);
+    free_timing(s->tm);
     for (void *p : s->allocs) (void)hipFree(p);
     for (int i = 0; i < 3; i++)
         if (s->prim_bufs[i]) (void)hipFree(s->prim_bufs[i]);
     if (s->stats_dev) (void)hipFree(s->stats_dev);
     delete s;
-}
-
-// samples per thread: keep >= 2^20 threads in flight when the image allows
-static void plan_chunks(const dr_render_options *o, int &per, int &chunks) {
-    long long pixels = (long long)o->width * o->height;
-    chunks = 1;
-    while ((long long)chunks * 2 <= o->num_samples && pixels * chunks < (1ll << 20)) chunks *= 2;
-    per = (o->num_samples + chunks - 1) / chunks;
-    chunks = (o->num_samples + per - 1) / per;
 }
 
 static SampleCtx make_ctx(const dr_render_options *o) {
@@ -607,132 +733,368 @@ static SampleCtx make_ctx(const dr_render_options *o) {
     return c;
 }
 
+// samples per thread: give a job >= min_threads threads when its image allows
+static void plan_chunks(const dr_render_options *o, long long min_threads, int &per, int &chunks) {
+    long long pixels = (long long)o->width * o->height;
+    chunks = 1;
+    while ((long long)chunks * 2 <= o->num_samples && pixels * chunks < min_threads) chunks *= 2;
+    per = (o->num_samples + chunks - 1) / chunks;
+    chunks = (o->num_samples + per - 1) / per;
+}
+
 static bool grow(void **p, size_t bytes) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
     return hipMalloc(p, bytes) == hipSuccess;
 }
 
-static dr_status ensure_slots(AdjBuffers &a, long long n) {
-    if (n <= a.slot_cap) return DR_OK;
-    if (!grow((void **)&a.pts, n * sizeof(PointRec)) || !grow((void **)&a.pt_dp, n * 3 * sizeof(float)) ||
-        !grow((void **)&a.pt_lastj, n * sizeof(int)) || !grow((void **)&a.counts, n * sizeof(int))) {
-        a.slot_cap = 0;
-        return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint point buffers");
-    }
-    a.slot_cap = n;
-    return DR_OK;
-}
-
-static dr_status ensure_queue(AdjBuffers &a, long long n) {
-    if (!a.counter) {
-        if (hipMalloc((void **)&a.counter, 2 * sizeof(unsigned long long)) != hipSuccess ||
-            hipHostMalloc((void **)&a.counter_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
-            return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint queue counter");
-    }
-    if (n <= a.q_cap) return DR_OK;
-    if (!grow((void **)&a.q, n * sizeof(EdgeEntry)) || !grow((void **)&a.entry_dp, n * sizeof(float4))) {
-        a.q_cap = 0;
-        return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint edge queue");
-    }
-    a.q_cap = n;
-    return DR_OK;
-}
-
 static unsigned grid_of(long long n) { return (unsigned)((n + DR_BLOCK - 1) / DR_BLOCK); }
 
-// samples per batch of the wavefront adjoint (bounds the point and edge
-// buffers; one host read of the edge count per batch)
-#define DR_BATCH_SAMPLES (1ll << 22)
+static dr_status check_opts(const dr_render_options *o) {
+    if (!o) return fail(DR_ERROR_INVALID_ARGUMENT, "null options");
+    if (o->width <= 0 || o->height <= 0 || o->num_samples <= 0 || o->max_bounces < 1)
+        return fail(DR_ERROR_INVALID_ARGUMENT, "width, height, num_samples must be > 0 and max_bounces >= 1");
+    if ((long long)o->width * o->height > (1ll << 28)) return fail(DR_ERROR_INVALID_ARGUMENT, "image too large");
+    return DR_OK;
+}
 
-// Resident workgroups of the persistent selection kernel on this device.
-static long long select_blocks() {
-    static long long cached = 0;
-    if (cached == 0) {
+static dr_status launch_check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(DR_ERROR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return DR_OK;
+}
+
+// Resident workgroups of a kernel on this device (for persistent launches).
+template <class K>
+static long long resident_blocks(K kernel, long long &cache) {
+    if (cache == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_select, DR_BLOCK, 0) != hipSuccess ||
-            cus <= 0 || per_cu <= 0) {
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, DR_BLOCK, 0) != hipSuccess || cus <= 0 ||
+            per_cu <= 0) {
             cus = 256;
             per_cu = 2;
         }
-        cached = (long long)cus * per_cu;
+        cache = (long long)cus * per_cu;
     }
-    return cached;
+    return cache;
 }
 
-template <int MAXB>
-static dr_status launch_adjoint(dr_scene *s, const dr_render_options *o, const float *d_image, float *image,
-                                float *grads, float *dx, hipStream_t hs) {
-    int per, chunks;
-    plan_chunks(o, per, chunks);
-    size_t n = 3ull * o->width * o->height;
-    if (chunks > 1) {
-        k_fill<<<1024, 256, 0, hs>>>(image, n, 0.f);
-        if (dx) k_fill<<<1024, 256, 0, hs>>>(dx, n, 0.f);
-    }
-    long long total = (long long)o->width * o->height * chunks;
-    long long bt = std::max(1ll, std::min(total, DR_BATCH_SAMPLES / per));
-    SampleCtx c = make_ctx(o);
-    AdjBuffers &A = s->adj;
-    dr_status st = ensure_slots(A, bt * per * MAXB);
-    if (st != DR_OK) return st;
-    st = ensure_queue(A, std::max(A.q_cap, bt * per * 4));
-    if (st != DR_OK) return st;
-    const bool edges = !(o->flags & DR_SKIP_SECONDARY_EDGES);
-    for (long long t0 = 0; t0 < total; t0 += bt) {
-        long long t1 = std::min(total, t0 + bt);
-        long long nslots = (t1 - t0) * per * MAXB;
-        {
-            LaunchTimer timer(s, TK_PATH, hs);
-            k_path<MAXB><<<grid_of(t1 - t0), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, t1, d_image, image,
-                                                                 grads, A.pts);
+// ------------------------------------------------------------ planning ----
+struct Pass {
+    int r0, r1;
+    long long nslots, nblocks;
+    bool edges, dp_last;
+};
+struct Plan {
+    std::vector<DrJob> jobs;
+    std::vector<DrRange> ranges;
+    std::vector<Pass> passes;
+    int prim_r0 = 0, prim_r1 = 0, fwd_r0 = 0, fwd_r1 = 0;
+    long long prim_blocks = 0, fwd_blocks = 0, max_slots = 0;
+    int maxb = 1;
+    std::vector<std::pair<float *, size_t>> zero;  // buffers cleared first
+    std::vector<unsigned long long> key;           // what a captured graph depends on
+};
+
+static int bucket(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : 8; }
+
+static dr_status build_plan(const dr_job *jobs, int n, const Tuning &tn, Plan &P) {
+    P = Plan();
+    if (n <= 0 || !jobs) return fail(DR_ERROR_INVALID_ARGUMENT, "no jobs");
+    std::vector<float *> grads_seen;
+    for (int i = 0; i < n; i++) {
+        const dr_job &jb = jobs[i];
+        dr_scene *sc = jb.scene;
+        if (!sc) return fail(DR_ERROR_INVALID_ARGUMENT, "job without a scene");
+        dr_status st = check_opts(&jb.opts);
+        if (st != DR_OK) return st;
+        if (!jb.image) return fail(DR_ERROR_INVALID_ARGUMENT, "job without an image buffer");
+        bool adj = jb.d_image != nullptr;
+        if (adj && !jb.grads) return fail(DR_ERROR_INVALID_ARGUMENT, "adjoint job without a gradient buffer");
+        if (adj && jb.opts.max_bounces > 8) return fail(DR_ERROR_UNSUPPORTED, "the adjoint supports max_bounces <= 8");
+        if (jb.view && jb.view->scene != sc) return fail(DR_ERROR_INVALID_ARGUMENT, "view of another scene");
+        DrJob dj;
+        std::memset(&dj, 0, sizeof(dj));
+        dj.S = sc->view;
+        if (jb.view) {
+            dj.S.cam = jb.view->cam;
+            dj.S.num_prim_edges = jb.view->num_prim_edges;
+            dj.S.prim_func_int = jb.view->prim_func_int;
+            dj.S.prim_edges = (const DrEdge *)jb.view->bufs[0];
+            dj.S.prim_func = (const float *)jb.view->bufs[1];
+            dj.S.prim_cdf = (const float *)jb.view->bufs[2];
         }
-        if (!edges) continue;
-        k_fill<<<1024, 256, 0, hs>>>(A.pt_dp, (size_t)nslots * 3, 0.f);
-        k_fill_i<<<1024, 256, 0, hs>>>(A.pt_lastj, (size_t)nslots, -1);
-        long long nedges = 0;
-        for (int attempt = 0; attempt < 2; attempt++) {
-            // queue counter and work counter: two 8-byte zeros
-            k_fill<<<1, 64, 0, hs>>>((float *)A.counter, 4, 0.f);
-            {
-                LaunchTimer timer(s, TK_SELECT, hs);
-                long long blocks = std::min<long long>(grid_of(nslots), select_blocks());
-                k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, nslots, MAXB, A.pts,
-                                                                A.q, A.counter, A.q_cap, A.counts, A.counter + 1);
+        dj.c = make_ctx(&jb.opts);
+        plan_chunks(&jb.opts, tn.min_threads, dj.per, dj.chunks);
+        dj.prim_n = jb.opts.num_primary_edge_samples > 0
+                        ? jb.opts.num_primary_edge_samples
+                        : (long long)jb.opts.num_samples * jb.opts.width * jb.opts.height;
+        dj.d_image = jb.d_image;
+        dj.image = jb.image;
+        dj.grads = adj ? jb.grads : nullptr;
+        dj.dx = (adj && (jb.opts.flags & DR_WRITE_DX_IMAGE)) ? jb.dx_image : nullptr;
+        size_t npix = 3ull * jb.opts.width * jb.opts.height;
+        if (dj.chunks > 1) P.zero.push_back({jb.image, npix});
+        if (dj.dx) P.zero.push_back({dj.dx, npix});
+        if (adj && std::find(grads_seen.begin(), grads_seen.end(), jb.grads) == grads_seen.end()) {
+            grads_seen.push_back(jb.grads);
+            P.zero.push_back({jb.grads, sc->hs.grad_size});
+        }
+        if (adj) P.maxb = std::max(P.maxb, bucket(jb.opts.max_bounces));
+        P.jobs.push_back(dj);
+        unsigned long long k[] = {(unsigned long long)(uintptr_t)sc, (unsigned long long)(uintptr_t)jb.view,
+                                  (unsigned long long)jb.opts.width, (unsigned long long)jb.opts.height,
+                                  (unsigned long long)jb.opts.num_samples, (unsigned long long)jb.opts.max_bounces,
+                                  (unsigned long long)jb.opts.flags, (unsigned long long)dj.prim_n,
+                                  (unsigned long long)(uintptr_t)jb.d_image, (unsigned long long)(uintptr_t)jb.image,
+                                  (unsigned long long)(uintptr_t)dj.grads, (unsigned long long)(uintptr_t)dj.dx,
+                                  (unsigned long long)(uintptr_t)sc->view.stats};
+        P.key.insert(P.key.end(), k, k + sizeof(k) / sizeof(k[0]));
+    }
+    // passes over the adjoint jobs' (pixel, chunk) threads
+    long long cur_samples = 0, cur_slots = 0, cur_blocks = 0;
+    Pass pass{0, 0, 0, 0, false, false};
+    auto close = [&]() {
+        if (pass.r1 > pass.r0) {
+            pass.nslots = cur_slots;
+            pass.nblocks = cur_blocks;
+            P.passes.push_back(pass);
+            P.max_slots = std::max(P.max_slots, cur_slots);
+        }
+        pass = Pass{(int)P.ranges.size(), (int)P.ranges.size(), 0, 0, false, false};
+        cur_samples = cur_slots = cur_blocks = 0;
+    };
+    for (int i = 0; i < n; i++) {
+        const DrJob &dj = P.jobs[i];
+        if (!dj.d_image) continue;
+        long long total = (long long)dj.c.W * dj.c.H * dj.chunks;
+        for (long long t = 0; t < total;) {
+            long long room = (tn.pass_samples - cur_samples) / dj.per;
+            if (room <= 0 && cur_samples > 0) {
+                close();
+                continue;
             }
-            if (hipMemcpyAsync(A.counter_host, A.counter, sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) !=
-                    hipSuccess ||
-                hipStreamSynchronize(hs) != hipSuccess)
-                return fail(DR_ERROR_DEVICE, "reading the edge queue size failed");
-            nedges = (long long)*A.counter_host;
-            if (nedges <= A.q_cap) break;
-            // the selection is deterministic: grow the queue and select again
-            st = ensure_queue(A, nedges + nedges / 4);
-            if (st != DR_OK) return st;
-        }
-        if (nedges > 0) {
-            LaunchTimer timer(s, TK_EDGE, hs);
-            k_edge<MAXB><<<grid_of(nedges), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, A.q, nedges, A.pts,
-                                                                d_image, grads, A.pt_dp, A.pt_lastj, A.entry_dp);
-        }
-        if (nedges > 0 && (o->flags & DR_QUIRK_DP_LAST))
-            k_dp_last<<<grid_of(nedges), DR_BLOCK, 0, hs>>>(A.q, nedges, A.entry_dp, A.pt_lastj, A.pt_dp);
-        {
-            LaunchTimer timer(s, TK_DP, hs);
-            k_dp<MAXB><<<grid_of(t1 - t0), DR_BLOCK, 0, hs>>>(s->view, c, per, chunks, t0, t1, A.pts, A.pt_dp,
-                                                               A.counts, grads, dx);
+            long long take = std::min(total - t, std::max(room, 1ll));
+            DrRange r{i, 0, t, t + take, cur_slots, cur_blocks};
+            P.ranges.push_back(r);
+            pass.r1 = (int)P.ranges.size();
+            pass.edges |= !(dj.c.flags & DR_SKIP_SECONDARY_EDGES);
+            pass.dp_last |= (dj.c.flags & DR_QUIRK_DP_LAST) != 0;
+            cur_samples += take * dj.per;
+            cur_slots += take * dj.per * P.maxb;
+            cur_blocks += (take + DR_BLOCK - 1) / DR_BLOCK;
+            t += take;
         }
     }
-    if (dx && !edges) k_fill<<<1024, 256, 0, hs>>>(dx, n, 0.f);
-    if (!(o->flags & DR_SKIP_PRIMARY_EDGES)) {
-        LaunchTimer timer(s, TK_PRIMARY, hs);
-        long long N = o->num_primary_edge_samples > 0 ? o->num_primary_edge_samples
-                                                      : (long long)o->num_samples * o->width * o->height;
-        k_primary<MAXB><<<grid_of(N), DR_BLOCK, 0, hs>>>(s->view, c, N, d_image, grads);
+    close();
+    // primary-edge samples of the adjoint jobs, one launch
+    P.prim_r0 = (int)P.ranges.size();
+    long long blocks = 0;
+    for (int i = 0; i < n; i++) {
+        const DrJob &dj = P.jobs[i];
+        if (!dj.d_image || (dj.c.flags & DR_SKIP_PRIMARY_EDGES) || dj.prim_n <= 0) continue;
+        P.ranges.push_back(DrRange{i, 0, 0, dj.prim_n, 0, blocks});
+        blocks += (dj.prim_n + DR_BLOCK - 1) / DR_BLOCK;
+    }
+    P.prim_r1 = (int)P.ranges.size();
+    P.prim_blocks = blocks;
+    // forward-only jobs, one launch
+    P.fwd_r0 = (int)P.ranges.size();
+    blocks = 0;
+    for (int i = 0; i < n; i++) {
+        const DrJob &dj = P.jobs[i];
+        if (dj.d_image) continue;
+        long long total = (long long)dj.c.W * dj.c.H * dj.chunks;
+        P.ranges.push_back(DrRange{i, 0, 0, total, 0, blocks});
+        blocks += (total + DR_BLOCK - 1) / DR_BLOCK;
+    }
+    P.fwd_r1 = (int)P.ranges.size();
+    P.fwd_blocks = blocks;
+    P.key.push_back((unsigned long long)tn.pass_samples);
+    P.key.push_back((unsigned long long)tn.min_threads);
+    return DR_OK;
+}
+
+static dr_status ensure_workspace(Workspace &w, const Plan &P) {
+    if (!w.cnt) {
+        if (hipMalloc((void **)&w.cnt, 4 * sizeof(unsigned long long)) != hipSuccess ||
+            hipHostMalloc((void **)&w.cnt_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+            return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint counters");
+    }
+    if (P.max_slots > w.slot_cap) {
+        long long n = P.max_slots;
+        if (!grow((void **)&w.pts, n * sizeof(PointRec)) || !grow((void **)&w.pt_dp, n * 3 * sizeof(float)) ||
+            !grow((void **)&w.pt_lastj, n * sizeof(int)) || !grow((void **)&w.counts, n * sizeof(int))) {
+            w.slot_cap = 0;
+            return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint point buffers");
+        }
+        w.slot_cap = n;
+    }
+    long long qn = P.max_slots * 2;
+    if (qn > w.q_cap) {
+        if (!grow((void **)&w.q, qn * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, qn * sizeof(float4))) {
+            w.q_cap = 0;
+            return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint edge queue");
+        }
+        w.q_cap = qn;
+    }
+    if ((int)P.jobs.size() > w.jobs_cap) {
+        if (!grow((void **)&w.jobs, P.jobs.size() * sizeof(DrJob))) return fail(DR_ERROR_OUT_OF_MEMORY, "job table");
+        w.jobs_cap = (int)P.jobs.size();
+    }
+    if ((int)P.ranges.size() > w.ranges_cap) {
+        if (!grow((void **)&w.ranges, std::max<size_t>(1, P.ranges.size()) * sizeof(DrRange)))
+            return fail(DR_ERROR_OUT_OF_MEMORY, "range table");
+        w.ranges_cap = (int)P.ranges.size();
     }
     return DR_OK;
+}
+
+static dr_status grow_queue(Workspace &w, long long n) {
+    if (!grow((void **)&w.q, n * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, n * sizeof(float4))) {
+        w.q_cap = 0;
+        return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint edge queue");
+    }
+    w.q_cap = n;
+    return DR_OK;
+}
+
+static long long g_select_blocks = 0, g_edge_blocks[4] = {0, 0, 0, 0};
+
+// All launches of a plan, stream-ordered, without host synchronisation (so
+// they can be captured in a hipGraph).  The tables are already on the device.
+template <int MAXB>
+static void enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
+    for (auto &z : P.zero) k_fill<<<256, 256, 0, hs>>>(z.first, z.second, 0.f);
+    k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 2), 2, 0.f);
+    const int bi = MAXB == 1 ? 0 : MAXB == 2 ? 1 : MAXB == 4 ? 2 : 3;
+    for (const Pass &ps : P.passes) {
+        const DrRange *R = w.ranges + ps.r0;
+        int nr = ps.r1 - ps.r0;
+        {
+            LaunchTimer t(tm, TK_PATH, hs);
+            k_path<MAXB><<<(unsigned)ps.nblocks, DR_BLOCK, 0, hs>>>(w.jobs, R, nr, w.pts);
+        }
+        if (!ps.edges) continue;
+        k_fill<<<1024, 256, 0, hs>>>(w.pt_dp, (size_t)ps.nslots * 3, 0.f);
+        k_fill_i<<<1024, 256, 0, hs>>>(w.pt_lastj, (size_t)ps.nslots, -1);
+        k_fill<<<1, 64, 0, hs>>>((float *)w.cnt, 4, 0.f);
+        {
+            LaunchTimer t(tm, TK_SELECT, hs);
+            long long blocks = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_select, g_select_blocks));
+            k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, R, nr, ps.nslots, MAXB, w.pts, w.q, w.cnt,
+                                                            w.q_cap, w.counts);
+        }
+        long long eb = resident_blocks(k_edge<MAXB>, g_edge_blocks[bi]);
+        {
+            LaunchTimer t(tm, TK_EDGE, hs);
+            k_edge<MAXB><<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.jobs, R, nr, w.q, w.cnt, w.q_cap, w.pts, w.pt_dp,
+                                                            w.pt_lastj, w.entry_dp);
+        }
+        if (ps.dp_last) k_dp_last<<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.q, w.cnt, w.q_cap, w.entry_dp, w.pt_lastj, w.pt_dp);
+        {
+            LaunchTimer t(tm, TK_DP, hs);
+            k_dp<MAXB><<<(unsigned)ps.nblocks, DR_BLOCK, 0, hs>>>(w.jobs, R, nr, w.pts, w.pt_dp, w.counts);
+        }
+    }
+    if (P.prim_r1 > P.prim_r0) {
+        LaunchTimer t(tm, TK_PRIMARY, hs);
+        k_primary<MAXB><<<(unsigned)P.prim_blocks, DR_BLOCK, 0, hs>>>(w.jobs, w.ranges + P.prim_r0,
+                                                                       P.prim_r1 - P.prim_r0);
+    }
+    if (P.fwd_r1 > P.fwd_r0) {
+        LaunchTimer t(tm, TK_FORWARD, hs);
+        k_forward<<<(unsigned)P.fwd_blocks, DR_BLOCK, 0, hs>>>(w.jobs, w.ranges + P.fwd_r0, P.fwd_r1 - P.fwd_r0);
+    }
+}
+
+static void enqueue_any(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
+    if (P.maxb <= 1) enqueue<1>(P, w, tm, hs);
+    else if (P.maxb <= 2) enqueue<2>(P, w, tm, hs);
+    else if (P.maxb <= 4) enqueue<4>(P, w, tm, hs);
+    else enqueue<8>(P, w, tm, hs);
+}
+
+static bool has_edges(const Plan &P) {
+    for (const Pass &ps : P.passes)
+        if (ps.edges) return true;
+    return false;
+}
+
+// Run a plan: tables to the device, launches (captured once and replayed as a
+// hipGraph with use_graph), then, when there are secondary edges, one host
+// read of the largest queue length; a queue that overflowed is grown and the
+// plan runs again (every output is rewritten from zero by a run).
+static dr_status run_plan(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs, bool use_graph) {
+    dr_status st = ensure_workspace(w, P);
+    if (st != DR_OK) return st;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if (hipMemcpyAsync(w.jobs, P.jobs.data(), P.jobs.size() * sizeof(DrJob), hipMemcpyHostToDevice, hs) !=
+                hipSuccess ||
+            (!P.ranges.empty() && hipMemcpyAsync(w.ranges, P.ranges.data(), P.ranges.size() * sizeof(DrRange),
+                                                 hipMemcpyHostToDevice, hs) != hipSuccess))
+            return fail(DR_ERROR_DEVICE, "job table upload failed");
+        if (use_graph) {
+            std::vector<unsigned long long> key = P.key;
+            unsigned long long extra[] = {(unsigned long long)(uintptr_t)w.pts, (unsigned long long)(uintptr_t)w.q,
+                                          (unsigned long long)w.q_cap, (unsigned long long)(uintptr_t)w.jobs,
+                                          (unsigned long long)(uintptr_t)w.ranges};
+            key.insert(key.end(), extra, extra + 5);
+            if (!w.exec || key != w.graph_key) {
+                if (w.exec) (void)hipGraphExecDestroy(w.exec);
+                w.exec = nullptr;
+                w.graph_key.clear();
+                if (!w.cap_stream && hipStreamCreate(&w.cap_stream) != hipSuccess)
+                    return fail(DR_ERROR_DEVICE, "stream creation failed");
+                hipGraph_t g = nullptr;
+                if (hipStreamBeginCapture(w.cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                    return fail(DR_ERROR_DEVICE, "graph capture failed to start");
+                enqueue_any(P, w, nullptr, w.cap_stream);
+                hipError_t e = hipStreamEndCapture(w.cap_stream, &g);
+                if (e != hipSuccess || !g) return fail(DR_ERROR_DEVICE, "graph capture failed");
+                e = hipGraphInstantiate(&w.exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    w.exec = nullptr;
+                    return fail(DR_ERROR_DEVICE, "graph instantiation failed");
+                }
+                w.graph_key = key;
+            }
+            if (hipGraphLaunch(w.exec, hs) != hipSuccess) return fail(DR_ERROR_DEVICE, "graph launch failed");
+        } else {
+            enqueue_any(P, w, tm, hs);
+        }
+        st = launch_check("render kernels");
+        if (st != DR_OK) return st;
+        if (!has_edges(P)) return DR_OK;
+        if (hipMemcpyAsync(w.cnt_host, w.cnt + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) !=
+                hipSuccess ||
+            hipStreamSynchronize(hs) != hipSuccess)
+            return fail(DR_ERROR_DEVICE, "reading the edge queue length failed");
+        long long len = (long long)*w.cnt_host;
+        if (len <= w.q_cap) return DR_OK;
+        st = grow_queue(w, len + len / 4);
+        if (st != DR_OK) return st;
+    }
+    return fail(DR_ERROR_DEVICE, "edge queue kept overflowing");
+}
+
+static dr_status single_job(dr_scene *s, const dr_render_options *o, const float *d_image, float *image, float *grads,
+                            float *dx, hipStream_t hs) {
+    dr_job jb;
+    std::memset(&jb, 0, sizeof(jb));
+    jb.scene = s;
+    jb.opts = *o;
+    jb.d_image = d_image;
+    jb.image = image;
+    jb.grads = grads;
+    jb.dx_image = dx;
+    Plan P;
+    dr_status st = build_plan(&jb, 1, Tuning(), P);
+    if (st != DR_OK) return st;
+    return run_plan(P, s->ws, &s->tm, hs, false);
 }
 
 extern "C" {
@@ -842,7 +1204,7 @@ dr_status dr_scene_stats(dr_scene *s, int enable, dr_stats *out) {
         out->primary_edge_samples = h[4];
         out->edge_nodes_visited = h[5];
         out->edges_tested = h[6];
-        for (auto &t : s->timed) {
+        for (auto &t : s->tm.timed) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) ms = 0.f;
             if (t.kind < 0 || t.kind >= TK_COUNT) continue;
@@ -850,65 +1212,139 @@ dr_status dr_scene_stats(dr_scene *s, int enable, dr_stats *out) {
             out->kernel_launches[t.kind]++;
         }
     }
-    for (auto &t : s->timed) {
-        s->event_pool.push_back(t.a);
-        s->event_pool.push_back(t.b);
+    for (auto &t : s->tm.timed) {
+        s->tm.pool.push_back(t.a);
+        s->tm.pool.push_back(t.b);
     }
-    s->timed.clear();
+    s->tm.timed.clear();
     (void)hipMemset(s->stats_dev, 0, 8 * sizeof(unsigned long long));
-    s->timing_on = (enable & DR_STATS_TIMING) != 0;
+    s->tm.on = (enable & DR_STATS_TIMING) != 0;
     s->stats_on = (enable & DR_STATS_COUNTERS) != 0;
     s->view.stats = s->stats_on ? s->stats_dev : nullptr;
     return DR_OK;
 }
 
-static dr_status check_opts(const dr_scene *s, const dr_render_options *o) {
-    if (!s || !o) return fail(DR_ERROR_INVALID_ARGUMENT, "null argument");
-    if (o->width <= 0 || o->height <= 0 || o->num_samples <= 0 || o->max_bounces < 1)
-        return fail(DR_ERROR_INVALID_ARGUMENT, "width, height, num_samples must be > 0 and max_bounces >= 1");
-    if ((long long)o->width * o->height > (1ll << 28)) return fail(DR_ERROR_INVALID_ARGUMENT, "image too large");
-    return DR_OK;
-}
-
-
-
-static dr_status launch_check(const char *what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(DR_ERROR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-    return DR_OK;
-}
-
 dr_status dr_render(dr_scene *s, const dr_render_options *o, float *image, void *stream) {
-    dr_status st = check_opts(s, o);
+    if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
+    dr_status st = check_opts(o);
     if (st != DR_OK) return st;
     if (!image) return fail(DR_ERROR_INVALID_ARGUMENT, "image is NULL");
-    hipStream_t hs = (hipStream_t)stream;
-    int per, chunks;
-    plan_chunks(o, per, chunks);
-    size_t n = 3ull * o->width * o->height;
-    if (chunks > 1) k_fill<<<1024, 256, 0, hs>>>(image, n, 0.f);
-    long long total = (long long)o->width * o->height * chunks;
-    LaunchTimer timer(s, TK_FORWARD, hs);
-    k_forward<<<(unsigned)((total + DR_BLOCK - 1) / DR_BLOCK), DR_BLOCK, 0, hs>>>(s->view, make_ctx(o), per, chunks,
-                                                                                image);
-    return launch_check("k_forward");
+    return single_job(s, o, nullptr, image, nullptr, nullptr, (hipStream_t)stream);
 }
 
 dr_status dr_render_adjoint(dr_scene *s, const dr_render_options *o, const float *d_image, float *image, float *grads,
                             float *dx_image, void *stream) {
-    dr_status st = check_opts(s, o);
+    if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
+    dr_status st = check_opts(o);
     if (st != DR_OK) return st;
     if (!d_image || !image || !grads) return fail(DR_ERROR_INVALID_ARGUMENT, "d_image, image and grads are required");
-    if (o->max_bounces > 8) return fail(DR_ERROR_UNSUPPORTED, "the adjoint supports max_bounces <= 8");
-    hipStream_t hs = (hipStream_t)stream;
-    k_fill<<<256, 256, 0, hs>>>(grads, s->hs.grad_size, 0.f);
-    float *dx = (o->flags & DR_WRITE_DX_IMAGE) ? dx_image : nullptr;
-    if (o->max_bounces <= 1) st = launch_adjoint<1>(s, o, d_image, image, grads, dx, hs);
-    else if (o->max_bounces <= 2) st = launch_adjoint<2>(s, o, d_image, image, grads, dx, hs);
-    else if (o->max_bounces <= 4) st = launch_adjoint<4>(s, o, d_image, image, grads, dx, hs);
-    else st = launch_adjoint<8>(s, o, d_image, image, grads, dx, hs);
+    return single_job(s, o, d_image, image, grads, dx_image, (hipStream_t)stream);
+}
+
+dr_status dr_view_create(dr_scene *s, const dr_camera *c, dr_view **out) {
+    if (!s || !c || !out) return fail(DR_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (c->fisheye) return fail(DR_ERROR_UNSUPPORTED, "fisheye cameras are not supported");
+    dr_view *v = new (std::nothrow) dr_view();
+    if (!v) return fail(DR_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    v->scene = s;
+    v->cam = drh::make_camera(*c);
+    std::vector<DrEdge> edges;
+    drh::Dist dist;
+    drh::primary_edges_for(s->hs, v->cam, edges, dist);
+    if (!upload_prim(edges, dist, v->bufs)) {
+        dr_view_destroy(v);
+        return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+    }
+    v->num_prim_edges = (int)edges.size();
+    v->prim_func_int = dist.func_int;
+    *out = v;
+    return DR_OK;
+}
+
+void dr_view_destroy(dr_view *v) {
+    if (!v) return;
+    for (int i = 0; i < 3; i++)
+        if (v->bufs[i]) (void)hipFree(v->bufs[i]);
+    delete v;
+}
+
+dr_status dr_batch_create(dr_batch **out) {
+    if (!out) return fail(DR_ERROR_INVALID_ARGUMENT, "out is NULL");
+    *out = new (std::nothrow) dr_batch();
+    if (!*out) return fail(DR_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    return DR_OK;
+}
+
+void dr_batch_destroy(dr_batch *b) {
+    if (!b) return;
+    free_workspace(b->ws);
+    free_timing(b->tm);
+    delete b;
+}
+
+dr_status dr_render_batch(dr_batch *b, const dr_job *jobs, int num_jobs, int flags, void *stream) {
+    if (!b) return fail(DR_ERROR_INVALID_ARGUMENT, "null batch");
+    Plan P;
+    dr_status st = build_plan(jobs, num_jobs, b->tune, P);
     if (st != DR_OK) return st;
-    return launch_check("adjoint kernels");
+    return run_plan(P, b->ws, &b->tm, (hipStream_t)stream, (flags & DR_BATCH_GRAPH) != 0);
+}
+
+dr_status dr_batch_set_tuning(dr_batch *b, const dr_tuning *t) {
+    if (!b || !t || t->pass_samples <= 0 || t->min_threads <= 0)
+        return fail(DR_ERROR_INVALID_ARGUMENT, "bad tuning");
+    b->tune.pass_samples = t->pass_samples;
+    b->tune.min_threads = t->min_threads;
+    return DR_OK;
+}
+
+dr_status dr_batch_autotune(dr_batch *b, const dr_job *jobs, int num_jobs, void *stream, dr_tuning *out) {
+    if (!b) return fail(DR_ERROR_INVALID_ARGUMENT, "null batch");
+    hipStream_t hs = (hipStream_t)stream;
+    static const long long kPass[] = {1ll << 20, 1ll << 21, 1ll << 22, 1ll << 23};
+    static const long long kThreads[] = {1ll << 19, 1ll << 20, 1ll << 21};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(DR_ERROR_DEVICE, "event creation failed");
+    Plan P;
+    dr_status st = build_plan(jobs, num_jobs, b->tune, P);
+    if (st == DR_OK) st = run_plan(P, b->ws, nullptr, hs, false);  // warm-up: buffers, code objects
+    double best = 1e300;
+    Tuning bt = b->tune;
+    int tried = 0;
+    for (long long ps : kPass) {
+        for (long long mt : kThreads) {
+            if (st != DR_OK) break;
+            Tuning t;
+            t.pass_samples = ps;
+            t.min_threads = mt;
+            st = build_plan(jobs, num_jobs, t, P);
+            if (st != DR_OK) break;
+            (void)hipEventRecord(e0, hs);
+            st = run_plan(P, b->ws, nullptr, hs, false);
+            (void)hipEventRecord(e1, hs);
+            if (st != DR_OK || hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tried++;
+            if (ms < best) {
+                best = ms;
+                bt = t;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != DR_OK) return st;
+    b->tune = bt;
+    if (out) {
+        out->pass_samples = bt.pass_samples;
+        out->min_threads = bt.min_threads;
+        out->best_ms = best;
+        out->candidates = tried;
+    }
+    return DR_OK;
 }
 
 dr_status dr_intersect(dr_scene *s, const float *rays, int64_t n, int32_t *hits, int any_hit, void *stream) {

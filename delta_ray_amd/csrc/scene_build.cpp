@@ -474,36 +474,42 @@ bool is_silhouette(const HostScene &hs, V3 p, const DrEdge &e) {
 
 }  // namespace
 
-void build_primary_edges(HostScene &hs) {
-    hs.prim_edges.clear();
+void primary_edges_for(const HostScene &hs, const DrCamera &cam, std::vector<DrEdge> &out, Dist &dist) {
+    out.clear();
     std::vector<float> lens;
-    V3 org = xfm_point(hs.cam.c2w, V3{0.f, 0.f, 0.f});
+    V3 org = xfm_point(cam.c2w, V3{0.f, 0.f, 0.f});
     for (const DrEdge &e : hs.edges) {
-        float dist = 0.f;
+        float len = 0.f;
         V2 a, b;
-        if (project_edge(hs.cam, getv(hs, e.shape, e.v0), getv(hs, e.shape, e.v1), a, b)) {
+        if (project_edge(cam, getv(hs, e.shape, e.v0), getv(hs, e.shape, e.v1), a, b)) {
             if (clip_line(a, b)) {
                 float dx = b.x - a.x, dy = b.y - a.y;
-                dist = std::sqrt(dx * dx + dy * dy);
+                len = std::sqrt(dx * dx + dy * dy);
             }
         }
-        if (!is_silhouette(hs, org, e)) dist = 0.f;
-        if (dist > 0.f) {
-            hs.prim_edges.push_back(e);
-            lens.push_back(dist);
+        if (!is_silhouette(hs, org, e)) len = 0.f;
+        if (len > 0.f) {
+            out.push_back(e);
+            lens.push_back(len);
         }
     }
-    hs.prim = lens.empty() ? Dist() : make_dist(lens);
+    dist = lens.empty() ? Dist() : make_dist(lens);
 }
 
-void set_camera(HostScene &hs, const dr_camera &c) {
-    std::memcpy(hs.cam.c2w, c.cam_to_world, sizeof(float) * 16);
-    std::memcpy(hs.cam.w2c, c.world_to_cam, sizeof(float) * 16);
-    hs.cam.fov = c.fov_factor;
-    hs.cam.aspect = c.aspect_ratio;
-    hs.cam.clip_near = c.clip_near;
-    hs.cam.pad = 0.f;
+void build_primary_edges(HostScene &hs) { primary_edges_for(hs, hs.cam, hs.prim_edges, hs.prim); }
+
+DrCamera make_camera(const dr_camera &c) {
+    DrCamera o;
+    std::memcpy(o.c2w, c.cam_to_world, sizeof(float) * 16);
+    std::memcpy(o.w2c, c.world_to_cam, sizeof(float) * 16);
+    o.fov = c.fov_factor;
+    o.aspect = c.aspect_ratio;
+    o.clip_near = c.clip_near;
+    o.pad = 0.f;
+    return o;
 }
+
+void set_camera(HostScene &hs, const dr_camera &c) { hs.cam = make_camera(c); }
 
 std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
     if (!d) return "null scene description";

@@ -38,8 +38,11 @@ struct HostScene {
 
 // Returns an empty string on success, else the reason for rejecting desc.
 std::string build_host_scene(const dr_scene_desc *desc, HostScene &hs);
+DrCamera make_camera(const dr_camera &c);
 void set_camera(HostScene &hs, const dr_camera &c);
-// Primary-edge list and distribution for the current camera
+// Primary-edge list and distribution (pmf proportional to the clipped screen
+// length of camera silhouettes) for the scene's camera / for a given camera
 void build_primary_edges(HostScene &hs);
+void primary_edges_for(const HostScene &hs, const DrCamera &cam, std::vector<DrEdge> &edges, Dist &dist);
 
 }  // namespace drh

@@ -136,16 +136,7 @@ class SceneDesc:
             raise NotImplementedError("fisheye cameras are not supported by delta_ray_amd")
         keep = []
         d = _abi.dr_scene_desc()
-        c = d.camera
-        for name in ("cam_to_world", "world_to_cam", "sample_to_cam", "cam_to_sample"):
-            m = _abi.f32(getattr(camera, name))
-            if m.shape != (4, 4):
-                raise ValueError("%s must be 4x4" % name)
-            getattr(c, name)[:] = m.reshape(-1).tolist()
-        c.fov_factor = camera.fov_factor
-        c.aspect_ratio = camera.aspect_ratio
-        c.clip_near = camera.clip_near
-        c.fisheye = 0
+        d.camera = SceneDesc.camera_struct(camera)
         mats = (_abi.dr_material * max(1, len(materials)))()
         mat_index = {}
         for i, m in enumerate(materials):
@@ -207,6 +198,25 @@ class SceneDesc:
         self._keep = (keep, mats, shs, lts)
         self.shapes, self.materials, self.lights = shapes, materials, lights
         self.layout = grad_layout(shapes, materials, lights)
+
+
+def _camera_struct(camera):
+    if camera.fisheye:
+        raise NotImplementedError("fisheye cameras are not supported by delta_ray_amd")
+    c = _abi.dr_camera()
+    for name in ("cam_to_world", "world_to_cam", "sample_to_cam", "cam_to_sample"):
+        m = _abi.f32(getattr(camera, name))
+        if m.shape != (4, 4):
+            raise ValueError("%s must be 4x4" % name)
+        getattr(c, name)[:] = m.reshape(-1).tolist()
+    c.fov_factor = float(camera.fov_factor)
+    c.aspect_ratio = float(camera.aspect_ratio)
+    c.clip_near = float(camera.clip_near)
+    c.fisheye = 0
+    return c
+
+
+SceneDesc.camera_struct = staticmethod(_camera_struct)
 
 
 def grad_layout(shapes, materials, lights):
@@ -331,6 +341,78 @@ class DeviceScene:
             out[k + "_ms"] = st.kernel_ms[i]
             out[k + "_launches"] = st.kernel_launches[i]
         return out
+
+
+class View:
+    """The scene of a DeviceScene seen by another camera (dr_view_create); keep
+    the DeviceScene alive while the view is used."""
+
+    def __init__(self, scene, camera):
+        self.scene = scene
+        self.lib = scene.lib
+        c = SceneDesc.camera_struct(camera)
+        h = ctypes.c_void_p()
+        _abi.check(self.lib, self.lib.dr_view_create(scene.handle, ctypes.byref(c), ctypes.byref(h)),
+                   "dr_view_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.dr_view_destroy(h)
+            self.handle = None
+
+
+class Batch:
+    """Many renders in one launch sequence (dr_render_batch).  A job is a dict
+    with scene (DeviceScene), optional view (View), options
+    (DeviceScene.options), image, optional d_image / grads / dx_image (torch
+    CUDA tensors); jobs with d_image are forward + adjoint."""
+
+    def __init__(self):
+        self.lib = _abi.load_library()
+        h = ctypes.c_void_p()
+        _abi.check(self.lib, self.lib.dr_batch_create(ctypes.byref(h)), "dr_batch_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.dr_batch_destroy(h)
+            self.handle = None
+
+    @staticmethod
+    def _jobs(jobs):
+        arr = (_abi.dr_job * len(jobs))()
+        for i, j in enumerate(jobs):
+            a = arr[i]
+            a.scene = j["scene"].handle
+            a.view = j["view"].handle if j.get("view") is not None else None
+            a.opts = j["options"]
+            ptr = lambda t: t.data_ptr() if t is not None else None
+            a.d_image = ptr(j.get("d_image"))
+            a.image = ptr(j["image"])
+            a.grads = ptr(j.get("grads"))
+            a.dx_image = ptr(j.get("dx_image"))
+        return arr
+
+    def render(self, jobs, graph=False, stream=None):
+        arr = self._jobs(jobs)
+        _abi.check(self.lib, self.lib.dr_render_batch(self.handle, arr, len(jobs),
+                                                      _abi.DR_BATCH_GRAPH if graph else 0, _stream_ptr(stream)),
+                   "dr_render_batch")
+
+    def autotune(self, jobs, stream=None):
+        arr = self._jobs(jobs)
+        t = _abi.dr_tuning()
+        _abi.check(self.lib, self.lib.dr_batch_autotune(self.handle, arr, len(jobs), _stream_ptr(stream),
+                                                        ctypes.byref(t)), "dr_batch_autotune")
+        return {"pass_samples": t.pass_samples, "min_threads": t.min_threads, "best_ms": t.best_ms,
+                "candidates": t.candidates}
+
+    def set_tuning(self, pass_samples, min_threads):
+        t = _abi.dr_tuning(pass_samples, min_threads, 0.0, 0)
+        _abi.check(self.lib, self.lib.dr_batch_set_tuning(self.handle, ctypes.byref(t)), "dr_batch_set_tuning")
 
 
 def _stream_ptr(stream):

@@ -241,4 +241,115 @@ def _orient_fast(v, ind, want):
     return out
 
 
+def icosphere(level):
+    """Unit icosphere: 20 * 4**level triangles (outward winding), with its
+    vertex normals (the positions)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    v = [np.asarray(p, np.float64) / np.linalg.norm(p) for p in v]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    for _ in range(level):
+        mid = {}
+
+        def midpoint(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                m = v[a] + v[b]
+                v.append(m / np.linalg.norm(m))
+                mid[k] = len(v) - 1
+            return mid[k]
+        nf = []
+        for a, b, c in f:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    V = np.asarray(v, np.float32)
+    F = np.asarray(f, np.int32)
+    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    flip = np.sum(np.cross(p1 - p0, p2 - p0) * (p0 + p1 + p2), axis=1) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V, F, V.copy()
+
+
+def random_affine(rng):
+    """Rotation * anisotropic scale with det in [0.5, 2], as a 3x3 matrix."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    while True:
+        sc = rng.uniform(0.7, 1.3, size=3)
+        if 0.5 <= np.prod(sc) <= 2.0:
+            return R @ np.diag(sc)
+
+
+def batched(seed=4, num_scenes=64, views=4, min_tris=1000, max_tris=200000, res_choices=(128, 192, 256, 384, 512),
+            spp_choices=(16, 32, 64), max_bounces=2):
+    """Uneven batch: scenes of log-uniform triangle counts (subdivided
+    icospheres or fBm heightfields under random affine maps, on a ground
+    quad), 1-4 area lights, diffuse or glossy materials, and `views` cameras
+    per scene at random resolutions and sample counts.  Returns a list of
+    scenes, each a dict of workload fields plus "views": [(camera,
+    resolution, spp)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(num_scenes):
+        target = float(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris))))
+        one = np.ones(2, np.float32)
+        glossy = rng.uniform() < 0.5
+        if glossy:
+            f0 = rng.uniform(0.04, 1.0)
+            mat = dr.Material(np.full(3, 0.5 * (1.0 - f0), np.float32), np.full(3, f0, np.float32),
+                              np.array([rng.uniform(0.05, 0.5)], np.float32), one, one, one, False)
+        else:
+            mat = diffuse_material(rng.uniform(0.2, 0.8, size=3))
+        A = random_affine(rng)
+        if rng.uniform() < 0.5:
+            # icosphere level closest to the target count (20 * 4**level)
+            level = int(np.clip(np.round(np.log(target / 20.0) / np.log(4.0)), 3, 7))
+            level = min(level, 6) if target < 3e5 else level
+            V, F, N = icosphere(level)
+            v = (V.astype(np.float64) @ A.T + np.array([0.0, 1.2, 0.0])).astype(np.float32)
+            nrm = (N.astype(np.float64) @ np.linalg.inv(A)).astype(np.float32)
+            nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+            obj = dr.Shape(v, F, None, nrm, mat, None)
+        else:
+            grid = int(np.clip(np.round(np.sqrt(target / 2.0)) + 1, 23, 317))
+            h = heightfield(seed=int(rng.integers(1 << 30)), grid=grid, resolution=(64, 64), num_samples=1)
+            hv = h["shapes"][0].vertices.astype(np.float64) * np.array([0.3, 1.0, 0.3])
+            v = (hv @ A.T + np.array([0.0, 1.0, 0.0])).astype(np.float32)
+            nrm = vertex_normals(v, h["shapes"][0].indices)
+            obj = dr.Shape(v, h["shapes"][0].indices, h["shapes"][0].uvs, nrm, mat, None)
+        grey = diffuse_material([0.5, 0.5, 0.5])
+        gv, gi = quad([0.0, -1.0, 0.0], [0.0, 1.0, 0.0], 12.0)
+        ground = dr.Shape(gv, gi, None, None, grey, None)
+        black = diffuse_material([0.0, 0.0, 0.0])
+        shapes, lights = [obj, ground], []
+        for _ in range(int(rng.integers(1, 5))):
+            ang = rng.uniform(0, 2 * np.pi)
+            c = np.array([3.0 * np.cos(ang), rng.uniform(4.0, 7.0), 3.0 * np.sin(ang)])
+            lv, li = quad(c, -c, rng.uniform(0.5, 1.5))
+            ls = dr.Shape(lv, li, None, None, black, None)
+            l = dr.Light(ls, np.full(3, rng.uniform(20.0, 60.0), np.float32))
+            ls.light = l
+            shapes.append(ls)
+            lights.append(l)
+        cams = []
+        for _ in range(views):
+            res = int(rng.choice(res_choices))
+            spp = int(rng.choice(spp_choices))
+            ang = rng.uniform(0, 2 * np.pi)
+            pos = [6.0 * np.cos(ang), rng.uniform(2.0, 4.0), 6.0 * np.sin(ang)]
+            cams.append((make_camera(pos, [0, 0.8, 0], [0, 1, 0], 45.0, (res, res)), (res, res), spp))
+        out.append(dict(name="batched_%d" % k, camera=cams[0][0], shapes=shapes, materials=[mat, grey, black],
+                        lights=lights, views=cams, max_bounces=max_bounces,
+                        num_triangles=int(sum(np.asarray(s.indices).shape[0] for s in shapes))))
+    return out
+
+
 WORKLOADS = {"single_triangle": single_triangle, "shadow": shadow, "heightfield": heightfield}

@@ -293,6 +293,57 @@ typedef struct dr_scene_info {
 } dr_scene_info;
 dr_status dr_scene_get_info(const dr_scene *scene, dr_scene_info *out);
 
+/* ---------------------------------------------------------------- batches
+ * Many renders of uneven sizes -- several scenes, several views of a scene,
+ * different resolutions and sample counts -- in one stream-ordered launch
+ * sequence: every stage (paths, edge selection, edge samples, Dirac sweep,
+ * primary edges, forward-only renders) is one kernel launch over all jobs.
+ * No reference counterpart: the reference renders one scene per call
+ * (delta_ray.cpp:1017-1570). */
+
+/* A scene seen by another camera (its own primary-edge list).  Must not
+ * outlive its scene. */
+typedef struct dr_view dr_view;
+dr_status dr_view_create(dr_scene *scene, const dr_camera *camera, dr_view **out);
+void dr_view_destroy(dr_view *view);
+
+typedef struct dr_job {
+    dr_scene *scene;
+    dr_view *view;               /* NULL: the scene's own camera */
+    dr_render_options opts;
+    const float *d_image;        /* device [H][W][3]; NULL: forward only */
+    float *image;                /* device [H][W][3] */
+    float *grads;                /* device, dr_scene_grad_size floats (adjoint jobs); jobs that
+                                  * share a buffer add into it (e.g. views of one scene) */
+    float *dx_image;             /* device [H][W][3] or NULL (DR_WRITE_DX_IMAGE) */
+} dr_job;
+
+typedef struct dr_batch dr_batch;
+dr_status dr_batch_create(dr_batch **out);
+void dr_batch_destroy(dr_batch *batch);
+
+/* DR_BATCH_GRAPH: capture the launches of this job list once in a hipGraph
+ * and replay it on later calls with the same jobs (same scenes, views, sizes,
+ * flags and buffers; seeds and cameras may change). */
+enum { DR_BATCH_GRAPH = 1 };
+/* Renders every job.  Every output buffer is cleared first.  When some job
+ * samples secondary edges the call ends with one host synchronisation of the
+ * stream (the edge queue's length is checked; an overflowing queue is grown
+ * and the jobs rendered again). */
+dr_status dr_render_batch(dr_batch *batch, const dr_job *jobs, int num_jobs, int flags, void *stream);
+
+/* Pass sizes of the pipeline: pixel samples per pass and the least number
+ * of threads a job is split into.  dr_batch_autotune renders the job list
+ * with each candidate (outputs are overwritten) and keeps the fastest. */
+typedef struct dr_tuning {
+    int64_t pass_samples;
+    int64_t min_threads;
+    double best_ms;              /* out: time of the chosen candidate */
+    int candidates;              /* out: candidates timed */
+} dr_tuning;
+dr_status dr_batch_autotune(dr_batch *batch, const dr_job *jobs, int num_jobs, void *stream, dr_tuning *out);
+dr_status dr_batch_set_tuning(dr_batch *batch, const dr_tuning *tuning);
+
 /* The reference's test_line_integral (delta_ray.cpp:1572-1836) on the GPU
  * with the device's LTC line sampler: four estimates of one line integral
  * (uniform / LTC-sampled position, cosine / glossy lobe) from num_samples
