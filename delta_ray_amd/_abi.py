@@ -94,6 +94,8 @@ class dr_tuning(ctypes.Structure):
 
 
 DR_BATCH_GRAPH = 1
+DR_MLP_FEATURES = 24
+DR_MLP_PARAMS = 5825
 
 DR_STATS_COUNTERS = 1
 DR_STATS_TIMING = 2
@@ -142,6 +144,14 @@ ENTRY_POINTS = [
     ("dr_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_job), ctypes.c_int, ctypes.c_void_p,
                                          ctypes.POINTER(dr_tuning)]),
     ("dr_batch_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_tuning)]),
+    ("dr_mlp_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("dr_mlp_destroy", None, [ctypes.c_void_p]),
+    ("dr_mlp_params", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("dr_mlp_infer", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                    ctypes.c_void_p]),
+    ("dr_mlp_train", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
     ("dr_test_line_integral", ctypes.c_int, [ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
                                              ctypes.c_void_p]),
     ("dr_last_error", ctypes.c_char_p, []),

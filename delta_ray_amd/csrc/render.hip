@@ -557,6 +557,8 @@ static dr_status fail(dr_status s, const std::string &msg) {
     g_err = msg;
     return s;
 }
+// error reporting for the library's other translation units
+extern "C" dr_status dr_internal_fail(dr_status s, const char *msg) { return fail(s, msg ? msg : "?"); }
 
 // kernel timing: event pairs recorded around launches when enabled
 enum { TK_FORWARD = 0, TK_PATH = 1, TK_PRIMARY = 2, TK_SELECT = 3, TK_EDGE = 4, TK_DP = 5, TK_COUNT = 6 };

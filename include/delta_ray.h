@@ -344,6 +344,29 @@ typedef struct dr_tuning {
 dr_status dr_batch_autotune(dr_batch *batch, const dr_job *jobs, int num_jobs, void *stream, dr_tuning *out);
 dr_status dr_batch_set_tuning(dr_batch *batch, const dr_tuning *tuning);
 
+/* ------------------------------------------------ learned edge proposal
+ * A small network, 24 per-edge features -> 64 -> 64 -> 1 (ReLU), whose
+ * output is the log of a per-edge sampling weight.  No reference
+ * counterpart.  Inference on the matrix cores (bf16 weights, fp32
+ * accumulation); training on the GPU (fp32 master weights, Adam) against
+ * target log-weights with per-edge loss weights (0 = no data).  Parameter
+ * layout (fp32): W1 [24][64], b1 [64], W2 [64][64] (in x out), b2 [64],
+ * w3 [64], b3. */
+#define DR_MLP_FEATURES 24
+#define DR_MLP_PARAMS 5825
+typedef struct dr_edge_mlp dr_edge_mlp;
+dr_status dr_mlp_create(uint64_t seed, dr_edge_mlp **out);
+void dr_mlp_destroy(dr_edge_mlp *mlp);
+/* get (host, DR_MLP_PARAMS floats) and / or set the fp32 parameters */
+dr_status dr_mlp_params(dr_edge_mlp *mlp, float *get, const float *set);
+/* logits[i] for features[i][24] (device buffers) */
+dr_status dr_mlp_infer(dr_edge_mlp *mlp, const float *features, int64_t n, float *logits, void *stream);
+/* steps Adam steps on random minibatches of `batch` rows (device buffers);
+ * last_loss (host, may be NULL) receives the last step's weighted mean
+ * squared error */
+dr_status dr_mlp_train(dr_edge_mlp *mlp, const float *features, const float *targets, const float *weights,
+                       int64_t n, int steps, int batch, float lr, uint64_t seed, float *last_loss, void *stream);
+
 /* The reference's test_line_integral (delta_ray.cpp:1572-1836) on the GPU
  * with the device's LTC line sampler: four estimates of one line integral
  * (uniform / LTC-sampled position, cosine / glossy lobe) from num_samples
