@@ -68,7 +68,7 @@ def test_render_function_matches_c_abi(gpu):
     g = torch.empty((scene.grad_size,), device=gpu)
     scene.render_adjoint(o, d.to(gpu), ref_img, g)
     torch.cuda.synchronize()
-    np.testing.assert_allclose(img.numpy(), ref_img.cpu().numpy(), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(img.detach().numpy(), ref_img.cpu().numpy(), rtol=1e-5, atol=1e-6)
     _, d_shapes, _, d_lights = dr.split_grads(g.cpu().numpy(), scene.sd.layout)
     np.testing.assert_allclose(v.grad.numpy(), d_shapes[0].vertices, rtol=1e-4, atol=1e-5)
     np.testing.assert_allclose(inten.grad.numpy(), d_lights[0].intensity, rtol=1e-4, atol=1e-6)
