@@ -192,43 +192,44 @@ DR_INLINE float node_importance(const DrEdgeNode &n, float bound, F3 p) {
     return bound * n.total_length / length_sq(c - p);
 }
 
-// Edge-tree traversal (edge_tree.cpp:511-683), iterative: a stack of
-// pending right children reproduces the recursion's depth-first order and
-// therefore its order of random draws.  The traversal of one shading point is
-// a cone pass then an "all" pass over the chosen tree; it is kept as explicit
-// state advanced one node per et_step() so that a persistent kernel can give a
-// lane a new point as soon as its traversal ends.  et_step() calls
-// `emit(edge, weight, in_normal_tree, index)` for every selected silhouette
-// edge, in order.
+// Edge-tree traversal (edge_tree.cpp:511-683), iterative.  The reference
+// walks the tree twice per shading point, a "cone" pass and an "all" pass,
+// depth first.  Here both passes walk together: a stack entry carries a pass
+// mask, a node both passes reach (all of them where the point lies inside the
+// boxes, which split in both passes) is expanded once, and each pass still
+// meets its nodes in its own depth-first order.  The cone pass draws from the
+// point's stream, the "all" pass from a stream of its own; selected edges get
+// ordinals counted per pass (bit 15 set for the "all" pass), which sort like
+// the reference's sequential order.  The traversal is explicit state advanced
+// one node per et_step() so that a persistent kernel can give a lane a new
+// point as soon as its traversal ends; et_step() calls
+// `emit(edge, weight, in_normal_tree, index, ordinal)` per selected edge.
 struct TreeEntry {
     int node;
     float bound, cone_pmf, all_pmf;
+    int mask;  // bit 0: the cone pass reaches the node, bit 1: the "all" pass
 };
 
 struct EdgeTraversal {
     const DrEdgeNode *nodes;
     const DrEdge *edges;
     int num_nodes;
-    bool directional, cone_mode;
+    bool directional;
     float root_bound, pmf;
     float roughness, dlum, slum;
     Cone cone;
+    Philox rc, ra;  // draws of the cone pass / the "all" pass
+    int jc, ja;     // ordinals of the next edge of each pass
     int sp_, pops;
     uint32_t n_inner, n_edges;
     TreeEntry stack[DR_TREE_STACK];
 };
 
-DR_INLINE void et_start_pass(EdgeTraversal &T, bool cone_mode) {
-    T.cone_mode = cone_mode;
-    T.sp_ = 0;
-    T.pops = 0;
-    T.stack[T.sp_++] = TreeEntry{0, T.root_bound, T.pmf, T.pmf};
-}
-
-// edge_tree.cpp:685-725: choose the tree (one draw when both exist) and set
-// up the cone pass.  Returns false when there is nothing to traverse.
+// edge_tree.cpp:685-725: choose the tree (one draw from rc when both exist)
+// and push the root for both passes.  Returns false when there is nothing
+// to traverse.
 DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp,
-                       const Cone &cone, Philox &rng) {
+                       const Cone &cone) {
     if (dot(wi, sp.fr.n) < 0.f) return false;
     bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;
     if (!has_n && !has_s) return false;
@@ -240,7 +241,7 @@ DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isec
     } else {
         float ni = S.nrm_nodes[0].total_length, si = S.sil_nodes[0].total_length;
         float npmf = ni / (ni + si), spmf = si / (ni + si);
-        float u = philox_next(rng);
+        float u = philox_next(T.rc);
         if (u < npmf) {
             T.nodes = S.nrm_nodes; T.edges = S.nrm_edges; T.num_nodes = S.num_nrm_nodes; T.directional = true;
             T.pmf = npmf;
@@ -257,25 +258,24 @@ DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isec
     T.root_bound = bsdf_bound(S, T.nodes[0], wi, sp, T.roughness, T.dlum, T.slum);
     T.n_inner = 0;
     T.n_edges = 0;
-    et_start_pass(T, true);
+    T.jc = 0;
+    T.ja = 0;
+    T.pops = 0;
+    T.sp_ = 0;
+    T.stack[T.sp_++] = TreeEntry{0, T.root_bound, T.pmf, T.pmf, 3};
     return true;
 }
 
-// One node of the current pass; at the end of the cone pass the "all" pass
-// starts.  Returns false when both passes are done.
+// One node.  Returns false when the traversal is done.
 template <class Emit>
-DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoint &sp, Philox &rng, Emit &emit) {
-    // each node is popped at most once per pass; the cap guards a malformed tree
-    if (T.sp_ == 0 || T.pops >= 2 * T.num_nodes + 8) {
-        if (!T.cone_mode) {
-            if (S.stats) {
-                atomicAdd(&S.stats[5], (unsigned long long)T.n_inner);
-                atomicAdd(&S.stats[6], (unsigned long long)T.n_edges);
-            }
-            return false;
+DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoint &sp, Emit &emit) {
+    // every node is pushed at most once; the cap guards a malformed tree
+    if (T.sp_ == 0 || T.pops >= T.num_nodes + 8) {
+        if (S.stats) {
+            atomicAdd(&S.stats[5], (unsigned long long)T.n_inner);
+            atomicAdd(&S.stats[6], (unsigned long long)T.n_edges);
         }
-        et_start_pass(T, false);
-        return true;
+        return false;
     }
     T.pops++;
     TreeEntry e = T.stack[--T.sp_];
@@ -283,11 +283,12 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
     const DrEdgeNode &node = T.nodes[e.node];
     if (node.nprims > 0) {
         float den = e.cone_pmf * e.cone_pmf + e.all_pmf * e.all_pmf;
-        float w = (T.cone_mode ? e.cone_pmf : e.all_pmf) / den;
         T.n_edges += node.nprims;
         for (int i = 0; i < node.nprims; i++) {
             const DrEdge &ed = T.edges[node.offset + i];
-            if (is_silhouette(S, sp.pos, ed)) emit(ed, w, T.directional, node.offset + i);
+            if (!is_silhouette(S, sp.pos, ed)) continue;
+            if (e.mask & 1) emit(ed, e.cone_pmf / den, T.directional, node.offset + i, T.jc++);
+            if (e.mask & 2) emit(ed, e.all_pmf / den, T.directional, node.offset + i, 0x8000 | T.ja++);
         }
         return true;
     }
@@ -309,7 +310,6 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
     bool rhc = aabb_cone_intersect(R.lo, R.hi, T.cone);
     bool cone_split = all_split || (lhc && rhc && lcs && rcs && T.cone.cos_angle > 0.95f);
     float ilc = lhc ? il : 0.f, irc = rhc ? ir : 0.f;
-    if (T.cone_mode && ilc + irc <= 1e-10f) return true;
     float plc = 0.f, prc = 0.f, pla, pra;
     if (cone_split) {
         plc = prc = 1.f;
@@ -323,28 +323,30 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
         pla = il / (il + ir);
         pra = 1.f - pla;
     }
-    bool split = T.cone_mode ? cone_split : all_split;
-    TreeEntry le = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf};
-    TreeEntry re = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf};
-    if (split) {
-        if (rcs && T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = re;
-        if (lcs && T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = le;
-    } else {
-        float u = philox_next(rng);
-        bool left = T.cone_mode ? u < plc : u < pla;
-        if (T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = left ? le : re;
+    int lm = 0, rm = 0;
+    if ((e.mask & 1) && ilc + irc > 1e-10f) {
+        if (cone_split) {
+            if (rcs) rm |= 1;
+            if (lcs) lm |= 1;
+        } else if (philox_next(T.rc) < plc) {
+            lm |= 1;
+        } else {
+            rm |= 1;
+        }
     }
+    if (e.mask & 2) {
+        if (all_split) {
+            if (rcs) rm |= 2;
+            if (lcs) lm |= 2;
+        } else if (philox_next(T.ra) < pla) {
+            lm |= 2;
+        } else {
+            rm |= 2;
+        }
+    }
+    if (rm && T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf, rm};
+    if (lm && T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf, lm};
     return true;
-}
-
-// Both passes at one shading point.
-template <class Emit>
-DR_INLINE void sample_shading_edges_idx(const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp, const Cone &cone,
-                                        Philox &rng, Emit &emit) {
-    EdgeTraversal T;
-    if (!et_init(T, S, wi, is, sp, cone, rng)) return;
-    while (et_step(T, S, wi, sp, rng, emit)) {
-    }
 }
 
 struct EdgeResult {

@@ -355,10 +355,11 @@ DR_INLINE uint64_t sample_id(const SampleCtx &c, int xi, int yi, int s) {
 // The random streams, light cone and tree choice of one point's selection;
 // false when the point selects nothing.
 DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, uint64_t sid, int b, const PointRec &p,
-                            EdgeTraversal &T, Philox &prng) {
-    philox_init(prng, c.seed, 0, sid, dr_stream_point(b));
-    philox_next(prng);  // bsdf component of the point's edge sample
-    philox_next(prng);  // position on the edge
+                            EdgeTraversal &T) {
+    philox_init(T.rc, c.seed, 0, sid, dr_stream_point(b));
+    philox_next(T.rc);  // bsdf component of the point's edge sample
+    philox_next(T.rc);  // position on the edge
+    philox_init(T.ra, c.seed, 0, sid, dr_stream_point_all(b));
     // the primary light sample of this pixel sample: its light_sel is the
     // third draw after the jitter of the main stream
     Philox mr;
@@ -368,23 +369,25 @@ DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, uint64_t s
     LightSample ls = draw_light_sample(mr);
     Cone cone = light_cone(S, ls.light_sel, p.sp.pos);
     Isect is = Isect{p.shape, p.tri, p.gtri};
-    return et_init(T, S, p.wi, is, p.sp, cone, prng);
+    return et_init(T, S, p.wi, is, p.sp, cone);
 }
 
+// Stage B: edge selection at one secondary point (edge_tree.cpp:685-725);
+// emit(tree, edge_index, weight, ordinal) per selected edge.  Returns the
+// number of selected edges.
 template <class Emit>
 DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, uint64_t sid, int b, const PointRec &p,
                            Emit &emit) {
     EdgeTraversal T;
-    Philox prng;
-    if (!select_begin(S, c, sid, b, p, T, prng)) return 0;
-    int j = 0;
-    auto em = [&](const DrEdge &e, float w, bool nrm, int idx) {
+    if (!select_begin(S, c, sid, b, p, T)) return 0;
+    int n = 0;
+    auto em = [&](const DrEdge &e, float w, bool nrm, int idx, int j) {
         emit(nrm, idx, w, j);
-        j++;
+        n++;
     };
-    while (et_step(T, S, p.wi, p.sp, prng, em)) {
+    while (et_step(T, S, p.wi, p.sp, em)) {
     }
-    return j;
+    return n;
 }
 
 // Stage C: one selected edge (reference delta_ray.cpp:904-966).  Scatters

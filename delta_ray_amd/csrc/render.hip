@@ -112,7 +112,7 @@ struct EdgeEntry {
     int slot;  // point slot in the pass
     int edge;  // index into the tree's edge list; bit 31 set: normal tree
     float w;
-    int j;     // ordinal of the edge in the point's selection
+    int j;     // ordinal of the edge in its pass (bit 15: the "all" pass)
 };
 
 struct SlotDecode {
@@ -203,7 +203,6 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
                                                      int *counts) {
     const int lane = (int)__lane_id();
     EdgeTraversal T;
-    Philox prng;
     SPoint sp;
     F3 wi = F3{0.f, 0.f, 0.f};
     const DrSceneView *Sp = &jobs[0].S;
@@ -218,10 +217,10 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
             if ((long long)(base + e) < cap) q[base + e] = buf[e];
         nbuf = 0;
     };
-    auto emit = [&](const DrEdge &ed, float w, bool nrm, int idx) {
+    auto emit = [&](const DrEdge &ed, float w, bool nrm, int idx, int ord) {
         // zero-weight edges carry no work (reference delta_ray.cpp:906-908)
         if (w != 0.f) {
-            buf[nbuf++] = EdgeEntry{(int)slot, idx | (nrm ? (int)0x80000000 : 0), w, j};
+            buf[nbuf++] = EdgeEntry{(int)slot, idx | (nrm ? (int)0x80000000 : 0), w, ord};
             if (nbuf == 8) flush();
         }
         j++;
@@ -247,7 +246,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
                         const DrJob &J = jobs[d.job];
                         Sp = &J.S;
                         if (!(J.c.flags & DR_SKIP_SECONDARY_EDGES))
-                            busy = select_begin(*Sp, J.c, d.sid, d.b, p, T, prng);
+                            busy = select_begin(*Sp, J.c, d.sid, d.b, p, T);
                     }
                     if (busy) {
                         slot = s;
@@ -264,7 +263,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
             if (!more) break;
             continue;
         }
-        if (!et_step(T, *Sp, wi, sp, prng, emit)) {
+        if (!et_step(T, *Sp, wi, sp, emit)) {
             if (nbuf > 0) flush();
             counts[slot] = j;
             if (Sp->stats) atomicAdd(&Sp->stats[3], (unsigned long long)j);

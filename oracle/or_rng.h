@@ -69,13 +69,15 @@ struct Philox {
 //   STREAM_MAIN: jitter (2), light sample (4), per bounce: light sample (4),
 //                bsdf sample (3), russian roulette (1)
 //   stream_point(b): secondary shading point at bounce b: edge sample (2),
-//                edge-tree choice and traversal draws
+//                edge-tree choice and the cone pass's traversal draws
+//   stream_point_all(b): the "all" pass's traversal draws at that point
 //   stream_edge(b, j): j-th selected edge at bounce b: light sample (4),
 //                per remaining bounce (8)
 // A primary-edge sample uses STREAM_MAIN of pass 1: edge choice (1), edge
 // sample (2), light sample (4), per bounce (8).
 static const uint32_t STREAM_MAIN = 0;
 inline uint32_t stream_point(int b) { return 1u + (uint32_t)b; }
+inline uint32_t stream_point_all(int b) { return 0x100u + (uint32_t)b; }
 inline uint32_t stream_edge(int b, int j) { return (1u << 20) | ((uint32_t)b << 16) | (uint32_t)(j & 0xffff); }
 
 }  // namespace oracle

@@ -694,9 +694,13 @@ bool is_silhouette(const Scene &sc, const V3f &p, const Edge &e) {
 }
 
 // ------------------------------------------------------- edge sampling ---
+// A selected edge, its weight and its ordinal j: the index among the
+// edges selected by its pass, with bit 15 set for the "all" pass (so that
+// ordinals sort like the reference's sequential selection: cone pass first).
 struct EdgeW {
     Edge e;
     float w;
+    int j;
 };
 
 static float min_sq_bound(float lo, float hi) {
@@ -792,6 +796,7 @@ struct TraverseCtx {
     Cone cone;
     std::vector<EdgeW> *out;
     Philox *rng;
+    int jc = 0, ja = 0;  // ordinals of the cone and the "all" pass
 };
 
 // edge_tree.cpp:511-652
@@ -804,7 +809,8 @@ void sample_edge_rec(TraverseCtx &c, bool cone_mode, int node_id, float nb, floa
                             : all_pmf / (cone_pmf * cone_pmf + all_pmf * all_pmf);
         for (int i = 0; i < node.nprims; i++) {
             const Edge &e = t.edges[node.offset + i];
-            if (is_silhouette(*c.sc, c.sp.pos, e)) c.out->push_back(EdgeW{e, w});
+            if (is_silhouette(*c.sc, c.sp.pos, e))
+                c.out->push_back(EdgeW{e, w, cone_mode ? c.jc++ : (0x8000 | c.ja++)});
         }
         return;
     }
@@ -851,8 +857,12 @@ void sample_edge_rec(TraverseCtx &c, bool cone_mode, int node_id, float nb, floa
     }
 }
 
+// The cone pass draws from `rng` (the point's stream), the "all" pass from
+// `rng_all` (a stream of its own), so that the two passes are independent
+// and may be walked together.
 void sample_tree(const Scene &sc, const EdgeTree &tree, bool directional, const V3f &wi, const Isect &is,
-                 const SP<float> &sp, const Cone &cone, float pmf, std::vector<EdgeW> &out, Philox &rng) {
+                 const SP<float> &sp, const Cone &cone, float pmf, std::vector<EdgeW> &out, Philox &rng,
+                 Philox &rng_all) {
     if (tree.nodes.empty()) return;
     int mid = sc.shapes[is.shape].mat;
     const Material &m = sc.mats[mid];
@@ -861,26 +871,27 @@ void sample_tree(const Scene &sc, const EdgeTree &tree, bool directional, const 
     TraverseCtx c{&sc, &tree, directional, wi, mid, sp, dlum, slum, cone, &out, &rng};
     float b = bsdf_bound(sc, tree.nodes[0], wi, mid, sp, dlum, slum);
     sample_edge_rec(c, true, 0, b, pmf, pmf);
+    c.rng = &rng_all;
     sample_edge_rec(c, false, 0, b, pmf, pmf);
 }
 
 // edge_tree.cpp:685-725
 void sample_shading_edges(const Scene &sc, const V3f &wi, const Isect &is, const SP<float> &sp, const Cone &cone,
-                          std::vector<EdgeW> &out, Philox &rng) {
+                          std::vector<EdgeW> &out, Philox &rng, Philox &rng_all) {
     if (dot(wi, sp.fr[2]) < 0.f) return;
     if (sc.nrm_tree.nodes.empty()) {
-        sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, 1.f, out, rng);
+        sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, 1.f, out, rng, rng_all);
         return;
     }
     if (sc.sil_tree.nodes.empty()) {
-        sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, 1.f, out, rng);
+        sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, 1.f, out, rng, rng_all);
         return;
     }
     float ni = sc.nrm_tree.nodes[0].total_length, si = sc.sil_tree.nodes[0].total_length;
     float npmf = ni / (ni + si), spmf = si / (ni + si);
     float u = rng.next();
-    if (u < npmf) sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, npmf, out, rng);
-    else sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, spmf, out, rng);
+    if (u < npmf) sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, npmf, out, rng, rng_all);
+    else sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, spmf, out, rng, rng_all);
 }
 
 struct EdgeResult {
@@ -1291,12 +1302,13 @@ V3f render_pixel_sample(const RenderCtx &rc, int xi, int yi, int s, Accum &acc, 
             float cos_a = tan_a <= 1.f ? 1.f / std::sqrt(tan_a * tan_a + 1.f) : 0.f;
             Cone cone{p.spf.pos, normalize(dl), cos_a};
             edges.clear();
-            sample_shading_edges(sc, p.wi, p.is, p.spf, cone, edges, prng);
+            Philox prng_all(o.seed, 0, sid, stream_point_all(p.bounce));
+            sample_shading_edges(sc, p.wi, p.is, p.spf, cone, edges, prng, prng_all);
             last_edge_count = (int)edges.size();
-            for (int j = 0; j < (int)edges.size(); j++) {
-                const EdgeW &ew = edges[j];
+            for (int k = 0; k < (int)edges.size(); k++) {
+                const EdgeW &ew = edges[k];
                 if (ew.w == 0.f) continue;
-                Philox erng(o.seed, 0, sid, stream_edge(p.bounce, j));
+                Philox erng(o.seed, 0, sid, stream_edge(p.bounce, ew.j));
                 LightSample els = draw_light_sample(erng);
                 std::vector<SecondarySample> ess;
                 for (int i = 0; i < o.max_bounces - p.bounce - 1; i++) ess.push_back(draw_secondary(erng));
