@@ -24,6 +24,8 @@ DR_SKIP_INTERIOR = 1 << 8
 DR_SKIP_SECONDARY_EDGES = 1 << 9
 DR_SKIP_PRIMARY_EDGES = 1 << 10
 DR_WRITE_DX_IMAGE = 1 << 11
+DR_LEARNED_EDGES = 1 << 12
+DR_EDGE_STATS = 1 << 13
 DR_FLAGS_PARITY = (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP |
                    DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE)
 
@@ -144,6 +146,10 @@ ENTRY_POINTS = [
     ("dr_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_job), ctypes.c_int, ctypes.c_void_p,
                                          ctypes.POINTER(dr_tuning)]),
     ("dr_batch_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_tuning)]),
+    ("dr_scene_edge_count", ctypes.c_int64, [ctypes.c_void_p]),
+    ("dr_scene_edge_features", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("dr_scene_set_edge_weights", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("dr_scene_edge_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     ("dr_mlp_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     ("dr_mlp_destroy", None, [ctypes.c_void_p]),
     ("dr_mlp_params", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

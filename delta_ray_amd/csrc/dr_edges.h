@@ -191,6 +191,11 @@ DR_INLINE float node_importance(const DrEdgeNode &n, float bound, F3 p) {
     F3 c = 0.5f * (F3{n.lo[0], n.lo[1], n.lo[2]} + F3{n.hi[0], n.hi[1], n.hi[2]});
     return bound * n.total_length / length_sq(c - p);
 }
+// the same with another node length (the learned proposal's)
+DR_INLINE float node_importance_len(const DrEdgeNode &n, float bound, F3 p, float len) {
+    F3 c = 0.5f * (F3{n.lo[0], n.lo[1], n.lo[2]} + F3{n.hi[0], n.hi[1], n.hi[2]});
+    return bound * len / length_sq(c - p);
+}
 
 // Edge-tree traversal (edge_tree.cpp:511-683), iterative.  The reference
 // walks the tree twice per shading point, a "cone" pass and an "all" pass,
@@ -207,6 +212,7 @@ DR_INLINE float node_importance(const DrEdgeNode &n, float bound, F3 p) {
 struct TreeEntry {
     int node;
     float bound, cone_pmf, all_pmf;
+    float cone_alt, all_alt;  // the same path's pmfs under the other proposal
     int mask;  // bit 0: the cone pass reaches the node, bit 1: the "all" pass
 };
 
@@ -220,6 +226,10 @@ struct EdgeTraversal {
     Cone cone;
     Philox rc, ra;  // draws of the cone pass / the "all" pass
     int jc, ja;     // ordinals of the next edge of each pass
+    // proposal: 0 the reference's (node length = edge lengths); with the
+    // learned mixture, 1 = this point uses the learned node lengths, 2 = the
+    // plain ones (probabilities 0.8 / 0.2)
+    int mode;
     int sp_, pops;
     uint32_t n_inner, n_edges;
     TreeEntry stack[DR_TREE_STACK];
@@ -229,7 +239,7 @@ struct EdgeTraversal {
 // and push the root for both passes.  Returns false when there is nothing
 // to traverse.
 DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp,
-                       const Cone &cone) {
+                       const Cone &cone, int mode) {
     if (dot(wi, sp.fr.n) < 0.f) return false;
     bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;
     if (!has_n && !has_s) return false;
@@ -262,9 +272,12 @@ DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isec
     T.ja = 0;
     T.pops = 0;
     T.sp_ = 0;
-    T.stack[T.sp_++] = TreeEntry{0, T.root_bound, T.pmf, T.pmf, 3};
+    T.mode = mode;
+    T.stack[T.sp_++] = TreeEntry{0, T.root_bound, T.pmf, T.pmf, T.pmf, T.pmf, 3};
     return true;
 }
+
+#define DR_LEARNED_SHARE 0.8f
 
 // One node.  Returns false when the traversal is done.
 template <class Emit>
@@ -282,13 +295,22 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
     if (e.bound <= 0.f) return true;
     const DrEdgeNode &node = T.nodes[e.node];
     if (node.nprims > 0) {
+        // power heuristic over the strategies that could have selected the
+        // edge: the two passes (and, with the learned mixture, the two
+        // proposals, run with probabilities pi and 1 - pi)
         float den = e.cone_pmf * e.cone_pmf + e.all_pmf * e.all_pmf;
+        float pi = 1.f;
+        if (T.mode != 0) {
+            pi = T.mode == 1 ? DR_LEARNED_SHARE : 1.f - DR_LEARNED_SHARE;
+            float po = 1.f - pi;
+            den = pi * pi * den + po * po * (e.cone_alt * e.cone_alt + e.all_alt * e.all_alt);
+        }
         T.n_edges += node.nprims;
         for (int i = 0; i < node.nprims; i++) {
             const DrEdge &ed = T.edges[node.offset + i];
             if (!is_silhouette(S, sp.pos, ed)) continue;
-            if (e.mask & 1) emit(ed, e.cone_pmf / den, T.directional, node.offset + i, T.jc++);
-            if (e.mask & 2) emit(ed, e.all_pmf / den, T.directional, node.offset + i, 0x8000 | T.ja++);
+            if (e.mask & 1) emit(ed, pi * e.cone_pmf / den, T.directional, node.offset + i, T.jc++);
+            if (e.mask & 2) emit(ed, pi * e.all_pmf / den, T.directional, node.offset + i, 0x8000 | T.ja++);
         }
         return true;
     }
@@ -305,6 +327,18 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
     float rb = rcs ? bsdf_bound(S, R, wi, sp, T.roughness, T.dlum, T.slum) : 0.f;
     float il = lcs ? node_importance(L, lb, sp.pos) : 0.f;
     float ir = rcs ? node_importance(R, rb, sp.pos) : 0.f;
+    // with the learned mixture: importances of the point's proposal (il,
+    // ir) and of the other one (il2, ir2), which only rescales node lengths
+    float il2 = il, ir2 = ir;
+    if (T.mode != 0) {
+        float ll = lcs ? node_importance_len(L, lb, sp.pos, L.pad) : 0.f;
+        float rl = rcs ? node_importance_len(R, rb, sp.pos, R.pad) : 0.f;
+        if (T.mode == 1) {
+            il = ll; ir = rl;
+        } else {
+            il2 = ll; ir2 = rl;
+        }
+    }
     if (il + ir <= 1e-10f) return true;
     bool lhc = aabb_cone_intersect(L.lo, L.hi, T.cone);
     bool rhc = aabb_cone_intersect(R.lo, R.hi, T.cone);
@@ -322,6 +356,23 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
     } else {
         pla = il / (il + ir);
         pra = 1.f - pla;
+    }
+    // the other proposal's split probabilities on this path
+    float plc2 = 0.f, prc2 = 0.f, pla2 = 0.f, pra2 = 0.f;
+    if (T.mode != 0 && il2 + ir2 > 1e-10f) {
+        float ilc2 = lhc ? il2 : 0.f, irc2 = rhc ? ir2 : 0.f;
+        if (cone_split) {
+            plc2 = prc2 = 1.f;
+        } else if (ilc2 + irc2 > 1e-10f) {
+            plc2 = ilc2 / (ilc2 + irc2);
+            prc2 = 1.f - plc2;
+        }
+        if (all_split) {
+            pla2 = pra2 = 1.f;
+        } else {
+            pla2 = il2 / (il2 + ir2);
+            pra2 = 1.f - pla2;
+        }
     }
     int lm = 0, rm = 0;
     if ((e.mask & 1) && ilc + irc > 1e-10f) {
@@ -344,8 +395,10 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
             rm |= 2;
         }
     }
-    if (rm && T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf, rm};
-    if (lm && T.sp_ < DR_TREE_STACK) T.stack[T.sp_++] = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf, lm};
+    if (rm && T.sp_ < DR_TREE_STACK)
+        T.stack[T.sp_++] = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf, prc2 * e.cone_alt, pra2 * e.all_alt, rm};
+    if (lm && T.sp_ < DR_TREE_STACK)
+        T.stack[T.sp_++] = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf, plc2 * e.cone_alt, pla2 * e.all_alt, lm};
     return true;
 }
 

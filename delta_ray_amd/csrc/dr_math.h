@@ -255,6 +255,8 @@ DR_INLINE float philox_next(Philox &r) {
 DR_INLINE uint32_t dr_stream_point(int b) { return 1u + (uint32_t)b; }
 // the edge-tree "all" pass at the point of bounce b
 DR_INLINE uint32_t dr_stream_point_all(int b) { return 0x100u + (uint32_t)b; }
+// the choice between the plain and the learned edge proposal at that point
+DR_INLINE uint32_t dr_stream_point_mix(int b) { return 0x200u + (uint32_t)b; }
 DR_INLINE uint32_t dr_stream_edge(int b, int j) {
     return (1u << 20) | ((uint32_t)b << 16) | (uint32_t)(j & 0xffff);
 }

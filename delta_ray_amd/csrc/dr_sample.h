@@ -369,7 +369,13 @@ DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, uint64_t s
     LightSample ls = draw_light_sample(mr);
     Cone cone = light_cone(S, ls.light_sel, p.sp.pos);
     Isect is = Isect{p.shape, p.tri, p.gtri};
-    return et_init(T, S, p.wi, is, p.sp, cone);
+    int mode = 0;
+    if (c.flags & DR_LEARNED_EDGES) {
+        Philox mix;
+        philox_init(mix, c.seed, 0, sid, dr_stream_point_mix(b));
+        mode = philox_next(mix) < DR_LEARNED_SHARE ? 1 : 2;
+    }
+    return et_init(T, S, p.wi, is, p.sp, cone, mode);
 }
 
 // Stage B: edge selection at one secondary point (edge_tree.cpp:685-725);
@@ -395,7 +401,7 @@ DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, uint64_t si
 // the shading point).
 template <int MAXB>
 DR_INLINE F3 stage_edge(const DrSceneView &S, const SampleCtx &c, GradSink &G, uint64_t sid, int b,
-                        const PointRec &p, const DrEdge &edge, float w, int j, F3 d_color) {
+                        const PointRec &p, const DrEdge &edge, float w, int j, F3 d_color, float *mag = nullptr) {
     Philox prng;
     philox_init(prng, c.seed, 0, sid, dr_stream_point(b));
     float bcomp = philox_next(prng), eu = philox_next(prng);
@@ -414,6 +420,7 @@ DR_INLINE F3 stage_edge(const DrSceneView &S, const SampleCtx &c, GradSink &G, u
         G.add3(es.vgrad_off + 3ll * edge.v0, er.dv0);
         G.add3(es.vgrad_off + 3ll * edge.v1, er.dv1);
     }
+    if (mag) *mag = length(er.dv0) + length(er.dv1);
     return er.dp;
 }
 

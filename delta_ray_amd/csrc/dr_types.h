@@ -105,6 +105,8 @@ struct DrSceneView {
     const DrEdge *nrm_edges;
     const DrEdgeNode *nrm_nodes;
     int num_sil_nodes, num_nrm_nodes;
+    int num_sil_edges;
+    float *edge_stat;  // per tree edge (silhouette tree, then normal tree): sum of |contribution|, count
     const DrEdge *prim_edges;
     const float *prim_func;
     const float *prim_cdf;

@@ -293,7 +293,13 @@ DR_INLINE void edge_entry(const DrJob *jobs, const DrRange *R, int nr, const Edg
     G.gmask = DR_GC_SLOTS_BIG - 1;
     G.job = (unsigned)d.job;
     F3 d_color = load3(J.d_image + 3 * (size_t)d.pix);
-    F3 dp = stage_edge<MAXB>(J.S, J.c, G, d.sid, d.b, p, edge, E.w, E.j, d_color);
+    float mag = 0.f;
+    F3 dp = stage_edge<MAXB>(J.S, J.c, G, d.sid, d.b, p, edge, E.w, E.j, d_color, &mag);
+    if ((J.c.flags & DR_EDGE_STATS) && J.S.edge_stat) {
+        long long ge = (E.edge & 0x80000000) ? (long long)J.S.num_sil_edges + idx : (long long)idx;
+        if (mag != 0.f) atomicAdd(J.S.edge_stat + 2 * ge, mag);
+        atomicAdd(J.S.edge_stat + 2 * ge + 1, 1.f);
+    }
     bool nz = !is_zero(dp);
     if (J.c.flags & DR_QUIRK_DP_LAST) {
         entry_dp[e] = make_float4(dp.x, dp.y, dp.z, nz ? __int_as_float(E.j) : __int_as_float(-1));
@@ -1148,6 +1154,17 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
     v.num_nodes = (int)hs.nodes.size();
     v.num_sil_nodes = (int)hs.sil_nodes.size();
     v.num_nrm_nodes = (int)hs.nrm_nodes.size();
+    v.num_sil_edges = (int)hs.sil_edges.size();
+    {
+        size_t ne = hs.sil_edges.size() + hs.nrm_edges.size();
+        std::vector<float> zeros(2 * std::max<size_t>(ne, 1), 0.f);
+        const float *p = nullptr;
+        if (!upload(s, zeros, &p)) {
+            free_scene(s);
+            return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+        }
+        v.edge_stat = const_cast<float *>(p);
+    }
     v.cam = hs.cam;
     v.stats = nullptr;
     auto t1 = std::chrono::steady_clock::now();
@@ -1240,6 +1257,55 @@ dr_status dr_render_adjoint(dr_scene *s, const dr_render_options *o, const float
     if (st != DR_OK) return st;
     if (!d_image || !image || !grads) return fail(DR_ERROR_INVALID_ARGUMENT, "d_image, image and grads are required");
     return single_job(s, o, d_image, image, grads, dx_image, (hipStream_t)stream);
+}
+
+int64_t dr_scene_edge_count(const dr_scene *s) {
+    return s ? (int64_t)(s->hs.sil_edges.size() + s->hs.nrm_edges.size()) : 0;
+}
+
+dr_status dr_scene_edge_features(dr_scene *s, const dr_view *view, float *features, void *stream) {
+    if (!s || !features) return fail(DR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (view && view->scene != s) return fail(DR_ERROR_INVALID_ARGUMENT, "view of another scene");
+    std::vector<float> f;
+    drh::edge_features(s->hs, view ? view->cam : s->hs.cam, f);
+    if (f.empty()) return DR_OK;
+    hipStream_t hs = (hipStream_t)stream;
+    if (hipMemcpyAsync(features, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, hs) != hipSuccess ||
+        hipStreamSynchronize(hs) != hipSuccess)
+        return fail(DR_ERROR_DEVICE, "feature upload failed");
+    return DR_OK;
+}
+
+dr_status dr_scene_set_edge_weights(dr_scene *s, const float *weights) {
+    if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
+    drh::HostScene &hs = s->hs;
+    size_t ns = hs.sil_edges.size();
+    if (weights) {
+        drh::set_learned_lengths(hs, hs.sil_nodes, hs.sil_edges, weights);
+        drh::set_learned_lengths(hs, hs.nrm_nodes, hs.nrm_edges, weights + ns);
+    } else {
+        for (DrEdgeNode &n : hs.sil_nodes) n.pad = n.total_length;
+        for (DrEdgeNode &n : hs.nrm_nodes) n.pad = n.total_length;
+    }
+    if ((!hs.sil_nodes.empty() &&
+         hipMemcpy(const_cast<DrEdgeNode *>(s->view.sil_nodes), hs.sil_nodes.data(),
+                   hs.sil_nodes.size() * sizeof(DrEdgeNode), hipMemcpyHostToDevice) != hipSuccess) ||
+        (!hs.nrm_nodes.empty() &&
+         hipMemcpy(const_cast<DrEdgeNode *>(s->view.nrm_nodes), hs.nrm_nodes.data(),
+                   hs.nrm_nodes.size() * sizeof(DrEdgeNode), hipMemcpyHostToDevice) != hipSuccess))
+        return fail(DR_ERROR_DEVICE, "edge tree upload failed");
+    return DR_OK;
+}
+
+dr_status dr_scene_edge_stats(dr_scene *s, float *out, int reset, void *stream) {
+    if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
+    hipStream_t hs = (hipStream_t)stream;
+    size_t n = 2 * (size_t)dr_scene_edge_count(s);
+    if (n == 0) return DR_OK;
+    if (out && hipMemcpyAsync(out, s->view.edge_stat, n * sizeof(float), hipMemcpyDeviceToDevice, hs) != hipSuccess)
+        return fail(DR_ERROR_DEVICE, "statistics copy failed");
+    if (reset) k_fill<<<256, 256, 0, hs>>>(s->view.edge_stat, n, 0.f);
+    return launch_check("dr_scene_edge_stats");
 }
 
 dr_status dr_view_create(dr_scene *s, const dr_camera *c, dr_view **out) {

@@ -397,7 +397,11 @@ void build_edge_tree(const HostScene &hs, const std::vector<DrEdge> &edges, bool
             n.total_length = nodes[i + 1].total_length + nodes[n.offset].total_length;
         }
     }
+    // sampling length of the learned proposal: the plain length until
+    // weights are set
+    for (DrEdgeNode &n : nodes) n.pad = n.total_length;
 }
+
 
 // ------------------------------------------------------- camera helpers --
 V3 xfm_point(const float *a, V3 p) {
@@ -672,6 +676,95 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
     build_edge_tree(hs, sil, false, hs.sil_nodes, hs.sil_edges);
     build_edge_tree(hs, nrm, true, hs.nrm_nodes, hs.nrm_edges);
     return "";
+}
+
+void set_learned_lengths(const HostScene &hs, std::vector<DrEdgeNode> &nodes, const std::vector<DrEdge> &ordered,
+                         const float *w) {
+    for (int i = (int)nodes.size() - 1; i >= 0; i--) {
+        DrEdgeNode &n = nodes[i];
+        if (n.nprims > 0) {
+            float s = 0.f;
+            for (int k = 0; k < n.nprims; k++) {
+                const DrEdge &ed = ordered[n.offset + k];
+                s += std::max(w[n.offset + k], 0.f) * length(getv(hs, ed.shape, ed.v1) - getv(hs, ed.shape, ed.v0));
+            }
+            n.pad = s;
+        } else {
+            n.pad = nodes[i + 1].pad + nodes[n.offset].pad;
+        }
+    }
+    if (!nodes.empty() && nodes[0].pad > 0.f) {
+        float scale = nodes[0].total_length / nodes[0].pad;
+        for (DrEdgeNode &n : nodes) n.pad *= scale;
+    }
+}
+
+void edge_features(const HostScene &hs, const DrCamera &cam, std::vector<float> &out) {
+    const int F = 24;
+    size_t n = hs.sil_edges.size() + hs.nrm_edges.size();
+    out.assign(n * F, 0.f);
+    // scene scale and centre from the vertex bounds
+    V3 lo{1e30f, 1e30f, 1e30f}, hi{-1e30f, -1e30f, -1e30f};
+    for (size_t i = 0; i + 2 < hs.verts.size(); i += 3) {
+        V3 p{hs.verts[i], hs.verts[i + 1], hs.verts[i + 2]};
+        lo = V3{std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+        hi = V3{std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+    }
+    V3 ctr = 0.5f * (lo + hi);
+    float scale = std::max(1e-6f, length(hi - lo));
+    auto face_normal = [&](int shape, int f) {
+        V3 a = getv(hs, shape, tri_index(hs, shape, f, 0)), b = getv(hs, shape, tri_index(hs, shape, f, 1)),
+           c = getv(hs, shape, tri_index(hs, shape, f, 2));
+        V3 nn = cross(b - a, c - a);
+        float l = length(nn);
+        return l > 0.f ? nn / l : V3{0.f, 0.f, 0.f};
+    };
+    auto fill = [&](const std::vector<DrEdge> &edges, const std::vector<DrEdgeNode> &nodes, size_t base) {
+        // leaf of every edge of the tree, for the node features
+        std::vector<int> leaf(edges.size(), 0);
+        for (size_t i = 0; i < nodes.size(); i++)
+            if (nodes[i].nprims > 0)
+                for (int k = 0; k < nodes[i].nprims; k++) leaf[nodes[i].offset + k] = (int)i;
+        float root = nodes.empty() ? 1.f : std::max(nodes[0].total_length, 1e-12f);
+        for (size_t i = 0; i < edges.size(); i++) {
+            const DrEdge &e = edges[i];
+            float *o = &out[(base + i) * F];
+            V3 v0 = getv(hs, e.shape, e.v0), v1 = getv(hs, e.shape, e.v1);
+            V2 s0, s1;
+            if (project_edge(cam, v0, v1, s0, s1)) {
+                o[0] = s0.x; o[1] = s0.y; o[2] = s1.x; o[3] = s1.y; o[4] = 1.f;
+            }
+            V3 n0 = face_normal(e.shape, e.f0);
+            V3 n1 = e.f1 >= 0 ? face_normal(e.shape, e.f1) : n0;
+            o[5] = n0.x; o[6] = n0.y; o[7] = n0.z;
+            o[8] = n1.x; o[9] = n1.y; o[10] = n1.z;
+            o[11] = dot(n0, n1);
+            o[12] = e.f1 < 0 ? 1.f : 0.f;
+            o[13] = length(v1 - v0) / scale;
+            V3 mid = 0.5f * (v0 + v1);
+            o[14] = (mid.x - ctr.x) / scale; o[15] = (mid.y - ctr.y) / scale; o[16] = (mid.z - ctr.z) / scale;
+            float best = 1e30f;
+            V3 ldir{0.f, 1.f, 0.f};
+            for (const DrLight &L : hs.lights) {
+                V3 c{L.sphere_c[0], L.sphere_c[1], L.sphere_c[2]};
+                float d = length(c - mid);
+                if (d < best) {
+                    best = d;
+                    ldir = d > 0.f ? (c - mid) / d : V3{0.f, 1.f, 0.f};
+                }
+            }
+            o[17] = best / scale;
+            o[18] = dot(n0, ldir);
+            o[19] = dot(n1, ldir);
+            const DrEdgeNode &lf = nodes[leaf[i]];
+            o[20] = lf.total_length / root * (float)edges.size();
+            o[21] = lf.angle;
+            o[22] = hs.shapes[e.shape].has_n ? 1.f : 0.f;
+            o[23] = 1.f;
+        }
+    };
+    fill(hs.sil_edges, hs.sil_nodes, 0);
+    fill(hs.nrm_edges, hs.nrm_nodes, hs.sil_edges.size());
 }
 
 }  // namespace drh

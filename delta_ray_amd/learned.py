@@ -1,0 +1,108 @@
+"""Learned secondary-edge proposal for a DeviceScene (no reference counterpart).
+
+The edge tree chooses silhouette edges with node importances built from edge
+lengths and a BSDF bound (edge_tree.cpp:499-509).  Here a small network
+(dr_mlp_*: 24 per-edge features -> 64 -> 64 -> 1, bf16 weights on the matrix
+cores) predicts a per-edge weight; node lengths become sums of weight x edge
+length (dr_scene_set_edge_weights), and DR_LEARNED_EDGES renders mix that
+proposal with the plain one per shading point (probabilities 0.8 / 0.2) and
+weight every selected edge with the power heuristic over both proposals and
+both tree passes, so the estimator stays unbiased whatever the network
+predicts and the plain proposal remains part of every estimate.
+
+Training data come from the plain estimator itself: DR_EDGE_STATS renders
+accumulate, per edge, the number of secondary-edge samples and the sum of
+their |vertex-gradient contribution| (already divided by the selection
+probability).  Their ratio, the mean contribution per selection, is
+proportional to how much more often the edge should be chosen than the plain
+proposal chooses it; its log (centred) is the network's regression target,
+with a loss weight growing with the sample count.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class EdgeProposal:
+    def __init__(self, scene, view=None, seed=0):
+        import torch
+        self.torch = torch
+        self.lib = scene.lib
+        self.scene = scene
+        self.n = int(self.lib.dr_scene_edge_count(scene.handle))
+        dev = torch.device("cuda")
+        self.features = torch.zeros((max(self.n, 1), _abi.DR_MLP_FEATURES), dtype=torch.float32, device=dev)
+        vh = view.handle if view is not None else None
+        _abi.check(self.lib, self.lib.dr_scene_edge_features(scene.handle, vh, _ptr(self.features), None),
+                   "dr_scene_edge_features")
+        self.stats = torch.zeros((max(self.n, 1), 2), dtype=torch.float32, device=dev)
+        self.acc = torch.zeros((max(self.n, 1), 2), dtype=torch.float64, device=dev)
+        h = ctypes.c_void_p()
+        _abi.check(self.lib, self.lib.dr_mlp_create(seed, ctypes.byref(h)), "dr_mlp_create")
+        self.mlp = h
+        self.logits = torch.zeros(max(self.n, 1), dtype=torch.float32, device=dev)
+        self.seed = seed
+        self.trained_steps = 0
+
+    def __del__(self):
+        h = getattr(self, "mlp", None)
+        if h is not None and h.value:
+            self.lib.dr_mlp_destroy(h)
+            self.mlp = None
+
+    def reset_stats(self):
+        _abi.check(self.lib, self.lib.dr_scene_edge_stats(self.scene.handle, None, 1, None), "dr_scene_edge_stats")
+
+    def collect(self):
+        """Add the statistics of the DR_EDGE_STATS renders since the last
+        call to the training set (and clear them in the scene)."""
+        _abi.check(self.lib, self.lib.dr_scene_edge_stats(self.scene.handle, _ptr(self.stats), 1, None),
+                   "dr_scene_edge_stats")
+        self.torch.cuda.synchronize()
+        self.acc += self.stats.double()
+
+    def targets(self, full_weight_at=64.0):
+        t = self.torch
+        cnt = self.acc[:, 1]
+        mean = self.acc[:, 0] / t.clamp(cnt, min=1.0)
+        seen = cnt > 0
+        y = t.log(mean + 1e-12)
+        w = t.clamp(cnt / full_weight_at, max=1.0)
+        w = t.where(seen, w, t.zeros_like(w))
+        if bool(seen.any()):
+            centre = (y * w).sum() / w.sum()
+            y = t.where(seen, y - centre, t.zeros_like(y))
+        return y.float().contiguous(), w.float().contiguous()
+
+    def train(self, steps=300, batch=1024, lr=3e-3):
+        y, w = self.targets()
+        loss = ctypes.c_float(0.0)
+        _abi.check(self.lib, self.lib.dr_mlp_train(self.mlp, _ptr(self.features), _ptr(y), _ptr(w), self.n, steps,
+                                                   batch, lr, self.seed + self.trained_steps, ctypes.byref(loss),
+                                                   None), "dr_mlp_train")
+        self.trained_steps += steps
+        return loss.value
+
+    def apply(self, clip=(0.05, 20.0)):
+        """Network weights -> the scene's learned node lengths."""
+        _abi.check(self.lib, self.lib.dr_mlp_infer(self.mlp, _ptr(self.features), self.n, _ptr(self.logits), None),
+                   "dr_mlp_infer")
+        self.torch.cuda.synchronize()
+        lg = self.logits[:self.n].double().cpu().numpy()
+        wts = np.exp(lg - lg.max())
+        wts = wts / max(wts.mean(), 1e-30)
+        wts = np.clip(wts, clip[0], clip[1]).astype(np.float32)
+        _abi.check(self.lib, self.lib.dr_scene_set_edge_weights(self.scene.handle, wts.ctypes.data),
+                   "dr_scene_set_edge_weights")
+        return wts
+
+    def set_weights(self, wts):
+        w = np.ascontiguousarray(wts, np.float32)
+        _abi.check(self.lib, self.lib.dr_scene_set_edge_weights(self.scene.handle, w.ctypes.data),
+                   "dr_scene_set_edge_weights")

@@ -138,7 +138,14 @@ enum {
     DR_SKIP_PRIMARY_EDGES = 1u << 10,
     /* write per-pixel secondary edge counts / 1000 into dx_image
      * (delta_ray.cpp:1251-1254) */
-    DR_WRITE_DX_IMAGE = 1u << 11
+    DR_WRITE_DX_IMAGE = 1u << 11,
+    /* secondary edges: mix the plain edge-tree proposal (weight 0.2) with the
+     * learned one (node lengths weighted by dr_scene_set_edge_weights); no
+     * reference counterpart */
+    DR_LEARNED_EDGES = 1u << 12,
+    /* accumulate per-edge statistics of the secondary-edge samples
+     * (dr_scene_edge_stats) */
+    DR_EDGE_STATS = 1u << 13
 };
 #define DR_FLAGS_PARITY (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP | \
                          DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE)
@@ -366,6 +373,25 @@ dr_status dr_mlp_infer(dr_edge_mlp *mlp, const float *features, int64_t n, float
  * squared error */
 dr_status dr_mlp_train(dr_edge_mlp *mlp, const float *features, const float *targets, const float *weights,
                        int64_t n, int steps, int batch, float lr, uint64_t seed, float *last_loss, void *stream);
+
+/* Edges of the scene's edge trees (silhouette tree, then normal tree): the
+ * rows of the features, weights and statistics below. */
+int64_t dr_scene_edge_count(const dr_scene *scene);
+/* DR_MLP_FEATURES floats per edge (device buffer): screen-space endpoints
+ * and visibility for the camera of `view` (NULL: the scene's), adjacent face
+ * normals and their cosine, boundary flag, length, midpoint, distance and
+ * direction cosines to the nearest light, the edge's leaf length share and
+ * normal-cone angle, has-normals flag, 1.  Synchronises the stream. */
+dr_status dr_scene_edge_features(dr_scene *scene, const dr_view *view, float *features, void *stream);
+/* Per-edge sampling weights of the learned proposal (host, one per edge;
+ * NULL: all 1): node lengths become sums of weight x edge length
+ * (DR_LEARNED_EDGES renders use them). */
+dr_status dr_scene_set_edge_weights(dr_scene *scene, const float *weights);
+/* Statistics accumulated by DR_EDGE_STATS renders, two floats per edge
+ * (device buffer `out`, may be NULL): sum over the edge's secondary-edge
+ * samples of |vertex-gradient contribution| (weights included), and the
+ * number of samples.  reset: clear them afterwards. */
+dr_status dr_scene_edge_stats(dr_scene *scene, float *out, int reset, void *stream);
 
 /* The reference's test_line_integral (delta_ray.cpp:1572-1836) on the GPU
  * with the device's LTC line sampler: four estimates of one line integral

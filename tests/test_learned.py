@@ -1,0 +1,94 @@
+"""The learned secondary-edge proposal (DR_LEARNED_EDGES).  Nothing in the
+reference pins it; it is pinned by unbiasedness: whatever the per-edge
+weights, the mean secondary-edge gradient equals the plain estimator's within
+a few standard errors, because every selected edge is weighted by the power
+heuristic over both proposals (0.8 learned, 0.2 plain) and both tree passes."""
+import numpy as np
+import pytest
+
+from delta_ray_amd import _abi
+from delta_ray_amd import delta_ray as dr
+from delta_ray_amd.learned import EdgeProposal
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+SECONDARY = _abi.DR_SKIP_INTERIOR | _abi.DR_SKIP_PRIMARY_EDGES
+
+
+def _grads(torch, scene, w, spp, seeds, flags, d):
+    W, H = w["resolution"]
+    out = []
+    img = torch.zeros((H, W, 3), device="cuda")
+    g = torch.zeros((scene.grad_size,), device="cuda")
+    for s in seeds:
+        o = dr.DeviceScene.options(W, H, spp, w["max_bounces"], s, _abi.DR_FLAGS_PARITY | flags)
+        scene.render_adjoint(o, d, img, g)
+        torch.cuda.synchronize()
+        out.append(g.cpu().numpy().copy())
+    return np.array(out)
+
+
+@pytest.mark.parametrize("name", ["shadow", "heightfield"])
+def test_learned_mixture_is_unbiased(gpu, name):
+    import torch
+    w = helpers.small_workload(name, res=32)
+    scene = helpers.gpu_scene(w)
+    W, H = w["resolution"]
+    d = torch.from_numpy(np.random.default_rng(0).normal(size=(H, W, 3)).astype(np.float32)).to(gpu)
+    prop = EdgeProposal(scene, seed=1)
+    # arbitrary, far-from-uniform weights: the estimate must stay unbiased
+    wts = np.exp(np.random.default_rng(1).normal(0.0, 1.5, size=prop.n)).astype(np.float32)
+    prop.set_weights(wts)
+    plain = _grads(torch, scene, w, 16, range(100, 164), SECONDARY, d)
+    learned = _grads(torch, scene, w, 16, range(200, 264), SECONDARY | _abi.DR_LEARNED_EDGES, d)
+    mp, ml = plain.mean(0), learned.mean(0)
+    se = np.sqrt(plain.var(0) / len(plain) + learned.var(0) / len(learned))
+    active = se > 0
+    z = np.abs(mp - ml)[active] / se[active]
+    # per parameter |z| is ~half-normal: few beyond 4, and the aggregate
+    # distance is what independent noise gives
+    assert np.mean(z > 4.0) < 0.01, np.sort(z)[-10:]
+    assert np.linalg.norm(mp - ml) < 4.0 * np.linalg.norm(se) + 1e-6
+
+
+def test_features_and_statistics(gpu):
+    import torch
+    w = helpers.small_workload("heightfield", res=32)
+    scene = helpers.gpu_scene(w)
+    prop = EdgeProposal(scene, seed=2)
+    f = prop.features[:prop.n].cpu().numpy()
+    assert f.shape == (prop.n, _abi.DR_MLP_FEATURES) and np.all(np.isfinite(f))
+    np.testing.assert_allclose(np.linalg.norm(f[:, 5:8], axis=1), 1.0, rtol=1e-4)  # face normal n0
+    assert np.all(f[:, 23] == 1.0) and np.all(f[:, 13] > 0)
+    W, H = w["resolution"]
+    d = torch.from_numpy(np.random.default_rng(0).normal(size=(H, W, 3)).astype(np.float32)).to(gpu)
+    prop.reset_stats()
+    _grads(torch, scene, w, 4, [5], SECONDARY | _abi.DR_EDGE_STATS, d)
+    prop.collect()
+    acc = prop.acc[:prop.n].cpu().numpy()
+    assert acc[:, 1].sum() > 0 and np.all(acc >= 0) and np.all(np.isfinite(acc))
+    # contributions only where there are samples
+    assert np.all(acc[acc[:, 1] == 0, 0] == 0)
+
+
+def test_train_and_apply_end_to_end(gpu):
+    """Statistics from plain renders -> training -> weights -> learned
+    renders that agree with plain ones in the mean."""
+    import torch
+    w = helpers.small_workload("shadow", res=32)
+    scene = helpers.gpu_scene(w)
+    W, H = w["resolution"]
+    d = torch.from_numpy(np.random.default_rng(3).normal(size=(H, W, 3)).astype(np.float32)).to(gpu)
+    prop = EdgeProposal(scene, seed=4)
+    prop.reset_stats()
+    _grads(torch, scene, w, 16, range(8), SECONDARY | _abi.DR_EDGE_STATS, d)
+    prop.collect()
+    loss = prop.train(steps=200, batch=256)
+    assert np.isfinite(loss)
+    wts = prop.apply()
+    assert wts.shape == (prop.n,) and np.all(wts > 0)
+    plain = _grads(torch, scene, w, 16, range(300, 340), SECONDARY, d)
+    learned = _grads(torch, scene, w, 16, range(400, 440), SECONDARY | _abi.DR_LEARNED_EDGES, d)
+    se = np.sqrt(plain.var(0) / len(plain) + learned.var(0) / len(learned))
+    assert np.linalg.norm(plain.mean(0) - learned.mean(0)) < 4.0 * np.linalg.norm(se) + 1e-6
