@@ -91,8 +91,8 @@ class dr_job(ctypes.Structure):
 
 
 class dr_tuning(ctypes.Structure):
-    _fields_ = [("pass_samples", ctypes.c_int64), ("min_threads", ctypes.c_int64), ("best_ms", ctypes.c_double),
-                ("candidates", ctypes.c_int)]
+    _fields_ = [("pass_samples", ctypes.c_int64), ("min_threads", ctypes.c_int64), ("select_refill", ctypes.c_int32),
+                ("best_ms", ctypes.c_double), ("candidates", ctypes.c_int)]
 
 
 DR_BATCH_GRAPH = 1

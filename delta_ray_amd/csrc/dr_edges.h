@@ -42,24 +42,31 @@ __constant__ static const signed char kConePolyIdx[27][6] = {
     {0, 1, 3, 7, 6, 4}, {0, 4, 5, 7, 6, 2}, {4, 5, 7, 6, 0, 0}, {1, 3, 7, 6, 4, 5}, {0, 4, 5, 7, 3, 2},
     {2, 6, 4, 5, 7, 3}, {1, 3, 2, 6, 4, 5}};
 
+// component k (0, 1, 2) of v, by value (no runtime indexing of a vector)
+DR_INLINE float comp(F3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
 DR_INLINE bool cone_edge_test(F3 D, float c2, F3 maxX, F3 maxPmV, float maxSqrLen, float maxDd, F3 CmV, int jA,
                               int jB, float flip, bool &decided) {
     int jDiff = jB - jA;
     float s = jDiff > 0 ? 1.f : -1.f;
     int k0 = abs(jDiff) >> 1;
-    float fder = flip * s * (D[k0] * maxSqrLen - maxDd * maxPmV[k0]);
+    float fder = flip * s * (comp(D, k0) * maxSqrLen - maxDd * comp(maxPmV, k0));
     decided = fder > 0;
     if (!decided) return false;
     int k1 = (k0 + 1) % 3, k2 = (k1 + 1) % 3;
-    float numer = maxPmV[k1] * maxPmV[k1] + maxPmV[k2] * maxPmV[k2];
-    float denom = D[k1] * maxPmV[k1] + D[k2] * maxPmV[k2];
-    F3 M;
-    M[k0] = numer * D[k0];
-    M[k1] = denom * (maxX[k1] + CmV[k1]);
-    M[k2] = denom * (maxX[k2] + CmV[k2]);
+    float p1 = comp(maxPmV, k1), p2 = comp(maxPmV, k2);
+    float d1 = comp(D, k1), d2 = comp(D, k2);
+    float numer = p1 * p1 + p2 * p2;
+    float denom = d1 * p1 + d2 * p2;
+    float m0 = numer * comp(D, k0);
+    float m1 = denom * (comp(maxX, k1) + comp(CmV, k1));
+    float m2 = denom * (comp(maxX, k2) + comp(CmV, k2));
+    // M has m0 at k0, m1 at k1, m2 at k2
+    F3 M = F3{k0 == 0 ? m0 : (k1 == 0 ? m1 : m2), k0 == 1 ? m0 : (k1 == 1 ? m1 : m2),
+              k0 == 2 ? m0 : (k1 == 2 ? m1 : m2)};
     float DdM = dot(D, M);
     if (DdM * DdM - c2 * dot(M, M) > 0) return true;
-    return s * (D[k1] * maxPmV[k2] - D[k2] * maxPmV[k1]) <= 0;
+    return s * (d1 * p2 - d2 * p1) <= 0;
 }
 
 DR_INLINE bool aabb_cone_intersect(const float *lo, const float *hi, const Cone &cone) {
@@ -114,9 +121,38 @@ DR_INLINE float min_sq_bound(float lo, float hi) {
 }
 DR_INLINE float max_sq_bound(float lo, float hi) { return fmaxf(lo * lo, hi * hi); }
 
-// edge_tree.cpp:345-480
-DR_INLINE float bsdf_bound(const DrSceneView &S, const DrEdgeNode &node, F3 wi, const SPoint &sp, float roughness,
-                           float dlum, float slum) {
+// The BSDF bound of edge_tree.cpp:345-480, split in two: the parts that
+// depend only on the shading point (the frame around wi, the normal in it,
+// the Phong exponent) are computed once per point in bound_ctx(); the per-node
+// part is bsdf_bound().  Same operations in the same order as one function,
+// so the bound is bitwise the same.
+struct BoundCtx {
+    Frame hf;
+    float lx, lz, lx2, lz2;  // normal in hf, and its squares
+    float e, e2;             // Phong exponent of the roughness, e + 2
+    float dterm, slum;       // diffuse luminance / pi, specular luminance
+};
+
+DR_INLINE void bound_ctx(BoundCtx &B, F3 wi, const SPoint &sp, float roughness, float dlum, float slum) {
+    B.dterm = dlum / DR_PI;
+    B.slum = slum;
+    if (slum > 0.f) {
+        F3 hy = normalize(cross(sp.fr.n, wi));
+        B.hf = Frame{normalize(cross(wi, hy)), hy, wi};
+        F3 ln = to_local(B.hf, sp.fr.n);
+        B.lx = ln.x;
+        B.lz = ln.z;
+        B.lx2 = ln.x * ln.x;
+        B.lz2 = ln.z * ln.z;
+        B.e = roughness_to_phong(roughness);
+        B.e2 = B.e + 2.f;
+    } else {
+        B.hf = Frame{F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
+        B.lx = B.lz = B.lx2 = B.lz2 = B.e = B.e2 = 0.f;
+    }
+}
+
+DR_INLINE float bsdf_bound(const DrEdgeNode &node, const SPoint &sp, const BoundCtx &B) {
     float cos_max = 0.f;
     for (int i = 0; i < 8; i++) {
         F3 c = F3{(i & 1) ? node.hi[0] : node.lo[0], (i & 2) ? node.hi[1] : node.lo[1],
@@ -125,14 +161,12 @@ DR_INLINE float bsdf_bound(const DrSceneView &S, const DrEdgeNode &node, F3 wi, 
     }
     if (cos_max <= 0.f) return 0.f;
     float phong_resp = 0.f;
-    if (slum > 0.f) {
-        F3 hy = normalize(cross(sp.fr.n, wi));
-        Frame hf = Frame{normalize(cross(wi, hy)), hy, wi};
+    if (B.slum > 0.f) {
         float blo[3] = {DR_INF, DR_INF, DR_INF}, bhi[3] = {-DR_INF, -DR_INF, -DR_INF};
         for (int i = 0; i < 8; i++) {
             F3 c = F3{(i & 1) ? node.hi[0] : node.lo[0], (i & 2) ? node.hi[1] : node.lo[1],
                       (i & 4) ? node.hi[2] : node.lo[2]};
-            F3 l = to_local(hf, c - sp.pos);
+            F3 l = to_local(B.hf, c - sp.pos);
             blo[0] = fminf(blo[0], l.x); blo[1] = fminf(blo[1], l.y); blo[2] = fminf(blo[2], l.z);
             bhi[0] = fmaxf(bhi[0], l.x); bhi[1] = fmaxf(bhi[1], l.y); bhi[2] = fmaxf(bhi[2], l.z);
         }
@@ -156,16 +190,14 @@ DR_INLINE float bsdf_bound(const DrSceneView &S, const DrEdgeNode &node, F3 wi, 
             cto_max = bhi[2] / (max_sq_bound(blo[0], bhi[0]) + max_sq_bound(blo[1], bhi[1]) + bhi[2] * bhi[2]);
         float cth_min = sqrtf((cto_min + 1.f) / 2.f);
         float cth_max = sqrtf((cto_max + 1.f) / 2.f);
-        F3 ln = to_local(hf, sp.fr.n);
-        float ch = cphm >= 0.f ? sqrtf(ln.z * ln.z / (ln.z * ln.z + ln.x * ln.x * cphm * cphm)) : 1.f;
+        float ch = cphm >= 0.f ? sqrtf(B.lz2 / (B.lz2 + B.lx2 * cphm * cphm)) : 1.f;
         if (ch > cth_max) ch = cth_max;
         else if (ch < cth_min) ch = cth_min;
         float sh = sqrtf(fmaxf(1.f - ch * ch, 0.f));
-        float hn = fminf(fabsf(ln.x * sh * cphm) + ln.z * ch, 1.f);
-        float e = roughness_to_phong(roughness);
-        phong_resp = powf(hn, e) * (e + 2.f) / (2.f * DR_PI);
+        float hn = fminf(fabsf(B.lx * sh * cphm) + B.lz * ch, 1.f);
+        phong_resp = powf(hn, B.e) * B.e2 / (2.f * DR_PI);
     }
-    return cos_max * (dlum / DR_PI + slum * phong_resp);
+    return cos_max * (B.dterm + B.slum * phong_resp);
 }
 
 // edge_tree.cpp:482-497
@@ -209,11 +241,15 @@ DR_INLINE float node_importance_len(const DrEdgeNode &n, float bound, F3 p, floa
 // one node per et_step() so that a persistent kernel can give a lane a new
 // point as soon as its traversal ends; et_step() calls
 // `emit(edge, weight, in_normal_tree, index, ordinal)` per selected edge.
+//
+// The stack is an array of its own (DR_TREE_STACK entries, owned by the
+// caller), not a member of EdgeTraversal: it is indexed at run time, so it
+// lives in private memory, while every other field of the traversal state can
+// stay in registers.
 struct TreeEntry {
-    int node;
+    int nm;  // node << 2 | pass mask (bit 0: the cone pass reaches the node, bit 1: the "all" pass)
     float bound, cone_pmf, all_pmf;
     float cone_alt, all_alt;  // the same path's pmfs under the other proposal
-    int mask;  // bit 0: the cone pass reaches the node, bit 1: the "all" pass
 };
 
 struct EdgeTraversal {
@@ -221,8 +257,7 @@ struct EdgeTraversal {
     const DrEdge *edges;
     int num_nodes;
     bool directional;
-    float root_bound, pmf;
-    float roughness, dlum, slum;
+    BoundCtx B;
     Cone cone;
     Philox rc, ra;  // draws of the cone pass / the "all" pass
     int jc, ja;     // ordinals of the next edge of each pass
@@ -232,18 +267,17 @@ struct EdgeTraversal {
     int mode;
     int sp_, pops;
     uint32_t n_inner, n_edges;
-    TreeEntry stack[DR_TREE_STACK];
 };
 
 // edge_tree.cpp:685-725: choose the tree (one draw from rc when both exist)
 // and push the root for both passes.  Returns false when there is nothing
 // to traverse.
-DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isect &is, const SPoint &sp,
-                       const Cone &cone, int mode) {
+DR_INLINE bool et_init(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S, F3 wi, const SPoint &sp,
+                       const Cone &cone, float roughness, float dlum, float slum, int mode) {
     if (dot(wi, sp.fr.n) < 0.f) return false;
     bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;
     if (!has_n && !has_s) return false;
-    T.pmf = 1.f;
+    float pmf = 1.f;
     if (!has_n) {
         T.nodes = S.sil_nodes; T.edges = S.sil_edges; T.num_nodes = S.num_sil_nodes; T.directional = false;
     } else if (!has_s) {
@@ -254,18 +288,15 @@ DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isec
         float u = philox_next(T.rc);
         if (u < npmf) {
             T.nodes = S.nrm_nodes; T.edges = S.nrm_edges; T.num_nodes = S.num_nrm_nodes; T.directional = true;
-            T.pmf = npmf;
+            pmf = npmf;
         } else {
             T.nodes = S.sil_nodes; T.edges = S.sil_edges; T.num_nodes = S.num_sil_nodes; T.directional = false;
-            T.pmf = spmf;
+            pmf = spmf;
         }
     }
-    const DrMaterial &m = S.materials[S.shapes[is.shape].mat];
-    T.dlum = luminance(get_diffuse(S, m, sp.uv));
-    T.slum = luminance(get_specular(S, m, sp.uv));
-    T.roughness = get_roughness(S, m, sp.uv);
+    bound_ctx(T.B, wi, sp, roughness, dlum, slum);
     T.cone = cone;
-    T.root_bound = bsdf_bound(S, T.nodes[0], wi, sp, T.roughness, T.dlum, T.slum);
+    float root_bound = bsdf_bound(T.nodes[0], sp, T.B);
     T.n_inner = 0;
     T.n_edges = 0;
     T.jc = 0;
@@ -273,7 +304,7 @@ DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isec
     T.pops = 0;
     T.sp_ = 0;
     T.mode = mode;
-    T.stack[T.sp_++] = TreeEntry{0, T.root_bound, T.pmf, T.pmf, T.pmf, T.pmf, 3};
+    stack[T.sp_++] = TreeEntry{3, root_bound, pmf, pmf, pmf, pmf};
     return true;
 }
 
@@ -281,7 +312,7 @@ DR_INLINE bool et_init(EdgeTraversal &T, const DrSceneView &S, F3 wi, const Isec
 
 // One node.  Returns false when the traversal is done.
 template <class Emit>
-DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoint &sp, Emit &emit) {
+DR_INLINE bool et_step(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S, const SPoint &sp, Emit &emit) {
     // every node is pushed at most once; the cap guards a malformed tree
     if (T.sp_ == 0 || T.pops >= T.num_nodes + 8) {
         if (S.stats) {
@@ -291,9 +322,10 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
         return false;
     }
     T.pops++;
-    TreeEntry e = T.stack[--T.sp_];
+    TreeEntry e = stack[--T.sp_];
     if (e.bound <= 0.f) return true;
-    const DrEdgeNode &node = T.nodes[e.node];
+    const int ni = e.nm >> 2, mask = e.nm & 3;
+    const DrEdgeNode &node = T.nodes[ni];
     if (node.nprims > 0) {
         // power heuristic over the strategies that could have selected the
         // edge: the two passes (and, with the learned mixture, the two
@@ -309,12 +341,12 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
         for (int i = 0; i < node.nprims; i++) {
             const DrEdge &ed = T.edges[node.offset + i];
             if (!is_silhouette(S, sp.pos, ed)) continue;
-            if (e.mask & 1) emit(ed, pi * e.cone_pmf / den, T.directional, node.offset + i, T.jc++);
-            if (e.mask & 2) emit(ed, pi * e.all_pmf / den, T.directional, node.offset + i, 0x8000 | T.ja++);
+            if (mask & 1) emit(ed, pi * e.cone_pmf / den, T.directional, node.offset + i, T.jc++);
+            if (mask & 2) emit(ed, pi * e.all_pmf / den, T.directional, node.offset + i, 0x8000 | T.ja++);
         }
         return true;
     }
-    int li = e.node + 1, ri = node.offset;
+    int li = ni + 1, ri = node.offset;
     const DrEdgeNode &L = T.nodes[li], &R = T.nodes[ri];
     T.n_inner++;
     bool all_split = e.bound > 1.f || node_inside(node, sp.pos);
@@ -323,8 +355,8 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
         lcs = contains_silhouette(L, sp.pos);
         rcs = contains_silhouette(R, sp.pos);
     }
-    float lb = lcs ? bsdf_bound(S, L, wi, sp, T.roughness, T.dlum, T.slum) : 0.f;
-    float rb = rcs ? bsdf_bound(S, R, wi, sp, T.roughness, T.dlum, T.slum) : 0.f;
+    float lb = lcs ? bsdf_bound(L, sp, T.B) : 0.f;
+    float rb = rcs ? bsdf_bound(R, sp, T.B) : 0.f;
     float il = lcs ? node_importance(L, lb, sp.pos) : 0.f;
     float ir = rcs ? node_importance(R, rb, sp.pos) : 0.f;
     // with the learned mixture: importances of the point's proposal (il,
@@ -375,7 +407,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
         }
     }
     int lm = 0, rm = 0;
-    if ((e.mask & 1) && ilc + irc > 1e-10f) {
+    if ((mask & 1) && ilc + irc > 1e-10f) {
         if (cone_split) {
             if (rcs) rm |= 1;
             if (lcs) lm |= 1;
@@ -385,7 +417,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
             rm |= 1;
         }
     }
-    if (e.mask & 2) {
+    if (mask & 2) {
         if (all_split) {
             if (rcs) rm |= 2;
             if (lcs) lm |= 2;
@@ -396,9 +428,11 @@ DR_INLINE bool et_step(EdgeTraversal &T, const DrSceneView &S, F3 wi, const SPoi
         }
     }
     if (rm && T.sp_ < DR_TREE_STACK)
-        T.stack[T.sp_++] = TreeEntry{ri, rb, prc * e.cone_pmf, pra * e.all_pmf, prc2 * e.cone_alt, pra2 * e.all_alt, rm};
+        stack[T.sp_++] = TreeEntry{ri << 2 | rm, rb, prc * e.cone_pmf, pra * e.all_pmf, prc2 * e.cone_alt,
+                                   pra2 * e.all_alt};
     if (lm && T.sp_ < DR_TREE_STACK)
-        T.stack[T.sp_++] = TreeEntry{li, lb, plc * e.cone_pmf, pla * e.all_pmf, plc2 * e.cone_alt, pla2 * e.all_alt, lm};
+        stack[T.sp_++] = TreeEntry{li << 2 | lm, lb, plc * e.cone_pmf, pla * e.all_pmf, plc2 * e.cone_alt,
+                                   pla2 * e.all_alt};
     return true;
 }
 

@@ -214,9 +214,11 @@ DR_INLINE void d_xfm_point(const float *a, F3 p, F3 dout, float *dm) {
 // Philox4x32-10 (Salmon et al., SC 2011).  Streams are keyed by the render
 // seed and counted by (sample id, pass, sub-stream, block); the CPU oracle
 // uses the same specification, so both draw identical numbers.
+// The four outputs of a block are held in b0..b3 and shifted down as they are
+// drawn (no indexed array: the state stays in registers).
 struct Philox {
     uint32_t k0, k1, c0, c1, c2, c3;
-    uint32_t buf[4];
+    uint32_t b0, b1, b2, b3;
     int idx;
 };
 DR_INLINE void philox_init(Philox &r, uint64_t seed, uint32_t pass, uint64_t sample, uint32_t stream) {
@@ -241,13 +243,16 @@ DR_INLINE void philox_refill(Philox &r) {
         a += 0x9E3779B9u;
         b += 0xBB67AE85u;
     }
-    r.buf[0] = x0; r.buf[1] = x1; r.buf[2] = x2; r.buf[3] = x3;
+    r.b0 = x0; r.b1 = x1; r.b2 = x2; r.b3 = x3;
     r.c3++;
     r.idx = 0;
 }
 DR_INLINE float philox_next(Philox &r) {
     if (r.idx >= 4) philox_refill(r);
-    uint32_t v = r.idx == 0 ? r.buf[0] : (r.idx == 1 ? r.buf[1] : (r.idx == 2 ? r.buf[2] : r.buf[3]));
+    uint32_t v = r.b0;
+    r.b0 = r.b1;
+    r.b1 = r.b2;
+    r.b2 = r.b3;
     r.idx++;
     return (float)(v >> 8) * (1.0f / 16777216.0f);
 }

@@ -298,42 +298,6 @@ DR_INLINE bool trace_path(const DrSceneView &S, const SampleCtx &c, int xi, int 
 // A secondary shading point (reference SecondaryPoint, pathtrace.h:170-176):
 // recorded for every bounce reached without a diffuse sample; the edge
 // stages read it.  valid = 0 marks an empty slot.
-struct PointRec {
-    SPoint sp;
-    F3 wi, thr;
-    int shape, tri, gtri, valid;
-};
-
-// Stage A: forward path + interior adjoint of one pixel sample; fills the
-// MAXB point slots of the sample.  Returns the colour / spp.
-template <int MAXB>
-DR_INLINE F3 stage_path(const DrSceneView &S, const SampleCtx &c, GradSink &G, int xi, int yi, int s, F3 d_color,
-                        PointRec *pts) {
-    PathRec<MAXB> P;
-    for (int b = 0; b < MAXB; b++) pts[b].valid = 0;
-    if (!trace_path<MAXB>(S, c, xi, yi, s, P)) return F3{0.f, 0.f, 0.f};
-    float sw = 1.f / c.spp;
-    if (!(c.flags & DR_SKIP_INTERIOR)) {
-        G.color_mode = true;
-        reverse_path<MAXB>(S, G, P.ls, P.org, P.dir, P.nd, P.is, P.sp, P.R, P.nused, P.chain, d_color * sw, nullptr,
-                           P.primary_light);
-    }
-    if (!(c.flags & DR_SKIP_SECONDARY_EDGES)) {
-        for (int b = 0; b < P.nused; b++) {
-            if (!P.chain[b]) continue;
-            PointRec &q = pts[b];
-            q.sp = P.sp[b];
-            q.wi = P.R[b].cur_wi;
-            q.thr = P.R[b].thr;
-            q.shape = P.is[b].shape;
-            q.tri = P.is[b].tri;
-            q.gtri = P.is[b].gtri;
-            q.valid = 1;
-        }
-    }
-    return P.color * sw;
-}
-
 // Edge-tree cone toward the light chosen by the primary light sample
 // (reference delta_ray.cpp:876-890)
 DR_INLINE Cone light_cone(const DrSceneView &S, float light_sel, F3 p) {
@@ -349,49 +313,96 @@ DR_INLINE uint64_t sample_id(const SampleCtx &c, int xi, int yi, int s) {
     return ((uint64_t)yi * c.W + xi) * (uint64_t)c.spp + s;
 }
 
-// Stage B: edge selection at one secondary point (edge_tree.cpp:685-725);
-// emit(tree, edge_index, weight, ordinal) per selected edge.  Returns the
-// number of selected edges.
-// The random streams, light cone and tree choice of one point's selection;
-// false when the point selects nothing.
-DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, uint64_t sid, int b, const PointRec &p,
-                            EdgeTraversal &T) {
-    philox_init(T.rc, c.seed, 0, sid, dr_stream_point(b));
+// One secondary shading point, written by the path stage.  Besides the
+// point itself it carries what starting its edge selection needs, computed
+// once here (where all lanes of a wave do it together) instead of in the
+// selection kernel: the sample's ids, the cone toward the light of the
+// primary light sample and the material's luminances and roughness.
+struct PointRec {
+    uint64_t sid;        // sample id ((y * W + x) * spp + s)
+    SPoint sp;
+    F3 wi, thr;
+    F3 cone_dir;
+    float cone_cos;
+    int shape, tri, gtri, valid;
+    float roughness, dlum, slum;
+    int job, b, pix;
+};
+
+// Stage A: forward path + interior adjoint of one pixel sample; fills the
+// MAXB point slots of the sample.  Returns the colour / spp.
+template <int MAXB>
+DR_INLINE F3 stage_path(const DrSceneView &S, const SampleCtx &c, GradSink &G, int xi, int yi, int s, F3 d_color,
+                        PointRec *pts, int job = 0) {
+    PathRec<MAXB> P;
+    for (int b = 0; b < MAXB; b++) pts[b].valid = 0;
+    if (!trace_path<MAXB>(S, c, xi, yi, s, P)) return F3{0.f, 0.f, 0.f};
+    float sw = 1.f / c.spp;
+    if (!(c.flags & DR_SKIP_INTERIOR)) {
+        G.color_mode = true;
+        reverse_path<MAXB>(S, G, P.ls, P.org, P.dir, P.nd, P.is, P.sp, P.R, P.nused, P.chain, d_color * sw, nullptr,
+                           P.primary_light);
+    }
+    if (!(c.flags & DR_SKIP_SECONDARY_EDGES)) {
+        for (int b = 0; b < P.nused; b++) {
+            if (!P.chain[b]) continue;
+            PointRec &q = pts[b];
+            q.sid = sample_id(c, xi, yi, s);
+            q.sp = P.sp[b];
+            q.wi = P.R[b].cur_wi;
+            q.thr = P.R[b].thr;
+            q.shape = P.is[b].shape;
+            q.tri = P.is[b].tri;
+            q.gtri = P.is[b].gtri;
+            q.valid = 1;
+            Cone cn = light_cone(S, P.ls.light_sel, q.sp.pos);
+            q.cone_dir = cn.dir;
+            q.cone_cos = cn.cos_angle;
+            const DrMaterial &m = S.materials[S.shapes[q.shape].mat];
+            q.roughness = get_roughness(S, m, q.sp.uv);
+            q.dlum = luminance(get_diffuse(S, m, q.sp.uv));
+            q.slum = luminance(get_specular(S, m, q.sp.uv));
+            q.job = job;
+            q.b = b;
+            q.pix = yi * c.W + xi;
+        }
+    }
+    return P.color * sw;
+}
+
+// The random streams, light cone and tree choice of one point's selection
+// (the light cone and material values come with the point record); false
+// when the point selects nothing.
+DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, const PointRec &p, EdgeTraversal &T,
+                            TreeEntry *stack) {
+    philox_init(T.rc, c.seed, 0, p.sid, dr_stream_point(p.b));
     philox_next(T.rc);  // bsdf component of the point's edge sample
     philox_next(T.rc);  // position on the edge
-    philox_init(T.ra, c.seed, 0, sid, dr_stream_point_all(b));
-    // the primary light sample of this pixel sample: its light_sel is the
-    // third draw after the jitter of the main stream
-    Philox mr;
-    philox_init(mr, c.seed, 0, sid, DR_STREAM_MAIN);
-    philox_next(mr);
-    philox_next(mr);
-    LightSample ls = draw_light_sample(mr);
-    Cone cone = light_cone(S, ls.light_sel, p.sp.pos);
-    Isect is = Isect{p.shape, p.tri, p.gtri};
+    philox_init(T.ra, c.seed, 0, p.sid, dr_stream_point_all(p.b));
     int mode = 0;
     if (c.flags & DR_LEARNED_EDGES) {
         Philox mix;
-        philox_init(mix, c.seed, 0, sid, dr_stream_point_mix(b));
+        philox_init(mix, c.seed, 0, p.sid, dr_stream_point_mix(p.b));
         mode = philox_next(mix) < DR_LEARNED_SHARE ? 1 : 2;
     }
-    return et_init(T, S, p.wi, is, p.sp, cone, mode);
+    return et_init(T, stack, S, p.wi, p.sp, Cone{p.sp.pos, p.cone_dir, p.cone_cos}, p.roughness, p.dlum, p.slum,
+                   mode);
 }
 
 // Stage B: edge selection at one secondary point (edge_tree.cpp:685-725);
 // emit(tree, edge_index, weight, ordinal) per selected edge.  Returns the
 // number of selected edges.
 template <class Emit>
-DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, uint64_t sid, int b, const PointRec &p,
-                           Emit &emit) {
+DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, const PointRec &p, Emit &emit) {
     EdgeTraversal T;
-    if (!select_begin(S, c, sid, b, p, T)) return 0;
+    TreeEntry stack[DR_TREE_STACK];
+    if (!select_begin(S, c, p, T, stack)) return 0;
     int n = 0;
     auto em = [&](const DrEdge &e, float w, bool nrm, int idx, int j) {
         emit(nrm, idx, w, j);
         n++;
     };
-    while (et_step(T, S, p.wi, p.sp, em)) {
+    while (et_step(T, stack, S, p.sp, em)) {
     }
     return n;
 }

@@ -237,7 +237,7 @@ __global__ void k_mlp_batch_idx(int *idx, int nbatch, long long n, uint64_t seed
     Philox r;
     philox_init(r, seed, 3, (uint64_t)step, (uint32_t)i);
     philox_refill(r);
-    uint64_t u = ((uint64_t)r.buf[0] << 32) | r.buf[1];
+    uint64_t u = ((uint64_t)r.b0 << 32) | r.b1;
     idx[i] = (int)(u % (uint64_t)n);
 }
 

@@ -70,15 +70,6 @@ DR_INLINE int range_of_block(const DrRange *r, int n, long long b) {
     }
     return lo;
 }
-DR_INLINE int range_of_slot(const DrRange *r, int n, long long s) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (r[mid].slot0 <= s) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 #define DR_GC_SLOTS_BIG 4096
 #define DR_GC_SLOTS_SMALL 2048
@@ -115,27 +106,6 @@ struct EdgeEntry {
     int j;     // ordinal of the edge in its pass (bit 15: the "all" pass)
 };
 
-struct SlotDecode {
-    int job, pix, xi, yi, s, b;
-    uint64_t sid;
-};
-DR_INLINE SlotDecode decode_slot(const DrJob *jobs, const DrRange *R, int nr, long long slot, int maxb) {
-    SlotDecode d;
-    const DrRange &rg = R[range_of_slot(R, nr, slot)];
-    const DrJob &J = jobs[rg.job];
-    long long local = slot - rg.slot0;
-    d.job = rg.job;
-    d.b = (int)(local % maxb);
-    long long rest = local / maxb;
-    int k = (int)(rest % J.per);
-    long long t = rg.t0 + rest / J.per;
-    d.pix = (int)(t / J.chunks);
-    d.s = (int)(t % J.chunks) * J.per + k;
-    d.xi = d.pix % J.c.W;
-    d.yi = d.pix / J.c.W;
-    d.sid = sample_id(J.c, d.xi, d.yi, d.s);
-    return d;
-}
 
 template <int MAXB>
 __global__ void __launch_bounds__(DR_BLOCK) k_path(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
@@ -174,7 +144,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(const DrJob *__restrict__ job
                 if (s < c.spp) acc += forward_sample(J.S, c, xi, yi, s);
                 continue;
             }
-            acc += stage_path<MAXB>(J.S, c, G, xi, yi, s, d_color, P);
+            acc += stage_path<MAXB>(J.S, c, G, xi, yi, s, d_color, P, rg.job);
         }
         float *px = J.image + 3 * (size_t)pix;
         if (chunks == 1) {
@@ -190,21 +160,25 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(const DrJob *__restrict__ job
 
 // B: edge selection, persistent.  Per-point traversal lengths vary by two
 // orders of magnitude (a point inside many node boxes splits at every level),
-// so a lane that has finished its point fetches the next one: idle lanes of a
-// wave take a batch of point slots from `work` with one atomic, and every
-// iteration advances each busy lane's traversal by one node.  Selected edges
-// are appended to the queue in groups of up to 8 per lane (one queue atomic
-// per group).  Every lane leaves the loop once `work` passes nslots.
-// cnt[0]: queue length (may exceed cap: the host grows the queue and renders
-// again), cnt[1]: work counter.
-__global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
-                                                     int nr, long long nslots, int maxb, const PointRec *pts,
-                                                     EdgeEntry *q, unsigned long long *cnt, long long cap,
-                                                     int *counts) {
+// so a lane that has finished its point takes the next one, and every
+// iteration advances each busy lane's traversal by one node.  Starting a point
+// (decoding its slot, the light cone, the material, the root bound) is a long
+// stretch of code that the whole wave executes whenever any lane does, so
+// idle lanes wait until at least `refill` of them are idle (or no lane is
+// busy) and then start new points together.  In a refill, idle lanes take
+// batches of point slots from `work` with one atomic per batch until each has
+// a valid point or the slots run out, so empty slots cost no iteration.
+// Selected edges are appended to the queue in groups of up to 8 per lane (one
+// queue atomic per group).  Every lane leaves the loop once `work` passes
+// nslots.  cnt[0]: queue length (may exceed cap: the host grows the queue and
+// renders again), cnt[1]: work counter.
+__global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ jobs, long long nslots,
+                                                     const PointRec *pts, EdgeEntry *q, unsigned long long *cnt,
+                                                     long long cap, int *counts, int refill) {
     const int lane = (int)__lane_id();
     EdgeTraversal T;
+    TreeEntry stack[DR_TREE_STACK];
     SPoint sp;
-    F3 wi = F3{0.f, 0.f, 0.f};
     const DrSceneView *Sp = &jobs[0].S;
     long long slot = -1;
     int j = 0;
@@ -226,36 +200,43 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
         j++;
     };
     while (true) {
-        bool want = slot < 0 && more;
-        unsigned long long idle = __ballot(want);
-        if (idle) {
-            int leader = __ffsll((long long)idle) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&cnt[1], (unsigned long long)__popcll(idle));
-            base = __shfl(base, leader);
-            if (want) {
-                unsigned long long below = lane == 0 ? 0ull : (idle & (~0ull >> (64 - lane)));
-                long long s = (long long)(base + __popcll(below));
-                if (s >= nslots) {
-                    more = false;
-                } else {
-                    const PointRec &p = pts[s];
-                    bool busy = false;
-                    if (p.valid) {
-                        SlotDecode d = decode_slot(jobs, R, nr, s, maxb);
-                        const DrJob &J = jobs[d.job];
-                        Sp = &J.S;
-                        if (!(J.c.flags & DR_SKIP_SECONDARY_EDGES))
-                            busy = select_begin(*Sp, J.c, d.sid, d.b, p, T);
-                    }
-                    if (busy) {
-                        slot = s;
-                        sp = p.sp;
-                        wi = p.wi;
-                        j = 0;
+        unsigned long long idle = __ballot(slot < 0 && more);
+        if (idle && (__popcll(idle) >= refill || __ballot(slot >= 0) == 0)) {
+            // fetch: until every idle lane holds a valid point or the slots run out
+            bool want = slot < 0 && more;
+            long long cand = -1;
+            while (true) {
+                unsigned long long m = __ballot(want);
+                if (!m) break;
+                int leader = __ffsll((long long)m) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(&cnt[1], (unsigned long long)__popcll(m));
+                base = __shfl(base, leader);
+                if (want) {
+                    unsigned long long below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+                    long long s = (long long)(base + __popcll(below));
+                    if (s >= nslots) {
+                        more = false;
+                        want = false;
+                    } else if (pts[s].valid) {  // (only jobs with secondary edges mark points valid)
+                        cand = s;
+                        want = false;
                     } else {
                         counts[s] = 0;
                     }
+                }
+            }
+            // start the new points together
+            if (cand >= 0) {
+                const PointRec &p = pts[cand];
+                const DrJob &J = jobs[p.job];
+                Sp = &J.S;
+                if (select_begin(*Sp, J.c, p, T, stack)) {
+                    slot = cand;
+                    sp = p.sp;
+                    j = 0;
+                } else {
+                    counts[cand] = 0;
                 }
             }
         }
@@ -263,7 +244,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
             if (!more) break;
             continue;
         }
-        if (!et_step(T, *Sp, wi, sp, emit)) {
+        if (!et_step(T, stack, *Sp, sp, emit)) {
             if (nbuf > 0) flush();
             counts[slot] = j;
             if (Sp->stats) atomicAdd(&Sp->stats[3], (unsigned long long)j);
@@ -275,12 +256,11 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
 // C: one selected edge per iteration of a grid-stride loop over the queue
 // (its length is read on the device, so no host round trip is needed).
 template <int MAXB>
-DR_INLINE void edge_entry(const DrJob *jobs, const DrRange *R, int nr, const EdgeEntry &E, long long e,
+DR_INLINE void edge_entry(const DrJob *jobs, const EdgeEntry &E, long long e,
                           const PointRec *pts, float *pt_dp, int *pt_lastj, float4 *entry_dp,
                           unsigned long long *gkey, float *gval) {
-    SlotDecode d = decode_slot(jobs, R, nr, E.slot, MAXB);
-    const DrJob &J = jobs[d.job];
     const PointRec &p = pts[E.slot];
+    const DrJob &J = jobs[p.job];
     int idx = E.edge & 0x7fffffff;
     const DrEdge &edge = (E.edge & 0x80000000) ? J.S.nrm_edges[idx] : J.S.sil_edges[idx];
     GradSink G;
@@ -291,10 +271,10 @@ DR_INLINE void edge_entry(const DrJob *jobs, const DrRange *R, int nr, const Edg
     G.gkey = gkey;
     G.gval = gval;
     G.gmask = DR_GC_SLOTS_BIG - 1;
-    G.job = (unsigned)d.job;
-    F3 d_color = load3(J.d_image + 3 * (size_t)d.pix);
+    G.job = (unsigned)p.job;
+    F3 d_color = load3(J.d_image + 3 * (size_t)p.pix);
     float mag = 0.f;
-    F3 dp = stage_edge<MAXB>(J.S, J.c, G, d.sid, d.b, p, edge, E.w, E.j, d_color, &mag);
+    F3 dp = stage_edge<MAXB>(J.S, J.c, G, p.sid, p.b, p, edge, E.w, E.j, d_color, &mag);
     if ((J.c.flags & DR_EDGE_STATS) && J.S.edge_stat) {
         long long ge = (E.edge & 0x80000000) ? (long long)J.S.num_sil_edges + idx : (long long)idx;
         if (mag != 0.f) atomicAdd(J.S.edge_stat + 2 * ge, mag);
@@ -318,8 +298,8 @@ DR_INLINE void edge_entry(const DrJob *jobs, const DrRange *R, int nr, const Edg
 
 // cnt[2] keeps the largest queue length seen, for the host's overflow check
 template <int MAXB>
-__global__ void __launch_bounds__(DR_BLOCK) k_edge(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
-                                                   int nr, const EdgeEntry *q, unsigned long long *cnt, long long cap,
+__global__ void __launch_bounds__(DR_BLOCK) k_edge(const DrJob *__restrict__ jobs, const EdgeEntry *q,
+                                                   unsigned long long *cnt, long long cap,
                                                    const PointRec *pts, float *pt_dp, int *pt_lastj,
                                                    float4 *entry_dp) {
     __shared__ unsigned long long gkey[DR_GC_SLOTS_BIG];
@@ -331,7 +311,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_edge(const DrJob *__restrict__ job
     long long n = (long long)len < cap ? (long long)len : cap;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
-        edge_entry<MAXB>(jobs, R, nr, q[e], e, pts, pt_dp, pt_lastj, entry_dp, gkey, gval);
+        edge_entry<MAXB>(jobs, q[e], e, pts, pt_dp, pt_lastj, entry_dp, gkey, gval);
     __syncthreads();
     gc_flush(gkey, gval, DR_GC_SLOTS_BIG, [&](int j) { return jobs[j].grads; });
 }
@@ -617,6 +597,7 @@ struct LaunchTimer {
 struct Tuning {
     long long pass_samples = 1ll << 22;
     long long min_threads = 1ll << 20;
+    int select_refill = 32;
 };
 
 // Device working set of the pipeline, grown on demand, and the hipGraph of
@@ -801,6 +782,7 @@ struct Plan {
     int prim_r0 = 0, prim_r1 = 0, fwd_r0 = 0, fwd_r1 = 0;
     long long prim_blocks = 0, fwd_blocks = 0, max_slots = 0;
     int maxb = 1;
+    int select_refill = 32;
     std::vector<std::pair<float *, size_t>> zero;  // buffers cleared first
     std::vector<unsigned long long> key;           // what a captured graph depends on
 };
@@ -919,8 +901,10 @@ static dr_status build_plan(const dr_job *jobs, int n, const Tuning &tn, Plan &P
     }
     P.fwd_r1 = (int)P.ranges.size();
     P.fwd_blocks = blocks;
+    P.select_refill = tn.select_refill;
     P.key.push_back((unsigned long long)tn.pass_samples);
     P.key.push_back((unsigned long long)tn.min_threads);
+    P.key.push_back((unsigned long long)tn.select_refill);
     return DR_OK;
 }
 
@@ -991,14 +975,14 @@ static void enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
         {
             LaunchTimer t(tm, TK_SELECT, hs);
             long long blocks = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_select, g_select_blocks));
-            k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, R, nr, ps.nslots, MAXB, w.pts, w.q, w.cnt,
-                                                            w.q_cap, w.counts);
+            k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, w.pts, w.q, w.cnt, w.q_cap,
+                                                            w.counts, P.select_refill);
         }
         long long eb = resident_blocks(k_edge<MAXB>, g_edge_blocks[bi]);
         {
             LaunchTimer t(tm, TK_EDGE, hs);
-            k_edge<MAXB><<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.jobs, R, nr, w.q, w.cnt, w.q_cap, w.pts, w.pt_dp,
-                                                            w.pt_lastj, w.entry_dp);
+            k_edge<MAXB><<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.jobs, w.q, w.cnt, w.q_cap, w.pts, w.pt_dp, w.pt_lastj,
+                                                            w.entry_dp);
         }
         if (ps.dp_last) k_dp_last<<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.q, w.cnt, w.q_cap, w.entry_dp, w.pt_lastj, w.pt_dp);
         {
@@ -1359,10 +1343,11 @@ dr_status dr_render_batch(dr_batch *b, const dr_job *jobs, int num_jobs, int fla
 }
 
 dr_status dr_batch_set_tuning(dr_batch *b, const dr_tuning *t) {
-    if (!b || !t || t->pass_samples <= 0 || t->min_threads <= 0)
+    if (!b || !t || t->pass_samples <= 0 || t->min_threads <= 0 || t->select_refill < 0 || t->select_refill > 64)
         return fail(DR_ERROR_INVALID_ARGUMENT, "bad tuning");
     b->tune.pass_samples = t->pass_samples;
     b->tune.min_threads = t->min_threads;
+    b->tune.select_refill = t->select_refill > 0 ? t->select_refill : Tuning().select_refill;
     return DR_OK;
 }
 
@@ -1401,6 +1386,28 @@ dr_status dr_batch_autotune(dr_batch *b, const dr_job *jobs, int num_jobs, void 
             }
         }
     }
+    // then the refill threshold of the edge selection, at the chosen pass sizes
+    static const int kRefill[] = {8, 16, 64};
+    if (has_edges(P)) {
+        for (int rf : kRefill) {
+            if (st != DR_OK) break;
+            Tuning t = bt;
+            t.select_refill = rf;
+            st = build_plan(jobs, num_jobs, t, P);
+            if (st != DR_OK) break;
+            (void)hipEventRecord(e0, hs);
+            st = run_plan(P, b->ws, nullptr, hs, false);
+            (void)hipEventRecord(e1, hs);
+            if (st != DR_OK || hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tried++;
+            if (ms < best) {
+                best = ms;
+                bt = t;
+            }
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != DR_OK) return st;
@@ -1408,6 +1415,7 @@ dr_status dr_batch_autotune(dr_batch *b, const dr_job *jobs, int num_jobs, void 
     if (out) {
         out->pass_samples = bt.pass_samples;
         out->min_threads = bt.min_threads;
+        out->select_refill = bt.select_refill;
         out->best_ms = best;
         out->candidates = tried;
     }

@@ -407,11 +407,13 @@ class Batch:
         t = _abi.dr_tuning()
         _abi.check(self.lib, self.lib.dr_batch_autotune(self.handle, arr, len(jobs), _stream_ptr(stream),
                                                         ctypes.byref(t)), "dr_batch_autotune")
-        return {"pass_samples": t.pass_samples, "min_threads": t.min_threads, "best_ms": t.best_ms,
-                "candidates": t.candidates}
+        return {"pass_samples": t.pass_samples, "min_threads": t.min_threads, "select_refill": t.select_refill,
+                "best_ms": t.best_ms, "candidates": t.candidates}
 
-    def set_tuning(self, pass_samples, min_threads):
-        t = _abi.dr_tuning(pass_samples, min_threads, 0.0, 0)
+    def set_tuning(self, pass_samples, min_threads, select_refill=0):
+        """select_refill: idle lanes an edge-selection wave gathers before
+        starting new shading points (0 = default)."""
+        t = _abi.dr_tuning(pass_samples, min_threads, select_refill, 0.0, 0)
         _abi.check(self.lib, self.lib.dr_batch_set_tuning(self.handle, ctypes.byref(t)), "dr_batch_set_tuning")
 
 

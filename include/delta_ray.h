@@ -340,11 +340,14 @@ enum { DR_BATCH_GRAPH = 1 };
 dr_status dr_render_batch(dr_batch *batch, const dr_job *jobs, int num_jobs, int flags, void *stream);
 
 /* Pass sizes of the pipeline: pixel samples per pass and the least number
- * of threads a job is split into.  dr_batch_autotune renders the job list
- * with each candidate (outputs are overwritten) and keeps the fastest. */
+ * of threads a job is split into; select_refill: idle lanes (1-64) a wave
+ * of the edge-selection kernel gathers before it starts new shading points
+ * (0 = the default, 32).  dr_batch_autotune renders the job list with each
+ * candidate (outputs are overwritten) and keeps the fastest. */
 typedef struct dr_tuning {
     int64_t pass_samples;
     int64_t min_threads;
+    int32_t select_refill;
     double best_ms;              /* out: time of the chosen candidate */
     int candidates;              /* out: candidates timed */
 } dr_tuning;

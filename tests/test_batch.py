@@ -64,12 +64,15 @@ def test_batch_matches_single_renders(gpu):
             assert _rel(j["grads"].cpu().numpy(), ref_g) < 1e-4
 
 
-def test_batch_small_passes_match(gpu):
-    """Jobs split over many passes (tiny pass budget) give the same result."""
+@pytest.mark.parametrize("refill", [0, 1, 64])
+def test_batch_small_passes_match(gpu, refill):
+    """Jobs split over many passes (tiny pass budget) give the same result,
+    whatever number of idle lanes the edge selection gathers before it starts
+    new shading points (the schedule changes, the estimate does not)."""
     import torch
     jobs = _job_list(torch, gpu)
     batch = dr.Batch()
-    batch.set_tuning(pass_samples=1 << 12, min_threads=1 << 10)
+    batch.set_tuning(pass_samples=1 << 12, min_threads=1 << 10, select_refill=refill)
     batch.render(jobs)
     torch.cuda.synchronize()
     for j in jobs:
@@ -132,8 +135,10 @@ def test_autotune_picks_a_candidate(gpu):
     jobs = _job_list(torch, gpu)[:3]
     batch = dr.Batch()
     t = batch.autotune(jobs)
-    assert t["candidates"] == 12 and t["best_ms"] > 0
+    # 4 x 3 pass sizes, then 3 more refill thresholds at the best of them
+    assert t["candidates"] == 15 and t["best_ms"] > 0
     assert t["pass_samples"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+    assert t["select_refill"] in (8, 16, 32, 64)
     batch.render(jobs)
     torch.cuda.synchronize()
     for j in jobs:
