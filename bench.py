@@ -17,8 +17,11 @@ Prints one JSON line.  `roofline` is for the dominant kernel (the one with
 the most device time per step): algorithmic bytes per launch (BVH traversal +
 edge sampling + image/d_image I/O, from the kernels' own counters in separate
 untimed runs) divided by its average launch time measured with HIP events on
-its launch stream during the timed steps.  `cpu_baseline` times the CPU restatement of the reference
-(oracle/) on the host cores on a bounded sample of the same workload.
+its launch stream during the timed steps; `traffic` is that kernel's HBM bytes
+per launch from the committed rocprofv3 summary of the same workload
+(profiles/), null when none covers it.  `cpu_baseline` times the CPU
+restatement of the reference (oracle/) on the host cores on a bounded sample
+of the same workload.
 """
 import argparse
 import json
@@ -96,6 +99,22 @@ def d_image_for(w, scene, dev):
     scene.render(dr.DeviceScene.options(W, H, spp, nb, 777), img)
     torch.cuda.synchronize()
     return (2.0 * (img - target)).contiguous()
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 summaries (a `traffic_bytes_per_launch <kernel> <workload>: N`
+    line in profiles/*.md, FETCH_SIZE + WRITE_SIZE scaled by a copy of known
+    size); (None, None) when no summary covers them."""
+    import glob
+    import re
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(f) as fh:
+            for line in fh:
+                m = re.match(r"traffic_bytes_per_launch (\S+) (.+): (\d+)\s*$", line)
+                if m and m.group(1) == kernel and m.group(2) == workload:
+                    return int(m.group(3)), os.path.relpath(f, ROOT)
+    return None, None
 
 
 def cpu_baseline(w, d_img, spp, threads):
@@ -245,8 +264,10 @@ def main():
     achieved = bytes_launch / (kms[dom] / 1000.0) / 1e9
     rays_total = cb["rays"] + per_kernel["primary"]["rays"]
     c = per_kernel[dom]
+    wl_key = "%s %dx%d %dspp %d-bounce" % (w["name"], W, H, spp, nb)
+    traffic, traffic_src = profiled_traffic("k_" + dom, wl_key)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "k_" + dom, "kernel_ms_per_launch": round(kms[dom], 3),
                 "bytes_per_launch": int(bytes_launch),
                 "per_ray": {"nodes": round(c["nodes_visited"] / max(c["rays"], 1), 2),
