@@ -7,11 +7,13 @@ RenderFunction.backward call: pixel samples with interior adjoint and
 secondary edges, then spp*W*H primary-edge samples), with scene, d_image and
 buffers resident on the GPU before the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--workload heightfield|shadow|single_triangle|batched]
+    python bench.py [--steps K] [--warmup W] [--workload heightfield|shadow|single_triangle|batched|learned]
 
 `--workload batched` times the uneven batch instead (64 scenes of 1k-200k
 triangles x 4 views, forward + adjoint each, one graph-captured batch per
-step, pass sizes from the autotuner).
+step, pass sizes from the autotuner).  `--workload learned` measures the time
+to a gradient relative RMSE of 1e-2 with the learned edge proposal and with
+the plain one (see bench_learned).
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the one with
 the most device time per step): algorithmic bytes per launch (BVH traversal +
@@ -57,6 +59,12 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched workload")
     ap.add_argument("--no-autotune", action="store_true", help="batched workload: default pass sizes")
     ap.add_argument("--cpu-spp", type=int, default=16, help="spp of the bounded CPU sample")
+    ap.add_argument("--lv-res", type=int, default=128, help="learned workload: image size")
+    ap.add_argument("--lv-spp", type=int, default=16, help="learned workload: samples per pixel")
+    ap.add_argument("--lv-renders", type=int, default=512, help="learned workload: renders per estimator")
+    ap.add_argument("--lv-train-renders", type=int, default=16, help="learned workload: statistics renders")
+    ap.add_argument("--lv-train-steps", type=int, default=300, help="learned workload: MLP training steps")
+    ap.add_argument("--lv-eps", type=float, default=1e-2, help="learned workload: target relative RMSE")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known HBM traffic: "
                          "256 MiB read + 256 MiB written) to scale hardware byte counters")
@@ -138,11 +146,14 @@ def bench_batched(args):
     t0 = time.perf_counter()
     scenes = workloads.batched(num_scenes=args.batch_scenes)
     gen_s = time.perf_counter() - t0
+    print("batched: %d scenes generated in %.1f s" % (len(scenes), gen_s), file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     keep, jobs = [], []
     rng = np.random.default_rng(5)
     samples = 0
-    for sc in scenes:
+    for si, sc in enumerate(scenes):
+        if si % 8 == 0:
+            print("batched: building scene %d" % si, file=sys.stderr, flush=True)
         ds = dr.DeviceScene(sc["views"][0][0], sc["shapes"], sc["materials"], sc["lights"])
         grads = torch.zeros((ds.grad_size,), dtype=torch.float32, device=dev)
         keep.append(ds)
@@ -155,15 +166,18 @@ def bench_batched(args):
                              image=img, d_image=d_img, grads=grads))
             samples += W * H * spp
     build_s = time.perf_counter() - t0
+    print("batched: %d jobs built in %.1f s" % (len(jobs), build_s), file=sys.stderr, flush=True)
     batch = dr.Batch()
     # the autotuner times 12 candidates: run it on every 8th job (a
     # representative eighth of the batch) and keep its choice for all jobs
     tune = None if args.no_autotune else batch.autotune(jobs[::8])
+    print("batched: tuning %s" % (tune,), file=sys.stderr, flush=True)
 
     def step(i):
         for k, j in enumerate(jobs):
             j["options"].seed = 1000003 * i + k
         batch.render(jobs, graph=True)
+        print("batched: step %d enqueued" % i, file=sys.stderr, flush=True)
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
@@ -185,6 +199,119 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """Time to a gradient relative RMSE of 1e-2, learned edge proposal vs
+    the plain one.  Both estimators are unbiased, so for each one the
+    per-render variance V of the gradient (sum over parameters) and the time
+    per render t give the time to reach relative RMSE eps as
+    t * V / (eps^2 |g|^2)  (n renders average to variance V / n); |g| is the
+    norm of the mean of all renders of both.  The secondary-edge estimator
+    is heavy-tailed (a rare sample through a tiny tree pmf carries a huge
+    weight), so V is estimated as the median over 16 groups of renders of the
+    groups' sample variances (median of means: what a run of typical length
+    sees); the plain sample variance over all renders, dominated by the few
+    largest renders, and tail percentiles of the per-render squared error are
+    reported beside it.  The learned time includes its training:
+    DR_EDGE_STATS renders (statistics of the plain estimator), MLP training
+    and applying the weights.  Measured on the secondary-edge term, the one
+    the proposal changes, and on the full gradient (interior + secondary +
+    primary edges)."""
+    import numpy as np
+    import torch
+    from delta_ray_amd import _abi
+    from delta_ray_amd import delta_ray as dr
+    from delta_ray_amd import workloads
+    from delta_ray_amd.learned import EdgeProposal
+    dev = torch.device("cuda:0")
+    res, spp = args.lv_res, args.lv_spp
+    w = workloads.heightfield(resolution=(res, res), num_samples=spp)
+    W, H = w["resolution"]
+    nb = w["max_bounces"]
+    scene = dr.DeviceScene(w["camera"], w["shapes"], w["materials"], w["lights"])
+    d_img = d_image_for(w, scene, dev)
+    img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    g = torch.empty((scene.grad_size,), dtype=torch.float32, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def renders(flags, seeds):
+        out, ms = [], 0.0
+        for sd in seeds:
+            o = dr.DeviceScene.options(W, H, spp, nb, sd, _abi.DR_FLAGS_PARITY | flags)
+            e0.record()
+            scene.render_adjoint(o, d_img, img, g)
+            e1.record()
+            torch.cuda.synchronize()
+            ms += e0.elapsed_time(e1)
+            out.append(g.double().cpu().numpy().copy())
+        return np.array(out), ms / len(seeds)
+
+    def var(a):
+        return float(((a - a.mean(0)) ** 2).sum(1).sum() / (len(a) - 1))
+
+    def mom_var(a, groups=16):
+        m = len(a) // groups
+        return float(np.median([var(a[i * m:(i + 1) * m]) for i in range(groups)]))
+
+    def tail(a):
+        e = ((a - a.mean(0)) ** 2).sum(1)
+        return [float("%.4g" % x) for x in np.percentile(e, [50, 90, 99, 100])]
+
+    secondary = _abi.DR_SKIP_INTERIOR | _abi.DR_SKIP_PRIMARY_EDGES
+    n = args.lv_renders
+    renders(secondary, [1])  # warm-up
+    # training of the learned proposal, timed as a whole
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prop = EdgeProposal(scene, seed=7)
+    prop.reset_stats()
+    renders(secondary | _abi.DR_EDGE_STATS, range(50, 50 + args.lv_train_renders))
+    prop.collect()
+    loss = prop.train(steps=args.lv_train_steps, batch=1024)
+    wts = prop.apply()
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    print("learned: trained in %.2f s" % train_s, file=sys.stderr, flush=True)
+    out = {}
+    eps = args.lv_eps
+    for term, flags in (("secondary", secondary), ("full", 0)):
+        gp, tp = renders(flags, range(1000, 1000 + n))
+        gl, tl = renders(flags | _abi.DR_LEARNED_EDGES, range(100000, 100000 + n))
+        print("learned: %s term measured" % term, file=sys.stderr, flush=True)
+        mean = np.concatenate([gp, gl]).mean(0)
+        vp, vl = mom_var(gp), mom_var(gl)
+        g2 = max(float((mean ** 2).sum()) - 0.25 * (vp + vl) / n, 1e-30)
+        t_plain = tp / 1e3 * vp / (eps ** 2 * g2)
+        t_learned = tl / 1e3 * vl / (eps ** 2 * g2) + train_s
+        se = np.sqrt(gp.var(0, ddof=1) / n + gl.var(0, ddof=1) / n)
+        diff = np.abs(gp.mean(0) - gl.mean(0))
+        act = se > 0
+        out[term] = {"time_to_eps_s_plain": round(t_plain, 3), "time_to_eps_s_learned": round(t_learned, 3),
+                     "learned_over_plain_time": round(t_learned / t_plain, 4),
+                     "variance_ratio_plain_over_learned": round(vp / vl, 4),
+                     "ms_per_render_plain": round(tp, 3), "ms_per_render_learned": round(tl, 3),
+                     "rel_rmse_per_render_plain": round(float(np.sqrt(vp / g2)), 5),
+                     "rel_rmse_per_render_learned": round(float(np.sqrt(vl / g2)), 5),
+                     "sample_variance_plain": float("%.4g" % var(gp)),
+                     "sample_variance_learned": float("%.4g" % var(gl)),
+                     "sq_error_p50_p90_p99_max_plain": tail(gp), "sq_error_p50_p90_p99_max_learned": tail(gl),
+                     "mean_difference_over_se": round(float(np.linalg.norm(diff) / np.linalg.norm(se)), 3),
+                     "params_beyond_4se": round(float(np.mean(diff[act] / se[act] > 4.0)), 5)}
+    sec = out["secondary"]
+    res_line = {"metric": "time to gradient rel-RMSE 1e-2 (secondary-edge term), learned proposal",
+                "value": sec["time_to_eps_s_learned"], "unit": "s", "n_gpus": 1, "steps": n, "warmup": 1,
+                "ms_per_step": sec["ms_per_render_learned"], "higher_is_better": False, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
+                "config": {"workload": "heightfield %dx%d %dspp %d-bounce, %d independent renders per estimator"
+                                       % (W, H, spp, nb, n), "parallelism": "single GPU"},
+                "extra": {"eps": eps, "training_s": round(train_s, 3),
+                          "training": {"stats_renders": args.lv_train_renders, "mlp_steps": args.lv_train_steps,
+                                       "final_loss": round(float(loss), 5),
+                                       "weight_p5_p50_p95": [round(float(x), 4)
+                                                             for x in np.percentile(wts, [5, 50, 95])]},
+                          "secondary": sec, "full": out["full"]}}
+    print(json.dumps(res_line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -193,6 +320,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "learned":
+        bench_learned(args)
         return
     import numpy as np
     import torch

@@ -327,22 +327,27 @@ DR_INLINE bool et_step(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S,
     const int ni = e.nm >> 2, mask = e.nm & 3;
     const DrEdgeNode &node = T.nodes[ni];
     if (node.nprims > 0) {
-        // power heuristic over the strategies that could have selected the
-        // edge: the two passes (and, with the learned mixture, the two
-        // proposals, run with probabilities pi and 1 - pi)
+        // power heuristic over the two passes, which both run
+        // (edge_tree.cpp:528-536): pass t selected the edge with pmf p_t and
+        // weighs it p_t / (p_cone^2 + p_all^2).  With the learned mixture the
+        // point ran one proposal s of two (probabilities pi_s, pi_o); the
+        // passes' weights are scaled by q_s / (pi_s q_s + pi_o q_o), q = p_cone
+        // + p_all of each proposal (balance heuristic over the proposals: the
+        // defensive mixture, unbiased for any q, and exactly the plain
+        // estimator when both proposals agree).
         float den = e.cone_pmf * e.cone_pmf + e.all_pmf * e.all_pmf;
-        float pi = 1.f;
+        float scale = 1.f;
         if (T.mode != 0) {
-            pi = T.mode == 1 ? DR_LEARNED_SHARE : 1.f - DR_LEARNED_SHARE;
-            float po = 1.f - pi;
-            den = pi * pi * den + po * po * (e.cone_alt * e.cone_alt + e.all_alt * e.all_alt);
+            float pi = T.mode == 1 ? DR_LEARNED_SHARE : 1.f - DR_LEARNED_SHARE;
+            float qs = e.cone_pmf + e.all_pmf, qo = e.cone_alt + e.all_alt;
+            scale = qs / (pi * qs + (1.f - pi) * qo);
         }
         T.n_edges += node.nprims;
         for (int i = 0; i < node.nprims; i++) {
             const DrEdge &ed = T.edges[node.offset + i];
             if (!is_silhouette(S, sp.pos, ed)) continue;
-            if (mask & 1) emit(ed, pi * e.cone_pmf / den, T.directional, node.offset + i, T.jc++);
-            if (mask & 2) emit(ed, pi * e.all_pmf / den, T.directional, node.offset + i, 0x8000 | T.ja++);
+            if (mask & 1) emit(ed, scale * e.cone_pmf / den, T.directional, node.offset + i, T.jc++);
+            if (mask & 2) emit(ed, scale * e.all_pmf / den, T.directional, node.offset + i, 0x8000 | T.ja++);
         }
         return true;
     }

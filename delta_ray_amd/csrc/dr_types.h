@@ -106,7 +106,7 @@ struct DrSceneView {
     const DrEdgeNode *nrm_nodes;
     int num_sil_nodes, num_nrm_nodes;
     int num_sil_edges;
-    float *edge_stat;  // per tree edge (silhouette tree, then normal tree): sum of |contribution|, count
+    float *edge_stat;  // per tree edge (silhouette tree, then normal tree): sum of |contribution|, count, sum of squares
     const DrEdge *prim_edges;
     const float *prim_func;
     const float *prim_cdf;

@@ -277,8 +277,11 @@ DR_INLINE void edge_entry(const DrJob *jobs, const EdgeEntry &E, long long e,
     F3 dp = stage_edge<MAXB>(J.S, J.c, G, p.sid, p.b, p, edge, E.w, E.j, d_color, &mag);
     if ((J.c.flags & DR_EDGE_STATS) && J.S.edge_stat) {
         long long ge = (E.edge & 0x80000000) ? (long long)J.S.num_sil_edges + idx : (long long)idx;
-        if (mag != 0.f) atomicAdd(J.S.edge_stat + 2 * ge, mag);
-        atomicAdd(J.S.edge_stat + 2 * ge + 1, 1.f);
+        if (mag != 0.f) {
+            atomicAdd(J.S.edge_stat + 3 * ge, mag);
+            atomicAdd(J.S.edge_stat + 3 * ge + 2, mag * mag);
+        }
+        atomicAdd(J.S.edge_stat + 3 * ge + 1, 1.f);
     }
     bool nz = !is_zero(dp);
     if (J.c.flags & DR_QUIRK_DP_LAST) {
@@ -1141,7 +1144,7 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
     v.num_sil_edges = (int)hs.sil_edges.size();
     {
         size_t ne = hs.sil_edges.size() + hs.nrm_edges.size();
-        std::vector<float> zeros(2 * std::max<size_t>(ne, 1), 0.f);
+        std::vector<float> zeros(3 * std::max<size_t>(ne, 1), 0.f);
         const float *p = nullptr;
         if (!upload(s, zeros, &p)) {
             free_scene(s);
@@ -1284,7 +1287,7 @@ dr_status dr_scene_set_edge_weights(dr_scene *s, const float *weights) {
 dr_status dr_scene_edge_stats(dr_scene *s, float *out, int reset, void *stream) {
     if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
     hipStream_t hs = (hipStream_t)stream;
-    size_t n = 2 * (size_t)dr_scene_edge_count(s);
+    size_t n = 3 * (size_t)dr_scene_edge_count(s);
     if (n == 0) return DR_OK;
     if (out && hipMemcpyAsync(out, s->view.edge_stat, n * sizeof(float), hipMemcpyDeviceToDevice, hs) != hipSuccess)
         return fail(DR_ERROR_DEVICE, "statistics copy failed");

@@ -5,18 +5,25 @@ lengths and a BSDF bound (edge_tree.cpp:499-509).  Here a small network
 (dr_mlp_*: 24 per-edge features -> 64 -> 64 -> 1, bf16 weights on the matrix
 cores) predicts a per-edge weight; node lengths become sums of weight x edge
 length (dr_scene_set_edge_weights), and DR_LEARNED_EDGES renders mix that
-proposal with the plain one per shading point (probabilities 0.8 / 0.2) and
-weight every selected edge with the power heuristic over both proposals and
-both tree passes, so the estimator stays unbiased whatever the network
-predicts and the plain proposal remains part of every estimate.
+proposal with the plain one per shading point (probabilities 0.8 / 0.2): the
+two tree passes of the chosen proposal are weighted with the reference's
+power heuristic, and the proposals with the balance heuristic (a defensive
+mixture), so the estimator stays unbiased whatever the network predicts, the
+plain proposal remains part of every estimate, and uninformative weights
+give the plain estimator back.
 
 Training data come from the plain estimator itself: DR_EDGE_STATS renders
-accumulate, per edge, the number of secondary-edge samples and the sum of
-their |vertex-gradient contribution| (already divided by the selection
-probability).  Their ratio, the mean contribution per selection, is
-proportional to how much more often the edge should be chosen than the plain
-proposal chooses it; its log (centred) is the network's regression target,
-with a loss weight growing with the sample count.
+accumulate, per edge, the number of secondary-edge samples and the sum and
+sum of squares of their |vertex-gradient contribution| (already divided by the
+selection probability).  For a proposal whose per-edge scale factors w_e
+multiply the plain one, the second moment of the estimate is (to first
+order, when the points' totals vary little) smallest for w_e proportional to
+the root mean square of the edge's weighted contributions (sqrt(sum f^2/p / sum p), over the points that select it); the
+log of that RMS (centred, with a floor) is the network's regression target,
+with a loss weight growing with the sample count.  The applied weights
+combine the network's prediction with the edge's own statistic, in
+proportion to its sample count (the network is the prior of edges with few
+or no samples), and are normalised by their geometric mean.
 """
 import ctypes
 
@@ -41,8 +48,8 @@ class EdgeProposal:
         vh = view.handle if view is not None else None
         _abi.check(self.lib, self.lib.dr_scene_edge_features(scene.handle, vh, _ptr(self.features), None),
                    "dr_scene_edge_features")
-        self.stats = torch.zeros((max(self.n, 1), 2), dtype=torch.float32, device=dev)
-        self.acc = torch.zeros((max(self.n, 1), 2), dtype=torch.float64, device=dev)
+        self.stats = torch.zeros((max(self.n, 1), 3), dtype=torch.float32, device=dev)
+        self.acc = torch.zeros((max(self.n, 1), 3), dtype=torch.float64, device=dev)
         h = ctypes.c_void_p()
         _abi.check(self.lib, self.lib.dr_mlp_create(seed, ctypes.byref(h)), "dr_mlp_create")
         self.mlp = h
@@ -67,12 +74,18 @@ class EdgeProposal:
         self.torch.cuda.synchronize()
         self.acc += self.stats.double()
 
-    def targets(self, full_weight_at=64.0):
+    def targets(self, full_weight_at=64.0, floor=1e-3):
+        """Per-edge regression targets (log RMS of the weighted contributions,
+        centred) and loss weights.  The RMS gets `floor` times the RMS over
+        all samples added, so edges that were selected but never contributed
+        get a finite target (about log(floor) below the typical edge) instead
+        of dominating the loss."""
         t = self.torch
         cnt = self.acc[:, 1]
-        mean = self.acc[:, 0] / t.clamp(cnt, min=1.0)
         seen = cnt > 0
-        y = t.log(mean + 1e-12)
+        rms = t.sqrt(self.acc[:, 2] / t.clamp(cnt, min=1.0))
+        typical = t.sqrt(self.acc[:, 2].sum() / t.clamp(cnt.sum(), min=1.0)).clamp(min=1e-30)
+        y = t.log(rms + floor * typical)
         w = t.clamp(cnt / full_weight_at, max=1.0)
         w = t.where(seen, w, t.zeros_like(w))
         if bool(seen.any()):
@@ -89,15 +102,24 @@ class EdgeProposal:
         self.trained_steps += steps
         return loss.value
 
-    def apply(self, clip=(0.05, 20.0)):
-        """Network weights -> the scene's learned node lengths."""
+    def apply(self, clip=(0.05, 20.0), prior_count=16.0, temper=0.5):
+        """Network weights -> the scene's learned node lengths.  Per edge the
+        log-weight is (n * y + k * prediction) / (n + k): its own target y
+        from n samples, shrunk toward the network's prediction with k =
+        `prior_count` pseudo-samples; centred (weights normalised by their
+        geometric mean), multiplied by `temper` (weights = RMS ** temper: the
+        statistic is noisy and the best scale factor differs from point to
+        point, so the step toward it is damped) and clipped to `clip`."""
         _abi.check(self.lib, self.lib.dr_mlp_infer(self.mlp, _ptr(self.features), self.n, _ptr(self.logits), None),
                    "dr_mlp_infer")
         self.torch.cuda.synchronize()
-        lg = self.logits[:self.n].double().cpu().numpy()
-        wts = np.exp(lg - lg.max())
-        wts = wts / max(wts.mean(), 1e-30)
-        wts = np.clip(wts, clip[0], clip[1]).astype(np.float32)
+        pred = self.logits[:self.n].double().cpu().numpy()
+        y, _ = self.targets()
+        y = y[:self.n].double().cpu().numpy()
+        cnt = self.acc[:self.n, 1].cpu().numpy()
+        lg = (cnt * y + prior_count * pred) / (cnt + prior_count)
+        lg = temper * (lg - lg.mean())
+        wts = np.clip(np.exp(lg), clip[0], clip[1]).astype(np.float32)
         _abi.check(self.lib, self.lib.dr_scene_set_edge_weights(self.scene.handle, wts.ctypes.data),
                    "dr_scene_set_edge_weights")
         return wts

@@ -390,10 +390,11 @@ dr_status dr_scene_edge_features(dr_scene *scene, const dr_view *view, float *fe
  * NULL: all 1): node lengths become sums of weight x edge length
  * (DR_LEARNED_EDGES renders use them). */
 dr_status dr_scene_set_edge_weights(dr_scene *scene, const float *weights);
-/* Statistics accumulated by DR_EDGE_STATS renders, two floats per edge
- * (device buffer `out`, may be NULL): sum over the edge's secondary-edge
- * samples of |vertex-gradient contribution| (weights included), and the
- * number of samples.  reset: clear them afterwards. */
+/* Statistics accumulated by DR_EDGE_STATS renders, three floats per edge
+ * (device buffer `out`, may be NULL): over the edge's secondary-edge
+ * samples, the sum of |vertex-gradient contribution| (weights included),
+ * the number of samples and the sum of squared contributions.  reset: clear
+ * them afterwards. */
 dr_status dr_scene_edge_stats(dr_scene *scene, float *out, int reset, void *stream);
 
 /* The reference's test_line_integral (delta_ray.cpp:1572-1836) on the GPU

@@ -1,8 +1,10 @@
 """The learned secondary-edge proposal (DR_LEARNED_EDGES).  Nothing in the
 reference pins it; it is pinned by unbiasedness: whatever the per-edge
 weights, the mean secondary-edge gradient equals the plain estimator's within
-a few standard errors, because every selected edge is weighted by the power
-heuristic over both proposals (0.8 learned, 0.2 plain) and both tree passes."""
+a few standard errors, because the two tree passes of the proposal a point
+runs (0.8 learned, 0.2 plain) are weighted with the power heuristic and the
+proposals with the balance heuristic.  With all weights 1 both proposals are
+the plain one, and the mixture gives the plain estimate sample by sample."""
 import numpy as np
 import pytest
 
@@ -92,3 +94,21 @@ def test_train_and_apply_end_to_end(gpu):
     learned = _grads(torch, scene, w, 16, range(400, 440), SECONDARY | _abi.DR_LEARNED_EDGES, d)
     se = np.sqrt(plain.var(0) / len(plain) + learned.var(0) / len(learned))
     assert np.linalg.norm(plain.mean(0) - learned.mean(0)) < 4.0 * np.linalg.norm(se) + 1e-6
+
+
+def test_uniform_weights_give_the_plain_estimate(gpu):
+    """Weights all 1: the learned node lengths equal the plain ones (up to
+    rounding), the balance heuristic over the proposals is 1, and the same
+    seed gives the plain gradient."""
+    import torch
+    w = helpers.small_workload("heightfield", res=32)
+    scene = helpers.gpu_scene(w)
+    W, H = w["resolution"]
+    d = torch.from_numpy(np.random.default_rng(4).normal(size=(H, W, 3)).astype(np.float32)).to(gpu)
+    prop = EdgeProposal(scene, seed=3)
+    prop.set_weights(np.ones(prop.n, np.float32))
+    plain = _grads(torch, scene, w, 4, [11, 12], SECONDARY, d)
+    learned = _grads(torch, scene, w, 4, [11, 12], SECONDARY | _abi.DR_LEARNED_EDGES, d)
+    assert np.abs(plain).sum() > 0
+    for a, b in zip(plain, learned):
+        assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(a), np.linalg.norm(a - b) / np.linalg.norm(a)
