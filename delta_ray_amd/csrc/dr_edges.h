@@ -242,14 +242,90 @@ DR_INLINE float node_importance_len(const DrEdgeNode &n, float bound, F3 p, floa
 // point as soon as its traversal ends; et_step() calls
 // `emit(edge, weight, in_normal_tree, index, ordinal)` per selected edge.
 //
-// The stack is an array of its own (DR_TREE_STACK entries, owned by the
-// caller), not a member of EdgeTraversal: it is indexed at run time, so it
-// lives in private memory, while every other field of the traversal state can
-// stay in registers.
+// The stack is an object of its own (owned by the caller), not a member of
+// EdgeTraversal: its storage is indexed at run time, while every other field
+// of the traversal state stays in registers.  An entry carries what the parent
+// already knew of the node (child offset, edge count, whether the point lies
+// in its box), so popping an interior node needs only its two children's
+// records, loaded side by side.  Two stacks: PlainStack, an array (the CPU
+// restatement, tests), and LdsStack (the selection kernel): the top entries
+// live in LDS and older ones spill to private memory in halves.
 struct TreeEntry {
     int nm;  // node << 2 | pass mask (bit 0: the cone pass reaches the node, bit 1: the "all" pass)
     float bound, cone_pmf, all_pmf;
-    float cone_alt, all_alt;  // the same path's pmfs under the other proposal
+    int off;   // the node's offset (interior: right child; leaf: first edge)
+    int npin;  // the node's edge count << 1 | the point lies inside its box
+};
+
+// The same path's pmfs under the other proposal (learned mixture only),
+// kept beside the entry at the same depth.
+struct TreeAlt {
+    float cone, all;
+};
+
+struct PlainStack {
+    TreeEntry e[DR_TREE_STACK];
+    TreeAlt alt[DR_TREE_STACK];
+    int sp;
+    DR_INLINE void clear() { sp = 0; }
+    DR_INLINE int size() const { return sp; }
+    DR_INLINE void push(const TreeEntry &x, TreeAlt a, bool with_alt) {
+        if (with_alt) alt[sp] = a;
+        e[sp++] = x;
+    }
+    DR_INLINE TreeEntry pop(TreeAlt &a, bool with_alt) {
+        --sp;
+        if (with_alt) a = alt[sp];
+        return e[sp];
+    }
+};
+
+#define DR_LDS_STACK 8  // entries per lane in LDS (a multiple of 2)
+
+// Top DR_LDS_STACK entries of each lane in LDS, one word of an entry per
+// row of the block's lanes (conflict-free: lane l always uses bank l % 64);
+// entries below them in private memory.  Entries [lo, sp) are in LDS, entry i
+// in slot i % DR_LDS_STACK; [0, lo) in `spill`.
+struct LdsStack {
+    float *lds;  // the block's stack area + threadIdx.x (stride: blockDim.x)
+    int stride;
+    TreeEntry spill[DR_TREE_STACK];
+    TreeAlt alt[DR_TREE_STACK];
+    int sp, lo;
+    DR_INLINE void clear() { sp = lo = 0; }
+    DR_INLINE int size() const { return sp; }
+    DR_INLINE void put(int slot, const TreeEntry &x) {
+        float *w = lds + slot * 6 * stride;
+        w[0] = __int_as_float(x.nm);
+        w[stride] = x.bound;
+        w[2 * stride] = x.cone_pmf;
+        w[3 * stride] = x.all_pmf;
+        w[4 * stride] = __int_as_float(x.off);
+        w[5 * stride] = __int_as_float(x.npin);
+    }
+    DR_INLINE TreeEntry get(int slot) const {
+        const float *w = lds + slot * 6 * stride;
+        return TreeEntry{__float_as_int(w[0]), w[stride], w[2 * stride], w[3 * stride], __float_as_int(w[4 * stride]),
+                         __float_as_int(w[5 * stride])};
+    }
+    DR_INLINE void push(const TreeEntry &x, TreeAlt a, bool with_alt) {
+        if (sp - lo == DR_LDS_STACK) {
+            for (int j = 0; j < DR_LDS_STACK / 2; j++) spill[lo + j] = get((lo + j) % DR_LDS_STACK);
+            lo += DR_LDS_STACK / 2;
+        }
+        if (with_alt) alt[sp] = a;
+        put(sp % DR_LDS_STACK, x);
+        sp++;
+    }
+    DR_INLINE TreeEntry pop(TreeAlt &a, bool with_alt) {
+        if (sp == lo) {
+            lo -= DR_LDS_STACK / 2;
+            for (int j = 0; j < DR_LDS_STACK / 2; j++) put((lo + j) % DR_LDS_STACK, spill[lo + j]);
+        }
+        --sp;
+        if (with_alt) a = alt[sp];
+        return get(sp % DR_LDS_STACK);
+    }
 };
 
 struct EdgeTraversal {
@@ -265,14 +341,17 @@ struct EdgeTraversal {
     // learned mixture, 1 = this point uses the learned node lengths, 2 = the
     // plain ones (probabilities 0.8 / 0.2)
     int mode;
-    int sp_, pops;
+    int pops;
     uint32_t n_inner, n_edges;
 };
+
+DR_INLINE int node_npin(const DrEdgeNode &n, F3 p) { return n.nprims << 1 | (node_inside(n, p) ? 1 : 0); }
 
 // edge_tree.cpp:685-725: choose the tree (one draw from rc when both exist)
 // and push the root for both passes.  Returns false when there is nothing
 // to traverse.
-DR_INLINE bool et_init(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S, F3 wi, const SPoint &sp,
+template <class Stack>
+DR_INLINE bool et_init(EdgeTraversal &T, Stack &stack, const DrSceneView &S, F3 wi, const SPoint &sp,
                        const Cone &cone, float roughness, float dlum, float slum, int mode) {
     if (dot(wi, sp.fr.n) < 0.f) return false;
     bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;
@@ -296,25 +375,26 @@ DR_INLINE bool et_init(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S,
     }
     bound_ctx(T.B, wi, sp, roughness, dlum, slum);
     T.cone = cone;
-    float root_bound = bsdf_bound(T.nodes[0], sp, T.B);
+    const DrEdgeNode &root = T.nodes[0];
+    float root_bound = bsdf_bound(root, sp, T.B);
     T.n_inner = 0;
     T.n_edges = 0;
     T.jc = 0;
     T.ja = 0;
     T.pops = 0;
-    T.sp_ = 0;
     T.mode = mode;
-    stack[T.sp_++] = TreeEntry{3, root_bound, pmf, pmf, pmf, pmf};
+    stack.clear();
+    stack.push(TreeEntry{3, root_bound, pmf, pmf, root.offset, node_npin(root, sp.pos)}, TreeAlt{pmf, pmf}, mode != 0);
     return true;
 }
 
 #define DR_LEARNED_SHARE 0.8f
 
 // One node.  Returns false when the traversal is done.
-template <class Emit>
-DR_INLINE bool et_step(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S, const SPoint &sp, Emit &emit) {
+template <class Stack, class Emit>
+DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, const SPoint &sp, Emit &emit) {
     // every node is pushed at most once; the cap guards a malformed tree
-    if (T.sp_ == 0 || T.pops >= T.num_nodes + 8) {
+    if (stack.size() == 0 || T.pops >= T.num_nodes + 8) {
         if (S.stats) {
             atomicAdd(&S.stats[5], (unsigned long long)T.n_inner);
             atomicAdd(&S.stats[6], (unsigned long long)T.n_edges);
@@ -322,11 +402,12 @@ DR_INLINE bool et_step(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S,
         return false;
     }
     T.pops++;
-    TreeEntry e = stack[--T.sp_];
+    TreeAlt ea{0.f, 0.f};
+    TreeEntry e = stack.pop(ea, T.mode != 0);
     if (e.bound <= 0.f) return true;
     const int ni = e.nm >> 2, mask = e.nm & 3;
-    const DrEdgeNode &node = T.nodes[ni];
-    if (node.nprims > 0) {
+    const int nprims = e.npin >> 1;
+    if (nprims > 0) {
         // power heuristic over the two passes, which both run
         // (edge_tree.cpp:528-536): pass t selected the edge with pmf p_t and
         // weighs it p_t / (p_cone^2 + p_all^2).  With the learned mixture the
@@ -339,22 +420,22 @@ DR_INLINE bool et_step(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S,
         float scale = 1.f;
         if (T.mode != 0) {
             float pi = T.mode == 1 ? DR_LEARNED_SHARE : 1.f - DR_LEARNED_SHARE;
-            float qs = e.cone_pmf + e.all_pmf, qo = e.cone_alt + e.all_alt;
+            float qs = e.cone_pmf + e.all_pmf, qo = ea.cone + ea.all;
             scale = qs / (pi * qs + (1.f - pi) * qo);
         }
-        T.n_edges += node.nprims;
-        for (int i = 0; i < node.nprims; i++) {
-            const DrEdge &ed = T.edges[node.offset + i];
+        T.n_edges += nprims;
+        for (int i = 0; i < nprims; i++) {
+            const DrEdge &ed = T.edges[e.off + i];
             if (!is_silhouette(S, sp.pos, ed)) continue;
-            if (mask & 1) emit(ed, scale * e.cone_pmf / den, T.directional, node.offset + i, T.jc++);
-            if (mask & 2) emit(ed, scale * e.all_pmf / den, T.directional, node.offset + i, 0x8000 | T.ja++);
+            if (mask & 1) emit(ed, scale * e.cone_pmf / den, T.directional, e.off + i, T.jc++);
+            if (mask & 2) emit(ed, scale * e.all_pmf / den, T.directional, e.off + i, 0x8000 | T.ja++);
         }
         return true;
     }
-    int li = ni + 1, ri = node.offset;
+    int li = ni + 1, ri = e.off;
     const DrEdgeNode &L = T.nodes[li], &R = T.nodes[ri];
     T.n_inner++;
-    bool all_split = e.bound > 1.f || node_inside(node, sp.pos);
+    bool all_split = e.bound > 1.f || (e.npin & 1);
     bool lcs = true, rcs = true;
     if (T.directional) {
         lcs = contains_silhouette(L, sp.pos);
@@ -432,12 +513,12 @@ DR_INLINE bool et_step(EdgeTraversal &T, TreeEntry *stack, const DrSceneView &S,
             rm |= 2;
         }
     }
-    if (rm && T.sp_ < DR_TREE_STACK)
-        stack[T.sp_++] = TreeEntry{ri << 2 | rm, rb, prc * e.cone_pmf, pra * e.all_pmf, prc2 * e.cone_alt,
-                                   pra2 * e.all_alt};
-    if (lm && T.sp_ < DR_TREE_STACK)
-        stack[T.sp_++] = TreeEntry{li << 2 | lm, lb, plc * e.cone_pmf, pla * e.all_pmf, plc2 * e.cone_alt,
-                                   pla2 * e.all_alt};
+    if (rm && stack.size() < DR_TREE_STACK)
+        stack.push(TreeEntry{ri << 2 | rm, rb, prc * e.cone_pmf, pra * e.all_pmf, R.offset, node_npin(R, sp.pos)},
+                   TreeAlt{prc2 * ea.cone, pra2 * ea.all}, T.mode != 0);
+    if (lm && stack.size() < DR_TREE_STACK)
+        stack.push(TreeEntry{li << 2 | lm, lb, plc * e.cone_pmf, pla * e.all_pmf, L.offset, node_npin(L, sp.pos)},
+                   TreeAlt{plc2 * ea.cone, pla2 * ea.all}, T.mode != 0);
     return true;
 }
 

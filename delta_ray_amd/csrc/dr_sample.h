@@ -373,8 +373,9 @@ DR_INLINE F3 stage_path(const DrSceneView &S, const SampleCtx &c, GradSink &G, i
 // The random streams, light cone and tree choice of one point's selection
 // (the light cone and material values come with the point record); false
 // when the point selects nothing.
+template <class Stack>
 DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, const PointRec &p, EdgeTraversal &T,
-                            TreeEntry *stack) {
+                            Stack &stack) {
     philox_init(T.rc, c.seed, 0, p.sid, dr_stream_point(p.b));
     philox_next(T.rc);  // bsdf component of the point's edge sample
     philox_next(T.rc);  // position on the edge
@@ -395,7 +396,7 @@ DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, const Poin
 template <class Emit>
 DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, const PointRec &p, Emit &emit) {
     EdgeTraversal T;
-    TreeEntry stack[DR_TREE_STACK];
+    PlainStack stack;
     if (!select_begin(S, c, p, T, stack)) return 0;
     int n = 0;
     auto em = [&](const DrEdge &e, float w, bool nrm, int idx, int j) {

@@ -172,12 +172,17 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(const DrJob *__restrict__ job
 // queue atomic per group).  Every lane leaves the loop once `work` passes
 // nslots.  cnt[0]: queue length (may exceed cap: the host grows the queue and
 // renders again), cnt[1]: work counter.
-__global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ jobs, long long nslots,
+__global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ jobs, long long nslots, int maxb,
                                                      const PointRec *pts, EdgeEntry *q, unsigned long long *cnt,
                                                      long long cap, int *counts, int refill) {
+    const long long per_bounce = nslots / maxb;  // nslots is a multiple of maxb
+    __shared__ float s_stack[DR_LDS_STACK * 6 * DR_BLOCK];
     const int lane = (int)__lane_id();
     EdgeTraversal T;
-    TreeEntry stack[DR_TREE_STACK];
+    LdsStack stack;
+    stack.lds = s_stack + threadIdx.x;
+    stack.stride = DR_BLOCK;
+    stack.clear();
     SPoint sp;
     const DrSceneView *Sp = &jobs[0].S;
     long long slot = -1;
@@ -214,7 +219,11 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
                 base = __shfl(base, leader);
                 if (want) {
                     unsigned long long below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-                    long long s = (long long)(base + __popcll(below));
+                    long long f = (long long)(base + __popcll(below));
+                    // points in the order bounce-major: a wave's points are the
+                    // same bounce of neighbouring samples and pixels, which
+                    // walk similar tree paths (slots are bounce-minor)
+                    long long s = f < nslots ? (f % per_bounce) * maxb + f / per_bounce : f;
                     if (s >= nslots) {
                         more = false;
                         want = false;
@@ -600,7 +609,7 @@ struct LaunchTimer {
 struct Tuning {
     long long pass_samples = 1ll << 22;
     long long min_threads = 1ll << 20;
-    int select_refill = 32;
+    int select_refill = 64;
 };
 
 // Device working set of the pipeline, grown on demand, and the hipGraph of
@@ -785,7 +794,7 @@ struct Plan {
     int prim_r0 = 0, prim_r1 = 0, fwd_r0 = 0, fwd_r1 = 0;
     long long prim_blocks = 0, fwd_blocks = 0, max_slots = 0;
     int maxb = 1;
-    int select_refill = 32;
+    int select_refill = 64;
     std::vector<std::pair<float *, size_t>> zero;  // buffers cleared first
     std::vector<unsigned long long> key;           // what a captured graph depends on
 };
@@ -978,7 +987,7 @@ static void enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
         {
             LaunchTimer t(tm, TK_SELECT, hs);
             long long blocks = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_select, g_select_blocks));
-            k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, w.pts, w.q, w.cnt, w.q_cap,
+            k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, MAXB, w.pts, w.q, w.cnt, w.q_cap,
                                                             w.counts, P.select_refill);
         }
         long long eb = resident_blocks(k_edge<MAXB>, g_edge_blocks[bi]);
@@ -1390,7 +1399,7 @@ dr_status dr_batch_autotune(dr_batch *b, const dr_job *jobs, int num_jobs, void 
         }
     }
     // then the refill threshold of the edge selection, at the chosen pass sizes
-    static const int kRefill[] = {8, 16, 64};
+    static const int kRefill[] = {8, 16, 32};
     if (has_edges(P)) {
         for (int rf : kRefill) {
             if (st != DR_OK) break;

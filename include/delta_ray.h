@@ -342,7 +342,7 @@ dr_status dr_render_batch(dr_batch *batch, const dr_job *jobs, int num_jobs, int
 /* Pass sizes of the pipeline: pixel samples per pass and the least number
  * of threads a job is split into; select_refill: idle lanes (1-64) a wave
  * of the edge-selection kernel gathers before it starts new shading points
- * (0 = the default, 32).  dr_batch_autotune renders the job list with each
+ * (0 = the default, 64).  dr_batch_autotune renders the job list with each
  * candidate (outputs are overwritten) and keeps the fastest. */
 typedef struct dr_tuning {
     int64_t pass_samples;
