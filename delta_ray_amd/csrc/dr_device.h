@@ -91,10 +91,12 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
     // every node is entered at most once; the cap only guards against a
     // malformed tree
     const uint32_t cap = 2u * (uint32_t)S.num_nodes + 8u;
+    const DR_GLOBAL DrNode *nodes = gptr(S.nodes);
     while (nn < cap) {
-        const DrNode &N = S.nodes[node];
-        float4 A = *(const float4 *)N.a, B = *(const float4 *)N.b, C = *(const float4 *)N.c;
-        int4 D = *(const int4 *)N.d;
+        const DR_GLOBAL DrNode &N = nodes[node];
+        float4 A = *(const DR_GLOBAL float4 *)N.a, B = *(const DR_GLOBAL float4 *)N.b,
+               C = *(const DR_GLOBAL float4 *)N.c;
+        int4 D = *(const DR_GLOBAL int4 *)N.d;
         nn++;
         float t0, t1;
         bool h0 = box_hit(A.x, A.y, A.z, A.w, B.x, B.y, org, inv, tmin, best_t, t0);
@@ -102,7 +104,7 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
         // leaves first
         if (h0 && D.z > 0) {
             for (int i = 0; i < D.z; i++) {
-                const DrTri &T = S.tris[D.x + i];
+                const DrTri T = gptr(S.tris)[D.x + i];
                 float u, v, t;
                 nt++;
                 if (tri_hit(T, org, dir, tmin, tmax, u, v, t)) {
@@ -116,7 +118,7 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
         }
         if (h1 && D.w > 0) {
             for (int i = 0; i < D.w; i++) {
-                const DrTri &T = S.tris[D.y + i];
+                const DrTri T = gptr(S.tris)[D.y + i];
                 float u, v, t;
                 nt++;
                 if (tri_hit(T, org, dir, tmin, tmax, u, v, t)) {
@@ -162,10 +164,12 @@ DR_INLINE bool trace_any(const DrSceneView &S, F3 org, F3 dir, float tmin, float
     uint32_t nn = 0, nt = 0;
     bool hit = false;
     const uint32_t cap = 2u * (uint32_t)S.num_nodes + 8u;
+    const DR_GLOBAL DrNode *nodes = gptr(S.nodes);
     while (nn < cap) {
-        const DrNode &N = S.nodes[node];
-        float4 A = *(const float4 *)N.a, B = *(const float4 *)N.b, C = *(const float4 *)N.c;
-        int4 D = *(const int4 *)N.d;
+        const DR_GLOBAL DrNode &N = nodes[node];
+        float4 A = *(const DR_GLOBAL float4 *)N.a, B = *(const DR_GLOBAL float4 *)N.b,
+               C = *(const DR_GLOBAL float4 *)N.c;
+        int4 D = *(const DR_GLOBAL int4 *)N.d;
         nn++;
         float t0, t1;
         bool h0 = box_hit(A.x, A.y, A.z, A.w, B.x, B.y, org, inv, tmin, tmax, t0);
@@ -174,7 +178,8 @@ DR_INLINE bool trace_any(const DrSceneView &S, F3 org, F3 dir, float tmin, float
             for (int i = 0; i < D.z && !hit; i++) {
                 float u, v, t;
                 nt++;
-                hit = tri_hit(S.tris[D.x + i], org, dir, tmin, tmax, u, v, t);
+                const DrTri T = gptr(S.tris)[D.x + i];
+                hit = tri_hit(T, org, dir, tmin, tmax, u, v, t);
             }
             h0 = false;
         }
@@ -182,7 +187,8 @@ DR_INLINE bool trace_any(const DrSceneView &S, F3 org, F3 dir, float tmin, float
             for (int i = 0; i < D.w && !hit; i++) {
                 float u, v, t;
                 nt++;
-                hit = tri_hit(S.tris[D.y + i], org, dir, tmin, tmax, u, v, t);
+                const DrTri T = gptr(S.tris)[D.y + i];
+                hit = tri_hit(T, org, dir, tmin, tmax, u, v, t);
             }
             h1 = false;
         }

@@ -287,7 +287,7 @@ struct PlainStack {
 // entries below them in private memory.  Entries [lo, sp) are in LDS, entry i
 // in slot i % DR_LDS_STACK; [0, lo) in `spill`.
 struct LdsStack {
-    float *lds;  // the block's stack area + threadIdx.x (stride: blockDim.x)
+    DR_LDS float *lds;  // the block's stack area + threadIdx.x (stride: blockDim.x)
     int stride;
     TreeEntry spill[DR_TREE_STACK];
     TreeAlt alt[DR_TREE_STACK];
@@ -295,7 +295,7 @@ struct LdsStack {
     DR_INLINE void clear() { sp = lo = 0; }
     DR_INLINE int size() const { return sp; }
     DR_INLINE void put(int slot, const TreeEntry &x) {
-        float *w = lds + slot * 6 * stride;
+        DR_LDS float *w = lds + slot * 6 * stride;
         w[0] = __int_as_float(x.nm);
         w[stride] = x.bound;
         w[2 * stride] = x.cone_pmf;
@@ -304,7 +304,7 @@ struct LdsStack {
         w[5 * stride] = __int_as_float(x.npin);
     }
     DR_INLINE TreeEntry get(int slot) const {
-        const float *w = lds + slot * 6 * stride;
+        const DR_LDS float *w = lds + slot * 6 * stride;
         return TreeEntry{__float_as_int(w[0]), w[stride], w[2 * stride], w[3 * stride], __float_as_int(w[4 * stride]),
                          __float_as_int(w[5 * stride])};
     }
@@ -425,7 +425,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         }
         T.n_edges += nprims;
         for (int i = 0; i < nprims; i++) {
-            const DrEdge &ed = T.edges[e.off + i];
+            const DrEdge ed = gptr(T.edges)[e.off + i];
             if (!is_silhouette(S, sp.pos, ed)) continue;
             if (mask & 1) emit(ed, scale * e.cone_pmf / den, T.directional, e.off + i, T.jc++);
             if (mask & 2) emit(ed, scale * e.all_pmf / den, T.directional, e.off + i, 0x8000 | T.ja++);
@@ -433,7 +433,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         return true;
     }
     int li = ni + 1, ri = e.off;
-    const DrEdgeNode &L = T.nodes[li], &R = T.nodes[ri];
+    const DrEdgeNode L = gptr(T.nodes)[li], R = gptr(T.nodes)[ri];
     T.n_inner++;
     bool all_split = e.bound > 1.f || (e.npin & 1);
     bool lcs = true, rcs = true;

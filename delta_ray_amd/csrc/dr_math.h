@@ -13,6 +13,26 @@
 #define DR_INLINE __device__ __forceinline__
 #define DR_PI 3.14159265358979323846f
 
+// Address spaces in device code: loads through DR_GLOBAL pointers compile to
+// global_* instructions and DR_LDS to ds_*, where the generic flat_* ones
+// would also hold the LDS / scalar-load wait counter.
+#if defined(__HIP_DEVICE_COMPILE__)
+#ifndef DR_GLOBAL
+#define DR_GLOBAL __attribute__((address_space(1)))
+#endif
+#define DR_LDS __attribute__((address_space(3)))
+#else
+#ifndef DR_GLOBAL
+#define DR_GLOBAL
+#endif
+#define DR_LDS
+#endif
+// a pointer into device (global) memory, for loads
+template <class T>
+DR_INLINE const DR_GLOBAL T *gptr(const T *p) {
+    return (const DR_GLOBAL T *)p;
+}
+
 struct F2 {
     float x, y;
 };
@@ -55,7 +75,11 @@ DR_INLINE float max3(F3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
 DR_INLINE bool is_zero(F3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
 DR_INLINE float luminance(F3 a) { return 0.212671f * a.x + 0.715160f * a.y + 0.072169f * a.z; }
 DR_INLINE F3 lum_w() { return F3{0.212671f, 0.715160f, 0.072169f}; }
-DR_INLINE F3 load3(const float *p) { return F3{p[0], p[1], p[2]}; }
+// three floats from device (global) memory
+DR_INLINE F3 load3(const float *p) {
+    const DR_GLOBAL float *q = gptr(p);
+    return F3{q[0], q[1], q[2]};
+}
 
 // adjoint of normalize: d(v/|v|) given d_out
 DR_INLINE F3 d_normalize(F3 v, F3 d_out) {

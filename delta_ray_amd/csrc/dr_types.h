@@ -81,38 +81,47 @@ struct DrCamera {
 };
 
 // Everything a kernel needs, passed by value.
+// Device code reads the scene through pointers in the global address space,
+// so its loads compile to global_* rather than generic flat_* instructions.
+#ifndef DR_GLOBAL
+#if defined(__HIP_DEVICE_COMPILE__)
+#define DR_GLOBAL __attribute__((address_space(1)))
+#else
+#define DR_GLOBAL
+#endif
+#endif
 struct DrSceneView {
-    const DrNode *nodes;
-    const DrTri *tris;
-    const float *verts;
-    const int *tri_vidx;
-    const int *tri_shape;
-    const float *uvs;
-    const float *normals;
-    const float *texels;
-    const DrShape *shapes;
-    const DrMaterial *materials;
-    const DrLight *lights;
-    const float *light_func;  // per light: triangle areas, then cdf (n+1)
-    const float *light_cdf;
-    const float *lights_func;  // light selection by luminance
-    const float *lights_cdf;
+    const DR_GLOBAL DrNode *nodes;
+    const DR_GLOBAL DrTri *tris;
+    const DR_GLOBAL float *verts;
+    const DR_GLOBAL int *tri_vidx;
+    const DR_GLOBAL int *tri_shape;
+    const DR_GLOBAL float *uvs;
+    const DR_GLOBAL float *normals;
+    const DR_GLOBAL float *texels;
+    const DR_GLOBAL DrShape *shapes;
+    const DR_GLOBAL DrMaterial *materials;
+    const DR_GLOBAL DrLight *lights;
+    const DR_GLOBAL float *light_func;  // per light: triangle areas, then cdf (n+1)
+    const DR_GLOBAL float *light_cdf;
+    const DR_GLOBAL float *lights_func;  // light selection by luminance
+    const DR_GLOBAL float *lights_cdf;
     float lights_func_int, light_total_intensity;
     int num_lights, num_shapes;
     int num_tris, num_nodes;
-    const DrEdge *sil_edges;
-    const DrEdgeNode *sil_nodes;
-    const DrEdge *nrm_edges;
-    const DrEdgeNode *nrm_nodes;
+    const DR_GLOBAL DrEdge *sil_edges;
+    const DR_GLOBAL DrEdgeNode *sil_nodes;
+    const DR_GLOBAL DrEdge *nrm_edges;
+    const DR_GLOBAL DrEdgeNode *nrm_nodes;
     int num_sil_nodes, num_nrm_nodes;
     int num_sil_edges;
     float *edge_stat;  // per tree edge (silhouette tree, then normal tree): sum of |contribution|, count, sum of squares
-    const DrEdge *prim_edges;
-    const float *prim_func;
-    const float *prim_cdf;
+    const DR_GLOBAL DrEdge *prim_edges;
+    const DR_GLOBAL float *prim_func;
+    const DR_GLOBAL float *prim_cdf;
     float prim_func_int;
     int num_prim_edges;
-    const float *ltc;
+    const DR_GLOBAL float *ltc;
     DrCamera cam;
     unsigned long long *stats;  // NULL or 5 counters
 };

@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
     const int lane = (int)__lane_id();
     EdgeTraversal T;
     LdsStack stack;
-    stack.lds = s_stack + threadIdx.x;
+    stack.lds = (DR_LDS float *)s_stack + threadIdx.x;
     stack.stride = DR_BLOCK;
     stack.clear();
     SPoint sp;
@@ -671,8 +671,8 @@ struct dr_batch {
     Tuning tune;
 };
 
-template <class T>
-static bool upload(dr_scene *s, const std::vector<T> &v, const T **out) {
+template <class T, class P>
+static bool upload(dr_scene *s, const std::vector<T> &v, P *out) {
     if (v.empty()) {
         *out = nullptr;
         return true;
@@ -681,7 +681,7 @@ static bool upload(dr_scene *s, const std::vector<T> &v, const T **out) {
     if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) return false;
     s->allocs.push_back(p);
     if (hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return false;
-    *out = (const T *)p;
+    *out = (P)p;
     return true;
 }
 
@@ -706,9 +706,9 @@ static bool upload_primary(dr_scene *s) {
     if (!upload_prim(hs.prim_edges, hs.prim, s->prim_bufs)) return false;
     s->view.num_prim_edges = (int)hs.prim_edges.size();
     s->view.prim_func_int = hs.prim.func_int;
-    s->view.prim_edges = (const DrEdge *)s->prim_bufs[0];
-    s->view.prim_func = (const float *)s->prim_bufs[1];
-    s->view.prim_cdf = (const float *)s->prim_bufs[2];
+    s->view.prim_edges = (decltype(s->view.prim_edges))s->prim_bufs[0];
+    s->view.prim_func = (decltype(s->view.prim_func))s->prim_bufs[1];
+    s->view.prim_cdf = (decltype(s->view.prim_cdf))s->prim_bufs[2];
     return true;
 }
 
@@ -823,9 +823,9 @@ static dr_status build_plan(const dr_job *jobs, int n, const Tuning &tn, Plan &P
             dj.S.cam = jb.view->cam;
             dj.S.num_prim_edges = jb.view->num_prim_edges;
             dj.S.prim_func_int = jb.view->prim_func_int;
-            dj.S.prim_edges = (const DrEdge *)jb.view->bufs[0];
-            dj.S.prim_func = (const float *)jb.view->bufs[1];
-            dj.S.prim_cdf = (const float *)jb.view->bufs[2];
+            dj.S.prim_edges = (decltype(dj.S.prim_edges))jb.view->bufs[0];
+            dj.S.prim_func = (decltype(dj.S.prim_func))jb.view->bufs[1];
+            dj.S.prim_cdf = (decltype(dj.S.prim_cdf))jb.view->bufs[2];
         }
         dj.c = make_ctx(&jb.opts);
         plan_chunks(&jb.opts, tn.min_threads, dj.per, dj.chunks);
@@ -1284,10 +1284,10 @@ dr_status dr_scene_set_edge_weights(dr_scene *s, const float *weights) {
         for (DrEdgeNode &n : hs.nrm_nodes) n.pad = n.total_length;
     }
     if ((!hs.sil_nodes.empty() &&
-         hipMemcpy(const_cast<DrEdgeNode *>(s->view.sil_nodes), hs.sil_nodes.data(),
+         hipMemcpy((void *)s->view.sil_nodes, hs.sil_nodes.data(),
                    hs.sil_nodes.size() * sizeof(DrEdgeNode), hipMemcpyHostToDevice) != hipSuccess) ||
         (!hs.nrm_nodes.empty() &&
-         hipMemcpy(const_cast<DrEdgeNode *>(s->view.nrm_nodes), hs.nrm_nodes.data(),
+         hipMemcpy((void *)s->view.nrm_nodes, hs.nrm_nodes.data(),
                    hs.nrm_nodes.size() * sizeof(DrEdgeNode), hipMemcpyHostToDevice) != hipSuccess))
         return fail(DR_ERROR_DEVICE, "edge tree upload failed");
     return DR_OK;
