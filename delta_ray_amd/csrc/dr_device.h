@@ -77,6 +77,39 @@ DR_INLINE void count_trace(const DrSceneView &S, uint32_t nn, uint32_t nt) {
     }
 }
 
+// Traversal stack of a ray query.  On the GPU its first DR_TSTACK_LDS entries
+// live in LDS (one column per lane of the workgroup, at most 256 lanes; a
+// thread runs one query at a time, so every query of a kernel shares the
+// array) and deeper ones in private memory.
+#define DR_TSTACK_LDS 16
+#if defined(__HIP_DEVICE_COMPILE__)
+DR_INLINE DR_LDS int *trace_stack_lds() {
+    __shared__ int s_tstack[DR_TSTACK_LDS * 256];
+    return (DR_LDS int *)s_tstack + threadIdx.x;
+}
+struct TraceStack {
+    DR_LDS int *lds;
+    int stride;
+    int priv[DR_STACK - DR_TSTACK_LDS];
+    DR_INLINE void init() {
+        lds = trace_stack_lds();
+        stride = (int)blockDim.x;
+    }
+    DR_INLINE void put(int i, int v) {
+        if (i < DR_TSTACK_LDS) lds[i * stride] = v;
+        else priv[i - DR_TSTACK_LDS] = v;
+    }
+    DR_INLINE int get(int i) const { return i < DR_TSTACK_LDS ? lds[i * stride] : priv[i - DR_TSTACK_LDS]; }
+};
+#else
+struct TraceStack {
+    int a[DR_STACK];
+    DR_INLINE void init() {}
+    DR_INLINE void put(int i, int v) { a[i] = v; }
+    DR_INLINE int get(int i) const { return a[i]; }
+};
+#endif
+
 DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, float tmax, float &ou, float &ov) {
     Isect best = no_hit();
     if (S.num_nodes == 0) return best;
@@ -84,7 +117,8 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
     float best_t = tmax;
     int best_g = -1;
     float bu = 0.f, bv = 0.f;
-    int stack[DR_STACK];
+    TraceStack stack;
+    stack.init();
     int sp = 0;
     int node = 0;
     uint32_t nn = 0, nt = 0;
@@ -133,7 +167,7 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
         if (h0 && h1) {
             int near = D.x, far = D.y;
             if (t1 < t0) { near = D.y; far = D.x; }
-            if (sp < DR_STACK) stack[sp++] = far;
+            if (sp < DR_STACK) stack.put(sp++, far);
             node = near;
         } else if (h0) {
             node = D.x;
@@ -141,7 +175,7 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
             node = D.y;
         } else {
             if (sp == 0) break;
-            node = stack[--sp];
+            node = stack.get(--sp);
         }
     }
     count_trace(S, nn, nt);
@@ -158,7 +192,8 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
 DR_INLINE bool trace_any(const DrSceneView &S, F3 org, F3 dir, float tmin, float tmax) {
     if (S.num_nodes == 0) return false;
     F3 inv = F3{1.f / dir.x, 1.f / dir.y, 1.f / dir.z};
-    int stack[DR_STACK];
+    TraceStack stack;
+    stack.init();
     int sp = 0;
     int node = 0;
     uint32_t nn = 0, nt = 0;
@@ -194,7 +229,7 @@ DR_INLINE bool trace_any(const DrSceneView &S, F3 org, F3 dir, float tmin, float
         }
         if (hit) break;
         if (h0 && h1) {
-            if (sp < DR_STACK) stack[sp++] = D.y;
+            if (sp < DR_STACK) stack.put(sp++, D.y);
             node = D.x;
         } else if (h0) {
             node = D.x;
@@ -202,7 +237,7 @@ DR_INLINE bool trace_any(const DrSceneView &S, F3 org, F3 dir, float tmin, float
             node = D.y;
         } else {
             if (sp == 0) break;
-            node = stack[--sp];
+            node = stack.get(--sp);
         }
     }
     count_trace(S, nn, nt);
