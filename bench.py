@@ -386,8 +386,13 @@ def main():
     n_points = samples * nb
     bytes_k["select"] += (B_POINT * n_points + B_TREE_EXPAND * cb["edge_nodes_visited"] +
                           B_EDGE_TEST * cb["edges_tested"] + B_QUEUE * cb["secondary_edges"])
+    # the edge samples run in two launches per pass (k_edge, then k_edge_shade
+    # on the samples with continuation paths): their work counters are one
+    # figure, so their times are taken together as "edge"
+    kt["edge_shade_ms"] = kt["shade_ms"]
+    kt["edge_ms"] += kt["shade_ms"]
     kms = {k: kt[k + "_ms"] / max(kt[k + "_launches"], 1) for k in ("path", "select", "edge", "dp", "primary")}
-    kms_step = {k: kt[k + "_ms"] / args.steps for k in ("path", "select", "edge", "dp", "primary")}
+    kms_step = {k: kt[k + "_ms"] / args.steps for k in ("path", "select", "edge", "dp", "primary", "edge_shade")}
     dom = max(("path", "select", "edge", "primary"), key=lambda k: kms_step[k])
     launches_per_step = max(kt[dom + "_launches"] // args.steps, 1)
     bytes_launch = bytes_k[dom] / launches_per_step

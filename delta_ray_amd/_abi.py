@@ -103,7 +103,7 @@ DR_STATS_COUNTERS = 1
 DR_STATS_TIMING = 2
 
 
-KERNELS = ["forward", "path", "primary", "select", "edge", "dp"]
+KERNELS = ["forward", "path", "primary", "select", "edge", "dp", "shade"]
 
 
 class dr_stats(ctypes.Structure):
@@ -111,7 +111,7 @@ class dr_stats(ctypes.Structure):
                 ("tris_visited", ctypes.c_uint64), ("secondary_edges", ctypes.c_uint64),
                 ("primary_edge_samples", ctypes.c_uint64),
                 ("edge_nodes_visited", ctypes.c_uint64), ("edges_tested", ctypes.c_uint64),
-                ("kernel_ms", ctypes.c_double * 6), ("kernel_launches", ctypes.c_uint64 * 6)]
+                ("kernel_ms", ctypes.c_double * 7), ("kernel_launches", ctypes.c_uint64 * 7)]
 
 
 class dr_scene_info(ctypes.Structure):

@@ -578,7 +578,7 @@ DR_INLINE bool ltc_line_sample(F3 a, F3 b, float eu, LineSample &L) {
 DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_component, float eu, const DrEdge &edge,
                                                const LightSample &ls, const SecondarySample *ss, int nss, F3 wi,
                                                const Isect &is, const SPoint &sp, F3 weight, F3 d_color,
-                                               bool direct_only) {
+                                               bool direct_only, bool *defer = nullptr) {
     EdgeResult R = EdgeResult{F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
     if (!valid(is)) return R;
     if (!is_silhouette(S, sp.pos, edge)) return R;
@@ -633,6 +633,13 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
     if (sum3(eb) < 1e-6f) return R;
     F3 shr = F3{0.f, 0.f, 0.f}, shl = F3{0.f, 0.f, 0.f};
     if (!direct_only) {
+        // a sample that goes on to the continuation paths: with `defer`, stop
+        // here and report it (the caller runs it again in full, in a launch of
+        // such samples only, so the long continuation code runs on full waves)
+        if (defer) {
+            *defer = true;
+            return R;
+        }
         shr = shade(S, -sdir, ir, spr, ls, ss, nss, false);
         shl = shade(S, -sdir, il, spl, ls, ss, nss, false);
         if (valid(ir)) {

@@ -263,11 +263,19 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
 }
 
 // C: one selected edge per iteration of a grid-stride loop over the queue
-// (its length is read on the device, so no host round trip is needed).
+// (its length is read on the device, so no host round trip is needed).  Only
+// one edge sample in seven or so goes on to the continuation paths (the rest
+// stop at the horizon, at the connectivity test of the two side rays or at a
+// zero BSDF), and those paths are the longest code of the kernel: run in the
+// same waves they would leave most lanes idle.  So k_edge stops such samples
+// where the continuation starts (returns true: nothing of theirs is written)
+// and appends them to a list; k_edge_shade then runs them in full, on full
+// waves.  Every sample is thus computed by exactly one launch, with the same
+// operations as in one pass.
 template <int MAXB>
-DR_INLINE void edge_entry(const DrJob *jobs, const EdgeEntry &E, long long e,
+DR_INLINE bool edge_entry(const DrJob *jobs, const EdgeEntry &E, long long e,
                           const PointRec *pts, float *pt_dp, int *pt_lastj, float4 *entry_dp,
-                          unsigned long long *gkey, float *gval) {
+                          unsigned long long *gkey, float *gval, int gslots, bool defer) {
     const PointRec &p = pts[E.slot];
     const DrJob &J = jobs[p.job];
     int idx = E.edge & 0x7fffffff;
@@ -279,11 +287,14 @@ DR_INLINE void edge_entry(const DrJob *jobs, const EdgeEntry &E, long long e,
     G.color_mode = true;
     G.gkey = gkey;
     G.gval = gval;
-    G.gmask = DR_GC_SLOTS_BIG - 1;
+    G.gmask = gslots - 1;
     G.job = (unsigned)p.job;
     F3 d_color = load3(J.d_image + 3 * (size_t)p.pix);
     float mag = 0.f;
-    F3 dp = stage_edge<MAXB>(J.S, J.c, G, p.sid, p.b, p, edge, E.w, E.j, d_color, &mag);
+    bool deferred = false;
+    F3 dp = stage_edge<MAXB>(J.S, J.c, G, p.sid, p.b, p, edge, E.w, E.j, d_color, &mag,
+                             defer ? &deferred : nullptr);
+    if (deferred) return true;
     if ((J.c.flags & DR_EDGE_STATS) && J.S.edge_stat) {
         long long ge = (E.edge & 0x80000000) ? (long long)J.S.num_sil_edges + idx : (long long)idx;
         if (mag != 0.f) {
@@ -306,24 +317,62 @@ DR_INLINE void edge_entry(const DrJob *jobs, const EdgeEntry &E, long long e,
             atomicAdd(pt_dp + 3ll * E.slot + 2, dp.z);
         }
     }
+    return false;
 }
 
-// cnt[2] keeps the largest queue length seen, for the host's overflow check
+// cnt[2] keeps the largest queue length seen, for the host's overflow check;
+// cnt[3] counts the samples left for k_edge_shade (listed in `cont`, which
+// has the queue's capacity).
 template <int MAXB>
 __global__ void __launch_bounds__(DR_BLOCK) k_edge(const DrJob *__restrict__ jobs, const EdgeEntry *q,
                                                    unsigned long long *cnt, long long cap,
                                                    const PointRec *pts, float *pt_dp, int *pt_lastj,
-                                                   float4 *entry_dp) {
-    __shared__ unsigned long long gkey[DR_GC_SLOTS_BIG];
-    __shared__ float gval[DR_GC_SLOTS_BIG];
-    gc_init(gkey, gval, DR_GC_SLOTS_BIG);
+                                                   float4 *entry_dp, int *cont) {
+    // the smaller gradient cache keeps three workgroups per CU
+    __shared__ unsigned long long gkey[DR_GC_SLOTS_SMALL];
+    __shared__ float gval[DR_GC_SLOTS_SMALL];
+    gc_init(gkey, gval, DR_GC_SLOTS_SMALL);
     __syncthreads();
     unsigned long long len = cnt[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt[2], len);
     long long n = (long long)len < cap ? (long long)len : cap;
     long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride)
-        edge_entry<MAXB>(jobs, q[e], e, pts, pt_dp, pt_lastj, entry_dp, gkey, gval);
+    const int lane = (int)__lane_id();
+    for (long long e0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x - lane); e0 < n; e0 += stride) {
+        // whole waves iterate together (e0: the wave's first entry), so the
+        // list append below is one atomic per wave
+        long long e = e0 + lane;
+        bool later = e < n && edge_entry<MAXB>(jobs, q[e], e, pts, pt_dp, pt_lastj, entry_dp, gkey, gval,
+                                               DR_GC_SLOTS_SMALL, true);
+        unsigned long long m = __ballot(later);
+        if (m) {
+            int leader = __ffsll((long long)m) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&cnt[3], (unsigned long long)__popcll(m));
+            base = __shfl(base, leader);
+            if (later) cont[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)e;
+        }
+    }
+    __syncthreads();
+    gc_flush(gkey, gval, DR_GC_SLOTS_SMALL, [&](int j) { return jobs[j].grads; });
+}
+
+// C': the samples k_edge left for the continuation paths, in full.
+template <int MAXB>
+__global__ void __launch_bounds__(DR_BLOCK) k_edge_shade(const DrJob *__restrict__ jobs, const EdgeEntry *q,
+                                                         const unsigned long long *cnt, const PointRec *pts,
+                                                         float *pt_dp, int *pt_lastj, float4 *entry_dp,
+                                                         const int *cont) {
+    __shared__ unsigned long long gkey[DR_GC_SLOTS_BIG];
+    __shared__ float gval[DR_GC_SLOTS_BIG];
+    gc_init(gkey, gval, DR_GC_SLOTS_BIG);
+    __syncthreads();
+    long long n = (long long)cnt[3];
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        long long e = cont[i];
+        edge_entry<MAXB>(jobs, q[e], e, pts, pt_dp, pt_lastj, entry_dp, gkey, gval, DR_GC_SLOTS_BIG, false);
+    }
     __syncthreads();
     gc_flush(gkey, gval, DR_GC_SLOTS_BIG, [&](int j) { return jobs[j].grads; });
 }
@@ -558,7 +607,9 @@ static dr_status fail(dr_status s, const std::string &msg) {
 extern "C" dr_status dr_internal_fail(dr_status s, const char *msg) { return fail(s, msg ? msg : "?"); }
 
 // kernel timing: event pairs recorded around launches when enabled
-enum { TK_FORWARD = 0, TK_PATH = 1, TK_PRIMARY = 2, TK_SELECT = 3, TK_EDGE = 4, TK_DP = 5, TK_COUNT = 6 };
+enum { TK_FORWARD = 0, TK_PATH = 1, TK_PRIMARY = 2, TK_SELECT = 3, TK_EDGE = 4, TK_DP = 5, TK_SHADE = 6, TK_COUNT = 7 };
+static_assert(TK_FORWARD == DR_K_FORWARD && TK_EDGE == DR_K_EDGE && TK_SHADE == DR_K_SHADE && TK_COUNT == DR_K_COUNT,
+              "kernel timing slots follow dr_stats");
 struct TimedLaunch {
     int kind;
     hipEvent_t a, b;
@@ -622,8 +673,9 @@ struct Workspace {
     long long slot_cap = 0;
     EdgeEntry *q = nullptr;
     float4 *entry_dp = nullptr;
+    int *cont = nullptr;                     // queue entries left for the continuation launch
     long long q_cap = 0;
-    unsigned long long *cnt = nullptr;       // queue length, work counter, largest queue length
+    unsigned long long *cnt = nullptr;       // queue length, work counter, largest queue length, continuations
     unsigned long long *cnt_host = nullptr;  // pinned
     DrJob *jobs = nullptr;
     int jobs_cap = 0;
@@ -635,7 +687,7 @@ struct Workspace {
 };
 
 static void free_workspace(Workspace &w) {
-    void *ps[] = {w.pts, w.pt_dp, w.pt_lastj, w.counts, w.q, w.entry_dp, w.cnt, w.jobs, w.ranges};
+    void *ps[] = {w.pts, w.pt_dp, w.pt_lastj, w.counts, w.q, w.entry_dp, w.cont, w.cnt, w.jobs, w.ranges};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (w.cnt_host) (void)hipHostFree(w.cnt_host);
@@ -937,7 +989,8 @@ static dr_status ensure_workspace(Workspace &w, const Plan &P) {
     }
     long long qn = P.max_slots * 2;
     if (qn > w.q_cap) {
-        if (!grow((void **)&w.q, qn * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, qn * sizeof(float4))) {
+        if (!grow((void **)&w.q, qn * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, qn * sizeof(float4)) ||
+            !grow((void **)&w.cont, qn * sizeof(int))) {
             w.q_cap = 0;
             return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint edge queue");
         }
@@ -956,7 +1009,8 @@ static dr_status ensure_workspace(Workspace &w, const Plan &P) {
 }
 
 static dr_status grow_queue(Workspace &w, long long n) {
-    if (!grow((void **)&w.q, n * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, n * sizeof(float4))) {
+    if (!grow((void **)&w.q, n * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, n * sizeof(float4)) ||
+        !grow((void **)&w.cont, n * sizeof(int))) {
         w.q_cap = 0;
         return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint edge queue");
     }
@@ -964,7 +1018,7 @@ static dr_status grow_queue(Workspace &w, long long n) {
     return DR_OK;
 }
 
-static long long g_select_blocks = 0, g_edge_blocks[4] = {0, 0, 0, 0};
+static long long g_select_blocks = 0, g_edge_blocks[4] = {0, 0, 0, 0}, g_shade_blocks[4] = {0, 0, 0, 0};
 
 // All launches of a plan, stream-ordered, without host synchronisation (so
 // they can be captured in a hipGraph).  The tables are already on the device.
@@ -983,7 +1037,8 @@ static void enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
         if (!ps.edges) continue;
         k_fill<<<1024, 256, 0, hs>>>(w.pt_dp, (size_t)ps.nslots * 3, 0.f);
         k_fill_i<<<1024, 256, 0, hs>>>(w.pt_lastj, (size_t)ps.nslots, -1);
-        k_fill<<<1, 64, 0, hs>>>((float *)w.cnt, 4, 0.f);
+        k_fill<<<1, 64, 0, hs>>>((float *)w.cnt, 4, 0.f);        // queue length, work counter
+        k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 3), 2, 0.f);  // continuations
         {
             LaunchTimer t(tm, TK_SELECT, hs);
             long long blocks = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_select, g_select_blocks));
@@ -994,7 +1049,13 @@ static void enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
         {
             LaunchTimer t(tm, TK_EDGE, hs);
             k_edge<MAXB><<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.jobs, w.q, w.cnt, w.q_cap, w.pts, w.pt_dp, w.pt_lastj,
-                                                            w.entry_dp);
+                                                            w.entry_dp, w.cont);
+        }
+        {
+            LaunchTimer t(tm, TK_SHADE, hs);
+            long long sb = resident_blocks(k_edge_shade<MAXB>, g_shade_blocks[bi]);
+            k_edge_shade<MAXB><<<(unsigned)sb, DR_BLOCK, 0, hs>>>(w.jobs, w.q, w.cnt, w.pts, w.pt_dp, w.pt_lastj,
+                                                                  w.entry_dp, w.cont);
         }
         if (ps.dp_last) k_dp_last<<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.q, w.cnt, w.q_cap, w.entry_dp, w.pt_lastj, w.pt_dp);
         {

@@ -271,9 +271,10 @@ dr_status dr_intersect(dr_scene *scene, const float *rays, int64_t n,
  * times are cleared by every call. */
 enum { DR_STATS_COUNTERS = 1, DR_STATS_TIMING = 2 };
 /* kernel_ms / kernel_launches index: forward, path (radiance + interior
- * adjoint), primary edges, edge selection, edge samples, Dirac sweep */
+ * adjoint), primary edges, edge selection, edge samples, Dirac sweep, edge
+ * samples with continuation paths */
 enum { DR_K_FORWARD = 0, DR_K_PATH = 1, DR_K_PRIMARY = 2, DR_K_SELECT = 3, DR_K_EDGE = 4, DR_K_DP = 5,
-       DR_K_COUNT = 6 };
+       DR_K_SHADE = 6, DR_K_COUNT = 7 };
 typedef struct dr_stats {
     uint64_t rays;
     uint64_t nodes_visited;
