@@ -24,6 +24,21 @@ DR_INLINE bool is_silhouette(const DrSceneView &S, F3 p, const DrEdge &e) {
     return f0 != f1;
 }
 
+// The same test for edge i of the normal tree, from its geometry record
+// (one 64-byte read instead of edge -> shape -> face indices -> vertices);
+// same operations on the same values.  Silhouette-tree edges (boundary or
+// without shading normals) always pass.
+DR_INLINE bool is_silhouette_nrm(const DrSceneView &S, F3 p, int i) {
+    const DR_GLOBAL float4 *g = (const DR_GLOBAL float4 *)S.nrm_geo + 4 * (size_t)i;
+    float4 A = g[0], B = g[1], C = g[2], D = g[3];
+    F3 v0 = F3{A.x, A.y, A.z}, v1 = F3{B.x, B.y, B.z}, va = F3{C.x, C.y, C.z}, vb = F3{D.x, D.y, D.z};
+    F3 n0 = normalize(cross(v0 - va, v1 - va));
+    F3 n1 = normalize(cross(v1 - vb, v0 - vb));
+    bool f0 = dot(p - v0, n0) > 0.f && dot(p - v1, n0) > 0.f && dot(p - va, n0) > 0.f;
+    bool f1 = dot(p - v0, n1) > 0.f && dot(p - v1, n1) > 0.f && dot(p - vb, n1) > 0.f;
+    return f0 != f1;
+}
+
 struct Cone {
     F3 org, dir;
     float cos_angle;
@@ -240,7 +255,7 @@ DR_INLINE float node_importance_len(const DrEdgeNode &n, float bound, F3 p, floa
 // the reference's sequential order.  The traversal is explicit state advanced
 // one node per et_step() so that a persistent kernel can give a lane a new
 // point as soon as its traversal ends; et_step() calls
-// `emit(edge, weight, in_normal_tree, index, ordinal)` per selected edge.
+// `emit(weight, in_normal_tree, index, ordinal)` per selected edge.
 //
 // The stack is an object of its own (owned by the caller), not a member of
 // EdgeTraversal: its storage is indexed at run time, while every other field
@@ -425,10 +440,9 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         }
         T.n_edges += nprims;
         for (int i = 0; i < nprims; i++) {
-            const DrEdge ed = gptr(T.edges)[e.off + i];
-            if (!is_silhouette(S, sp.pos, ed)) continue;
-            if (mask & 1) emit(ed, scale * e.cone_pmf / den, T.directional, e.off + i, T.jc++);
-            if (mask & 2) emit(ed, scale * e.all_pmf / den, T.directional, e.off + i, 0x8000 | T.ja++);
+            if (T.directional && !is_silhouette_nrm(S, sp.pos, e.off + i)) continue;
+            if (mask & 1) emit(scale * e.cone_pmf / den, T.directional, e.off + i, T.jc++);
+            if (mask & 2) emit(scale * e.all_pmf / den, T.directional, e.off + i, 0x8000 | T.ja++);
         }
         return true;
     }

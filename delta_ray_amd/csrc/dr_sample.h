@@ -399,7 +399,7 @@ DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, const Point
     PlainStack stack;
     if (!select_begin(S, c, p, T, stack)) return 0;
     int n = 0;
-    auto em = [&](const DrEdge &e, float w, bool nrm, int idx, int j) {
+    auto em = [&](float w, bool nrm, int idx, int j) {
         emit(nrm, idx, w, j);
         n++;
     };

@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
             if ((long long)(base + e) < cap) q[base + e] = buf[e];
         nbuf = 0;
     };
-    auto emit = [&](const DrEdge &ed, float w, bool nrm, int idx, int ord) {
+    auto emit = [&](float w, bool nrm, int idx, int ord) {
         // zero-weight edges carry no work (reference delta_ray.cpp:906-908)
         if (w != 0.f) {
             buf[nbuf++] = EdgeEntry{(int)slot, idx | (nrm ? (int)0x80000000 : 0), w, ord};
@@ -1192,7 +1192,7 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
               upload(s, hs.light_cdf, &v.light_cdf) && upload(s, hs.light_sel.func, &v.lights_func) &&
               upload(s, hs.light_sel.cdf, &v.lights_cdf) && upload(s, hs.sil_edges, &v.sil_edges) &&
               upload(s, hs.sil_nodes, &v.sil_nodes) && upload(s, hs.nrm_edges, &v.nrm_edges) &&
-              upload(s, hs.nrm_nodes, &v.nrm_nodes);
+              upload(s, hs.nrm_nodes, &v.nrm_nodes) && upload(s, hs.nrm_geo, &v.nrm_geo);
     std::vector<float> ltc(dr_ltc_blob, dr_ltc_blob + 16384 * 9);
     if ((size_t)(dr_ltc_blob_end - dr_ltc_blob) != 16384 * 9) {
         free_scene(s);

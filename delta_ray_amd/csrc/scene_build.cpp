@@ -675,6 +675,21 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
     }
     build_edge_tree(hs, sil, false, hs.sil_nodes, hs.sil_edges);
     build_edge_tree(hs, nrm, true, hs.nrm_nodes, hs.nrm_edges);
+    // the vertices the silhouette test of a normal-tree edge reads
+    // (edge.h:59-79): endpoints, and each face's vertex off the edge
+    hs.nrm_geo.assign(16 * hs.nrm_edges.size(), 0.f);
+    for (size_t i = 0; i < hs.nrm_edges.size(); i++) {
+        const DrEdge &e = hs.nrm_edges[i];
+        const DrShape &sh = hs.shapes[e.shape];
+        int g0 = sh.vert_off + e.v0, g1 = sh.vert_off + e.v1;
+        const int *i0 = &hs.tri_vidx[3 * (size_t)(sh.tri_off + e.f0)];
+        const int *i1 = &hs.tri_vidx[3 * (size_t)(sh.tri_off + e.f1)];
+        int a = i0[0] != g0 && i0[0] != g1 ? i0[0] : (i0[1] != g0 && i0[1] != g1 ? i0[1] : i0[2]);
+        int b = i1[0] != g0 && i1[0] != g1 ? i1[0] : (i1[1] != g0 && i1[1] != g1 ? i1[1] : i1[2]);
+        const int gv[4] = {g0, g1, a, b};
+        for (int k = 0; k < 4; k++)
+            for (int c = 0; c < 3; c++) hs.nrm_geo[16 * i + 4 * k + c] = hs.verts[3 * (size_t)gv[k] + c];
+    }
     return "";
 }
 
