@@ -592,12 +592,24 @@ DR_INLINE bool ltc_line_sample(F3 a, F3 b, float eu, LineSample &L) {
 DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_component, float eu, const DrEdge &edge,
                                                const LightSample &ls, const SecondarySample *ss, int nss, F3 wi,
                                                const Isect &is, const SPoint &sp, F3 weight, F3 d_color,
-                                               bool direct_only, bool *defer = nullptr) {
+                                               bool direct_only, bool *defer = nullptr,
+                                               const float *geo = nullptr) {
     EdgeResult R = EdgeResult{F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
     if (!valid(is)) return R;
-    if (!is_silhouette(S, sp.pos, edge)) return R;
-    const DrShape &es = S.shapes[edge.shape];
-    F3 v0 = vert(S, es.vert_off + edge.v0), v1 = vert(S, es.vert_off + edge.v1);
+    F3 v0, v1;
+    if (geo) {
+        // an edge the tree selected at this point (a silhouette there), with
+        // its endpoints from the tree's geometry record
+        const DR_GLOBAL float4 *g = (const DR_GLOBAL float4 *)geo;
+        float4 A = g[0], B = g[1];
+        v0 = F3{A.x, A.y, A.z};
+        v1 = F3{B.x, B.y, B.z};
+    } else {
+        if (!is_silhouette(S, sp.pos, edge)) return R;
+        const DrShape &es = S.shapes[edge.shape];
+        v0 = vert(S, es.vert_off + edge.v0);
+        v1 = vert(S, es.vert_off + edge.v1);
+    }
     if (abs_sum(v1 - v0) == 1e-10f) return R;
     int mid = S.shapes[is.shape].mat;
     const DrMaterial &m = S.materials[mid];

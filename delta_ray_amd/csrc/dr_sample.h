@@ -414,7 +414,7 @@ DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, const Point
 template <int MAXB>
 DR_INLINE F3 stage_edge(const DrSceneView &S, const SampleCtx &c, GradSink &G, uint64_t sid, int b,
                         const PointRec &p, const DrEdge &edge, float w, int j, F3 d_color, float *mag = nullptr,
-                        bool *defer = nullptr) {
+                        bool *defer = nullptr, const float *geo = nullptr) {
     Philox prng;
     philox_init(prng, c.seed, 0, sid, dr_stream_point(b));
     float bcomp = philox_next(prng), eu = philox_next(prng);
@@ -427,7 +427,7 @@ DR_INLINE F3 stage_edge(const DrSceneView &S, const SampleCtx &c, GradSink &G, u
     F3 weight = p.thr * ((1.f / c.spp) * w);
     Isect is = Isect{p.shape, p.tri, p.gtri};
     EdgeResult er = sample_shading_edge_point(S, bcomp, eu, edge, els, ess, nss, p.wi, is, p.sp, weight, d_color,
-                                              c.nb == 1, defer);
+                                              c.nb == 1, defer, geo);
     if (defer && *defer) return F3{0.f, 0.f, 0.f};
     if (!is_zero(er.dv0) || !is_zero(er.dv1)) {
         const DrShape &es = S.shapes[edge.shape];

@@ -114,6 +114,7 @@ struct DrSceneView {
     const DR_GLOBAL DrEdge *nrm_edges;
     const DR_GLOBAL DrEdgeNode *nrm_nodes;
     const DR_GLOBAL float *nrm_geo;  // per normal-tree edge: v0, v1 and the faces' third vertices (float4 each)
+    const DR_GLOBAL float *sil_geo;  // per silhouette-tree edge: v0, v1 (same layout)
     int num_sil_nodes, num_nrm_nodes;
     int num_sil_edges;
     float *edge_stat;  // per tree edge (silhouette tree, then normal tree): sum of |contribution|, count, sum of squares

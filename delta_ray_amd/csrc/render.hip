@@ -292,8 +292,11 @@ DR_INLINE bool edge_entry(const DrJob *jobs, const EdgeEntry &E, long long e,
     F3 d_color = load3(J.d_image + 3 * (size_t)p.pix);
     float mag = 0.f;
     bool deferred = false;
+    // the edge was selected at this point, so it is a silhouette there: its
+    // endpoints come from the tree's geometry record
+    const float *geo = ((E.edge & 0x80000000) ? J.S.nrm_geo : J.S.sil_geo) + 16 * (size_t)idx;
     F3 dp = stage_edge<MAXB>(J.S, J.c, G, p.sid, p.b, p, edge, E.w, E.j, d_color, &mag,
-                             defer ? &deferred : nullptr);
+                             defer ? &deferred : nullptr, geo);
     if (deferred) return true;
     if ((J.c.flags & DR_EDGE_STATS) && J.S.edge_stat) {
         long long ge = (E.edge & 0x80000000) ? (long long)J.S.num_sil_edges + idx : (long long)idx;
@@ -359,7 +362,8 @@ __global__ void __launch_bounds__(DR_BLOCK) k_edge(const DrJob *__restrict__ job
 
 // C': the samples k_edge left for the continuation paths, in full.
 template <int MAXB>
-__global__ void __launch_bounds__(DR_BLOCK) k_edge_shade(const DrJob *__restrict__ jobs, const EdgeEntry *q,
+__global__ void __launch_bounds__(DR_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+k_edge_shade(const DrJob *__restrict__ jobs, const EdgeEntry *q,
                                                          const unsigned long long *cnt, const PointRec *pts,
                                                          float *pt_dp, int *pt_lastj, float4 *entry_dp,
                                                          const int *cont) {
@@ -1192,7 +1196,8 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
               upload(s, hs.light_cdf, &v.light_cdf) && upload(s, hs.light_sel.func, &v.lights_func) &&
               upload(s, hs.light_sel.cdf, &v.lights_cdf) && upload(s, hs.sil_edges, &v.sil_edges) &&
               upload(s, hs.sil_nodes, &v.sil_nodes) && upload(s, hs.nrm_edges, &v.nrm_edges) &&
-              upload(s, hs.nrm_nodes, &v.nrm_nodes) && upload(s, hs.nrm_geo, &v.nrm_geo);
+              upload(s, hs.nrm_nodes, &v.nrm_nodes) && upload(s, hs.nrm_geo, &v.nrm_geo) &&
+              upload(s, hs.sil_geo, &v.sil_geo);
     std::vector<float> ltc(dr_ltc_blob, dr_ltc_blob + 16384 * 9);
     if ((size_t)(dr_ltc_blob_end - dr_ltc_blob) != 16384 * 9) {
         free_scene(s);

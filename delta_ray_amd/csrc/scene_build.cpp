@@ -690,6 +690,14 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
         for (int k = 0; k < 4; k++)
             for (int c = 0; c < 3; c++) hs.nrm_geo[16 * i + 4 * k + c] = hs.verts[3 * (size_t)gv[k] + c];
     }
+    hs.sil_geo.assign(16 * hs.sil_edges.size(), 0.f);
+    for (size_t i = 0; i < hs.sil_edges.size(); i++) {
+        const DrEdge &e = hs.sil_edges[i];
+        const DrShape &sh = hs.shapes[e.shape];
+        const int gv[2] = {sh.vert_off + e.v0, sh.vert_off + e.v1};
+        for (int k = 0; k < 2; k++)
+            for (int c = 0; c < 3; c++) hs.sil_geo[16 * i + 4 * k + c] = hs.verts[3 * (size_t)gv[k] + c];
+    }
     return "";
 }
 

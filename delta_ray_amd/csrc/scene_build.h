@@ -32,8 +32,9 @@ struct HostScene {
     std::vector<DrEdge> sil_edges, nrm_edges;
     std::vector<DrEdgeNode> sil_nodes, nrm_nodes;
     // per normal-tree edge (tree order), 16 floats: its endpoints and the
-    // third vertices of its two faces, xyz + 1 pad each (the silhouette test)
-    std::vector<float> nrm_geo;
+    // third vertices of its two faces, xyz + 1 pad each (the silhouette test);
+    // per silhouette-tree edge the same layout with the endpoints only
+    std::vector<float> nrm_geo, sil_geo;
     std::vector<DrEdge> prim_edges;
     Dist prim;
     size_t grad_size = 0;
