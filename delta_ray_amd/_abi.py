@@ -158,6 +158,8 @@ ENTRY_POINTS = [
     ("dr_mlp_train", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
                                     ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
+    ("dr_test_tree_stack", ctypes.c_int, [ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.c_void_p]),
     ("dr_test_line_integral", ctypes.c_int, [ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
                                              ctypes.c_void_p]),
     ("dr_last_error", ctypes.c_char_p, []),

@@ -533,6 +533,45 @@ __global__ void k_intersect(DrSceneView S, const float *rays, long long n, int *
 // (0) uniform position on the segment, cosine lobe; (1) the LTC line sampler,
 // cosine lobe; (2) uniform, glossy; (3) the LTC line sampler, glossy.
 // out: per estimator the sum and the sum of squares (double).
+// dr_test_tree_stack: every lane runs `ops` random pushes and pops on the
+// selection kernel's LDS stack and on a plain array stack side by side;
+// out[0] counts pops that differ, out[1] the deepest stack reached.
+__global__ void __launch_bounds__(DR_BLOCK) k_tree_stack(long long ops, uint64_t seed, unsigned long long *out) {
+    __shared__ float s_stack[DR_LDS_STACK * 6 * DR_BLOCK];
+    LdsStack A;
+    A.lds = (DR_LDS float *)s_stack + threadIdx.x;
+    A.stride = DR_BLOCK;
+    A.clear();
+    PlainStack B;
+    B.clear();
+    Philox r;
+    philox_init(r, seed, 4, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, 0);
+    unsigned long long bad = 0;
+    int deepest = 0;
+    for (long long k = 0; k < ops; k++) {
+        bool alt = (k >> 6) & 1;
+        float u = philox_next(r);
+        bool push = B.size() == 0 || (B.size() < DR_TREE_STACK && u < 0.55f);
+        if (push) {
+            int x = (int)(philox_next(r) * 16777216.f);
+            TreeEntry e{x, philox_next(r), philox_next(r), philox_next(r), x ^ 0x5555, x + 7};
+            TreeAlt t{philox_next(r), philox_next(r)};
+            A.push(e, t, alt);
+            B.push(e, t, alt);
+            deepest = max(deepest, B.size());
+        } else {
+            TreeAlt ta{0.f, 0.f}, tb{0.f, 0.f};
+            TreeEntry a = A.pop(ta, alt), b = B.pop(tb, alt);
+            if (a.nm != b.nm || a.bound != b.bound || a.cone_pmf != b.cone_pmf || a.all_pmf != b.all_pmf ||
+                a.off != b.off || a.npin != b.npin || (alt && (ta.cone != tb.cone || ta.all != tb.all)))
+                bad++;
+        }
+        if (A.size() != B.size()) bad++;
+    }
+    if (bad) atomicAdd(&out[0], bad);
+    atomicMax(&out[1], (unsigned long long)deepest);
+}
+
 __global__ void __launch_bounds__(256) k_line_integral(long long n, uint64_t seed, double *out) {
     __shared__ double red[8][256];
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1506,6 +1545,24 @@ dr_status dr_intersect(dr_scene *s, const float *rays, int64_t n, int32_t *hits,
     hipStream_t hs = (hipStream_t)stream;
     k_intersect<<<(unsigned)((n + 255) / 256), 256, 0, hs>>>(s->view, rays, n, hits, any_hit);
     return launch_check("k_intersect");
+}
+
+dr_status dr_test_tree_stack(int64_t ops, uint64_t seed, int64_t *out, void *stream) {
+    if (ops <= 0 || !out) return fail(DR_ERROR_INVALID_ARGUMENT, "bad arguments");
+    hipStream_t hs = (hipStream_t)stream;
+    unsigned long long *dev = nullptr;
+    if (hipMalloc(&dev, 2 * sizeof(unsigned long long)) != hipSuccess)
+        return fail(DR_ERROR_OUT_OF_MEMORY, "hipMalloc");
+    k_fill<<<1, 64, 0, hs>>>((float *)dev, 4, 0.f);
+    k_tree_stack<<<64, DR_BLOCK, 0, hs>>>(ops, seed, dev);
+    unsigned long long r[2];
+    hipError_t e = hipMemcpyAsync(r, dev, sizeof(r), hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    hipFree(dev);
+    if (e != hipSuccess) return fail(DR_ERROR_DEVICE, hipGetErrorString(e));
+    out[0] = (int64_t)r[0];
+    out[1] = (int64_t)r[1];
+    return launch_check("k_tree_stack");
 }
 
 dr_status dr_test_line_integral(int64_t num_samples, uint64_t seed, double *out, void *stream) {

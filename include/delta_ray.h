@@ -406,6 +406,14 @@ dr_status dr_scene_edge_stats(dr_scene *scene, float *out, int reset, void *stre
  * stream. */
 dr_status dr_test_line_integral(int64_t num_samples, uint64_t seed, double *out, void *stream);
 
+/* Check of the edge-tree traversal stack the selection kernel keeps in LDS
+ * (top entries in LDS, older ones spilled to private memory): every GPU lane
+ * runs `ops` random pushes and pops on it and on a plain array stack.  out
+ * (host, 2 int64): pops that differ (0 expected), deepest stack reached.  No
+ * reference counterpart (a self-check of this implementation).  Synchronises
+ * the stream. */
+dr_status dr_test_tree_stack(int64_t ops, uint64_t seed, int64_t *out, void *stream);
+
 const char *dr_last_error(void);
 
 #ifdef __cplusplus
