@@ -64,6 +64,7 @@ def parse():
     ap.add_argument("--lv-renders", type=int, default=512, help="learned workload: renders per estimator")
     ap.add_argument("--lv-train-renders", type=int, default=16, help="learned workload: statistics renders")
     ap.add_argument("--lv-train-steps", type=int, default=300, help="learned workload: MLP training steps")
+    ap.add_argument("--lv-rounds", type=int, default=2, help="learned workload: rounds of statistics + training")
     ap.add_argument("--lv-eps", type=float, default=1e-2, help="learned workload: target relative RMSE")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known HBM traffic: "
@@ -212,8 +213,9 @@ def bench_learned(args):
     sees); the plain sample variance over all renders, dominated by the few
     largest renders, and tail percentiles of the per-render squared error are
     reported beside it.  The learned time includes its training:
-    DR_EDGE_STATS renders (statistics of the plain estimator), MLP training
-    and applying the weights.  Measured on the secondary-edge term, the one
+    `--lv-rounds` rounds of DR_EDGE_STATS renders (statistics of the plain
+    estimator, then of the learned mixture), MLP training and applying the
+    weights.  Measured on the secondary-edge term, the one
     the proposal changes, and on the full gradient (interior + secondary +
     primary edges)."""
     import numpy as np
@@ -263,11 +265,15 @@ def bench_learned(args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prop = EdgeProposal(scene, seed=7)
-    prop.reset_stats()
-    renders(secondary | _abi.DR_EDGE_STATS, range(50, 50 + args.lv_train_renders))
-    prop.collect()
-    loss = prop.train(steps=args.lv_train_steps, batch=1024)
-    wts = prop.apply()
+    for rnd in range(args.lv_rounds):
+        # round 0 gathers statistics of the plain estimator, later rounds of
+        # the learned mixture itself
+        prop.reset_stats()
+        flags = secondary | _abi.DR_EDGE_STATS | (_abi.DR_LEARNED_EDGES if rnd > 0 else 0)
+        renders(flags, range(50 + 100 * rnd, 50 + 100 * rnd + args.lv_train_renders))
+        prop.collect()
+        loss = prop.train(steps=args.lv_train_steps, batch=1024)
+        wts = prop.apply()
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     print("learned: trained in %.2f s" % train_s, file=sys.stderr, flush=True)
@@ -304,7 +310,8 @@ def bench_learned(args):
                 "config": {"workload": "heightfield %dx%d %dspp %d-bounce, %d independent renders per estimator"
                                        % (W, H, spp, nb, n), "parallelism": "single GPU"},
                 "extra": {"eps": eps, "training_s": round(train_s, 3),
-                          "training": {"stats_renders": args.lv_train_renders, "mlp_steps": args.lv_train_steps,
+                          "training": {"rounds": args.lv_rounds, "stats_renders_per_round": args.lv_train_renders,
+                                       "mlp_steps_per_round": args.lv_train_steps,
                                        "final_loss": round(float(loss), 5),
                                        "weight_p5_p50_p95": [round(float(x), 4)
                                                              for x in np.percentile(wts, [5, 50, 95])]},

@@ -56,6 +56,7 @@ class EdgeProposal:
         self.logits = torch.zeros(max(self.n, 1), dtype=torch.float32, device=dev)
         self.seed = seed
         self.trained_steps = 0
+        self.log_w = np.zeros(self.n)  # current log-weights (all 1: the plain proposal)
 
     def __del__(self):
         h = getattr(self, "mlp", None)
@@ -64,7 +65,10 @@ class EdgeProposal:
             self.mlp = None
 
     def reset_stats(self):
+        """Start a statistics window: clear the scene's counters and the
+        statistics gathered so far."""
         _abi.check(self.lib, self.lib.dr_scene_edge_stats(self.scene.handle, None, 1, None), "dr_scene_edge_stats")
+        self.acc.zero_()
 
     def collect(self):
         """Add the statistics of the DR_EDGE_STATS renders since the last
@@ -103,13 +107,17 @@ class EdgeProposal:
         return loss.value
 
     def apply(self, clip=(0.05, 20.0), prior_count=16.0, temper=0.5):
-        """Network weights -> the scene's learned node lengths.  Per edge the
-        log-weight is (n * y + k * prediction) / (n + k): its own target y
-        from n samples, shrunk toward the network's prediction with k =
+        """Network + statistics -> the scene's learned node lengths.  Per edge
+        the step is (n * y + k * prediction) / (n + k): its own target y from
+        n samples, shrunk toward the network's prediction with k =
         `prior_count` pseudo-samples; centred (weights normalised by their
-        geometric mean), multiplied by `temper` (weights = RMS ** temper: the
-        statistic is noisy and the best scale factor differs from point to
-        point, so the step toward it is damped) and clipped to `clip`."""
+        geometric mean) and multiplied by `temper` (the statistic is noisy and
+        the best scale factor differs from point to point, so the step toward
+        it is damped).  The step is added to the current log-weights, so that
+        rounds of statistics gathered under the learned proposal itself
+        (DR_LEARNED_EDGES | DR_EDGE_STATS renders) refine the weights: each
+        round's RMS measures what the current proposal still under- or
+        over-samples.  Weights are clipped to `clip`."""
         _abi.check(self.lib, self.lib.dr_mlp_infer(self.mlp, _ptr(self.features), self.n, _ptr(self.logits), None),
                    "dr_mlp_infer")
         self.torch.cuda.synchronize()
@@ -118,13 +126,14 @@ class EdgeProposal:
         y = y[:self.n].double().cpu().numpy()
         cnt = self.acc[:self.n, 1].cpu().numpy()
         lg = (cnt * y + prior_count * pred) / (cnt + prior_count)
-        lg = temper * (lg - lg.mean())
-        wts = np.clip(np.exp(lg), clip[0], clip[1]).astype(np.float32)
+        self.log_w = np.clip(self.log_w + temper * (lg - lg.mean()), np.log(clip[0]), np.log(clip[1]))
+        wts = np.exp(self.log_w).astype(np.float32)
         _abi.check(self.lib, self.lib.dr_scene_set_edge_weights(self.scene.handle, wts.ctypes.data),
                    "dr_scene_set_edge_weights")
         return wts
 
     def set_weights(self, wts):
         w = np.ascontiguousarray(wts, np.float32)
+        self.log_w = np.log(np.maximum(w.astype(np.float64), 1e-30))
         _abi.check(self.lib, self.lib.dr_scene_set_edge_weights(self.scene.handle, w.ctypes.data),
                    "dr_scene_set_edge_weights")

@@ -112,3 +112,38 @@ def test_uniform_weights_give_the_plain_estimate(gpu):
     assert np.abs(plain).sum() > 0
     for a, b in zip(plain, learned):
         assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(a), np.linalg.norm(a - b) / np.linalg.norm(a)
+
+
+def test_trained_proposal_reduces_variance(gpu):
+    """What the trained network gains over the method without it: on the
+    heightfield (grid 128, 32 x 32 pixels, 4 spp), two rounds of statistics
+    and training, then 128 renders of each estimator.  The secondary-edge
+    estimator is heavy-tailed, so the variance is the median over 16 groups
+    of the groups' variances; the learned one is clearly lower (1.5x here,
+    2.4x at 128 x 128 x 16 spp in bench.py), and the means agree."""
+    import torch
+    from delta_ray_amd import workloads
+    w = workloads.heightfield(resolution=(32, 32), num_samples=4)
+    scene = helpers.gpu_scene(w)
+    W, H = w["resolution"]
+    d = torch.from_numpy(np.random.default_rng(6).normal(size=(H, W, 3)).astype(np.float32)).to(gpu)
+    prop = EdgeProposal(scene, seed=7)
+    for rnd in range(2):
+        prop.reset_stats()
+        flags = SECONDARY | _abi.DR_EDGE_STATS | (_abi.DR_LEARNED_EDGES if rnd else 0)
+        _grads(torch, scene, w, 4, range(50 + 100 * rnd, 66 + 100 * rnd), flags, d)
+        prop.collect()
+        prop.train(steps=300, batch=1024)
+        prop.apply()
+    plain = _grads(torch, scene, w, 4, range(1000, 1128), SECONDARY, d)
+    learned = _grads(torch, scene, w, 4, range(5000, 5128), SECONDARY | _abi.DR_LEARNED_EDGES, d)
+
+    def mom_var(a, groups=16):
+        m = len(a) // groups
+        return float(np.median([((a[i * m:(i + 1) * m] - a[i * m:(i + 1) * m].mean(0)) ** 2).sum(1).sum() / (m - 1)
+                                for i in range(groups)]))
+    ratio = mom_var(plain) / mom_var(learned)
+    print("variance ratio plain / learned", ratio)
+    assert ratio > 1.3, ratio
+    se = np.sqrt(plain.var(0, ddof=1) / len(plain) + learned.var(0, ddof=1) / len(learned))
+    assert np.linalg.norm(plain.mean(0) - learned.mean(0)) < 4.0 * np.linalg.norm(se) + 1e-6
