@@ -368,6 +368,18 @@ def main():
     samples = spp * W * H
     value = samples / (ms_per_step / 1000.0)
 
+    # forward only (the reference's render() with a 0-d d_image), same
+    # workload, after the timed adjoint steps
+    fo = dr.DeviceScene.options(W, H, spp, nb, 4242)
+    scene.render(fo, image, stream)
+    torch.cuda.synchronize()
+    tf0 = time.perf_counter()
+    for i in range(args.steps):
+        fo.seed = 5000 + i
+        scene.render(fo, image, stream)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - tf0) * 1000.0 / args.steps
+
     # per-kernel counters from untimed runs: path only, path + secondary
     # edges, path + primary edges; differences attribute rays to kernels
     def counters(extra_flags):
@@ -425,7 +437,9 @@ def main():
                       "samples_per_step": samples, "primary_edge_samples_per_step": samples,
                       "parallelism": "single GPU"},
            "roofline": roofline,
-           "extra": {"rays_per_s": round(rays_total / (ms_per_step / 1000.0), 1),
+           "extra": {"forward_samples_per_s": round(samples / (fwd_ms / 1000.0), 1),
+                     "forward_ms_per_step": round(fwd_ms, 3),
+                     "rays_per_s": round(rays_total / (ms_per_step / 1000.0), 1),
                      "rays_per_step": int(rays_total),
                      "secondary_edges_per_step": int(cb["secondary_edges"]),
                      "edge_tree_nodes_per_point": round(cb["edge_nodes_visited"] / max(n_points, 1), 2),
