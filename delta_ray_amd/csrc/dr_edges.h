@@ -139,8 +139,12 @@ DR_INLINE float max_sq_bound(float lo, float hi) { return fmaxf(lo * lo, hi * hi
 // The BSDF bound of edge_tree.cpp:345-480, split in two: the parts that
 // depend only on the shading point (the frame around wi, the normal in it,
 // the Phong exponent) are computed once per point in bound_ctx(); the per-node
-// part is bsdf_bound().  Same operations in the same order as one function,
-// so the bound is bitwise the same.
+// part is bsdf_bound().  The per-node part uses the hardware square root,
+// reciprocal and exp2/log2 (about 1 ulp): the bound only sets node
+// importances, and the same value serves for drawing and for weighting, so
+// the estimator stays unbiased; against the CPU restatement's exact
+// arithmetic the pmfs differ in the last bits (a split decision flips only
+// when its uniform draw falls within them).
 struct BoundCtx {
     Frame hf;
     float lx, lz, lx2, lz2;  // normal in hf, and its squares
@@ -188,29 +192,29 @@ DR_INLINE float bsdf_bound(const DrEdgeNode &node, const SPoint &sp, const Bound
         float cphm = 0.f;
         float hx = bhi[0];
         if (hx > 0.f) {
-            cphm = hx / sqrtf(min_sq_bound(blo[1], bhi[1]) + hx * hx);
+            cphm = hx * approx_rsqrt(min_sq_bound(blo[1], bhi[1]) + hx * hx);
         } else if (hx == 0.f) {
             cphm = fmaxf(cphm, 0.f);
         } else {
-            cphm = hx / sqrtf(max_sq_bound(blo[1], bhi[1]) + hx * hx);
+            cphm = hx * approx_rsqrt(max_sq_bound(blo[1], bhi[1]) + hx * hx);
         }
         float cto_min, cto_max;
         if (blo[2] > 0.f)
-            cto_min = blo[2] / (max_sq_bound(blo[0], bhi[0]) + max_sq_bound(blo[1], bhi[1]) + blo[2] * blo[2]);
+            cto_min = blo[2] * approx_rcp(max_sq_bound(blo[0], bhi[0]) + max_sq_bound(blo[1], bhi[1]) + blo[2] * blo[2]);
         else
-            cto_min = blo[2] / (min_sq_bound(blo[0], bhi[0]) + min_sq_bound(blo[1], bhi[1]) + blo[2] * blo[2]);
+            cto_min = blo[2] * approx_rcp(min_sq_bound(blo[0], bhi[0]) + min_sq_bound(blo[1], bhi[1]) + blo[2] * blo[2]);
         if (bhi[2] > 0.f)
-            cto_max = bhi[2] / (min_sq_bound(blo[0], bhi[0]) + min_sq_bound(blo[1], bhi[1]) + bhi[2] * bhi[2]);
+            cto_max = bhi[2] * approx_rcp(min_sq_bound(blo[0], bhi[0]) + min_sq_bound(blo[1], bhi[1]) + bhi[2] * bhi[2]);
         else
-            cto_max = bhi[2] / (max_sq_bound(blo[0], bhi[0]) + max_sq_bound(blo[1], bhi[1]) + bhi[2] * bhi[2]);
-        float cth_min = sqrtf((cto_min + 1.f) / 2.f);
-        float cth_max = sqrtf((cto_max + 1.f) / 2.f);
-        float ch = cphm >= 0.f ? sqrtf(B.lz2 / (B.lz2 + B.lx2 * cphm * cphm)) : 1.f;
+            cto_max = bhi[2] * approx_rcp(max_sq_bound(blo[0], bhi[0]) + max_sq_bound(blo[1], bhi[1]) + bhi[2] * bhi[2]);
+        float cth_min = approx_sqrt((cto_min + 1.f) * 0.5f);
+        float cth_max = approx_sqrt((cto_max + 1.f) * 0.5f);
+        float ch = cphm >= 0.f ? approx_sqrt(B.lz2 * approx_rcp(B.lz2 + B.lx2 * cphm * cphm)) : 1.f;
         if (ch > cth_max) ch = cth_max;
         else if (ch < cth_min) ch = cth_min;
-        float sh = sqrtf(fmaxf(1.f - ch * ch, 0.f));
+        float sh = approx_sqrt(fmaxf(1.f - ch * ch, 0.f));
         float hn = fminf(fabsf(B.lx * sh * cphm) + B.lz * ch, 1.f);
-        phong_resp = powf(hn, B.e) * B.e2 / (2.f * DR_PI);
+        phong_resp = approx_pow01(hn, B.e) * B.e2 * (1.f / (2.f * DR_PI));
     }
     return cos_max * (B.dterm + B.slum * phong_resp);
 }
@@ -657,15 +661,16 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
         return R;
     F3 eb = bsdf(S, mid, sp, wi, sdir);
     if (sum3(eb) < 1e-6f) return R;
+    // with `defer`, a sample that passed the connectivity test stops here
+    // and is reported: the caller runs it again in full in a launch of such
+    // samples only (about one in five), so the evaluation and continuation
+    // code runs on full waves
+    if (defer) {
+        *defer = true;
+        return R;
+    }
     F3 shr = F3{0.f, 0.f, 0.f}, shl = F3{0.f, 0.f, 0.f};
     if (!direct_only) {
-        // a sample that goes on to the continuation paths: with `defer`, stop
-        // here and report it (the caller runs it again in full, in a launch of
-        // such samples only, so the long continuation code runs on full waves)
-        if (defer) {
-            *defer = true;
-            return R;
-        }
         shr = shade(S, -sdir, ir, spr, ls, ss, nss, false);
         shl = shade(S, -sdir, il, spl, ls, ss, nss, false);
         if (valid(ir)) {

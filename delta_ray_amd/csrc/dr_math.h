@@ -75,6 +75,22 @@ DR_INLINE float max3(F3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
 DR_INLINE bool is_zero(F3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
 DR_INLINE float luminance(F3 a) { return 0.212671f * a.x + 0.715160f * a.y + 0.072169f * a.z; }
 DR_INLINE F3 lum_w() { return F3{0.212671f, 0.715160f, 0.072169f}; }
+// Hardware approximations (about 1 ulp) for sampling bounds, where only
+// consistency between sampling and weighting matters, not the last bit:
+// v_sqrt_f32, v_rsq_f32, v_rcp_f32, and pow as v_exp_f32(e * v_log_f32(x))
+// for x in [0, 1].  The host build (CPU checks) uses the exact functions.
+#if defined(__HIP_DEVICE_COMPILE__)
+DR_INLINE float approx_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+DR_INLINE float approx_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+DR_INLINE float approx_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+DR_INLINE float approx_pow01(float x, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x)); }
+#else
+DR_INLINE float approx_sqrt(float x) { return sqrtf(x); }
+DR_INLINE float approx_rsqrt(float x) { return 1.f / sqrtf(x); }
+DR_INLINE float approx_rcp(float x) { return 1.f / x; }
+DR_INLINE float approx_pow01(float x, float e) { return powf(x, e); }
+#endif
+
 // three floats from device (global) memory
 DR_INLINE F3 load3(const float *p) {
     const DR_GLOBAL float *q = gptr(p);

@@ -264,14 +264,14 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
 
 // C: one selected edge per iteration of a grid-stride loop over the queue
 // (its length is read on the device, so no host round trip is needed).  Only
-// one edge sample in seven or so goes on to the continuation paths (the rest
-// stop at the horizon, at the connectivity test of the two side rays or at a
-// zero BSDF), and those paths are the longest code of the kernel: run in the
-// same waves they would leave most lanes idle.  So k_edge stops such samples
-// where the continuation starts (returns true: nothing of theirs is written)
-// and appends them to a list; k_edge_shade then runs them in full, on full
-// waves.  Every sample is thus computed by exactly one launch, with the same
-// operations as in one pass.
+// about one edge sample in five passes the connectivity test of its two side
+// rays (the rest stop at the horizon, at that test or at a zero BSDF), and
+// what follows (evaluation at both hits, continuation paths, the Dirac
+// Jacobians, the scatter) is the longest code of the kernel: run in the same
+// waves it would leave most lanes idle.  So k_edge stops such samples there
+// (returns true: nothing of theirs is written) and appends them to a list;
+// k_edge_shade then runs them in full, on full waves.  Every sample is thus
+// computed by exactly one launch, with the same operations as in one pass.
 template <int MAXB>
 DR_INLINE bool edge_entry(const DrJob *jobs, const EdgeEntry &E, long long e,
                           const PointRec *pts, float *pt_dp, int *pt_lastj, float4 *entry_dp,
