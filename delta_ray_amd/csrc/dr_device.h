@@ -189,6 +189,105 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
     return best;
 }
 
+// Closest hits of two rays from one origin in one walk of the BVH (the two
+// side rays of an edge sample are a few 1e-3 rad apart and visit nearly the
+// same nodes, so one node fetch serves both).  A node is entered when either
+// ray's box test passes; each ray keeps its own closest hit and makes its
+// own box and triangle tests with trace_closest's arithmetic and tie rule.
+// Culling with a ray's current closest distance is conservative and the
+// closest hit does not depend on the visiting order, so each result is the
+// one trace_closest returns for that ray.
+DR_INLINE void trace_closest2(const DrSceneView &S, F3 org, F3 da, F3 db, float tmin, float tmax, Isect &ha,
+                              float &ua, float &va, Isect &hb, float &ub, float &vb) {
+    ha = no_hit();
+    hb = no_hit();
+    if (S.num_nodes == 0) return;
+    F3 ia = F3{1.f / da.x, 1.f / da.y, 1.f / da.z};
+    F3 ib = F3{1.f / db.x, 1.f / db.y, 1.f / db.z};
+    float ta_best = tmax, tb_best = tmax;
+    int ga = -1, gb = -1;
+    float au = 0.f, av = 0.f, bu = 0.f, bv = 0.f;
+    TraceStack stack;
+    stack.init();
+    int sp = 0;
+    int node = 0;
+    uint32_t nn = 0, nt = 0;
+    const uint32_t cap = 2u * (uint32_t)S.num_nodes + 8u;
+    const DR_GLOBAL DrNode *nodes = gptr(S.nodes);
+    auto test = [&](int first, int count, bool ta_on, bool tb_on) {
+        for (int i = 0; i < count; i++) {
+            const DrTri T = gptr(S.tris)[first + i];
+            int g = __float_as_int(T.v0[3]);
+            float u, v, t;
+            nt++;
+            if (ta_on && tri_hit(T, org, da, tmin, tmax, u, v, t) &&
+                (ga < 0 || t < ta_best || (t == ta_best && g < ga))) {
+                ta_best = t; ga = g; au = u; av = v;
+            }
+            if (tb_on && tri_hit(T, org, db, tmin, tmax, u, v, t) &&
+                (gb < 0 || t < tb_best || (t == tb_best && g < gb))) {
+                tb_best = t; gb = g; bu = u; bv = v;
+            }
+        }
+    };
+    while (nn < cap) {
+        const DR_GLOBAL DrNode &N = nodes[node];
+        float4 A = *(const DR_GLOBAL float4 *)N.a, B = *(const DR_GLOBAL float4 *)N.b,
+               C = *(const DR_GLOBAL float4 *)N.c;
+        int4 D = *(const DR_GLOBAL int4 *)N.d;
+        nn++;
+        float a0, a1, b0, b1;
+        bool ha0 = box_hit(A.x, A.y, A.z, A.w, B.x, B.y, org, ia, tmin, ta_best, a0);
+        bool ha1 = box_hit(B.z, B.w, C.x, C.y, C.z, C.w, org, ia, tmin, ta_best, a1);
+        bool hb0 = box_hit(A.x, A.y, A.z, A.w, B.x, B.y, org, ib, tmin, tb_best, b0);
+        bool hb1 = box_hit(B.z, B.w, C.x, C.y, C.z, C.w, org, ib, tmin, tb_best, b1);
+        bool h0 = ha0 || hb0, h1 = ha1 || hb1;
+        // leaves first
+        if (h0 && D.z > 0) {
+            test(D.x, D.z, ha0, hb0);
+            h0 = false;
+        }
+        if (h1 && D.w > 0) {
+            test(D.y, D.w, ha1, hb1);
+            h1 = false;
+        }
+        if (h0 && h1) {
+            // nearer child first, by the first ray that enters both
+            float t0 = ha0 && ha1 ? a0 : b0, t1 = ha0 && ha1 ? a1 : b1;
+            int near = D.x, far = D.y;
+            if (t1 < t0) { near = D.y; far = D.x; }
+            if (sp < DR_STACK) stack.put(sp++, far);
+            node = near;
+        } else if (h0) {
+            node = D.x;
+        } else if (h1) {
+            node = D.y;
+        } else {
+            if (sp == 0) break;
+            node = stack.get(--sp);
+        }
+    }
+    if (S.stats) {
+        atomicAdd(&S.stats[0], 2ull);
+        atomicAdd(&S.stats[1], (unsigned long long)nn);
+        atomicAdd(&S.stats[2], (unsigned long long)nt);
+    }
+    if (ga >= 0) {
+        ha.gtri = ga;
+        ha.shape = S.tri_shape[ga];
+        ha.tri = ga - S.shapes[ha.shape].tri_off;
+        ua = au;
+        va = av;
+    }
+    if (gb >= 0) {
+        hb.gtri = gb;
+        hb.shape = S.tri_shape[gb];
+        hb.tri = gb - S.shapes[hb.shape].tri_off;
+        ub = bu;
+        vb = bv;
+    }
+}
+
 DR_INLINE bool trace_any(const DrSceneView &S, F3 org, F3 dir, float tmin, float tmax) {
     if (S.num_nodes == 0) return false;
     F3 inv = F3{1.f / dir.x, 1.f / dir.y, 1.f / dir.z};
@@ -334,6 +433,14 @@ DR_INLINE Isect intersect(const DrSceneView &S, F3 org, F3 dir, SPoint &sp, bool
     Isect is = trace_closest(S, org, dir, 1e-3f, DR_INF, u, v);
     if (valid(is)) sp = surface_point(load_tri(S, is), org, dir, rcp);
     return is;
+}
+// intersect() of two rays from one origin, in one BVH walk
+DR_INLINE void intersect2(const DrSceneView &S, F3 org, F3 da, F3 db, Isect &ia, SPoint &spa, Isect &ib,
+                          SPoint &spb) {
+    float ua, va, ub, vb;
+    trace_closest2(S, org, da, db, 1e-3f, DR_INF, ia, ua, va, ib, ub, vb);
+    if (valid(ia)) spa = surface_point(load_tri(S, ia), org, da, false);
+    if (valid(ib)) spb = surface_point(load_tri(S, ib), org, db, false);
 }
 
 // --------------------------------------------------------- textures -------

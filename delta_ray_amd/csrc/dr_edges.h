@@ -653,8 +653,8 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
     F3 rdir = normalize(sdir + delta * spn);
     F3 ldir = normalize(sdir - delta * spn);
     SPoint spr, spl;
-    Isect ir = intersect(S, sp.pos, rdir, spr);
-    Isect il = intersect(S, sp.pos, ldir, spl);
+    Isect ir, il;
+    intersect2(S, sp.pos, rdir, ldir, ir, spr, il, spl);
     bool rc = ir.shape == edge.shape && (ir.tri == edge.f0 || ir.tri == edge.f1);
     bool lc = il.shape == edge.shape && (il.tri == edge.f0 || il.tri == edge.f1);
     if (rc == lc || (ir.shape == is.shape && ir.tri == is.tri) || (il.shape == is.shape && il.tri == is.tri))
