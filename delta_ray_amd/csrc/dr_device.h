@@ -44,25 +44,34 @@ DR_INLINE bool tri_hit(const DrTri &T, F3 org, F3 dir, float tmin, float tmax, f
 }
 
 // Slab test widened by a relative 1e-5 (+1e-7) so that box rounding can
-// never hide a triangle the exact test would accept.
+// never hide a triangle the exact test would accept.  An axis whose slab
+// distances are NaN (0 * inf: the ray lies in a slab plane), or whose entry
+// distance is +inf (exit -inf), does not constrain the interval.  The
+// widening is monotone, so the entry and exit are widened once, after the
+// max / min over the axes, which is the same as widening every axis.
 DR_INLINE bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz, F3 org, F3 inv, float tmin,
                        float tmax, float &tenter) {
     tenter = tmax;
     // an empty slot (lo = +inf, hi = -inf) never hits
     if (!(lx <= hx)) return false;
-    float t0 = tmin, t1 = tmax;
-    float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        float tn = (lo[a] - org[a]) * inv[a];
-        float tf = (hi[a] - org[a]) * inv[a];
-        float mn = fminf(tn, tf), mx = fmaxf(tn, tf);
-        if (tn != tn || tf != tf) continue;
-        mn = mn - fabsf(mn) * 1e-5f - 1e-7f;
-        mx = mx + fabsf(mx) * 1e-5f + 1e-7f;
-        t0 = mn > t0 ? mn : t0;
-        t1 = mx < t1 ? mx : t1;
-    }
+    float tnx = (lx - org.x) * inv.x, tfx = (hx - org.x) * inv.x;
+    float tny = (ly - org.y) * inv.y, tfy = (hy - org.y) * inv.y;
+    float tnz = (lz - org.z) * inv.z, tfz = (hz - org.z) * inv.z;
+    bool bx = tnx != tnx || tfx != tfx, by = tny != tny || tfy != tfy, bz = tnz != tnz || tfz != tfz;
+    float mnx = fminf(tnx, tfx), mxx = fmaxf(tnx, tfx);
+    float mny = fminf(tny, tfy), mxy = fmaxf(tny, tfy);
+    float mnz = fminf(tnz, tfz), mxz = fmaxf(tnz, tfz);
+    mnx = (bx || mnx == DR_INF) ? -DR_INF : mnx;
+    mxx = (bx || mxx == -DR_INF) ? DR_INF : mxx;
+    mny = (by || mny == DR_INF) ? -DR_INF : mny;
+    mxy = (by || mxy == -DR_INF) ? DR_INF : mxy;
+    mnz = (bz || mnz == DR_INF) ? -DR_INF : mnz;
+    mxz = (bz || mxz == -DR_INF) ? DR_INF : mxz;
+    float mn = fmaxf(fmaxf(mnx, mny), mnz), mx = fminf(fminf(mxx, mxy), mxz);
+    mn = mn - fabsf(mn) * 1e-5f - 1e-7f;
+    mx = mx + fabsf(mx) * 1e-5f + 1e-7f;
+    float t0 = mn > tmin ? mn : tmin;
+    float t1 = mx < tmax ? mx : tmax;
     tenter = t0;
     return t0 <= t1;
 }
