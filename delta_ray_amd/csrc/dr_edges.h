@@ -476,8 +476,12 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         }
     }
     if (il + ir <= 1e-10f) return true;
-    bool lhc = aabb_cone_intersect(L.lo, L.hi, T.cone);
-    bool rhc = aabb_cone_intersect(R.lo, R.hi, T.cone);
+    // the cone tests only matter while the cone pass can still reach the
+    // node's subtree (its own mask bit or a nonzero pmf of either proposal);
+    // otherwise every quantity they feed is multiplied by zero
+    bool need_cone = (mask & 1) || e.cone_pmf != 0.f || (T.mode != 0 && ea.cone != 0.f);
+    bool lhc = need_cone && aabb_cone_intersect(L.lo, L.hi, T.cone);
+    bool rhc = need_cone && aabb_cone_intersect(R.lo, R.hi, T.cone);
     bool cone_split = all_split || (lhc && rhc && lcs && rcs && T.cone.cos_angle > 0.95f);
     float ilc = lhc ? il : 0.f, irc = rhc ? ir : 0.f;
     float plc = 0.f, prc = 0.f, pla, pra;
