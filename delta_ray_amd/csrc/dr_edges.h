@@ -269,11 +269,26 @@ DR_INLINE float node_importance_len(const DrEdgeNode &n, float bound, F3 p, floa
 // records, loaded side by side.  Two stacks: PlainStack, an array (the CPU
 // restatement, tests), and LdsStack (the selection kernel): the top entries
 // live in LDS and older ones spill to private memory in halves.
+//
+// An entry is 16 bytes.  What popping needs of the node's bound is two flags,
+// kept in the sign bits of the two pmfs (a pmf is never negative):
+//   code  (interior: node index; leaf: -1 - edge count) << 2 | pass mask
+//         (bit 0: the cone pass reaches the node, bit 1: the "all" pass)
+//   cpmf  the cone pass's pmf of the path to the node; sign bit: the "all"
+//         pass takes both children (bound > 1, or the point lies in the box)
+//   apmf  the "all" pass's pmf; sign bit: the bound is <= 0 (nothing to take)
+//   off   the node's offset (interior: right child; leaf: first edge)
 struct TreeEntry {
-    int nm;  // node << 2 | pass mask (bit 0: the cone pass reaches the node, bit 1: the "all" pass)
-    float bound, cone_pmf, all_pmf;
-    int off;   // the node's offset (interior: right child; leaf: first edge)
-    int npin;  // the node's edge count << 1 | the point lies inside its box
+    int code;
+    uint32_t cpmf, apmf;
+    int off;
+    DR_INLINE int mask() const { return code & 3; }
+    DR_INLINE int node() const { return code >> 2; }
+    DR_INLINE int nprims() const { return code < 0 ? -1 - (code >> 2) : 0; }
+    DR_INLINE float cone_pmf() const { return __uint_as_float(cpmf & 0x7fffffffu); }
+    DR_INLINE float all_pmf() const { return __uint_as_float(apmf & 0x7fffffffu); }
+    DR_INLINE bool split_all() const { return (cpmf >> 31) != 0; }
+    DR_INLINE bool empty() const { return (apmf >> 31) != 0; }
 };
 
 // The same path's pmfs under the other proposal (learned mixture only),
@@ -299,7 +314,8 @@ struct PlainStack {
     }
 };
 
-#define DR_LDS_STACK 8  // entries per lane in LDS (a multiple of 2)
+#define DR_LDS_STACK 12  // entries per lane in LDS (a multiple of 2)
+#define DR_LDS_WORDS 4   // words per entry
 
 // Top DR_LDS_STACK entries of each lane in LDS, one word of an entry per
 // row of the block's lanes (conflict-free: lane l always uses bank l % 64);
@@ -314,18 +330,16 @@ struct LdsStack {
     DR_INLINE void clear() { sp = lo = 0; }
     DR_INLINE int size() const { return sp; }
     DR_INLINE void put(int slot, const TreeEntry &x) {
-        DR_LDS float *w = lds + slot * 6 * stride;
-        w[0] = __int_as_float(x.nm);
-        w[stride] = x.bound;
-        w[2 * stride] = x.cone_pmf;
-        w[3 * stride] = x.all_pmf;
-        w[4 * stride] = __int_as_float(x.off);
-        w[5 * stride] = __int_as_float(x.npin);
+        DR_LDS float *w = lds + slot * DR_LDS_WORDS * stride;
+        w[0] = __int_as_float(x.code);
+        w[stride] = __uint_as_float(x.cpmf);
+        w[2 * stride] = __uint_as_float(x.apmf);
+        w[3 * stride] = __int_as_float(x.off);
     }
     DR_INLINE TreeEntry get(int slot) const {
-        const DR_LDS float *w = lds + slot * 6 * stride;
-        return TreeEntry{__float_as_int(w[0]), w[stride], w[2 * stride], w[3 * stride], __float_as_int(w[4 * stride]),
-                         __float_as_int(w[5 * stride])};
+        const DR_LDS float *w = lds + slot * DR_LDS_WORDS * stride;
+        return TreeEntry{__float_as_int(w[0]), __float_as_uint(w[stride]), __float_as_uint(w[2 * stride]),
+                         __float_as_int(w[3 * stride])};
     }
     DR_INLINE void push(const TreeEntry &x, TreeAlt a, bool with_alt) {
         if (sp - lo == DR_LDS_STACK) {
@@ -364,7 +378,16 @@ struct EdgeTraversal {
     uint32_t n_inner, n_edges;
 };
 
-DR_INLINE int node_npin(const DrEdgeNode &n, F3 p) { return n.nprims << 1 | (node_inside(n, p) ? 1 : 0); }
+// The entry of node `node` (record n, bound computed by its parent).  Edge
+// counts and node indices below 2^29 fit the code (checked by the builder).
+DR_INLINE TreeEntry tree_entry(int node, int mask, const DrEdgeNode &n, float bound, float cone_pmf, float all_pmf,
+                               F3 p) {
+    int v = n.nprims > 0 ? -1 - n.nprims : node;
+    bool split = bound > 1.f || node_inside(n, p);
+    return TreeEntry{(int)((uint32_t)v << 2 | (uint32_t)mask),
+                     (__float_as_uint(cone_pmf) & 0x7fffffffu) | (split ? 0x80000000u : 0u),
+                     (__float_as_uint(all_pmf) & 0x7fffffffu) | (bound <= 0.f ? 0x80000000u : 0u), n.offset};
+}
 
 // edge_tree.cpp:685-725: choose the tree (one draw from rc when both exist)
 // and push the root for both passes.  Returns false when there is nothing
@@ -403,7 +426,7 @@ DR_INLINE bool et_init(EdgeTraversal &T, Stack &stack, const DrSceneView &S, F3 
     T.pops = 0;
     T.mode = mode;
     stack.clear();
-    stack.push(TreeEntry{3, root_bound, pmf, pmf, root.offset, node_npin(root, sp.pos)}, TreeAlt{pmf, pmf}, mode != 0);
+    stack.push(tree_entry(0, 3, root, root_bound, pmf, pmf, sp.pos), TreeAlt{pmf, pmf}, mode != 0);
     return true;
 }
 
@@ -423,9 +446,10 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
     T.pops++;
     TreeAlt ea{0.f, 0.f};
     TreeEntry e = stack.pop(ea, T.mode != 0);
-    if (e.bound <= 0.f) return true;
-    const int ni = e.nm >> 2, mask = e.nm & 3;
-    const int nprims = e.npin >> 1;
+    if (e.empty()) return true;
+    const int ni = e.node(), mask = e.mask();
+    const int nprims = e.nprims();
+    const float cone_pmf = e.cone_pmf(), all_pmf = e.all_pmf();
     if (nprims > 0) {
         // power heuristic over the two passes, which both run
         // (edge_tree.cpp:528-536): pass t selected the edge with pmf p_t and
@@ -435,25 +459,25 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         // + p_all of each proposal (balance heuristic over the proposals: the
         // defensive mixture, unbiased for any q, and exactly the plain
         // estimator when both proposals agree).
-        float den = e.cone_pmf * e.cone_pmf + e.all_pmf * e.all_pmf;
+        float den = cone_pmf * cone_pmf + all_pmf * all_pmf;
         float scale = 1.f;
         if (T.mode != 0) {
             float pi = T.mode == 1 ? DR_LEARNED_SHARE : 1.f - DR_LEARNED_SHARE;
-            float qs = e.cone_pmf + e.all_pmf, qo = ea.cone + ea.all;
+            float qs = cone_pmf + all_pmf, qo = ea.cone + ea.all;
             scale = qs / (pi * qs + (1.f - pi) * qo);
         }
         T.n_edges += nprims;
         for (int i = 0; i < nprims; i++) {
             if (T.directional && !is_silhouette_nrm(S, sp.pos, e.off + i)) continue;
-            if (mask & 1) emit(scale * e.cone_pmf / den, T.directional, e.off + i, T.jc++);
-            if (mask & 2) emit(scale * e.all_pmf / den, T.directional, e.off + i, 0x8000 | T.ja++);
+            if (mask & 1) emit(scale * cone_pmf / den, T.directional, e.off + i, T.jc++);
+            if (mask & 2) emit(scale * all_pmf / den, T.directional, e.off + i, 0x8000 | T.ja++);
         }
         return true;
     }
     int li = ni + 1, ri = e.off;
     const DrEdgeNode L = gptr(T.nodes)[li], R = gptr(T.nodes)[ri];
     T.n_inner++;
-    bool all_split = e.bound > 1.f || (e.npin & 1);
+    bool all_split = e.split_all();
     bool lcs = true, rcs = true;
     if (T.directional) {
         lcs = contains_silhouette(L, sp.pos);
@@ -479,7 +503,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
     // the cone tests only matter while the cone pass can still reach the
     // node's subtree (its own mask bit or a nonzero pmf of either proposal);
     // otherwise every quantity they feed is multiplied by zero
-    bool need_cone = (mask & 1) || e.cone_pmf != 0.f || (T.mode != 0 && ea.cone != 0.f);
+    bool need_cone = (mask & 1) || cone_pmf != 0.f || (T.mode != 0 && ea.cone != 0.f);
     bool lhc = need_cone && aabb_cone_intersect(L.lo, L.hi, T.cone);
     bool rhc = need_cone && aabb_cone_intersect(R.lo, R.hi, T.cone);
     bool cone_split = all_split || (lhc && rhc && lcs && rcs && T.cone.cos_angle > 0.95f);
@@ -536,10 +560,10 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         }
     }
     if (rm && stack.size() < DR_TREE_STACK)
-        stack.push(TreeEntry{ri << 2 | rm, rb, prc * e.cone_pmf, pra * e.all_pmf, R.offset, node_npin(R, sp.pos)},
+        stack.push(tree_entry(ri, rm, R, rb, prc * cone_pmf, pra * all_pmf, sp.pos),
                    TreeAlt{prc2 * ea.cone, pra2 * ea.all}, T.mode != 0);
     if (lm && stack.size() < DR_TREE_STACK)
-        stack.push(TreeEntry{li << 2 | lm, lb, plc * e.cone_pmf, pla * e.all_pmf, L.offset, node_npin(L, sp.pos)},
+        stack.push(tree_entry(li, lm, L, lb, plc * cone_pmf, pla * all_pmf, sp.pos),
                    TreeAlt{plc2 * ea.cone, pla2 * ea.all}, T.mode != 0);
     return true;
 }

@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
                                                      const PointRec *pts, EdgeEntry *q, unsigned long long *cnt,
                                                      long long cap, int *counts, int refill) {
     const long long per_bounce = nslots / maxb;  // nslots is a multiple of maxb
-    __shared__ float s_stack[DR_LDS_STACK * 6 * DR_BLOCK];
+    __shared__ float s_stack[DR_LDS_STACK * DR_LDS_WORDS * DR_BLOCK];
     const int lane = (int)__lane_id();
     EdgeTraversal T;
     LdsStack stack;
@@ -537,7 +537,7 @@ __global__ void k_intersect(DrSceneView S, const float *rays, long long n, int *
 // selection kernel's LDS stack and on a plain array stack side by side;
 // out[0] counts pops that differ, out[1] the deepest stack reached.
 __global__ void __launch_bounds__(DR_BLOCK) k_tree_stack(long long ops, uint64_t seed, unsigned long long *out) {
-    __shared__ float s_stack[DR_LDS_STACK * 6 * DR_BLOCK];
+    __shared__ float s_stack[DR_LDS_STACK * DR_LDS_WORDS * DR_BLOCK];
     LdsStack A;
     A.lds = (DR_LDS float *)s_stack + threadIdx.x;
     A.stride = DR_BLOCK;
@@ -554,7 +554,8 @@ __global__ void __launch_bounds__(DR_BLOCK) k_tree_stack(long long ops, uint64_t
         bool push = B.size() == 0 || (B.size() < DR_TREE_STACK && u < 0.55f);
         if (push) {
             int x = (int)(philox_next(r) * 16777216.f);
-            TreeEntry e{x, philox_next(r), philox_next(r), philox_next(r), x ^ 0x5555, x + 7};
+            TreeEntry e{x - 8388608, __float_as_uint(philox_next(r)) ^ (uint32_t)x << 8,
+                        __float_as_uint(philox_next(r)) ^ (uint32_t)x << 11, x ^ 0x5555};
             TreeAlt t{philox_next(r), philox_next(r)};
             A.push(e, t, alt);
             B.push(e, t, alt);
@@ -562,8 +563,8 @@ __global__ void __launch_bounds__(DR_BLOCK) k_tree_stack(long long ops, uint64_t
         } else {
             TreeAlt ta{0.f, 0.f}, tb{0.f, 0.f};
             TreeEntry a = A.pop(ta, alt), b = B.pop(tb, alt);
-            if (a.nm != b.nm || a.bound != b.bound || a.cone_pmf != b.cone_pmf || a.all_pmf != b.all_pmf ||
-                a.off != b.off || a.npin != b.npin || (alt && (ta.cone != tb.cone || ta.all != tb.all)))
+            if (a.code != b.code || a.cpmf != b.cpmf || a.apmf != b.apmf || a.off != b.off ||
+                (alt && (ta.cone != tb.cone || ta.all != tb.all)))
                 bad++;
         }
         if (A.size() != B.size()) bad++;

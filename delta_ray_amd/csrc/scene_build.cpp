@@ -675,6 +675,9 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
     }
     build_edge_tree(hs, sil, false, hs.sil_nodes, hs.sil_edges);
     build_edge_tree(hs, nrm, true, hs.nrm_nodes, hs.nrm_edges);
+    // a traversal stack entry holds a node index or an edge count in 29 bits
+    if (hs.sil_nodes.size() >= (1u << 29) || hs.nrm_nodes.size() >= (1u << 29) || hs.edges.size() >= (1u << 29))
+        return "too many edges for the edge trees (at most 2^29 edges and nodes each)";
     // the vertices the silhouette test of a normal-tree edge reads
     // (edge.h:59-79): endpoints, and each face's vertex off the edge
     hs.nrm_geo.assign(16 * hs.nrm_edges.size(), 0.f);
