@@ -168,10 +168,14 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(const DrJob *__restrict__ job
 // busy) and then start new points together.  In a refill, idle lanes take
 // batches of point slots from `work` with one atomic per batch until each has
 // a valid point or the slots run out, so empty slots cost no iteration.
-// Selected edges are appended to the queue in groups of up to 8 per lane (one
-// queue atomic per group).  Every lane leaves the loop once `work` passes
-// nslots.  cnt[0]: queue length (may exceed cap: the host grows the queue and
-// renders again), cnt[1]: work counter.
+// Selected edges are appended to the queue in groups of up to DR_EMIT_GROUP
+// per lane (one queue atomic per group; a point's edges flush together at its
+// end, so they sit side by side in the queue, which keeps k_edge's waves on
+// rays from one point: 32 rather than 8 makes k_edge 6 % faster).  Every
+// lane leaves the loop once `work` passes nslots.  cnt[0]: queue length (may
+// exceed cap: the host grows the queue and renders again), cnt[1]: work
+// counter.
+#define DR_EMIT_GROUP 32
 __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ jobs, long long nslots, int maxb,
                                                      const PointRec *pts, EdgeEntry *q, unsigned long long *cnt,
                                                      long long cap, int *counts, int refill) {
@@ -188,7 +192,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
     long long slot = -1;
     int j = 0;
     bool more = true;
-    EdgeEntry buf[8];
+    EdgeEntry buf[DR_EMIT_GROUP];
     int nbuf = 0;
     auto flush = [&]() {
         unsigned long long base = atomicAdd(&cnt[0], (unsigned long long)nbuf);
@@ -200,7 +204,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
         // zero-weight edges carry no work (reference delta_ray.cpp:906-908)
         if (w != 0.f) {
             buf[nbuf++] = EdgeEntry{(int)slot, idx | (nrm ? (int)0x80000000 : 0), w, ord};
-            if (nbuf == 8) flush();
+            if (nbuf == DR_EMIT_GROUP) flush();
         }
         j++;
     };
