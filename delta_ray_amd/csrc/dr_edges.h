@@ -264,11 +264,12 @@ DR_INLINE float node_importance_len(const DrEdgeNode &n, float bound, F3 p, floa
 // The stack is an object of its own (owned by the caller), not a member of
 // EdgeTraversal: its storage is indexed at run time, while every other field
 // of the traversal state stays in registers.  An entry carries what the parent
-// already knew of the node (child offset, edge count, whether the point lies
-// in its box), so popping an interior node needs only its two children's
-// records, loaded side by side.  Two stacks: PlainStack, an array (the CPU
-// restatement, tests), and LdsStack (the selection kernel): the top entries
-// live in LDS and older ones spill to private memory in halves.
+// already knew of the node (child offset, edge count, what its bound and
+// whether the point lies in its box decide), so popping an interior node needs
+// only its two children's records, loaded side by side.  Two stacks:
+// LdsStack (the selection kernel): the top entries live in LDS and older ones
+// spill to private memory in halves; and PlainStack, an array, against which
+// the stack self-test (k_tree_stack) checks it.
 //
 // An entry is 16 bytes.  What popping needs of the node's bound is two flags,
 // kept in the sign bits of the two pmfs (a pmf is never negative):

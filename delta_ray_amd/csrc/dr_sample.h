@@ -390,24 +390,6 @@ DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, const Poin
                    mode);
 }
 
-// Stage B: edge selection at one secondary point (edge_tree.cpp:685-725);
-// emit(tree, edge_index, weight, ordinal) per selected edge.  Returns the
-// number of selected edges.
-template <class Emit>
-DR_INLINE int stage_select(const DrSceneView &S, const SampleCtx &c, const PointRec &p, Emit &emit) {
-    EdgeTraversal T;
-    PlainStack stack;
-    if (!select_begin(S, c, p, T, stack)) return 0;
-    int n = 0;
-    auto em = [&](float w, bool nrm, int idx, int j) {
-        emit(nrm, idx, w, j);
-        n++;
-    };
-    while (et_step(T, stack, S, p.sp, em)) {
-    }
-    return n;
-}
-
 // Stage C: one selected edge (reference delta_ray.cpp:904-966).  Scatters
 // the vertex derivatives of the edge and returns dp (the Dirac derivative at
 // the shading point).
