@@ -111,6 +111,7 @@ class dr_stats(ctypes.Structure):
                 ("tris_visited", ctypes.c_uint64), ("secondary_edges", ctypes.c_uint64),
                 ("primary_edge_samples", ctypes.c_uint64),
                 ("edge_nodes_visited", ctypes.c_uint64), ("edges_tested", ctypes.c_uint64),
+                ("stack_overflows", ctypes.c_uint64),
                 ("kernel_ms", ctypes.c_double * 7), ("kernel_launches", ctypes.c_uint64 * 7)]
 
 
@@ -119,7 +120,12 @@ class dr_scene_info(ctypes.Structure):
                 ("bvh_nodes", ctypes.c_int), ("num_edges", ctypes.c_int),
                 ("num_primary_edges", ctypes.c_int),
                 ("silhouette_tree_nodes", ctypes.c_int),
-                ("normal_tree_nodes", ctypes.c_int), ("build_ms", ctypes.c_double)]
+                ("normal_tree_nodes", ctypes.c_int), ("build_ms", ctypes.c_double),
+                ("refits", ctypes.c_int), ("rebuilds", ctypes.c_int)]
+
+
+class dr_shape_update(ctypes.Structure):
+    _fields_ = [("shape_id", ctypes.c_int), ("vertices", ctypes.c_void_p), ("normals", ctypes.c_void_p)]
 
 
 # entry points declared in include/delta_ray.h: (name, restype, argtypes)
@@ -127,6 +133,10 @@ ENTRY_POINTS = [
     ("dr_scene_create", ctypes.c_int, [ctypes.POINTER(dr_scene_desc), ctypes.POINTER(ctypes.c_void_p)]),
     ("dr_scene_destroy", None, [ctypes.c_void_p]),
     ("dr_scene_set_camera", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_camera)]),
+    ("dr_scene_update_vertices", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_shape_update), ctypes.c_int,
+                                                ctypes.c_void_p]),
+    ("dr_scene_update_materials", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_material), ctypes.c_int]),
+    ("dr_scene_update_lights", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_light), ctypes.c_int]),
     ("dr_scene_grad_size", ctypes.c_size_t, [ctypes.c_void_p]),
     ("dr_render", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_render_options), ctypes.c_void_p,
                                  ctypes.c_void_p]),

@@ -2,7 +2,7 @@
 
     python -m delta_ray_amd.build_lib        # or delta_ray_amd.build_lib.build()
 
-Compiles csrc/render.hip and csrc/mlp.hip with hipcc, csrc/scene_build.cpp with the host C++
+Compiles csrc/render.hip, csrc/mlp.hip and csrc/refit.hip with hipcc, csrc/scene_build.cpp with the host C++
 compiler and csrc/ltc_blob.S (the embedded LTC table), then links the shared
 library next to this file.  Everything is compiled with -ffp-contract=off so
 that ray/triangle arithmetic rounds exactly like the CPU oracle's.
@@ -49,23 +49,25 @@ def build(verbose=False):
     objs = {
         "render": os.path.join(BUILD, "render.o"),
         "mlp": os.path.join(BUILD, "mlp.o"),
+        "refit": os.path.join(BUILD, "refit.o"),
         "scene": os.path.join(BUILD, "scene_build.o"),
         "ltc": os.path.join(BUILD, "ltc_blob.o"),
     }
     steps = [
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "render.hip"), "-o", objs["render"]],
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "mlp.hip"), "-o", objs["mlp"]],
+        [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "refit.hip"), "-o", objs["refit"]],
         [cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene_build.cpp"), "-o", objs["scene"]],
         ["gcc", "-c", "-DLTC_FILE=\"%s\"" % LTC, os.path.join(CSRC, "ltc_blob.S"), "-o", objs["ltc"]],
     ]
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=5) as ex:
         outs = list(ex.map(_run, steps))
     if verbose:
         for o in outs:
             sys.stdout.write(o)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp,
-          objs["render"], objs["mlp"], objs["scene"], objs["ltc"]])
+          objs["render"], objs["mlp"], objs["refit"], objs["scene"], objs["ltc"]])
     os.replace(tmp, LIB)
     return LIB
 

@@ -5,8 +5,6 @@
 #include "dr_math.h"
 #include "dr_types.h"
 
-#define DR_STACK 64
-#define DR_TREE_STACK 32
 #define DR_INF __builtin_huge_valf()
 
 struct Isect {
@@ -176,7 +174,10 @@ DR_INLINE Isect trace_closest(const DrSceneView &S, F3 org, F3 dir, float tmin, 
         if (h0 && h1) {
             int near = D.x, far = D.y;
             if (t1 < t0) { near = D.y; far = D.x; }
+            // the builder bounds the depth, so the stack cannot fill; a full
+            // stack would be counted, not silently dropped
             if (sp < DR_STACK) stack.put(sp++, far);
+            else if (S.stats) atomicAdd(&S.stats[7], 1ull);
             node = near;
         } else if (h0) {
             node = D.x;
@@ -265,7 +266,10 @@ DR_INLINE void trace_closest2(const DrSceneView &S, F3 org, F3 da, F3 db, float 
             float t0 = ha0 && ha1 ? a0 : b0, t1 = ha0 && ha1 ? a1 : b1;
             int near = D.x, far = D.y;
             if (t1 < t0) { near = D.y; far = D.x; }
+            // the builder bounds the depth, so the stack cannot fill; a full
+            // stack would be counted, not silently dropped
             if (sp < DR_STACK) stack.put(sp++, far);
+            else if (S.stats) atomicAdd(&S.stats[7], 1ull);
             node = near;
         } else if (h0) {
             node = D.x;
@@ -338,6 +342,7 @@ DR_INLINE bool trace_any(const DrSceneView &S, F3 org, F3 dir, float tmin, float
         if (hit) break;
         if (h0 && h1) {
             if (sp < DR_STACK) stack.put(sp++, D.y);
+            else if (S.stats) atomicAdd(&S.stats[7], 1ull);
             node = D.x;
         } else if (h0) {
             node = D.x;

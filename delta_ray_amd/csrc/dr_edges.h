@@ -45,7 +45,9 @@ struct Cone {
 };
 
 // Box-cone overlap of Eberly's GeometricTools (IntrAlignedBox3Cone3), as the
-// reference uses it (aabb.cpp:98-311).  Box corners in CCW order for each
+// reference uses it (aabb.cpp:98-311).  Algorithm and case table after the
+// Geometric Tools Engine, David Eberly, Geometric Tools LLC, distributed
+// under the Boost Software License 1.0.  Box corners in CCW order for each
 // of the 27 apex regions; geometric data that must match the reference.
 __constant__ static const signed char kConePolyN[27] = {6, 6, 6, 6, 4, 6, 6, 6, 6, 6, 4, 6, 4, 0,
                                                          4, 6, 4, 6, 6, 6, 6, 6, 4, 6, 6, 6, 6};
@@ -471,7 +473,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         for (int i = 0; i < nprims; i++) {
             if (T.directional && !is_silhouette_nrm(S, sp.pos, e.off + i)) continue;
             if (mask & 1) emit(scale * cone_pmf / den, T.directional, e.off + i, T.jc++);
-            if (mask & 2) emit(scale * all_pmf / den, T.directional, e.off + i, 0x8000 | T.ja++);
+            if (mask & 2) emit(scale * all_pmf / den, T.directional, e.off + i, DR_ORD_ALL_PASS | T.ja++);
         }
         return true;
     }
@@ -560,12 +562,18 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
             rm |= 2;
         }
     }
+    // the builder bounds the tree depth so that the stack cannot fill (a
+    // full stack would be counted in the statistics, not silently dropped)
     if (rm && stack.size() < DR_TREE_STACK)
         stack.push(tree_entry(ri, rm, R, rb, prc * cone_pmf, pra * all_pmf, sp.pos),
                    TreeAlt{prc2 * ea.cone, pra2 * ea.all}, T.mode != 0);
+    else if (rm && S.stats)
+        atomicAdd(&S.stats[7], 1ull);
     if (lm && stack.size() < DR_TREE_STACK)
         stack.push(tree_entry(li, lm, L, lb, plc * cone_pmf, pla * all_pmf, sp.pos),
                    TreeAlt{plc2 * ea.cone, pla2 * ea.all}, T.mode != 0);
+    else if (lm && S.stats)
+        atomicAdd(&S.stats[7], 1ull);
     return true;
 }
 

@@ -302,6 +302,9 @@ DR_INLINE uint32_t dr_stream_point(int b) { return 1u + (uint32_t)b; }
 DR_INLINE uint32_t dr_stream_point_all(int b) { return 0x100u + (uint32_t)b; }
 // the choice between the plain and the learned edge proposal at that point
 DR_INLINE uint32_t dr_stream_point_mix(int b) { return 0x200u + (uint32_t)b; }
+// the selected edge of ordinal j (bit 30: the "all" pass; 27 bits of count)
+// at bounce b: bit 31 marks an edge stream, bits 28-30 the bounce, bit 27
+// the pass, bits 0-26 the count
 DR_INLINE uint32_t dr_stream_edge(int b, int j) {
-    return (1u << 20) | ((uint32_t)b << 16) | (uint32_t)(j & 0xffff);
+    return 0x80000000u | ((uint32_t)b << 28) | ((((uint32_t)j >> 30) & 1u) << 27) | ((uint32_t)j & 0x7ffffffu);
 }

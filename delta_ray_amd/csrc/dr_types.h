@@ -19,6 +19,14 @@
 #include <stdint.h>
 
 #define DR_MAX_SHAPES 65536
+// Traversal stack sizes of ray queries (entries, one per BVH level) and of
+// the edge-tree walk (depth + 1 entries); the builder rejects deeper trees.
+#define DR_STACK 64
+#define DR_TREE_STACK 32
+// The ordinal of a selected edge within its pass takes 27 bits of the
+// random-number stream word (the pass flag is bit 30 of the ordinal).
+#define DR_EDGE_ORDINAL_BITS 27
+#define DR_ORD_ALL_PASS 0x40000000
 
 struct DrNode {
     // child 0 box: a.xyz lo, a.w hi.x, b.xy hi.yz; child 1 box: b.zw lo.xy,

@@ -19,11 +19,13 @@
 // skipped).
 #include "../../include/delta_ray.h"
 #include "dr_sample.h"
+#include "refit.h"
 #include "scene_build.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -103,7 +105,7 @@ struct EdgeEntry {
     int slot;  // point slot in the pass
     int edge;  // index into the tree's edge list; bit 31 set: normal tree
     float w;
-    int j;     // ordinal of the edge in its pass (bit 15: the "all" pass)
+    int j;     // ordinal of the edge in its pass (bit 30: the "all" pass)
 };
 
 
@@ -732,6 +734,7 @@ struct Workspace {
     hipStream_t cap_stream = nullptr;        // stream graphs are captured on
     hipGraphExec_t exec = nullptr;
     std::vector<unsigned long long> graph_key;
+    unsigned long long epoch = 0;            // counts reallocations (a captured graph is stale after one)
 };
 
 static void free_workspace(Workspace &w) {
@@ -744,25 +747,50 @@ static void free_workspace(Workspace &w) {
     w = Workspace();
 }
 
+// Device arrays of the scene's structures (BVH, edges, edge trees, their
+// refit tables and scratch, primary-edge buffers): replaced together when the
+// edge set changes and the structures are rebuilt.
+struct SceneStruct {
+    std::vector<void *> allocs;
+    const int *bvh_lv = nullptr, *bvh_lv_off = nullptr;
+    const int *sil_lv = nullptr, *sil_lv_off = nullptr, *sil_rng = nullptr;
+    const int *nrm_lv = nullptr, *nrm_lv_off = nullptr, *nrm_rng = nullptr;
+    int bvh_nlev = 0, sil_nlev = 0, nrm_nlev = 0;
+    const DrEdge *edges = nullptr, *cand = nullptr;
+    const int *cand_keep = nullptr;
+    float *nrm_on = nullptr;              // 6 floats per normal-tree edge
+    float *nsum = nullptr, *wsum = nullptr;  // 3 / 1 floats per node of the larger tree
+    float *prim_len = nullptr;            // one float per mesh edge
+    float *edge_w = nullptr;              // learned per-edge weights (tree order), or NULL
+    size_t prim_cap = 0;                  // capacity of the primary-edge buffers (mesh edges)
+    void *prim_bufs[3] = {nullptr, nullptr, nullptr};  // edges, func, cdf
+};
+
 struct dr_scene {
     drh::HostScene hs;
+    unsigned long long serial = 0;  // unique per created scene (keys captured graphs)
     DrSceneView view;
-    std::vector<void *> allocs;
+    std::vector<void *> allocs;     // arrays whose sizes never change (vertices, textures, lights, LTC)
+    SceneStruct st;
     unsigned long long *stats_dev = nullptr;
     bool stats_on = false;
     Timing tm;
     Workspace ws;
     double build_ms = 0.0;
-    void *prim_bufs[3] = {nullptr, nullptr, nullptr};
+    bool host_stale = false;        // device vertices newer than hs.verts / hs.normals
+    void *small = nullptr;          // device: changed-edge count, primary count, integral
+    int refits = 0, rebuilds = 0;
+    std::vector<dr_view *> views;   // views of this scene (their primary edges follow its vertices)
 };
 
 // a scene seen by another camera: its own primary-edge list
 struct dr_view {
-    dr_scene *scene;
+    dr_scene *scene;                   // NULL once the scene is destroyed
     DrCamera cam;
     int num_prim_edges = 0;
     float prim_func_int = 0.f;
     void *bufs[3] = {nullptr, nullptr, nullptr};
+    size_t cap = 0;                    // capacity of bufs (the scene's mesh edges when allocated)
 };
 
 struct dr_batch {
@@ -772,53 +800,135 @@ struct dr_batch {
 };
 
 template <class T, class P>
-static bool upload(dr_scene *s, const std::vector<T> &v, P *out) {
+static bool upload_to(std::vector<void *> &allocs, const std::vector<T> &v, P *out) {
     if (v.empty()) {
         *out = nullptr;
         return true;
     }
     void *p = nullptr;
     if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) return false;
-    s->allocs.push_back(p);
+    allocs.push_back(p);
     if (hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return false;
     *out = (P)p;
     return true;
 }
+template <class T, class P>
+static bool upload(dr_scene *s, const std::vector<T> &v, P *out) {
+    return upload_to(s->allocs, v, out);
+}
+static bool alloc_to(std::vector<void *> &allocs, size_t bytes, void **out) {
+    *out = nullptr;
+    if (bytes == 0) return true;
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        *out = nullptr;
+        return false;
+    }
+    allocs.push_back(*out);
+    return true;
+}
+// copy into an existing device array of the same size
+template <class T, class P>
+static bool refresh(const std::vector<T> &v, P dst) {
+    return v.empty() || hipMemcpy((void *)dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+}
 
-// primary edges and their distribution to the device (three buffers)
-static bool upload_prim(const std::vector<DrEdge> &edges, const drh::Dist &dist, void *bufs[3]) {
+// capacity buffers for a primary-edge list of at most `cap` edges
+static bool alloc_prim(void *bufs[3], size_t cap) {
     for (int i = 0; i < 3; i++) {
         if (bufs[i]) (void)hipFree(bufs[i]);
         bufs[i] = nullptr;
     }
-    if (edges.empty()) return true;
-    size_t sz[3] = {edges.size() * sizeof(DrEdge), dist.func.size() * sizeof(float), dist.cdf.size() * sizeof(float)};
-    const void *src[3] = {edges.data(), dist.func.data(), dist.cdf.data()};
-    for (int i = 0; i < 3; i++) {
+    size_t c = std::max<size_t>(cap, 1);
+    size_t sz[3] = {c * sizeof(DrEdge), c * sizeof(float), (c + 1) * sizeof(float)};
+    for (int i = 0; i < 3; i++)
         if (hipMalloc(&bufs[i], sz[i]) != hipSuccess) return false;
-        if (hipMemcpy(bufs[i], src[i], sz[i], hipMemcpyHostToDevice) != hipSuccess) return false;
-    }
     return true;
 }
 
-static bool upload_primary(dr_scene *s) {
+static void free_struct(SceneStruct &st) {
+    for (void *p : st.allocs) (void)hipFree(p);
+    for (int i = 0; i < 3; i++)
+        if (st.prim_bufs[i]) (void)hipFree(st.prim_bufs[i]);
+    st = SceneStruct();
+}
+
+// Upload the structures of hs (BVH, triangles, edges, edge trees, geometry
+// records, refit tables, primary edges from the host build, statistics) and
+// point the view at them.  Frees the previous ones.
+static bool upload_structures(dr_scene *s) {
     const drh::HostScene &hs = s->hs;
-    if (!upload_prim(hs.prim_edges, hs.prim, s->prim_bufs)) return false;
-    s->view.num_prim_edges = (int)hs.prim_edges.size();
-    s->view.prim_func_int = hs.prim.func_int;
-    s->view.prim_edges = (decltype(s->view.prim_edges))s->prim_bufs[0];
-    s->view.prim_func = (decltype(s->view.prim_func))s->prim_bufs[1];
-    s->view.prim_cdf = (decltype(s->view.prim_cdf))s->prim_bufs[2];
+    DrSceneView &v = s->view;
+    free_struct(s->st);
+    SceneStruct &st = s->st;
+    std::vector<void *> &A = st.allocs;
+    bool ok = upload_to(A, hs.nodes, &v.nodes) && upload_to(A, hs.tris, &v.tris) &&
+              upload_to(A, hs.sil_edges, &v.sil_edges) && upload_to(A, hs.sil_nodes, &v.sil_nodes) &&
+              upload_to(A, hs.nrm_edges, &v.nrm_edges) && upload_to(A, hs.nrm_nodes, &v.nrm_nodes) &&
+              upload_to(A, hs.nrm_geo, &v.nrm_geo) && upload_to(A, hs.sil_geo, &v.sil_geo) &&
+              upload_to(A, hs.bvh_lv, &st.bvh_lv) && upload_to(A, hs.bvh_lv_off, &st.bvh_lv_off) &&
+              upload_to(A, hs.sil_lv, &st.sil_lv) && upload_to(A, hs.sil_lv_off, &st.sil_lv_off) &&
+              upload_to(A, hs.sil_rng, &st.sil_rng) && upload_to(A, hs.nrm_lv, &st.nrm_lv) &&
+              upload_to(A, hs.nrm_lv_off, &st.nrm_lv_off) && upload_to(A, hs.nrm_rng, &st.nrm_rng) &&
+              upload_to(A, hs.edges, &st.edges) && upload_to(A, hs.cand_edges, &st.cand) &&
+              upload_to(A, hs.cand_keep, &st.cand_keep);
+    if (!ok) return false;
+    st.bvh_nlev = hs.bvh_lv_off.empty() ? 0 : (int)hs.bvh_lv_off.size() - 1;
+    st.sil_nlev = hs.sil_lv_off.empty() ? 0 : (int)hs.sil_lv_off.size() - 1;
+    st.nrm_nlev = hs.nrm_lv_off.empty() ? 0 : (int)hs.nrm_lv_off.size() - 1;
+    size_t maxn = std::max(hs.sil_nodes.size(), hs.nrm_nodes.size());
+    size_t ne = hs.sil_edges.size() + hs.nrm_edges.size();
+    void *p = nullptr;
+    ok = alloc_to(A, 6 * hs.nrm_edges.size() * sizeof(float), (void **)&st.nrm_on) &&
+         alloc_to(A, 3 * maxn * sizeof(float), (void **)&st.nsum) &&
+         alloc_to(A, maxn * sizeof(float), (void **)&st.wsum) &&
+         alloc_to(A, hs.edges.size() * sizeof(float), (void **)&st.prim_len) &&
+         alloc_to(A, 3 * std::max<size_t>(ne, 1) * sizeof(float), &p);
+    if (!ok) return false;
+    v.edge_stat = (float *)p;
+    if (hipMemset(p, 0, 3 * std::max<size_t>(ne, 1) * sizeof(float)) != hipSuccess) return false;
+    // primary edges: capacity of every mesh edge, filled from the host build
+    st.prim_cap = hs.edges.size();
+    if (!alloc_prim(st.prim_bufs, st.prim_cap)) return false;
+    if (!hs.prim_edges.empty() &&
+        (hipMemcpy(st.prim_bufs[0], hs.prim_edges.data(), hs.prim_edges.size() * sizeof(DrEdge),
+                   hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(st.prim_bufs[1], hs.prim.func.data(), hs.prim.func.size() * sizeof(float),
+                   hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(st.prim_bufs[2], hs.prim.cdf.data(), hs.prim.cdf.size() * sizeof(float),
+                   hipMemcpyHostToDevice) != hipSuccess))
+        return false;
+    v.num_prim_edges = (int)hs.prim_edges.size();
+    v.prim_func_int = hs.prim.func_int;
+    v.prim_edges = (decltype(v.prim_edges))st.prim_bufs[0];
+    v.prim_func = (decltype(v.prim_func))st.prim_bufs[1];
+    v.prim_cdf = (decltype(v.prim_cdf))st.prim_bufs[2];
+    v.num_tris = (int)hs.tri_shape.size();
+    v.num_nodes = (int)hs.nodes.size();
+    v.num_sil_nodes = (int)hs.sil_nodes.size();
+    v.num_nrm_nodes = (int)hs.nrm_nodes.size();
+    v.num_sil_edges = (int)hs.sil_edges.size();
     return true;
+}
+
+// light arrays of hs into the existing device arrays (same sizes)
+static bool refresh_lights(dr_scene *s) {
+    const drh::HostScene &hs = s->hs;
+    DrSceneView &v = s->view;
+    bool ok = refresh(hs.lights, v.lights) && refresh(hs.light_func, v.light_func) &&
+              refresh(hs.light_cdf, v.light_cdf) && refresh(hs.light_sel.func, v.lights_func) &&
+              refresh(hs.light_sel.cdf, v.lights_cdf);
+    v.lights_func_int = hs.light_sel.func_int;
+    v.light_total_intensity = hs.light_total_intensity;
+    return ok;
 }
 
 static void free_scene(dr_scene *s) {
     free_workspace(s->ws);
     free_timing(s->tm);
     for (void *p : s->allocs) (void)hipFree(p);
-    for (int i = 0; i < 3; i++)
-        if (s->prim_bufs[i]) (void)hipFree(s->prim_bufs[i]);
+    free_struct(s->st);
     if (s->stats_dev) (void)hipFree(s->stats_dev);
+    if (s->small) (void)hipFree(s->small);
     delete s;
 }
 
@@ -842,7 +952,9 @@ static void plan_chunks(const dr_render_options *o, long long min_threads, int &
     chunks = (o->num_samples + per - 1) / per;
 }
 
+static unsigned long long g_alloc_epoch = 0;
 static bool grow(void **p, size_t bytes) {
+    g_alloc_epoch++;
     if (*p) (void)hipFree(*p);
     *p = nullptr;
     return hipMalloc(p, bytes) == hipSuccess;
@@ -945,7 +1057,8 @@ static dr_status build_plan(const dr_job *jobs, int n, const Tuning &tn, Plan &P
         }
         if (adj) P.maxb = std::max(P.maxb, bucket(jb.opts.max_bounces));
         P.jobs.push_back(dj);
-        unsigned long long k[] = {(unsigned long long)(uintptr_t)sc, (unsigned long long)(uintptr_t)jb.view,
+        unsigned long long k[] = {(unsigned long long)(uintptr_t)sc, sc->serial, (unsigned long long)sc->hs.grad_size,
+                                  (unsigned long long)(uintptr_t)jb.view,
                                   (unsigned long long)jb.opts.width, (unsigned long long)jb.opts.height,
                                   (unsigned long long)jb.opts.num_samples, (unsigned long long)jb.opts.max_bounces,
                                   (unsigned long long)jb.opts.flags, (unsigned long long)dj.prim_n,
@@ -1020,7 +1133,14 @@ static dr_status build_plan(const dr_job *jobs, int n, const Tuning &tn, Plan &P
     return DR_OK;
 }
 
+static dr_status ensure_workspace_(Workspace &w, const Plan &P);
 static dr_status ensure_workspace(Workspace &w, const Plan &P) {
+    unsigned long long e0 = g_alloc_epoch;
+    dr_status st = ensure_workspace_(w, P);
+    if (g_alloc_epoch != e0) w.epoch++;
+    return st;
+}
+static dr_status ensure_workspace_(Workspace &w, const Plan &P) {
     if (!w.cnt) {
         if (hipMalloc((void **)&w.cnt, 4 * sizeof(unsigned long long)) != hipSuccess ||
             hipHostMalloc((void **)&w.cnt_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
@@ -1035,7 +1155,9 @@ static dr_status ensure_workspace(Workspace &w, const Plan &P) {
         }
         w.slot_cap = n;
     }
-    long long qn = P.max_slots * 2;
+    // four queue entries per point slot (the heightfield at 2 bounces selects
+    // about 3.5); a pass that selects more grows the queue
+    long long qn = P.max_slots * 4;
     if (qn > w.q_cap) {
         if (!grow((void **)&w.q, qn * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, qn * sizeof(float4)) ||
             !grow((void **)&w.cont, qn * sizeof(int))) {
@@ -1057,6 +1179,7 @@ static dr_status ensure_workspace(Workspace &w, const Plan &P) {
 }
 
 static dr_status grow_queue(Workspace &w, long long n) {
+    w.epoch++;
     if (!grow((void **)&w.q, n * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, n * sizeof(float4)) ||
         !grow((void **)&w.cont, n * sizeof(int))) {
         w.q_cap = 0;
@@ -1070,8 +1193,33 @@ static long long g_select_blocks = 0, g_edge_blocks[4] = {0, 0, 0, 0}, g_shade_b
 
 // All launches of a plan, stream-ordered, without host synchronisation (so
 // they can be captured in a hipGraph).  The tables are already on the device.
+static dr_status grow_queue(Workspace &w, long long n);
+
+// After a pass's edge selection, when the host may wait (no graph capture):
+// read the queue length and, if the queue overflowed, grow it and run the
+// selection of that pass again (it only writes the queue and the per-point
+// counts, so a second run replaces the first).
+template <class Launch>
+static dr_status settle_selection(Workspace &w, hipStream_t hs, Launch &&select) {
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if (hipMemcpyAsync(w.cnt_host, w.cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) != hipSuccess ||
+            hipStreamSynchronize(hs) != hipSuccess)
+            return fail(DR_ERROR_DEVICE, "reading the edge queue length failed");
+        long long len = (long long)*w.cnt_host;
+        if (len <= w.q_cap) return DR_OK;
+        dr_status st = grow_queue(w, len + len / 4);
+        if (st != DR_OK) return st;
+        k_fill<<<1, 64, 0, hs>>>((float *)w.cnt, 4, 0.f);
+        select();
+    }
+    return fail(DR_ERROR_DEVICE, "edge queue kept overflowing");
+}
+
+// sync_select: settle every pass's queue length on the host before its edge
+// samples (uncaptured runs); otherwise the caller checks the largest length
+// once at the end.
 template <int MAXB>
-static void enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
+static dr_status enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs, bool sync_select) {
     for (auto &z : P.zero) k_fill<<<256, 256, 0, hs>>>(z.first, z.second, 0.f);
     k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 2), 2, 0.f);
     const int bi = MAXB == 1 ? 0 : MAXB == 2 ? 1 : MAXB == 4 ? 2 : 3;
@@ -1087,11 +1235,16 @@ static void enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
         k_fill_i<<<1024, 256, 0, hs>>>(w.pt_lastj, (size_t)ps.nslots, -1);
         k_fill<<<1, 64, 0, hs>>>((float *)w.cnt, 4, 0.f);        // queue length, work counter
         k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 3), 2, 0.f);  // continuations
-        {
+        auto select = [&]() {
             LaunchTimer t(tm, TK_SELECT, hs);
             long long blocks = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_select, g_select_blocks));
             k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, MAXB, w.pts, w.q, w.cnt, w.q_cap,
                                                             w.counts, P.select_refill);
+        };
+        select();
+        if (sync_select) {
+            dr_status st = settle_selection(w, hs, select);
+            if (st != DR_OK) return st;
         }
         long long eb = resident_blocks(k_edge<MAXB>, g_edge_blocks[bi]);
         {
@@ -1120,13 +1273,14 @@ static void enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
         LaunchTimer t(tm, TK_FORWARD, hs);
         k_forward<<<(unsigned)P.fwd_blocks, DR_BLOCK, 0, hs>>>(w.jobs, w.ranges + P.fwd_r0, P.fwd_r1 - P.fwd_r0);
     }
+    return DR_OK;
 }
 
-static void enqueue_any(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs) {
-    if (P.maxb <= 1) enqueue<1>(P, w, tm, hs);
-    else if (P.maxb <= 2) enqueue<2>(P, w, tm, hs);
-    else if (P.maxb <= 4) enqueue<4>(P, w, tm, hs);
-    else enqueue<8>(P, w, tm, hs);
+static dr_status enqueue_any(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs, bool sync_select) {
+    if (P.maxb <= 1) return enqueue<1>(P, w, tm, hs, sync_select);
+    if (P.maxb <= 2) return enqueue<2>(P, w, tm, hs, sync_select);
+    if (P.maxb <= 4) return enqueue<4>(P, w, tm, hs, sync_select);
+    return enqueue<8>(P, w, tm, hs, sync_select);
 }
 
 static bool has_edges(const Plan &P) {
@@ -1135,10 +1289,13 @@ static bool has_edges(const Plan &P) {
     return false;
 }
 
-// Run a plan: tables to the device, launches (captured once and replayed as a
-// hipGraph with use_graph), then, when there are secondary edges, one host
-// read of the largest queue length; a queue that overflowed is grown and the
-// plan runs again (every output is rewritten from zero by a run).
+// Run a plan: tables to the device, then the launches.  Uncaptured, every
+// pass's queue length is settled after its edge selection (an overflowing
+// queue is grown and only that selection runs again).  With use_graph the
+// launches are captured once and replayed as a hipGraph, and, when there are
+// secondary edges, the largest queue length is read once at the end; a queue
+// that overflowed is grown and the plan runs again (every output is
+// rewritten from zero by a run).
 static dr_status run_plan(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs, bool use_graph) {
     dr_status st = ensure_workspace(w, P);
     if (st != DR_OK) return st;
@@ -1149,11 +1306,15 @@ static dr_status run_plan(const Plan &P, Workspace &w, Timing *tm, hipStream_t h
                                                  hipMemcpyHostToDevice, hs) != hipSuccess))
             return fail(DR_ERROR_DEVICE, "job table upload failed");
         if (use_graph) {
+            // every device buffer the captured launches use, and the
+            // workspace's reallocation count
             std::vector<unsigned long long> key = P.key;
-            unsigned long long extra[] = {(unsigned long long)(uintptr_t)w.pts, (unsigned long long)(uintptr_t)w.q,
-                                          (unsigned long long)w.q_cap, (unsigned long long)(uintptr_t)w.jobs,
-                                          (unsigned long long)(uintptr_t)w.ranges};
-            key.insert(key.end(), extra, extra + 5);
+            const void *bufs[] = {w.pts, w.pt_dp, w.pt_lastj, w.counts, w.q, w.entry_dp, w.cont, w.cnt, w.jobs,
+                                  w.ranges};
+            for (const void *b : bufs) key.push_back((unsigned long long)(uintptr_t)b);
+            key.push_back((unsigned long long)w.q_cap);
+            key.push_back((unsigned long long)w.slot_cap);
+            key.push_back(w.epoch);
             if (!w.exec || key != w.graph_key) {
                 if (w.exec) (void)hipGraphExecDestroy(w.exec);
                 w.exec = nullptr;
@@ -1163,7 +1324,7 @@ static dr_status run_plan(const Plan &P, Workspace &w, Timing *tm, hipStream_t h
                 hipGraph_t g = nullptr;
                 if (hipStreamBeginCapture(w.cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                     return fail(DR_ERROR_DEVICE, "graph capture failed to start");
-                enqueue_any(P, w, nullptr, w.cap_stream);
+                (void)enqueue_any(P, w, nullptr, w.cap_stream, false);
                 hipError_t e = hipStreamEndCapture(w.cap_stream, &g);
                 if (e != hipSuccess || !g) return fail(DR_ERROR_DEVICE, "graph capture failed");
                 e = hipGraphInstantiate(&w.exec, g, nullptr, nullptr, 0);
@@ -1176,7 +1337,8 @@ static dr_status run_plan(const Plan &P, Workspace &w, Timing *tm, hipStream_t h
             }
             if (hipGraphLaunch(w.exec, hs) != hipSuccess) return fail(DR_ERROR_DEVICE, "graph launch failed");
         } else {
-            enqueue_any(P, w, tm, hs);
+            st = enqueue_any(P, w, tm, hs, true);
+            if (st != DR_OK) return st;
         }
         st = launch_check("render kernels");
         if (st != DR_OK) return st;
@@ -1219,6 +1381,8 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
     auto t0 = std::chrono::steady_clock::now();
     dr_scene *s = new (std::nothrow) dr_scene();
     if (!s) return fail(DR_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    static std::atomic<unsigned long long> serial{0};
+    s->serial = ++serial;
     std::string err;
     try {
         err = drh::build_host_scene(desc, s->hs);
@@ -1232,22 +1396,19 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
     drh::HostScene &hs = s->hs;
     DrSceneView &v = s->view;
     std::memset(&v, 0, sizeof(v));
-    bool ok = upload(s, hs.nodes, &v.nodes) && upload(s, hs.tris, &v.tris) && upload(s, hs.verts, &v.verts) &&
-              upload(s, hs.tri_vidx, &v.tri_vidx) && upload(s, hs.tri_shape, &v.tri_shape) &&
-              upload(s, hs.uvs, &v.uvs) && upload(s, hs.normals, &v.normals) && upload(s, hs.texels, &v.texels) &&
+    bool ok = upload(s, hs.verts, &v.verts) && upload(s, hs.tri_vidx, &v.tri_vidx) &&
+              upload(s, hs.tri_shape, &v.tri_shape) && upload(s, hs.uvs, &v.uvs) &&
+              upload(s, hs.normals, &v.normals) && upload(s, hs.texels, &v.texels) &&
               upload(s, hs.shapes, &v.shapes) && upload(s, hs.materials, &v.materials) &&
               upload(s, hs.lights, &v.lights) && upload(s, hs.light_func, &v.light_func) &&
               upload(s, hs.light_cdf, &v.light_cdf) && upload(s, hs.light_sel.func, &v.lights_func) &&
-              upload(s, hs.light_sel.cdf, &v.lights_cdf) && upload(s, hs.sil_edges, &v.sil_edges) &&
-              upload(s, hs.sil_nodes, &v.sil_nodes) && upload(s, hs.nrm_edges, &v.nrm_edges) &&
-              upload(s, hs.nrm_nodes, &v.nrm_nodes) && upload(s, hs.nrm_geo, &v.nrm_geo) &&
-              upload(s, hs.sil_geo, &v.sil_geo);
+              upload(s, hs.light_sel.cdf, &v.lights_cdf);
     std::vector<float> ltc(dr_ltc_blob, dr_ltc_blob + 16384 * 9);
     if ((size_t)(dr_ltc_blob_end - dr_ltc_blob) != 16384 * 9) {
         free_scene(s);
         return fail(DR_ERROR_DEVICE, "embedded LTC table has the wrong size");
     }
-    ok = ok && upload(s, ltc, &v.ltc) && upload_primary(s);
+    ok = ok && upload(s, ltc, &v.ltc) && upload_structures(s) && hipMalloc(&s->small, 16) == hipSuccess;
     if (!ok) {
         free_scene(s);
         return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation or copy failed");
@@ -1256,21 +1417,6 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
     v.light_total_intensity = hs.light_total_intensity;
     v.num_lights = (int)hs.lights.size();
     v.num_shapes = (int)hs.shapes.size();
-    v.num_tris = (int)hs.tri_shape.size();
-    v.num_nodes = (int)hs.nodes.size();
-    v.num_sil_nodes = (int)hs.sil_nodes.size();
-    v.num_nrm_nodes = (int)hs.nrm_nodes.size();
-    v.num_sil_edges = (int)hs.sil_edges.size();
-    {
-        size_t ne = hs.sil_edges.size() + hs.nrm_edges.size();
-        std::vector<float> zeros(3 * std::max<size_t>(ne, 1), 0.f);
-        const float *p = nullptr;
-        if (!upload(s, zeros, &p)) {
-            free_scene(s);
-            return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation failed");
-        }
-        v.edge_stat = const_cast<float *>(p);
-    }
     v.cam = hs.cam;
     v.stats = nullptr;
     auto t1 = std::chrono::steady_clock::now();
@@ -1279,17 +1425,245 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
     return DR_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------- device refit -----
+static drrefit::RefitView refit_view(dr_scene *s) {
+    const DrSceneView &v = s->view;
+    const SceneStruct &st = s->st;
+    drrefit::RefitView r;
+    r.verts = v.verts;
+    r.normals = v.normals;
+    r.tri_vidx = v.tri_vidx;
+    r.shapes = v.shapes;
+    r.tris = (DrTri *)(v.tris);
+    r.num_tris = (int)s->hs.tris.size();
+    r.nodes = (DrNode *)(v.nodes);
+    r.bvh_lv = st.bvh_lv;
+    r.bvh_lv_off = st.bvh_lv_off;
+    r.bvh_nlev = st.bvh_nlev;
+    r.edges = st.edges;
+    r.num_edges = (int)s->hs.edges.size();
+    r.cand = st.cand;
+    r.cand_keep = st.cand_keep;
+    r.num_cand = (int)s->hs.cand_edges.size();
+    return r;
+}
+
+// The primary-edge list and distribution of camera `cam` on the device into
+// the capacity buffers `bufs`; reads back the count and the integral
+// (synchronises the stream).
+static dr_status device_primary(dr_scene *s, const DrCamera &cam, void *bufs[3], int &count, float &fint,
+                                hipStream_t hs) {
+    drrefit::PrimaryOut o;
+    o.len = s->st.prim_len;
+    o.edges = (DrEdge *)bufs[0];
+    o.func = (float *)bufs[1];
+    o.cdf = (float *)bufs[2];
+    o.count = (int *)s->small + 1;
+    o.func_int = (float *)s->small + 2;
+    k_fill<<<1, 64, 0, hs>>>((float *)s->small + 1, 2, 0.f);
+    drrefit::primary_edges(refit_view(s), cam, o, hs);
+    dr_status st = launch_check("primary-edge kernels");
+    if (st != DR_OK) return st;
+    int r[4];
+    if (hipMemcpyAsync(r, s->small, 16, hipMemcpyDeviceToHost, hs) != hipSuccess || hipStreamSynchronize(hs) != hipSuccess)
+        return fail(DR_ERROR_DEVICE, "reading the primary-edge count failed");
+    count = r[1];
+    std::memcpy(&fint, &r[2], sizeof(float));
+    return DR_OK;
+}
+
+// device edge trees from the current device vertices (and learned weights)
+static dr_status refit_edge_trees(dr_scene *s, hipStream_t hs) {
+    const drh::HostScene &h = s->hs;
+    const SceneStruct &st = s->st;
+    DrSceneView &v = s->view;
+    drrefit::RefitView rv = refit_view(s);
+    drrefit::EdgeTreeRefit t;
+    t.nsum = st.nsum;
+    t.wsum = st.wsum;
+    // silhouette tree: endpoints only, no cones
+    t.nodes = (DrEdgeNode *)(v.sil_nodes);
+    t.num_nodes = (int)h.sil_nodes.size();
+    t.edges = v.sil_edges;
+    t.num_edges = (int)h.sil_edges.size();
+    t.directional = false;
+    t.geo = (float *)(v.sil_geo);
+    t.on = nullptr;
+    t.weights = st.edge_w;
+    t.lv = st.sil_lv;
+    t.lv_off = st.sil_lv_off;
+    t.rng = st.sil_rng;
+    t.nlev = st.sil_nlev;
+    drrefit::refit_edge_tree(rv, t, hs);
+    t.nodes = (DrEdgeNode *)(v.nrm_nodes);
+    t.num_nodes = (int)h.nrm_nodes.size();
+    t.edges = v.nrm_edges;
+    t.num_edges = (int)h.nrm_edges.size();
+    t.directional = true;
+    t.geo = (float *)(v.nrm_geo);
+    t.on = st.nrm_on;
+    t.weights = st.edge_w ? st.edge_w + h.sil_edges.size() : nullptr;
+    t.lv = st.nrm_lv;
+    t.lv_off = st.nrm_lv_off;
+    t.rng = st.nrm_rng;
+    t.nlev = st.nrm_nlev;
+    drrefit::refit_edge_tree(rv, t, hs);
+    return launch_check("edge-tree refit");
+}
+
+static dr_status sync_host_vertices(dr_scene *s) {
+    if (!s->host_stale) return DR_OK;
+    drh::HostScene &h = s->hs;
+    if ((!h.verts.empty() &&
+         hipMemcpy(h.verts.data(), s->view.verts, h.verts.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) ||
+        (!h.normals.empty() && hipMemcpy(h.normals.data(), s->view.normals, h.normals.size() * sizeof(float),
+                                         hipMemcpyDeviceToHost) != hipSuccess))
+        return fail(DR_ERROR_DEVICE, "vertex read-back failed");
+    s->host_stale = false;
+    return DR_OK;
+}
+
+// primary edges of every view of the scene, for its current vertices
+static dr_status refresh_views(dr_scene *s, hipStream_t hs) {
+    for (dr_view *v : s->views) {
+        if (v->cap != s->st.prim_cap || !v->bufs[0]) {
+            if (!alloc_prim(v->bufs, s->st.prim_cap)) return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+            v->cap = s->st.prim_cap;
+        }
+        dr_status st = device_primary(s, v->cam, v->bufs, v->num_prim_edges, v->prim_func_int, hs);
+        if (st != DR_OK) return st;
+    }
+    return DR_OK;
+}
+
+extern "C" {
+
 void dr_scene_destroy(dr_scene *s) {
-    if (s) free_scene(s);
+    if (!s) return;
+    for (dr_view *v : s->views) v->scene = nullptr;
+    free_scene(s);
 }
 
 dr_status dr_scene_set_camera(dr_scene *s, const dr_camera *c) {
     if (!s || !c) return fail(DR_ERROR_INVALID_ARGUMENT, "null argument");
     if (c->fisheye) return fail(DR_ERROR_UNSUPPORTED, "fisheye cameras are not supported");
-    drh::set_camera(s->hs, *c);
-    drh::build_primary_edges(s->hs);
-    if (!upload_primary(s)) return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation failed");
-    s->view.cam = s->hs.cam;
+    DrCamera cam = drh::make_camera(*c);
+    int n = 0;
+    float fi = 0.f;
+    dr_status st = device_primary(s, cam, s->st.prim_bufs, n, fi, nullptr);
+    if (st != DR_OK) return st;
+    s->hs.cam = cam;
+    s->view.cam = cam;
+    s->view.num_prim_edges = n;
+    s->view.prim_func_int = fi;
+    return DR_OK;
+}
+
+dr_status dr_scene_update_vertices(dr_scene *s, const dr_shape_update *updates, int num_updates, void *stream) {
+    if (!s || (num_updates > 0 && !updates) || num_updates < 0) return fail(DR_ERROR_INVALID_ARGUMENT, "bad arguments");
+    hipStream_t hs = (hipStream_t)stream;
+    drh::HostScene &h = s->hs;
+    bool lights = false;
+    for (int i = 0; i < num_updates; i++) {
+        const dr_shape_update &u = updates[i];
+        if (u.shape_id < 0 || u.shape_id >= (int)h.shapes.size()) return fail(DR_ERROR_INVALID_ARGUMENT, "shape id out of range");
+        const DrShape &sh = h.shapes[u.shape_id];
+        if (!u.vertices && sh.nv > 0) return fail(DR_ERROR_INVALID_ARGUMENT, "vertices are required");
+        if (u.normals && !sh.has_n) return fail(DR_ERROR_INVALID_ARGUMENT, "the shape was built without normals");
+        if (sh.nv > 0 && hipMemcpyAsync((float *)s->view.verts + 3 * (size_t)sh.vert_off, u.vertices,
+                                        3 * (size_t)sh.nv * sizeof(float), hipMemcpyDeviceToDevice, hs) != hipSuccess)
+            return fail(DR_ERROR_DEVICE, "vertex copy failed");
+        if (u.normals && hipMemcpyAsync((float *)s->view.normals + 3 * (size_t)sh.vert_off, u.normals,
+                                        3 * (size_t)sh.nv * sizeof(float), hipMemcpyDeviceToDevice, hs) != hipSuccess)
+            return fail(DR_ERROR_DEVICE, "normal copy failed");
+        lights |= sh.light >= 0;
+    }
+    s->host_stale = true;
+    // the edge filter's verdicts on the new vertices
+    k_fill<<<1, 64, 0, hs>>>((float *)s->small, 1, 0.f);
+    drrefit::RefitView rv = refit_view(s);
+    drrefit::edge_filter(rv, (unsigned int *)s->small, hs);
+    unsigned int changed = 0;
+    if (launch_check("edge filter") != DR_OK ||
+        hipMemcpyAsync(&changed, s->small, sizeof(changed), hipMemcpyDeviceToHost, hs) != hipSuccess ||
+        hipStreamSynchronize(hs) != hipSuccess)
+        return fail(DR_ERROR_DEVICE, "edge filter failed");
+    if (changed) {
+        // the edge set changed: rebuild the structures on the host
+        dr_status st = sync_host_vertices(s);
+        if (st != DR_OK) return st;
+        std::string err = drh::build_structures(h);
+        if (!err.empty()) return fail(DR_ERROR_INVALID_ARGUMENT, err);
+        if (!upload_structures(s)) return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation or copy failed");
+        s->rebuilds++;
+    } else {
+        drrefit::refit_geometry(rv, hs);
+        dr_status st = refit_edge_trees(s, hs);
+        if (st == DR_OK) st = launch_check("BVH refit");
+        if (st != DR_OK) return st;
+        int n = 0;
+        float fi = 0.f;
+        st = device_primary(s, s->view.cam, s->st.prim_bufs, n, fi, hs);
+        if (st != DR_OK) return st;
+        s->view.num_prim_edges = n;
+        s->view.prim_func_int = fi;
+        s->refits++;
+    }
+    if (lights) {
+        dr_status st = sync_host_vertices(s);
+        if (st != DR_OK) return st;
+        drh::build_lights(h);
+        if (!refresh_lights(s)) return fail(DR_ERROR_DEVICE, "light upload failed");
+    }
+    return refresh_views(s, hs);
+}
+
+dr_status dr_scene_update_materials(dr_scene *s, const dr_material *materials, int num_materials) {
+    if (!s || (num_materials > 0 && !materials)) return fail(DR_ERROR_INVALID_ARGUMENT, "bad arguments");
+    drh::HostScene &h = s->hs;
+    if (num_materials != (int)h.materials.size()) return fail(DR_ERROR_INVALID_ARGUMENT, "material count differs");
+    std::vector<float> texels(h.texels.size());
+    std::vector<DrMaterial> mats = h.materials;
+    for (int i = 0; i < num_materials; i++) {
+        const dr_material &m = materials[i];
+        DrMaterial &o = mats[i];
+        const dr_texture *t[3] = {&m.diffuse_reflectance, &m.specular_reflectance, &m.roughness};
+        DrTex *d[3] = {&o.diffuse, &o.specular, &o.roughness};
+        for (int k = 0; k < 3; k++) {
+            if (t[k]->height != d[k]->h || (t[k]->height > 0 && t[k]->width != d[k]->w) || t[k]->channels != d[k]->c ||
+                !t[k]->data)
+                return fail(DR_ERROR_INVALID_ARGUMENT, "material texture sizes differ from the scene's");
+            std::memcpy(&texels[d[k]->off], t[k]->data, dr_texture_size(t[k]) * sizeof(float));
+        }
+        for (int k = 0; k < 2; k++) {
+            o.duv[k] = m.diffuse_uv_scale[k];
+            o.suv[k] = m.specular_uv_scale[k];
+            o.ruv[k] = m.roughness_uv_scale[k];
+        }
+        o.two_sided = m.two_sided ? 1 : 0;
+    }
+    h.texels.swap(texels);
+    h.materials.swap(mats);
+    if (!refresh(h.texels, s->view.texels) || !refresh(h.materials, s->view.materials))
+        return fail(DR_ERROR_DEVICE, "material upload failed");
+    return DR_OK;
+}
+
+dr_status dr_scene_update_lights(dr_scene *s, const dr_light *lights, int num_lights) {
+    if (!s || (num_lights > 0 && !lights)) return fail(DR_ERROR_INVALID_ARGUMENT, "bad arguments");
+    drh::HostScene &h = s->hs;
+    if (num_lights != (int)h.light_shapes.size()) return fail(DR_ERROR_INVALID_ARGUMENT, "light count differs");
+    for (int i = 0; i < num_lights; i++)
+        if (lights[i].shape_id != h.light_shapes[i])
+            return fail(DR_ERROR_INVALID_ARGUMENT, "a light's shape differs from the scene's");
+    dr_status st = sync_host_vertices(s);
+    if (st != DR_OK) return st;
+    for (int i = 0; i < num_lights; i++)
+        for (int k = 0; k < 3; k++) h.light_I[3 * i + k] = lights[i].intensity[k];
+    drh::build_lights(h);
+    if (!refresh_lights(s)) return fail(DR_ERROR_DEVICE, "light upload failed");
     return DR_OK;
 }
 
@@ -1301,10 +1675,12 @@ dr_status dr_scene_get_info(const dr_scene *s, dr_scene_info *o) {
     o->num_vertices = (int)(s->hs.verts.size() / 3);
     o->bvh_nodes = (int)s->hs.nodes.size();
     o->num_edges = (int)s->hs.edges.size();
-    o->num_primary_edges = (int)s->hs.prim_edges.size();
+    o->num_primary_edges = s->view.num_prim_edges;
     o->silhouette_tree_nodes = (int)s->hs.sil_nodes.size();
     o->normal_tree_nodes = (int)s->hs.nrm_nodes.size();
     o->build_ms = s->build_ms;
+    o->refits = s->refits;
+    o->rebuilds = s->rebuilds;
     return DR_OK;
 }
 
@@ -1328,6 +1704,7 @@ dr_status dr_scene_stats(dr_scene *s, int enable, dr_stats *out) {
         out->primary_edge_samples = h[4];
         out->edge_nodes_visited = h[5];
         out->edges_tested = h[6];
+        out->stack_overflows = h[7];
         for (auto &t : s->tm.timed) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) ms = 0.f;
@@ -1372,6 +1749,8 @@ int64_t dr_scene_edge_count(const dr_scene *s) {
 dr_status dr_scene_edge_features(dr_scene *s, const dr_view *view, float *features, void *stream) {
     if (!s || !features) return fail(DR_ERROR_INVALID_ARGUMENT, "null argument");
     if (view && view->scene != s) return fail(DR_ERROR_INVALID_ARGUMENT, "view of another scene");
+    dr_status st = sync_host_vertices(s);
+    if (st != DR_OK) return st;
     std::vector<float> f;
     drh::edge_features(s->hs, view ? view->cam : s->hs.cam, f);
     if (f.empty()) return DR_OK;
@@ -1384,22 +1763,21 @@ dr_status dr_scene_edge_features(dr_scene *s, const dr_view *view, float *featur
 
 dr_status dr_scene_set_edge_weights(dr_scene *s, const float *weights) {
     if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
-    drh::HostScene &hs = s->hs;
-    size_t ns = hs.sil_edges.size();
-    if (weights) {
-        drh::set_learned_lengths(hs, hs.sil_nodes, hs.sil_edges, weights);
-        drh::set_learned_lengths(hs, hs.nrm_nodes, hs.nrm_edges, weights + ns);
+    SceneStruct &st = s->st;
+    size_t ne = s->hs.sil_edges.size() + s->hs.nrm_edges.size();
+    if (weights && ne > 0) {
+        if (!st.edge_w && !alloc_to(st.allocs, ne * sizeof(float), (void **)&st.edge_w))
+            return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+        if (hipMemcpy(st.edge_w, weights, ne * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(DR_ERROR_DEVICE, "weight upload failed");
     } else {
-        for (DrEdgeNode &n : hs.sil_nodes) n.pad = n.total_length;
-        for (DrEdgeNode &n : hs.nrm_nodes) n.pad = n.total_length;
+        st.edge_w = nullptr;  // (the allocation stays with the structures)
     }
-    if ((!hs.sil_nodes.empty() &&
-         hipMemcpy((void *)s->view.sil_nodes, hs.sil_nodes.data(),
-                   hs.sil_nodes.size() * sizeof(DrEdgeNode), hipMemcpyHostToDevice) != hipSuccess) ||
-        (!hs.nrm_nodes.empty() &&
-         hipMemcpy((void *)s->view.nrm_nodes, hs.nrm_nodes.data(),
-                   hs.nrm_nodes.size() * sizeof(DrEdgeNode), hipMemcpyHostToDevice) != hipSuccess))
-        return fail(DR_ERROR_DEVICE, "edge tree upload failed");
+    // node lengths of the learned proposal, recomputed on the device with the
+    // tree refit (which leaves the rest of the trees as they are)
+    dr_status r = refit_edge_trees(s, nullptr);
+    if (r != DR_OK) return r;
+    if (hipStreamSynchronize(nullptr) != hipSuccess) return fail(DR_ERROR_DEVICE, "edge-tree refit failed");
     return DR_OK;
 }
 
@@ -1422,21 +1800,22 @@ dr_status dr_view_create(dr_scene *s, const dr_camera *c, dr_view **out) {
     if (!v) return fail(DR_ERROR_OUT_OF_MEMORY, "host allocation failed");
     v->scene = s;
     v->cam = drh::make_camera(*c);
-    std::vector<DrEdge> edges;
-    drh::Dist dist;
-    drh::primary_edges_for(s->hs, v->cam, edges, dist);
-    if (!upload_prim(edges, dist, v->bufs)) {
+    s->views.push_back(v);
+    dr_status st = refresh_views(s, nullptr);
+    if (st != DR_OK) {
         dr_view_destroy(v);
-        return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+        return st;
     }
-    v->num_prim_edges = (int)edges.size();
-    v->prim_func_int = dist.func_int;
     *out = v;
     return DR_OK;
 }
 
 void dr_view_destroy(dr_view *v) {
     if (!v) return;
+    if (v->scene) {
+        auto &vs = v->scene->views;
+        vs.erase(std::remove(vs.begin(), vs.end(), v), vs.end());
+    }
     for (int i = 0; i < 3; i++)
         if (v->bufs[i]) (void)hipFree(v->bufs[i]);
     delete v;
@@ -1563,7 +1942,7 @@ dr_status dr_test_tree_stack(int64_t ops, uint64_t seed, int64_t *out, void *str
     unsigned long long r[2];
     hipError_t e = hipMemcpyAsync(r, dev, sizeof(r), hipMemcpyDeviceToHost, hs);
     if (e == hipSuccess) e = hipStreamSynchronize(hs);
-    hipFree(dev);
+    (void)hipFree(dev);
     if (e != hipSuccess) return fail(DR_ERROR_DEVICE, hipGetErrorString(e));
     out[0] = (int64_t)r[0];
     out[1] = (int64_t)r[1];
@@ -1580,7 +1959,7 @@ dr_status dr_test_line_integral(int64_t num_samples, uint64_t seed, double *out,
     double sums[8];
     hipError_t e = hipMemcpyAsync(sums, dev, sizeof(sums), hipMemcpyDeviceToHost, hs);
     if (e == hipSuccess) e = hipStreamSynchronize(hs);
-    hipFree(dev);
+    (void)hipFree(dev);
     if (e != hipSuccess) return fail(DR_ERROR_DEVICE, hipGetErrorString(e));
     for (int k = 0; k < 4; k++) {
         double mean = sums[2 * k] / (double)num_samples;

@@ -105,7 +105,14 @@ struct BNode {
     int left = -1, right = -1, first = 0, count = 0;
 };
 
-int bvh_rec(std::vector<BNode> &nodes, std::vector<int> &order, std::vector<BPrim> &p, int b, int e) {
+// Binned SAH split; below depth kSahDepth the split is the object median on
+// the largest centroid axis, so the depth stays below kSahDepth + log2(n) and
+// the device traversal stack (DR_STACK entries, one per level) cannot fill.
+const int kSahDepth = 32;
+
+int bvh_rec(std::vector<BNode> &nodes, std::vector<int> &order, std::vector<BPrim> &p, int b, int e, int depth,
+            int &max_depth) {
+    max_depth = std::max(max_depth, depth);
     int id = (int)nodes.size();
     nodes.emplace_back();
     Box box, cb;
@@ -129,7 +136,10 @@ int bvh_rec(std::vector<BNode> &nodes, std::vector<int> &order, std::vector<BPri
     if (ext.y > ext[axis]) axis = 1;
     if (ext.z > ext[axis]) axis = 2;
     int mid = (b + e) / 2;
-    if (ext[axis] > 0.f) {
+    if (depth >= kSahDepth) {
+        std::nth_element(p.begin() + b, p.begin() + mid, p.begin() + e,
+                         [axis](const BPrim &x, const BPrim &y) { return x.c[axis] < y.c[axis]; });
+    } else if (ext[axis] > 0.f) {
         const int NB = 32;
         Box bb[NB];
         int bc[NB] = {0};
@@ -167,8 +177,8 @@ int bvh_rec(std::vector<BNode> &nodes, std::vector<int> &order, std::vector<BPri
             if (mid == b || mid == e) mid = (b + e) / 2;
         }
     }
-    int l = bvh_rec(nodes, order, p, b, mid);
-    int r = bvh_rec(nodes, order, p, mid, e);
+    int l = bvh_rec(nodes, order, p, b, mid, depth + 1, max_depth);
+    int r = bvh_rec(nodes, order, p, mid, e, depth + 1, max_depth);
     nodes[id].left = l;
     nodes[id].right = r;
     return id;
@@ -178,6 +188,7 @@ void build_bvh(HostScene &hs) {
     int nt = (int)hs.tri_shape.size();
     hs.nodes.clear();
     hs.tris.clear();
+    hs.bvh_depth = 0;
     if (nt == 0) return;
     std::vector<BPrim> p(nt);
     for (int g = 0; g < nt; g++) {
@@ -193,7 +204,7 @@ void build_bvh(HostScene &hs) {
     }
     std::vector<BNode> bn;
     std::vector<int> order;
-    bvh_rec(bn, order, p, 0, nt);
+    bvh_rec(bn, order, p, 0, nt, 0, hs.bvh_depth);
     // flatten into 2-wide nodes holding both child boxes; leaves are folded
     // into their parent.  A root that is itself a leaf gets an empty sibling.
     hs.tris.resize(nt);
@@ -279,7 +290,9 @@ struct EPrim {
 };
 
 int edge_rec(const std::vector<DrEdge> &edges, std::vector<EPrim> &p, int b, int e, bool directional,
-             std::vector<DrEdgeNode> &nodes, std::vector<DrEdge> &ordered, std::vector<int> &child1) {
+             std::vector<DrEdgeNode> &nodes, std::vector<DrEdge> &ordered, std::vector<int> &child1,
+             std::vector<V3> &on, int depth, int &max_depth) {
+    max_depth = std::max(max_depth, depth);
     int id = (int)nodes.size();
     nodes.emplace_back();
     child1.push_back(-1);
@@ -288,22 +301,10 @@ int edge_rec(const std::vector<DrEdge> &edges, std::vector<EPrim> &p, int b, int
         box.grow(p[i].box);
         if (directional) dbox.grow(p[i].dbox);
     }
-    V3 nsum = V3{0.f, 0.f, 0.f};
-    for (int i = b; i < e; i++) {
-        nsum = nsum + p[i].n0;
-        nsum = nsum + p[i].n1;
-    }
-    V3 axis = normalize(nsum);
-    float cmin = 1.f;
-    for (int i = b; i < e; i++) {
-        cmin = std::min(cmin, dot(p[i].n0, axis));
-        cmin = std::min(cmin, dot(p[i].n1, axis));
-    }
+    // the normal cone is set bottom-up once the tree is built
     DrEdgeNode &N = nodes[id];
     N.lo[0] = box.lo.x; N.lo[1] = box.lo.y; N.lo[2] = box.lo.z;
     N.hi[0] = box.hi.x; N.hi[1] = box.hi.y; N.hi[2] = box.hi.z;
-    N.axis[0] = axis.x; N.axis[1] = axis.y; N.axis[2] = axis.z;
-    N.angle = std::acos(cmin);
     N.nprims = 0;
     N.offset = 0;
     N.total_length = 0.f;
@@ -312,7 +313,11 @@ int edge_rec(const std::vector<DrEdge> &edges, std::vector<EPrim> &p, int b, int
     auto leaf = [&]() {
         nodes[id].offset = (int)ordered.size();
         nodes[id].nprims = n;
-        for (int i = b; i < e; i++) ordered.push_back(edges[p[i].id]);
+        for (int i = b; i < e; i++) {
+            ordered.push_back(edges[p[i].id]);
+            on.push_back(p[i].n0);
+            on.push_back(p[i].n1);
+        }
     };
     if (n == 1) {
         leaf();
@@ -346,17 +351,19 @@ int edge_rec(const std::vector<DrEdge> &edges, std::vector<EPrim> &p, int b, int
     std::nth_element(p.begin() + b, p.begin() + mid, p.begin() + e, [dim, spatial](const EPrim &x, const EPrim &y) {
         return spatial ? x.c[dim] < y.c[dim] : x.dc[dim] < y.dc[dim];
     });
-    edge_rec(edges, p, b, mid, directional, nodes, ordered, child1);
-    int r = edge_rec(edges, p, mid, e, directional, nodes, ordered, child1);
+    edge_rec(edges, p, b, mid, directional, nodes, ordered, child1, on, depth + 1, max_depth);
+    int r = edge_rec(edges, p, mid, e, directional, nodes, ordered, child1, on, depth + 1, max_depth);
     child1[id] = r;
     return id;
 }
 
-void build_edge_tree(const HostScene &hs, const std::vector<DrEdge> &edges, bool directional,
-                     std::vector<DrEdgeNode> &nodes, std::vector<DrEdge> &ordered) {
+// Returns the depth of the deepest node (root: 0).  Median splits keep it at
+// most ceil(log2(edges)).
+int build_edge_tree(const HostScene &hs, const std::vector<DrEdge> &edges, bool directional,
+                    std::vector<DrEdgeNode> &nodes, std::vector<DrEdge> &ordered) {
     nodes.clear();
     ordered.clear();
-    if (edges.empty()) return;
+    if (edges.empty()) return 0;
     std::vector<EPrim> p(edges.size());
     Box world;
     for (size_t i = 0; i < edges.size(); i++) {
@@ -383,23 +390,52 @@ void build_edge_tree(const HostScene &hs, const std::vector<DrEdge> &edges, bool
         }
     }
     std::vector<int> child1;
-    edge_rec(edges, p, 0, (int)edges.size(), directional, nodes, ordered, child1);
-    for (int i = (int)nodes.size() - 1; i >= 0; i--) {
+    std::vector<V3> on;  // per ordered edge: its two face normals
+    int depth = 0;
+    edge_rec(edges, p, 0, (int)edges.size(), directional, nodes, ordered, child1, on, 0, depth);
+    // Bottom-up (children follow parents): interior offset = second child,
+    // total length, the sum of the face normals (leaf: in edge order;
+    // interior: left + right) and the node's range of ordered edges; the
+    // cone axis is the normalised sum, its angle the acos of the least cosine
+    // to it over the node's edges.  The device refit (dr_refit.h) repeats
+    // these steps in the same order, so a refit to the same vertices
+    // reproduces this tree.
+    int nn = (int)nodes.size();
+    std::vector<V3> nsum(nn);
+    std::vector<int> first(nn), last(nn);
+    for (int i = nn - 1; i >= 0; i--) {
         DrEdgeNode &n = nodes[i];
         if (n.nprims > 0) {
             n.total_length = 0.f;
+            nsum[i] = V3{0.f, 0.f, 0.f};
             for (int k = 0; k < n.nprims; k++) {
                 const DrEdge &ed = ordered[n.offset + k];
                 n.total_length += length(getv(hs, ed.shape, ed.v1) - getv(hs, ed.shape, ed.v0));
+                nsum[i] = nsum[i] + on[2 * (n.offset + k)];
+                nsum[i] = nsum[i] + on[2 * (n.offset + k) + 1];
             }
+            first[i] = n.offset;
+            last[i] = n.offset + n.nprims;
         } else {
             n.offset = child1[i];
             n.total_length = nodes[i + 1].total_length + nodes[n.offset].total_length;
+            nsum[i] = nsum[i + 1] + nsum[n.offset];
+            first[i] = first[i + 1];
+            last[i] = last[n.offset];
         }
+        V3 axis = normalize(nsum[i]);
+        float cmin = 1.f;
+        for (int k = first[i]; k < last[i]; k++) {
+            cmin = std::min(cmin, dot(on[2 * k], axis));
+            cmin = std::min(cmin, dot(on[2 * k + 1], axis));
+        }
+        n.axis[0] = axis.x; n.axis[1] = axis.y; n.axis[2] = axis.z;
+        n.angle = std::acos(cmin);
     }
     // sampling length of the learned proposal: the plain length until
     // weights are set
     for (DrEdgeNode &n : nodes) n.pad = n.total_length;
+    return depth;
 }
 
 
@@ -474,6 +510,18 @@ bool is_silhouette(const HostScene &hs, V3 p, const DrEdge &e) {
     bool f0 = dot(p - v0, n0) > 0.f && dot(p - v1, n0) > 0.f && dot(p - a, n0) > 0.f;
     bool f1 = dot(p - v0, n1) > 0.f && dot(p - v1, n1) > 0.f && dot(p - b, n1) > 0.f;
     return f0 != f1;
+}
+
+// reference scene.cpp:80-96: drop degenerate edges and edges between
+// coplanar faces
+bool edge_kept(const HostScene &hs, const DrEdge &e) {
+    V3 dv = getv(hs, e.shape, e.v0) - getv(hs, e.shape, e.v1);
+    if (std::fabs(dv.x) + std::fabs(dv.y) + std::fabs(dv.z) < 1e-10f) return false;
+    if (e.f1 != -1) {
+        float ang = std::fabs(dot(face_normal(hs, e.shape, e.f0), face_normal(hs, e.shape, e.f1)));
+        if (ang == 1.f || ang == -1.f) return false;
+    }
+    return true;
 }
 
 }  // namespace
@@ -591,20 +639,36 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
         tbase += s.num_triangles;
         hs.shapes.push_back(o);
     }
-    // lights (reference scene.cpp:24-52)
-    std::vector<float> imp;
-    hs.light_total_intensity = 0.f;
+    hs.light_shapes.clear();
     for (int i = 0; i < d->num_lights; i++) {
         const dr_light &l = d->lights[i];
         if (l.shape_id < 0 || l.shape_id >= d->num_shapes) return "light shape id out of range";
-        DrShape &sh = hs.shapes[l.shape_id];
-        if (sh.nt == 0) return "light shape has no triangles";
-        sh.light = i;
+        if (hs.shapes[l.shape_id].nt == 0) return "light shape has no triangles";
+        hs.light_shapes.push_back(l.shape_id);
+        hs.light_I.insert(hs.light_I.end(), l.intensity, l.intensity + 3);
+    }
+    hs.light_goff = ioff;
+    build_lights(hs);
+    return build_structures(hs);
+}
+
+// lights (reference scene.cpp:24-52): selection pmf by luminance, per light a
+// triangle pmf by area, total area and bounding sphere
+void build_lights(HostScene &hs) {
+    std::vector<float> imp;
+    hs.lights.clear();
+    hs.light_func.clear();
+    hs.light_cdf.clear();
+    hs.light_total_intensity = 0.f;
+    for (size_t i = 0; i < hs.light_shapes.size(); i++) {
+        int sid = hs.light_shapes[i];
+        DrShape &sh = hs.shapes[sid];
+        sh.light = (int)i;
         DrLight o;
         std::memset(&o, 0, sizeof(o));
-        o.shape = l.shape_id;
-        for (int k = 0; k < 3; k++) o.I[k] = l.intensity[k];
-        float lum = luminance(V3{l.intensity[0], l.intensity[1], l.intensity[2]});
+        o.shape = sid;
+        for (int k = 0; k < 3; k++) o.I[k] = hs.light_I[3 * i + k];
+        float lum = luminance(V3{o.I[0], o.I[1], o.I[2]});
         o.lum = lum;
         imp.push_back(lum);
         hs.light_total_intensity += lum;
@@ -612,9 +676,9 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
         Box bounds;
         float asum = 0.f;
         for (int t = 0; t < sh.nt; t++) {
-            V3 a = getv(hs, l.shape_id, tri_index(hs, l.shape_id, t, 0));
-            V3 b = getv(hs, l.shape_id, tri_index(hs, l.shape_id, t, 1));
-            V3 c = getv(hs, l.shape_id, tri_index(hs, l.shape_id, t, 2));
+            V3 a = getv(hs, sid, tri_index(hs, sid, t, 0));
+            V3 b = getv(hs, sid, tri_index(hs, sid, t, 1));
+            V3 c = getv(hs, sid, tri_index(hs, sid, t, 2));
             float ar = 0.5f * length(cross(b - a, c - a));
             asum += ar;
             areas.push_back(ar);
@@ -633,12 +697,46 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
         V3 c = (bounds.lo + bounds.hi) / 2.f;
         o.sphere_c[0] = c.x; o.sphere_c[1] = c.y; o.sphere_c[2] = c.z;
         o.sphere_r = length(bounds.hi - c);
-        o.grad_off = (long long)ioff[i];
+        o.grad_off = (long long)hs.light_goff[i];
         hs.lights.push_back(o);
     }
     hs.light_sel = make_dist(imp);
+}
+
+namespace {
+// Nodes of a tree grouped by depth, deepest level first (a bottom-up refit
+// handles one level at a time).  children(i, out) lists node i's children.
+template <class Children>
+void tree_levels(int n, Children children, std::vector<int> &order, std::vector<int> &off) {
+    order.clear();
+    off.clear();
+    if (n == 0) return;
+    std::vector<int> depth(n, 0);
+    int maxd = 0;
+    for (int i = 0; i < n; i++) {  // preorder: a parent precedes its children
+        int c[2];
+        int k = children(i, c);
+        for (int j = 0; j < k; j++) {
+            depth[c[j]] = depth[i] + 1;
+            maxd = std::max(maxd, depth[i] + 1);
+        }
+    }
+    std::vector<int> cnt(maxd + 1, 0);
+    for (int i = 0; i < n; i++) cnt[maxd - depth[i]]++;
+    off.assign(maxd + 2, 0);
+    for (int l = 0; l <= maxd; l++) off[l + 1] = off[l] + cnt[l];
+    order.assign(n, 0);
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int i = 0; i < n; i++) order[pos[maxd - depth[i]]++] = i;
+}
+}  // namespace
+
+// Triangle BVH, mesh edges (reference scene.cpp:54-97), edge trees, their
+// per-edge geometry records, the primary-edge distribution and the refit
+// tables, from the shapes' current vertices.
+std::string build_structures(HostScene &hs) {
     build_bvh(hs);
-    // edges (reference scene.cpp:54-97)
+    // candidate edges, sorted by (shape, v0, v1) as the reference's std::map
     std::map<std::tuple<int, int, int>, DrEdge> emap;
     for (int i = 0; i < (int)hs.shapes.size(); i++) {
         const DrShape &s = hs.shapes[i];
@@ -657,15 +755,14 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
         }
     }
     hs.edges.clear();
+    hs.cand_edges.clear();
+    hs.cand_keep.clear();
     for (auto &kv : emap) {
         const DrEdge &e = kv.second;
-        V3 dv = getv(hs, e.shape, e.v0) - getv(hs, e.shape, e.v1);
-        if (std::fabs(dv.x) + std::fabs(dv.y) + std::fabs(dv.z) < 1e-10f) continue;
-        if (e.f1 != -1) {
-            float ang = std::fabs(dot(face_normal(hs, e.shape, e.f0), face_normal(hs, e.shape, e.f1)));
-            if (ang == 1.f || ang == -1.f) continue;
-        }
-        hs.edges.push_back(e);
+        bool keep = edge_kept(hs, e);
+        hs.cand_edges.push_back(e);
+        hs.cand_keep.push_back(keep ? 1 : 0);
+        if (keep) hs.edges.push_back(e);
     }
     build_primary_edges(hs);
     std::vector<DrEdge> sil, nrm;
@@ -673,13 +770,56 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
         if (e.f0 == -1 || e.f1 == -1 || !hs.shapes[e.shape].has_n) sil.push_back(e);
         else nrm.push_back(e);
     }
-    build_edge_tree(hs, sil, false, hs.sil_nodes, hs.sil_edges);
-    build_edge_tree(hs, nrm, true, hs.nrm_nodes, hs.nrm_edges);
-    // a traversal stack entry holds a node index or an edge count in 29 bits
-    if (hs.sil_nodes.size() >= (1u << 29) || hs.nrm_nodes.size() >= (1u << 29) || hs.edges.size() >= (1u << 29))
-        return "too many edges for the edge trees (at most 2^29 edges and nodes each)";
-    // the vertices the silhouette test of a normal-tree edge reads
-    // (edge.h:59-79): endpoints, and each face's vertex off the edge
+    // edge ordinals in the random-number streams of edge samples have 27 bits
+    // (and a traversal stack entry holds a node index or an edge count in 29)
+    if (hs.edges.size() >= (1u << DR_EDGE_ORDINAL_BITS))
+        return "too many edges for the edge trees (at most 2^27 - 1)";
+    hs.sil_depth = build_edge_tree(hs, sil, false, hs.sil_nodes, hs.sil_edges);
+    hs.nrm_depth = build_edge_tree(hs, nrm, true, hs.nrm_nodes, hs.nrm_edges);
+    // a depth-first walk holds at most one pending sibling per level plus the
+    // two children just pushed: depth + 1 entries
+    if (hs.sil_depth + 1 > DR_TREE_STACK || hs.nrm_depth + 1 > DR_TREE_STACK)
+        return "edge tree deeper than the traversal stack";
+    // the ray traversal stack holds at most one far child per interior level
+    if (hs.bvh_depth > DR_STACK) return "triangle BVH deeper than the traversal stack";
+    build_geo_records(hs);
+    // refit tables: levels of the three trees, edge ranges of the edge-tree nodes
+    tree_levels((int)hs.nodes.size(), [&](int i, int *c) {
+        int k = 0;
+        const DrNode &n = hs.nodes[i];
+        for (int sl = 0; sl < 2; sl++)
+            if (n.d[2 + sl] == 0 && n.d[sl] > 0) c[k++] = n.d[sl];
+        return k;
+    }, hs.bvh_lv, hs.bvh_lv_off);
+    auto etree = [&](const std::vector<DrEdgeNode> &nodes, std::vector<int> &lv, std::vector<int> &off,
+                     std::vector<int> &rng) {
+        tree_levels((int)nodes.size(), [&](int i, int *c) {
+            if (nodes[i].nprims > 0) return 0;
+            c[0] = i + 1;
+            c[1] = nodes[i].offset;
+            return 2;
+        }, lv, off);
+        int nn = (int)nodes.size();
+        rng.assign(2 * (size_t)nn, 0);
+        for (int i = nn - 1; i >= 0; i--) {
+            if (nodes[i].nprims > 0) {
+                rng[2 * i] = nodes[i].offset;
+                rng[2 * i + 1] = nodes[i].offset + nodes[i].nprims;
+            } else {
+                rng[2 * i] = rng[2 * (i + 1)];
+                rng[2 * i + 1] = rng[2 * nodes[i].offset + 1];
+            }
+        }
+    };
+    etree(hs.sil_nodes, hs.sil_lv, hs.sil_lv_off, hs.sil_rng);
+    etree(hs.nrm_nodes, hs.nrm_lv, hs.nrm_lv_off, hs.nrm_rng);
+    return "";
+}
+
+// the vertices the silhouette test of a normal-tree edge reads (edge.h:59-79):
+// endpoints, and each face's vertex off the edge; the endpoints of
+// silhouette-tree edges
+void build_geo_records(HostScene &hs) {
     hs.nrm_geo.assign(16 * hs.nrm_edges.size(), 0.f);
     for (size_t i = 0; i < hs.nrm_edges.size(); i++) {
         const DrEdge &e = hs.nrm_edges[i];
@@ -701,7 +841,6 @@ std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
         for (int k = 0; k < 2; k++)
             for (int c = 0; c < 3; c++) hs.sil_geo[16 * i + 4 * k + c] = hs.verts[3 * (size_t)gv[k] + c];
     }
-    return "";
 }
 
 void set_learned_lengths(const HostScene &hs, std::vector<DrEdgeNode> &nodes, const std::vector<DrEdge> &ordered,

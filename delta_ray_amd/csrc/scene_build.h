@@ -38,10 +38,28 @@ struct HostScene {
     std::vector<DrEdge> prim_edges;
     Dist prim;
     size_t grad_size = 0;
+    int bvh_depth = 0, sil_depth = 0, nrm_depth = 0;  // deepest node (root 0)
+    // lights as given: shape, intensity (3 per light), gradient offset
+    std::vector<int> light_shapes;
+    std::vector<float> light_I;
+    std::vector<size_t> light_goff;
+    // every mesh edge before the degenerate / coplanar filter, in the order
+    // of `edges`, and whether the filter kept it
+    std::vector<DrEdge> cand_edges;
+    std::vector<int> cand_keep;
+    // refit tables: nodes by depth, deepest level first, with level offsets;
+    // per edge-tree node the range [first, last) of its ordered edges
+    std::vector<int> bvh_lv, bvh_lv_off, sil_lv, sil_lv_off, nrm_lv, nrm_lv_off;
+    std::vector<int> sil_rng, nrm_rng;
 };
 
 // Returns an empty string on success, else the reason for rejecting desc.
 std::string build_host_scene(const dr_scene_desc *desc, HostScene &hs);
+// From the current vertices: the light distributions / the BVH, edges, edge
+// trees, geometry records, primary edges and refit tables (a rebuild)
+void build_lights(HostScene &hs);
+std::string build_structures(HostScene &hs);
+void build_geo_records(HostScene &hs);
 DrCamera make_camera(const dr_camera &c);
 void set_camera(HostScene &hs, const dr_camera &c);
 // Primary-edge list and distribution (pmf proportional to the clipped screen

@@ -137,21 +137,9 @@ class SceneDesc:
         keep = []
         d = _abi.dr_scene_desc()
         d.camera = SceneDesc.camera_struct(camera)
-        mats = (_abi.dr_material * max(1, len(materials)))()
-        mat_index = {}
-        for i, m in enumerate(materials):
-            mat_index[id(m)] = i
-            mm = mats[i]
-            mm.diffuse_reflectance, a = _texture(m.diffuse_reflectance, 3)
-            keep.append(a)
-            mm.specular_reflectance, a = _texture(m.specular_reflectance, 3)
-            keep.append(a)
-            mm.roughness, a = _texture(m.roughness, 1)
-            keep.append(a)
-            mm.diffuse_uv_scale[:] = _abi.f32(m.diffuse_uv_scale).reshape(-1)[:2].tolist()
-            mm.specular_uv_scale[:] = _abi.f32(m.specular_uv_scale).reshape(-1)[:2].tolist()
-            mm.roughness_uv_scale[:] = _abi.f32(m.roughness_uv_scale).reshape(-1)[:2].tolist()
-            mm.two_sided = 1 if m.two_sided else 0
+        mats, mkeep = _material_structs(materials)
+        keep += mkeep
+        mat_index = {id(m): i for i, m in enumerate(materials)}
         light_index = {id(l): i for i, l in enumerate(lights)}
         shape_index = {id(s): i for i, s in enumerate(shapes)}
         shs = (_abi.dr_shape * max(1, len(shapes)))()
@@ -198,6 +186,25 @@ class SceneDesc:
         self._keep = (keep, mats, shs, lts)
         self.shapes, self.materials, self.lights = shapes, materials, lights
         self.layout = grad_layout(shapes, materials, lights)
+
+
+def _material_structs(materials):
+    """dr_material array for Material objects, and the arrays it points into"""
+    keep = []
+    mats = (_abi.dr_material * max(1, len(materials)))()
+    for i, m in enumerate(materials):
+        mm = mats[i]
+        mm.diffuse_reflectance, a = _texture(m.diffuse_reflectance, 3)
+        keep.append(a)
+        mm.specular_reflectance, a = _texture(m.specular_reflectance, 3)
+        keep.append(a)
+        mm.roughness, a = _texture(m.roughness, 1)
+        keep.append(a)
+        mm.diffuse_uv_scale[:] = _abi.f32(m.diffuse_uv_scale).reshape(-1)[:2].tolist()
+        mm.specular_uv_scale[:] = _abi.f32(m.specular_uv_scale).reshape(-1)[:2].tolist()
+        mm.roughness_uv_scale[:] = _abi.f32(m.roughness_uv_scale).reshape(-1)[:2].tolist()
+        mm.two_sided = 1 if m.two_sided else 0
+    return mats, keep
 
 
 def _camera_struct(camera):
@@ -322,6 +329,52 @@ class DeviceScene:
             ctypes.c_void_p(dx_image.data_ptr() if dx_image is not None else None), s),
             "dr_render_adjoint")
 
+    def update_vertices(self, updates, stream=None):
+        """updates: [(shape_index, vertices, normals or None)], torch float32
+        CUDA tensors [N, 3] of the shapes' sizes (dr_scene_update_vertices:
+        device refit, or a host rebuild when the edge set changes)."""
+        import torch
+        arr = (_abi.dr_shape_update * max(1, len(updates)))()
+        keep = []
+        for i, (sid, v, n) in enumerate(updates):
+            v = v.detach().to(torch.float32).contiguous()
+            if not v.is_cuda:
+                v = v.cuda()
+            keep.append(v)
+            arr[i].shape_id = int(sid)
+            arr[i].vertices = v.data_ptr()
+            if n is not None:
+                n = n.detach().to(torch.float32).contiguous()
+                if not n.is_cuda:
+                    n = n.cuda()
+                keep.append(n)
+                arr[i].normals = n.data_ptr()
+            else:
+                arr[i].normals = None
+        _abi.check(self.lib, self.lib.dr_scene_update_vertices(self.handle, arr, len(updates), _stream_ptr(stream)),
+                   "dr_scene_update_vertices")
+        del keep
+
+    def update_materials(self, materials):
+        """new parameters of the scene's materials (same texture sizes)"""
+        mats, keep = _material_structs(materials)
+        _abi.check(self.lib, self.lib.dr_scene_update_materials(self.handle, mats, len(materials)),
+                   "dr_scene_update_materials")
+        del keep
+
+    def update_lights(self, shape_ids, intensities):
+        """new intensities of the scene's lights (same shapes)"""
+        lts = (_abi.dr_light * max(1, len(shape_ids)))()
+        for i, (sid, inten) in enumerate(zip(shape_ids, intensities)):
+            lts[i].shape_id = int(sid)
+            lts[i].intensity[:] = _abi.f32(inten).reshape(-1)[:3].tolist()
+        _abi.check(self.lib, self.lib.dr_scene_update_lights(self.handle, lts, len(shape_ids)),
+                   "dr_scene_update_lights")
+
+    def set_camera(self, camera):
+        c = _camera_struct(camera)
+        _abi.check(self.lib, self.lib.dr_scene_set_camera(self.handle, ctypes.byref(c)), "dr_scene_set_camera")
+
     def intersect(self, rays, hits, any_hit=False, stream=None):
         s = _stream_ptr(stream)
         _abi.check(self.lib, self.lib.dr_intersect(self.handle, ctypes.c_void_p(rays.data_ptr()),
@@ -336,7 +389,8 @@ class DeviceScene:
         _abi.check(self.lib, self.lib.dr_scene_stats(self.handle, mask, ctypes.byref(st)),
                    "dr_scene_stats")
         out = {k: getattr(st, k) for k in ("rays", "nodes_visited", "tris_visited", "secondary_edges",
-                                           "primary_edge_samples", "edge_nodes_visited", "edges_tested")}
+                                           "primary_edge_samples", "edge_nodes_visited", "edges_tested",
+                                           "stack_overflows")}
         for i, k in enumerate(_abi.KERNELS):
             out[k + "_ms"] = st.kernel_ms[i]
             out[k + "_launches"] = st.kernel_launches[i]

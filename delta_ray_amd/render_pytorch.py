@@ -9,8 +9,12 @@ the four camera matrices, material parameters, vertices, 2-D normals and light
 intensities.
 
 Differences that follow from running on the GPU:
-* the scene is uploaded once in forward and kept for backward (the reference
-  builds it twice per step);
+* the scene is uploaded once and kept across calls (the reference builds it
+  twice per step): while the scene's topology (counts, triangle indices, uvs,
+  texture sizes, light shapes) stays the same, a call only sends what changed
+  -- new vertex positions are refitted on the device from the caller's CUDA
+  tensors (dr_scene_update_vertices), materials, light intensities and the
+  camera are replaced -- and the device workspace is reused;
 * the image is returned on the GPU when the vertices live there, otherwise as a
   CPU tensor like the reference's;
 * the backward render uses the same random numbers as the forward render, so
@@ -32,6 +36,49 @@ def _np(t):
     if isinstance(t, torch.Tensor):
         return t.detach().to("cpu").numpy()
     return np.asarray(t)
+
+
+def _same(a, b):
+    """a equals the cached copy b (contents, compared where a lives)"""
+    if a is None or b is None:
+        return a is None and b is None
+    if isinstance(a, torch.Tensor):
+        if not isinstance(b, torch.Tensor) or tuple(a.shape) != tuple(b.shape):
+            return False
+        return bool(torch.equal(a.detach(), b.to(a.device)))
+    a = np.asarray(a)
+    b = b.cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
+def _copy(a):
+    if a is None:
+        return None
+    if isinstance(a, torch.Tensor):
+        return a.detach().clone()
+    return np.array(a, copy=True)
+
+
+class _SceneCache:
+    """The device scene of the last call and copies of what it holds now:
+    topology (counts, array shapes, ids), triangle indices and uvs (compared by
+    content), vertices and normals, camera."""
+    key = None
+    scene = None
+    indices = uvs = verts = normals = None
+    camera = None
+
+
+def _key(cam_scalars, mats, shapes, lights):
+    key = [len(mats), len(shapes), len(lights), bool(cam_scalars[3])]
+    for m in mats:
+        key.append(tuple(tuple(np.shape(a)) for a in m[:3]))
+    for v, ind, uv, nrm, mid in shapes:
+        key.append((tuple(np.shape(v)), tuple(np.shape(ind)), uv is None, nrm is None,
+                    np.ndim(nrm) if nrm is not None else -1, int(mid)))
+    for sid, _ in lights:
+        key.append(int(sid))
+    return tuple(key)
 
 
 class RenderFunction(torch.autograd.Function):
@@ -70,12 +117,38 @@ class RenderFunction(torch.autograd.Function):
         return cam, (fov, aspect, clip_near, fisheye), mats, shapes, lights, resolution, num_samples, max_bounces
 
     @staticmethod
-    def forward(ctx, seed, *args):
-        cam, (fov, aspect, clip_near, fisheye), mats, shapes, lights, resolution, spp, nb = \
-            RenderFunction._unpack(args)
+    def _scene(cam, cam_scalars, mats, shapes, lights):
+        """the cached device scene brought up to date (vertices refitted on the
+        device, materials, lights and camera replaced), or a new scene when the
+        topology changed"""
+        fov, aspect, clip_near, fisheye = cam_scalars
         camera = dr.Camera(*[_np(m).astype(np.float32) for m in cam], float(fov), float(aspect),
                            float(clip_near), bool(fisheye))
         materials = [dr.Material(*[_np(a).astype(np.float32) for a in m[:6]], bool(m[6])) for m in mats]
+        key = _key(cam_scalars, mats, shapes, lights)
+        C = _SceneCache
+        cam_now = [camera.cam_to_world, camera.world_to_cam, camera.sample_to_cam, camera.cam_to_sample,
+                   camera.fov_factor, camera.aspect_ratio, camera.clip_near]
+        same = (C.scene is not None and C.key == key and
+                all(_same(sh[1], C.indices[i]) and _same(sh[2], C.uvs[i]) for i, sh in enumerate(shapes)))
+        if same:
+            scene = C.scene
+            ups = []
+            for i, (v, ind, uv, nrm, mid) in enumerate(shapes):
+                n2 = nrm if (nrm is not None and np.ndim(nrm) == 2) else None
+                if not _same(v, C.verts[i]) or (n2 is not None and not _same(n2, C.normals[i])):
+                    vt = v if isinstance(v, torch.Tensor) else torch.as_tensor(_np(v))
+                    nt = None if n2 is None else (n2 if isinstance(n2, torch.Tensor) else torch.as_tensor(_np(n2)))
+                    ups.append((i, vt, nt))
+                    C.verts[i], C.normals[i] = _copy(v), _copy(n2)
+            if ups:
+                scene.update_vertices(ups)
+            scene.update_materials(materials)
+            scene.update_lights([int(sid) for sid, _ in lights], [_np(inten) for _, inten in lights])
+            if any(not np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(cam_now, C.camera)):
+                scene.set_camera(camera)
+                C.camera = [np.array(x, copy=True) for x in cam_now]
+            return scene
         shape_objs = []
         for v, ind, uv, nrm, mid in shapes:
             shape_objs.append(dr.Shape(_np(v).astype(np.float32), _np(ind).astype(np.int32),
@@ -88,11 +161,23 @@ class RenderFunction(torch.autograd.Function):
             shape_objs[sid].light = l
             light_objs.append(l)
         scene = dr.DeviceScene(camera, shape_objs, materials, light_objs)
+        C.scene, C.key = scene, key
+        C.indices = [_copy(sh[1]) for sh in shapes]
+        C.uvs = [_copy(sh[2]) for sh in shapes]
+        C.verts = [_copy(sh[0]) for sh in shapes]
+        C.normals = [_copy(sh[3]) if (sh[3] is not None and np.ndim(sh[3]) == 2) else None for sh in shapes]
+        C.camera = [np.array(x, copy=True) for x in cam_now]
+        return scene
+
+    @staticmethod
+    def forward(ctx, seed, *args):
+        cam, cam_scalars, mats, shapes, lights, resolution, spp, nb = RenderFunction._unpack(args)
+        scene = RenderFunction._scene(cam, cam_scalars, mats, shapes, lights)
         W, H = int(resolution[0]), int(resolution[1])
         dev = torch.device("cuda")
         image = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
         scene.render(dr.DeviceScene.options(W, H, spp, nb, seed, RenderFunction.flags), image)
-        ctx.scene, ctx.seed, ctx.size, ctx.spp, ctx.nb = scene, seed, (W, H), spp, nb
+        ctx.seed, ctx.size, ctx.spp, ctx.nb = seed, (W, H), spp, nb
         ctx.args = args
         on_gpu = len(shapes) > 0 and isinstance(shapes[0][0], torch.Tensor) and shapes[0][0].is_cuda
         ctx.on_gpu = on_gpu
@@ -100,7 +185,10 @@ class RenderFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_img):
-        scene = ctx.scene
+        # the cached scene may have been moved on by later calls: bring it back
+        # to this call's arguments (a no-op when nothing changed since)
+        cam, cam_scalars, mats, shapes, lights, _, _, _ = RenderFunction._unpack(ctx.args)
+        scene = RenderFunction._scene(cam, cam_scalars, mats, shapes, lights)
         W, H = ctx.size
         dev = torch.device("cuda")
         d_image = grad_img.detach().to(dev, torch.float32).contiguous()

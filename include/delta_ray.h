@@ -227,9 +227,40 @@ typedef struct dr_scene dr_scene;
 dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out);
 void dr_scene_destroy(dr_scene *scene);
 
-/* Replace the camera (no rebuild of geometry; the primary-edge distribution,
- * which depends on the camera, is recomputed on the device). */
+/* Replace the camera (no rebuild of geometry; the primary-edge list and its
+ * distribution, which depend on the camera, are recomputed on the device).
+ * Synchronises the default stream. */
 dr_status dr_scene_set_camera(dr_scene *scene, const dr_camera *camera);
+
+/* New vertex positions (and shading normals) of shapes, from DEVICE buffers:
+ * vertices [num_vertices, 3] of the shape's size, normals the same or NULL
+ * (keep).  Replaces what the reference does by building a new Scene in every
+ * render() call (scene.cpp:19-136, edge_tree.cpp:263-316, delta_ray.cpp:
+ * 1027-1030) with a refit on the device: the triangle records and BVH boxes,
+ * the edge trees' boxes, normal cones, lengths and geometry records, and the
+ * primary-edge distributions of the scene's camera and views are recomputed
+ * for the new vertices in the trees' existing topology, with the builder's
+ * arithmetic (a refit to the vertices a scene was built from reproduces it).
+ * When the new positions change which mesh edges the degenerate / coplanar
+ * filter keeps (scene.cpp:80-96), the BVH, edges and edge trees are rebuilt
+ * on the host instead (learned edge weights and edge statistics are then
+ * cleared).  Light distributions are recomputed when an emissive shape
+ * moves.  Ordered on `stream`; synchronises it (one read of the filter's
+ * verdict and of the primary-edge count). */
+typedef struct dr_shape_update {
+    int shape_id;
+    const float *vertices;
+    const float *normals;
+} dr_shape_update;
+dr_status dr_scene_update_vertices(dr_scene *scene, const dr_shape_update *updates, int num_updates,
+                                   void *stream);
+
+/* New material parameters (HOST arrays of the same sizes as the scene's) and
+ * new light intensities (same lights, same shapes): what the reference reads
+ * from its Material / Light objects at every render() call
+ * (delta_ray.cpp:1027-1030, scene.cpp:24-52). */
+dr_status dr_scene_update_materials(dr_scene *scene, const dr_material *materials, int num_materials);
+dr_status dr_scene_update_lights(dr_scene *scene, const dr_light *lights, int num_lights);
 
 /* Number of floats of the gradient buffer for this scene. */
 size_t dr_scene_grad_size(const dr_scene *scene);
@@ -242,8 +273,8 @@ dr_status dr_render(dr_scene *scene, const dr_render_options *opts,
 /* Forward + adjoint render (reference render() with d_image [H, W, 3]:
  * The secondary-edge stage appends the selected edges of a batch of pixel
  * samples to a queue whose length the host reads (one stream synchronisation
- * per batch of 2^22 samples); a queue that overflows is grown and the
- * deterministic selection is repeated.
+ * per batch of 2^22 samples); a queue that overflows is grown and only the
+ * deterministic selection of that batch is repeated.
  * pass 1 with render_sample, delta_ray.cpp:694-1005, then pass 2, the
  * primary-edge samples, delta_ray.cpp:1468-1550).  d_image: device [H,W,3].
  * image: device [H,W,3] (written).  grads: device, dr_scene_grad_size floats
@@ -283,6 +314,8 @@ typedef struct dr_stats {
     uint64_t primary_edge_samples;
     uint64_t edge_nodes_visited;           /* interior edge-tree nodes expanded */
     uint64_t edges_tested;                 /* leaf edges given the silhouette test */
+    uint64_t stack_overflows;              /* traversal stack pushes that found the stack full (0: the
+                                            * scene builder bounds tree depths by the stack sizes) */
     double kernel_ms[DR_K_COUNT];          /* summed kernel times */
     uint64_t kernel_launches[DR_K_COUNT];
 } dr_stats;
@@ -298,6 +331,8 @@ typedef struct dr_scene_info {
     int silhouette_tree_nodes;
     int normal_tree_nodes;
     double build_ms;
+    int refits;              /* dr_scene_update_vertices calls refitted on the device */
+    int rebuilds;            /* ... and rebuilt on the host (the edge set changed) */
 } dr_scene_info;
 dr_status dr_scene_get_info(const dr_scene *scene, dr_scene_info *out);
 

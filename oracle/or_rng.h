@@ -78,6 +78,10 @@ struct Philox {
 static const uint32_t STREAM_MAIN = 0;
 inline uint32_t stream_point(int b) { return 1u + (uint32_t)b; }
 inline uint32_t stream_point_all(int b) { return 0x100u + (uint32_t)b; }
-inline uint32_t stream_edge(int b, int j) { return (1u << 20) | ((uint32_t)b << 16) | (uint32_t)(j & 0xffff); }
+// bit 31: an edge stream; bits 28-30: bounce; bit 27: the "all" pass (bit 30
+// of the ordinal); bits 0-26: the ordinal's count
+inline uint32_t stream_edge(int b, int j) {
+    return 0x80000000u | ((uint32_t)b << 28) | ((((uint32_t)j >> 30) & 1u) << 27) | ((uint32_t)j & 0x7ffffffu);
+}
 
 }  // namespace oracle

@@ -239,7 +239,7 @@ struct EPrim {
 
 int build_edge_rec(const std::vector<Edge> &edges, std::vector<EPrim> &p, int b, int e,
                    bool directional, std::vector<EdgeNode> &nodes, std::vector<Edge> &ordered,
-                   std::vector<int> &child1) {
+                   std::vector<int> &child1, std::vector<V3f> &on) {
     int id = (int)nodes.size();
     nodes.emplace_back();
     child1.push_back(-1);
@@ -248,25 +248,17 @@ int build_edge_rec(const std::vector<Edge> &edges, std::vector<EPrim> &p, int b,
         box = merge(box, p[i].box);
         if (directional) dbox = merge(dbox, p[i].dbox);
     }
-    V3f nsum = v3(0.f, 0.f, 0.f);
-    for (int i = b; i < e; i++) {
-        nsum += p[i].n0;
-        nsum += p[i].n1;
-    }
-    V3f axis = normalize(nsum);
-    float cmin = 1.f;
-    for (int i = b; i < e; i++) {
-        cmin = std::min(cmin, dot(p[i].n0, axis));
-        cmin = std::min(cmin, dot(p[i].n1, axis));
-    }
+    // the normal cone is set bottom-up once the tree is built
     nodes[id].bounds = box;
-    nodes[id].cone_axis = axis;
-    nodes[id].cone_angle = std::acos(cmin);
     int n = e - b;
     auto leaf = [&]() {
         nodes[id].offset = (int)ordered.size();
         nodes[id].nprims = n;
-        for (int i = b; i < e; i++) ordered.push_back(edges[p[i].id]);
+        for (int i = b; i < e; i++) {
+            ordered.push_back(edges[p[i].id]);
+            on.push_back(p[i].n0);
+            on.push_back(p[i].n1);
+        }
     };
     if (n == 1) {
         leaf();
@@ -302,8 +294,8 @@ int build_edge_rec(const std::vector<Edge> &edges, std::vector<EPrim> &p, int b,
                          return spatial ? x.c[dim] < y.c[dim] : x.dc[dim] < y.dc[dim];
                      });
     nodes[id].nprims = 0;
-    build_edge_rec(edges, p, b, mid, directional, nodes, ordered, child1);
-    int r = build_edge_rec(edges, p, mid, e, directional, nodes, ordered, child1);
+    build_edge_rec(edges, p, b, mid, directional, nodes, ordered, child1, on);
+    int r = build_edge_rec(edges, p, mid, e, directional, nodes, ordered, child1, on);
     child1[id] = r;
     return id;
 }
@@ -341,21 +333,45 @@ static EdgeTree build_edge_tree(const Scene &sc, const std::vector<Edge> &edges,
         }
     }
     std::vector<int> child1;
-    build_edge_rec(edges, p, 0, (int)edges.size(), directional, tree.nodes, tree.edges, child1);
-    // interior: offset = second child; total_length bottom-up (children follow parents)
-    for (int i = (int)tree.nodes.size() - 1; i >= 0; i--) {
+    std::vector<V3f> on;  // per ordered edge: its two face normals
+    build_edge_rec(edges, p, 0, (int)edges.size(), directional, tree.nodes, tree.edges, child1, on);
+    // Bottom-up (children follow parents): interior offset = second child,
+    // total length, the sum of the face normals (leaf: in edge order;
+    // interior: left + right) and the range of the node's edges in the
+    // ordered list.  The cone: axis = normalised sum, angle = acos of the
+    // least cosine to it over the node's edges.
+    int nn = (int)tree.nodes.size();
+    std::vector<V3f> nsum(nn);
+    std::vector<int> first(nn), last(nn);
+    for (int i = nn - 1; i >= 0; i--) {
         EdgeNode &n = tree.nodes[i];
         if (n.nprims > 0) {
             n.total_length = 0.f;
+            nsum[i] = v3(0.f, 0.f, 0.f);
             for (int k = 0; k < n.nprims; k++) {
                 const Edge &ed = tree.edges[n.offset + k];
                 const Shape &s = sc.shapes[ed.shape];
                 n.total_length += length(s.vert(ed.v1) - s.vert(ed.v0));
+                nsum[i] += on[2 * (n.offset + k)];
+                nsum[i] += on[2 * (n.offset + k) + 1];
             }
+            first[i] = n.offset;
+            last[i] = n.offset + n.nprims;
         } else {
             n.offset = child1[i];
             n.total_length = tree.nodes[i + 1].total_length + tree.nodes[n.offset].total_length;
+            nsum[i] = nsum[i + 1] + nsum[n.offset];
+            first[i] = first[i + 1];
+            last[i] = last[n.offset];
         }
+        V3f axis = normalize(nsum[i]);
+        float cmin = 1.f;
+        for (int k = first[i]; k < last[i]; k++) {
+            cmin = std::min(cmin, dot(on[2 * k], axis));
+            cmin = std::min(cmin, dot(on[2 * k + 1], axis));
+        }
+        n.cone_axis = axis;
+        n.cone_angle = std::acos(cmin);
     }
     return tree;
 }
@@ -531,6 +547,8 @@ void build_scene(Scene &sc, const dr_scene_desc *d) {
 // cone's supporting plane, then a spherical-polygon search over the box
 // silhouette seen from the apex.  The polygon table (box corners in CCW order
 // for each of the 27 apex regions) is geometric data and must be the same.
+// Algorithm and case table after the Geometric Tools Engine, David Eberly,
+// Geometric Tools LLC, distributed under the Boost Software License 1.0.
 namespace {
 struct ConePoly {
     int n;

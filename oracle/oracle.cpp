@@ -810,7 +810,7 @@ void sample_edge_rec(TraverseCtx &c, bool cone_mode, int node_id, float nb, floa
         for (int i = 0; i < node.nprims; i++) {
             const Edge &e = t.edges[node.offset + i];
             if (is_silhouette(*c.sc, c.sp.pos, e))
-                c.out->push_back(EdgeW{e, w, cone_mode ? c.jc++ : (0x8000 | c.ja++)});
+                c.out->push_back(EdgeW{e, w, cone_mode ? c.jc++ : (0x40000000 | c.ja++)});
         }
         return;
     }

@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only.  A Python module (built into
 // oracle/_ref/ by oracle/Makefile.ref, in a container that holds the
-// reference sources; it never travels and nothing in the product loads it)
+// reference sources; it is git-ignored, and no GPU test, smoke run or bench
+// loads it: only the CPU tests of tests/test_oracle_ref.py do)
 // that calls single functions of the reference's own Embree-free headers, so
 // that tests can compare the CPU restatement in oracle/ with them:
 //   intersect.h:71-90      Moller-Trumbore (u, v, t)

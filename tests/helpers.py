@@ -100,3 +100,44 @@ def random_rays(w, n, rng):
     rays[:, 4:7] = d
     rays[:, 7] = np.inf
     return rays
+
+
+def grad_groups(w):
+    """Parameter groups of the flat gradient buffer: {name: (begin, end)} --
+    the camera matrices, each material texture, each shape's vertices and
+    normals, each light's intensity (layout of include/delta_ray.h)."""
+    lay = dr.grad_layout(w["shapes"], w["materials"], w["lights"])
+    groups = {"camera": (0, 64)}
+    for i, m in enumerate(lay["materials"]):
+        for k, (o, shp) in m.items():
+            groups["material%d.%s" % (i, k)] = (o, o + int(np.prod(shp)))
+    for i, sh in enumerate(lay["shapes"]):
+        for k, (o, shp) in sh.items():
+            groups["shape%d.%s" % (i, k)] = (o, o + int(np.prod(shp)))
+    for i, l in enumerate(lay["lights"]):
+        o, shp = l["intensity"]
+        groups["light%d.intensity" % i] = (o, o + 3)
+    return groups
+
+
+def group_errors(w, g, gref):
+    """per group: (||g - gref||, ||gref||) in float64"""
+    out = {}
+    for k, (a, b) in grad_groups(w).items():
+        d = np.asarray(g[a:b], np.float64) - np.asarray(gref[a:b], np.float64)
+        out[k] = (float(np.linalg.norm(d)), float(np.linalg.norm(np.asarray(gref[a:b], np.float64))))
+    return out
+
+
+def assert_groups(w, g, gref, rtol, floor):
+    """Every parameter group of g matches gref: ||g - gref|| <= rtol ||gref||
+    + floor ||gref_all|| (the floor covers groups whose gradient is zero or
+    tiny next to the rest, where a single rounding-flipped sample dominates).
+    Returns the per-group errors."""
+    tot = float(np.linalg.norm(np.asarray(gref, np.float64)))
+    errs = group_errors(w, g, gref)
+    bad = {k: v for k, v in errs.items() if v[0] > rtol * v[1] + floor * tot}
+    for k, (e, n) in errs.items():
+        print("  %-28s err %.3e  norm %.3e  rel %.2e" % (k, e, n, e / max(n, 1e-30)), flush=True)
+    assert not bad, bad
+    return errs

@@ -145,3 +145,41 @@ def test_autotune_picks_a_candidate(gpu):
         ref_img, ref_g = _single(torch, j["scene"], j["options"], j, gpu)
         assert _rel(j["image"].cpu().numpy(), ref_img) < 1e-5
         assert _rel(j["grads"].cpu().numpy(), ref_g) < 1e-4
+
+
+def test_batched_workload_matches_oracle(gpu, oracle):
+    """Four scenes of the uneven batched workload (the largest icosphere, the
+    largest heightfield and two small ones, 1 k - 144 k triangles, 2 bounces),
+    two 64 x 64 views each, rendered in one captured batch: every job's image
+    and every parameter group of its gradient against the oracle rendering
+    that scene from that camera."""
+    import torch
+    scenes = workloads.batched()
+    pick = [0, 62, 2, 6]
+    jobs, refs = [], []
+    for k in pick:
+        s = scenes[k]
+        print("case: batched scene", k, s["num_triangles"], "triangles", flush=True)
+        w = dict(s, resolution=(64, 64))
+        ds = helpers.gpu_scene(w)
+        for v, (cam, _, _) in enumerate(s["views"][:2]):
+            cam64 = workloads.make_camera(
+                np.asarray(cam.cam_to_world)[:3, 3], [0, 0.8, 0], [0, 1, 0], 45.0, (64, 64))
+            view = dr.View(ds, cam64)
+            seed = 1000 * k + v
+            o = dr.DeviceScene.options(64, 64, 2, 2, seed)
+            bufs = _buffers(torch, ds, 64, 64, True, seed, gpu)
+            jobs.append(dict(scene=ds, view=view, options=o, **bufs))
+            wv = dict(w, camera=cam64)
+            refs.append((wv, seed, bufs["d_image"].cpu().numpy()))
+    batch = dr.Batch()
+    batch.render(jobs, graph=True)
+    torch.cuda.synchronize()
+    for j, (wv, seed, d_img) in zip(jobs, refs):
+        ref_img, gref, _ = oracle.scene_from_workload(wv).render(64, 64, 2, 2, seed, d_image=d_img)
+        img = j["image"].cpu().numpy()
+        g = j["grads"].cpu().numpy()
+        assert ref_img.sum() > 0 and np.abs(gref).sum() > 0
+        print("image rel err", _rel(img, ref_img), flush=True)
+        assert _rel(img, ref_img) < 2e-3
+        helpers.assert_groups(wv, g, gref, 1e-3, 2e-3)

@@ -79,19 +79,45 @@ TERMS = {
 }
 
 
-@pytest.mark.parametrize("name", SCENES)
-@pytest.mark.parametrize("term", list(TERMS))
-def test_adjoint_matches_oracle(gpu, oracle, name, term):
-    _print_case("adjoint", name, term)
-    w = small_workload(name)
+# Per parameter group (camera, each material texture, each shape's vertices
+# and normals, each light): ||g - g_oracle|| <= RTOL ||g_oracle|| + FLOOR
+# ||g_oracle (all)||.  Both sides trace the same paths with the same random
+# numbers; what is left is float rounding of the sums (atomics in a different
+# order) and the rare sample whose path flips on the last bit of a
+# transcendental (ocml vs glibc), which the floor absorbs.
+RTOL = {"interior": 1e-3, "secondary": 1e-3, "primary": 1e-3}
+FLOOR = {"interior": 1e-4, "secondary": 2e-3, "primary": 1e-4}
+
+
+def _adjoint_case(oracle, w, term, spp, seed):
     W, H = w["resolution"]
     nb = w["max_bounces"]
     rng = np.random.default_rng(3)
     d_img = rng.normal(size=(H, W, 3)).astype(np.float32)
     flags = _abi.DR_FLAGS_PARITY | TERMS[term]
-    img, g, _ = gpu_render(gpu_scene(w), w, 2, nb, 5, flags=flags, d_image=d_img)
-    ref_img, gref, _ = oracle.scene_from_workload(w).render(W, H, 2, nb, 5, flags=flags, d_image=d_img)
+    img, g, _ = gpu_render(gpu_scene(w), w, spp, nb, seed, flags=flags, d_image=d_img)
+    ref_img, gref, _ = oracle.scene_from_workload(w).render(W, H, spp, nb, seed, flags=flags, d_image=d_img)
+    return g, gref
+
+
+@pytest.mark.parametrize("name", SCENES)
+@pytest.mark.parametrize("term", list(TERMS))
+def test_adjoint_matches_oracle(gpu, oracle, name, term):
+    _print_case("adjoint", name, term)
+    w = small_workload(name)
+    g, gref = _adjoint_case(oracle, w, term, 2, 5)
     assert np.abs(gref).sum() > 0 or term == "secondary" and name == "single_triangle", "oracle gradient is zero"
-    e = rel_err(g, gref)
-    print("rel err", e, "norm", float(np.linalg.norm(gref)))
-    assert e < 2e-2, e
+    print("rel err (all)", rel_err(g, gref), "norm", float(np.linalg.norm(gref)))
+    helpers.assert_groups(w, g, gref, RTOL[term], FLOOR[term])
+
+
+@pytest.mark.parametrize("term", list(TERMS))
+def test_adjoint_matches_oracle_full_heightfield_mesh(gpu, oracle, term):
+    """The benchmark's 128 x 128 heightfield mesh (32 258 triangles, normal
+    edge tree of ~96 k nodes, 2 bounces) at 64 x 64 pixels, 1 sample."""
+    _print_case("adjoint heightfield-128", term)
+    w = small_workload("heightfield", res=64, grid=128)
+    g, gref = _adjoint_case(oracle, w, term, 1, 9)
+    assert np.abs(gref).sum() > 0
+    print("rel err (all)", rel_err(g, gref), "norm", float(np.linalg.norm(gref)))
+    helpers.assert_groups(w, g, gref, RTOL[term], FLOOR[term])

@@ -72,3 +72,39 @@ def test_render_function_matches_c_abi(gpu):
     _, d_shapes, _, d_lights = dr.split_grads(g.cpu().numpy(), scene.sd.layout)
     np.testing.assert_allclose(v.grad.numpy(), d_shapes[0].vertices, rtol=1e-4, atol=1e-5)
     np.testing.assert_allclose(inten.grad.numpy(), d_lights[0].intensity, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_render_function_keeps_and_refits_its_scene(gpu):
+    """An optimisation loop on CUDA vertices: the scene is built once and
+    refitted on the device every step (dr_scene_update_vertices), with the
+    gradients of a scene built from scratch for the same vertices."""
+    from delta_ray_amd import render_pytorch as rp
+    cam, mats, shapes, lights, v, inten = _scene(device="cuda")
+    opt = torch.optim.Adam([v], lr=0.05)
+    d = torch.from_numpy(np.random.default_rng(1).normal(size=(32, 32, 3)).astype(np.float32)).cuda()
+    first = None
+    for step in range(3):
+        print("step", step, flush=True)
+        opt.zero_grad()
+        args = RenderFunction.serialize_scene(cam, mats, shapes, lights, (32, 32), 4, 1)
+        img = RenderFunction.apply(11 + step, *args)
+        assert img.is_cuda
+        (img * d).sum().backward()
+        kept = rp._SceneCache.scene
+        if first is None:
+            first = kept
+        assert kept is first, "the scene was rebuilt"
+        g_kept = v.grad.detach().clone()
+        # the same step on a scene built from scratch
+        rp._SceneCache.scene = None
+        v.grad = None
+        img2 = RenderFunction.apply(11 + step, *args)
+        (img2 * d).sum().backward()
+        np.testing.assert_allclose(img.detach().cpu().numpy(), img2.detach().cpu().numpy(), rtol=0, atol=0)
+        np.testing.assert_allclose(g_kept.cpu().numpy(), v.grad.cpu().numpy(), rtol=1e-4, atol=1e-6)
+        rp._SceneCache.scene = first
+        rp._SceneCache.verts = [rp._copy(s.vertices) for s in shapes]
+        v.grad = g_kept
+        opt.step()
+    assert first.info()["refits"] >= 2
