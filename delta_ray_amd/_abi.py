@@ -20,6 +20,9 @@ DR_QUIRK_TEXEL_SWAP = 1 << 1
 DR_QUIRK_NORMAL_DN0 = 1 << 2
 DR_QUIRK_DP_LAST = 1 << 3
 DR_QUIRK_PROJECT_SCALE = 1 << 4
+DR_QUIRK_EDGE_BSDF_SUM = 1 << 5
+DR_QUIRK_EDGE_HIT_JACOBIAN = 1 << 6
+DR_QUIRK_CONE_PMF = 1 << 7
 DR_SKIP_INTERIOR = 1 << 8
 DR_SKIP_SECONDARY_EDGES = 1 << 9
 DR_SKIP_PRIMARY_EDGES = 1 << 10
@@ -27,7 +30,8 @@ DR_WRITE_DX_IMAGE = 1 << 11
 DR_LEARNED_EDGES = 1 << 12
 DR_EDGE_STATS = 1 << 13
 DR_FLAGS_PARITY = (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP |
-                   DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE)
+                   DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE |
+                   DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN | DR_QUIRK_CONE_PMF)
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)

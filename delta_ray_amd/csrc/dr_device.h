@@ -533,8 +533,9 @@ DR_INLINE float smith_g1(F3 v, F3 h, F3 n, const Frame &fr, float roughness) {
     return (3.535f * a + 2.181f * a2) / (1.0f + 2.276f * a + 2.577f * a2);
 }
 
-// material.h:257-351
-DR_INLINE F3 bsdf(const DrSceneView &S, int mid, const SPoint &sp, F3 wi, F3 wo) {
+// material.h:257-351.  comp: -1 the whole BSDF, 0 its diffuse part, 1 its
+// specular part.
+DR_INLINE F3 bsdf(const DrSceneView &S, int mid, const SPoint &sp, F3 wi, F3 wo, int comp = -1) {
     const DrMaterial &m = S.materials[mid];
     F3 n = sp.fr.n;
     if (m.two_sided && dot(wi, n) < 0.f) n = -n;
@@ -543,8 +544,8 @@ DR_INLINE F3 bsdf(const DrSceneView &S, int mid, const SPoint &sp, F3 wi, F3 wo)
     F3 diffuse = get_diffuse(S, m, sp.uv);
     F3 specular = get_specular(S, m, sp.uv);
     float roughness = get_roughness(S, m, sp.uv);
-    F3 result = diffuse * bsdf_cos / DR_PI;
-    if (sum3(specular) > 0.f && dot(wi, n) > 0.f) {
+    F3 result = comp == 1 ? F3{0.f, 0.f, 0.f} : diffuse * bsdf_cos / DR_PI;
+    if (comp != 0 && sum3(specular) > 0.f && dot(wi, n) > 0.f) {
         F3 h = normalize(wi + wo);
         F3 hl = to_local(sp.fr, h);
         if (hl.z > 0.f) {
@@ -716,7 +717,8 @@ DR_INLINE float light_pdf(const DrSceneView &S, const Isect &is) {
 }
 
 // pathtrace.h:7-57 for an area-light point (visibility always tested)
-DR_INLINE F3 eval_light(const DrSceneView &S, F3 wi, const Isect &sis, const SPoint &sp, int light, F3 lpos, F3 ln) {
+DR_INLINE F3 eval_light(const DrSceneView &S, F3 wi, const Isect &sis, const SPoint &sp, int light, F3 lpos, F3 ln,
+                        int comp = -1) {
     F3 zero = F3{0.f, 0.f, 0.f};
     if (!valid(sis) || light < 0) return zero;
     F3 d = lpos - sp.pos;
@@ -724,7 +726,7 @@ DR_INLINE F3 eval_light(const DrSceneView &S, F3 wi, const Isect &sis, const SPo
     if (dist == 0.f) return zero;
     F3 nd = d / dist;
     if (occluded(S, sp.pos, nd, dist)) return zero;
-    F3 b = bsdf(S, S.shapes[sis.shape].mat, sp, wi, nd);
+    F3 b = bsdf(S, S.shapes[sis.shape].mat, sp, wi, nd, comp);
     float lcos = dot(ln, -nd);
     if (lcos <= 0.f) return zero;
     const DrLight &l = S.lights[light];

@@ -257,7 +257,7 @@ DR_INLINE float node_importance_len(const DrEdgeNode &n, float bound, F3 p, floa
 // boxes, which split in both passes) is expanded once, and each pass still
 // meets its nodes in its own depth-first order.  The cone pass draws from the
 // point's stream, the "all" pass from a stream of its own; selected edges get
-// ordinals counted per pass (bit 15 set for the "all" pass), which sort like
+// ordinals counted per pass (bit 30 set for the "all" pass), which sort like
 // the reference's sequential order.  The traversal is explicit state advanced
 // one node per et_step() so that a persistent kernel can give a lane a new
 // point as soon as its traversal ends; et_step() calls
@@ -379,6 +379,7 @@ struct EdgeTraversal {
     int mode;
     int pops;
     uint32_t n_inner, n_edges;
+    bool cone_pmf_quirk;  // DR_QUIRK_CONE_PMF
 };
 
 // The entry of node `node` (record n, bound computed by its parent).  Edge
@@ -511,6 +512,11 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
     bool rhc = need_cone && aabb_cone_intersect(R.lo, R.hi, T.cone);
     bool cone_split = all_split || (lhc && rhc && lcs && rcs && T.cone.cos_angle > 0.95f);
     float ilc = lhc ? il : 0.f, irc = rhc ? ir : 0.f;
+    // The reference's cone pass stops where neither child meets the cone,
+    // even where the split rule would take both (edge_tree.cpp:587-589
+    // before 590-601), yet its all pass weighs the edges below with cone pmf
+    // 1: without the quirk they get the cone pmf 0 the cone pass really has.
+    if (!T.cone_pmf_quirk && ilc + irc <= 1e-10f) cone_split = false;
     float plc = 0.f, prc = 0.f, pla, pra;
     if (cone_split) {
         plc = prc = 1.f;
@@ -634,7 +640,8 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
                                                const LightSample &ls, const SecondarySample *ss, int nss, F3 wi,
                                                const Isect &is, const SPoint &sp, F3 weight, F3 d_color,
                                                bool direct_only, bool *defer = nullptr,
-                                               const float *geo = nullptr) {
+                                               const float *geo = nullptr,
+                                               unsigned quirks = DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN) {
     EdgeResult R = EdgeResult{F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}};
     if (!valid(is)) return R;
     F3 v0, v1;
@@ -665,15 +672,30 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
     M3 iso = m3_rows(fx, fy, n);
     M3 M, Minv;
     float m_pmf;
+    int comp;
     if (bsdf_component <= dpmf) {
         Minv = iso;
         M = inverse(Minv);
         m_pmf = dpmf;
+        comp = 0;
     } else {
         Minv = mul(inverse(get_ltc_matrix(S, mid, sp, wi)), iso);
         M = inverse(Minv);
         m_pmf = spmf;
+        comp = 1;
     }
+    // The reference weighs the whole BSDF by the chosen component's pmf
+    // (delta_ray.cpp:86-101, 249, 296-330), so each component's line
+    // density integrates the whole BSDF: twice the derivative when both
+    // components are nonzero.  Without that quirk a sample counts only the
+    // part of the BSDF of the component it was drawn from (unbiased).
+    if (quirks & DR_QUIRK_EDGE_BSDF_SUM) comp = -1;
+    // The reference's Jacobian of a side ray's hit point w.r.t. its direction
+    // treats the ray as leaving the world origin (intersect_jacobian of
+    // vector.h:690-716 gets the absolute hit position, delta_ray.cpp:329-331),
+    // which scales it by (x . n) / ((x - p) . n); without that quirk the
+    // hit is taken relative to the shading point p.
+    const bool from_origin = (quirks & DR_QUIRK_EDGE_HIT_JACOBIAN) != 0;
     F3 a = mul(Minv, v0 - sp.pos);
     F3 b = mul(Minv, v1 - sp.pos);
     if (a.z <= 0.f && b.z <= 0.f) return R;
@@ -698,6 +720,7 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
         return R;
     F3 eb = bsdf(S, mid, sp, wi, sdir);
     if (sum3(eb) < 1e-6f) return R;
+    if (comp >= 0) eb = bsdf(S, mid, sp, wi, sdir, comp);
     // with `defer`, a sample that passed the connectivity test stops here
     // and is reported: the caller runs it again in full in a launch of such
     // samples only (about one in five), so the evaluation and continuation
@@ -733,8 +756,8 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
     }
     int rl = valid(ir) ? S.shapes[ir.shape].light : -1;
     int ll = valid(il) ? S.shapes[il.shape].light : -1;
-    F3 er = eval_light(S, wi, is, sp, rl, spr.pos, spr.fr.n) + eb * shr;
-    F3 el = eval_light(S, wi, is, sp, ll, spl.pos, spl.fr.n) + eb * shl;
+    F3 er = eval_light(S, wi, is, sp, rl, spr.pos, spr.fr.n, comp) + eb * shr;
+    F3 el = eval_light(S, wi, is, sp, ll, spl.pos, spl.fr.n, comp) + eb * shl;
     if (valid(ir)) {
         F3 dd = spr.pos - sp.pos;
         if (length(dd) * fabsf(dot(normalize(dd), spr.ng)) < 1e-3f) return R;
@@ -747,7 +770,7 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
     float dirac_j = length(cross(d0, d1));
     auto accum = [&](const SPoint &hp, F3 ev) {
         M3 nj = normalize_jacobian(sample_p);
-        M3 ij = intersect_jacobian(sdir, hp.pos, hp.ng);
+        M3 ij = intersect_jacobian(sdir, from_origin ? hp.pos : hp.pos - sp.pos, hp.ng);
         float line_j = length(mul(mul(mul(ij, nj), M), wt)) / (m_pmf * pl);
         float w = line_j / dirac_j;
         F3 x = hp.pos - sp.pos;

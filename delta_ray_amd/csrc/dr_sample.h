@@ -386,6 +386,7 @@ DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, const Poin
         philox_init(mix, c.seed, 0, p.sid, dr_stream_point_mix(p.b));
         mode = philox_next(mix) < DR_LEARNED_SHARE ? 1 : 2;
     }
+    T.cone_pmf_quirk = (c.flags & DR_QUIRK_CONE_PMF) != 0;
     return et_init(T, stack, S, p.wi, p.sp, Cone{p.sp.pos, p.cone_dir, p.cone_cos}, p.roughness, p.dlum, p.slum,
                    mode);
 }
@@ -409,7 +410,7 @@ DR_INLINE F3 stage_edge(const DrSceneView &S, const SampleCtx &c, GradSink &G, u
     F3 weight = p.thr * ((1.f / c.spp) * w);
     Isect is = Isect{p.shape, p.tri, p.gtri};
     EdgeResult er = sample_shading_edge_point(S, bcomp, eu, edge, els, ess, nss, p.wi, is, p.sp, weight, d_color,
-                                              c.nb == 1, defer, geo);
+                                              c.nb == 1, defer, geo, c.flags);
     if (defer && *defer) return F3{0.f, 0.f, 0.f};
     if (!is_zero(er.dv0) || !is_zero(er.dv1)) {
         const DrShape &es = S.shapes[edge.shape];

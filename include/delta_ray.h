@@ -131,6 +131,25 @@ enum {
      * d_project_local (camera.h:204-213) drops the 0.5 of project_local
      * (camera.h:170-171) */
     DR_QUIRK_PROJECT_SCALE = 1u << 4,
+    /* a secondary-edge sample weighs the whole BSDF by the pmf of the
+     * component (diffuse / specular) whose LTC density drew it
+     * (delta_ray.cpp:86-101, 249, 296-330): twice the derivative on
+     * materials with both components; without it a sample counts its own
+     * component's part of the BSDF */
+    DR_QUIRK_EDGE_BSDF_SUM = 1u << 5,
+    /* the Jacobian of a secondary-edge side ray's hit point w.r.t. the ray
+     * direction treats the ray as leaving the world origin (the absolute hit
+     * position goes to intersect_jacobian, vector.h:690-716, delta_ray.cpp:
+     * 329-331): derivatives scaled by (x . n) / ((x - p) . n) for a hit x of
+     * normal n seen from p, zero on planes through the origin */
+    DR_QUIRK_EDGE_HIT_JACOBIAN = 1u << 6,
+    /* the edge tree's cone pass stops at a node where neither child meets
+     * the light cone even when its split rule would take both children
+     * (edge_tree.cpp:587-589 before 590-601), while the all pass weighs the
+     * edges below as if the cone pass reached them: those edges are
+     * under-weighted; without it the all pass uses the cone pmf 0 the cone
+     * pass really has there */
+    DR_QUIRK_CONE_PMF = 1u << 7,
     /* terms (for tests): skip the interior adjoint / secondary edges /
      * primary edges */
     DR_SKIP_INTERIOR = 1u << 8,
@@ -148,7 +167,8 @@ enum {
     DR_EDGE_STATS = 1u << 13
 };
 #define DR_FLAGS_PARITY (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP | \
-                         DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE)
+                         DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE | \
+                         DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN | DR_QUIRK_CONE_PMF)
 
 typedef struct dr_render_options {
     int width;

@@ -24,10 +24,21 @@
 
 #include <atomic>
 #include <cstdio>
+
+// Planted mistakes for tests/test_oracle_mutants.py, which builds the oracle
+// with -DOR_MUTANT=k and checks that the pin of each function fails: 1 drops
+// the pdf from the path throughput, 2 divides the specular edge sample by
+// the diffuse pmf, 3 drops the hit cosine of an edge sample's continuation,
+// 4 drops the absolute value of the node bound's azimuth term.  0: none.
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
 #include <fstream>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
+#include <algorithm>
 
 namespace oracle {
 
@@ -193,9 +204,11 @@ template <class T>
 inline T roughness_to_phong(const T &r) { return max(2.f / r, T(0.f)); }
 inline float roughness_to_phong(float r) { return max(2.f / r, 0.f); }
 
-// material.h:257-351
+// material.h:257-351.  comp: -1 the whole BSDF, 0 / 1 its diffuse /
+// specular part only.
 template <class T>
-V3<T> bsdf(const Scene &sc, int mid, const SP<T> &sp, const V3<T> &wi, const V3<T> &wo, Rec<T> *rec) {
+V3<T> bsdf(const Scene &sc, int mid, const SP<T> &sp, const V3<T> &wi, const V3<T> &wo, Rec<T> *rec,
+           int comp = -1) {
     const Material &m = sc.mats[mid];
     V3<T> n = sp.fr[2];
     if (m.two_sided && dot(wi, n) < 0.f) n = -n;
@@ -245,6 +258,8 @@ V3<T> bsdf(const Scene &sc, int mid, const SP<T> &sp, const V3<T> &wi, const V3<
             }
         }
     }
+    if (comp == 0) return diffuse_contrib;
+    if (comp == 1) return specular_contrib;
     return diffuse_contrib + specular_contrib;
 }
 
@@ -438,7 +453,7 @@ T light_pdf(const Scene &sc, const Isect &is, Rec<T> *rec) {
 // argument is not read there)
 template <class T>
 V3<T> eval(const Scene &sc, const V3<T> &wi, const Isect &sis, const SP<T> &sp, const Isect &lis,
-           const SP<T> &lp, Rec<T> *rec) {
+           const SP<T> &lp, Rec<T> *rec, int comp = -1) {
     V3<T> zero = v3<T>(T(0.f), T(0.f), T(0.f));
     if (!sis.valid() || !lis.valid()) return zero;
     int lid = sc.shapes[lis.shape].light;
@@ -448,7 +463,7 @@ V3<T> eval(const Scene &sc, const V3<T> &wi, const Isect &sis, const SP<T> &sp, 
     if (dist == 0.f) return zero;
     V3<T> nd = d / dist;
     if (occluded(sc, conv3<float>(sp.pos), conv3<float>(nd), val(dist))) return zero;
-    V3<T> b = bsdf(sc, sc.shapes[sis.shape].mat, sp, wi, nd, rec);
+    V3<T> b = bsdf(sc, sc.shapes[sis.shape].mat, sp, wi, nd, rec, comp);
     T lcos = dot(lp.fr[2], -nd);
     if (lcos <= 0.f) return zero;
     T dist2 = square(dist);
@@ -560,7 +575,7 @@ V3<T> shade(const Scene &sc, const V3<T> &wi, const Isect &pis, const SP<T> &psp
                 color += thr * contrib * (mis / pdf);
             }
         }
-        thr = thr * (be / pdf);
+        thr = OR_MUTANT == 1 ? thr * be : thr * (be / pdf);
         if (b >= 5) {
             float q = std::min(maxc(conv3<float>(thr)), 0.95f);
             if (ss[b].rr >= q) break;
@@ -695,7 +710,7 @@ bool is_silhouette(const Scene &sc, const V3f &p, const Edge &e) {
 
 // ------------------------------------------------------- edge sampling ---
 // A selected edge, its weight and its ordinal j: the index among the
-// edges selected by its pass, with bit 15 set for the "all" pass (so that
+// edges selected by its pass, with bit 30 set for the "all" pass (so that
 // ordinals sort like the reference's sequential selection: cone pass first).
 struct EdgeW {
     Edge e;
@@ -762,7 +777,8 @@ float bsdf_bound(const Scene &sc, const EdgeNode &node, const V3f &wi, int mid, 
         if (ch > cth_max) ch = cth_max;
         else if (ch < cth_min) ch = cth_min;
         float sh = std::sqrt(max(1.f - ch * ch, 0.f));
-        float hn = min(std::fabs(ln[0] * sh * cos_phi_h_max) + ln[2] * ch, 1.f);
+        float hn = OR_MUTANT == 4 ? min(ln[0] * sh * cos_phi_h_max + ln[2] * ch, 1.f)
+                                  : min(std::fabs(ln[0] * sh * cos_phi_h_max) + ln[2] * ch, 1.f);
         float e = roughness_to_phong(get_roughness<float>(sc.mats[mid], sp.uv, nullptr));
         phong_resp = std::pow(hn, e) * (e + 2.f) / float(2 * M_PI);
     }
@@ -797,6 +813,7 @@ struct TraverseCtx {
     std::vector<EdgeW> *out;
     Philox *rng;
     int jc = 0, ja = 0;  // ordinals of the cone and the "all" pass
+    uint32_t flags = 0;  // render flags (DR_QUIRK_CONE_PMF)
 };
 
 // edge_tree.cpp:511-652
@@ -832,6 +849,12 @@ void sample_edge_rec(TraverseCtx &c, bool cone_mode, int node_id, float nb, floa
     bool cone_split = all_split || (lhc && rhc && lcs && rcs && c.cone.cos_angle > 0.95f);
     float ilc = lhc ? il : 0.f, irc = rhc ? ir : 0.f;
     if (cone_mode && ilc + irc <= 1e-10f) return;
+    // The cone pass stops here when neither child meets the cone, even where
+    // the split rule would take both (edge_tree.cpp:587-589 come before
+    // 590-601), yet the all pass counts both children with cone pmf 1, so
+    // edges below are under-weighted.  Without the quirk the all pass gives
+    // them the cone pmf 0 the cone pass really has.
+    if (!(c.flags & DR_QUIRK_CONE_PMF) && ilc + irc <= 1e-10f) cone_split = false;
     float plc = 0.f, prc = 0.f, pla = 0.f, pra = 0.f;
     if (cone_split) {
         plc = prc = 1.f;
@@ -862,13 +885,13 @@ void sample_edge_rec(TraverseCtx &c, bool cone_mode, int node_id, float nb, floa
 // and may be walked together.
 void sample_tree(const Scene &sc, const EdgeTree &tree, bool directional, const V3f &wi, const Isect &is,
                  const SP<float> &sp, const Cone &cone, float pmf, std::vector<EdgeW> &out, Philox &rng,
-                 Philox &rng_all) {
+                 Philox &rng_all, uint32_t flags) {
     if (tree.nodes.empty()) return;
     int mid = sc.shapes[is.shape].mat;
     const Material &m = sc.mats[mid];
     float dlum = luminance(get_diffuse<float>(m, sp.uv, nullptr));
     float slum = luminance(get_specular<float>(m, sp.uv, nullptr));
-    TraverseCtx c{&sc, &tree, directional, wi, mid, sp, dlum, slum, cone, &out, &rng};
+    TraverseCtx c{&sc, &tree, directional, wi, mid, sp, dlum, slum, cone, &out, &rng, 0, 0, flags};
     float b = bsdf_bound(sc, tree.nodes[0], wi, mid, sp, dlum, slum);
     sample_edge_rec(c, true, 0, b, pmf, pmf);
     c.rng = &rng_all;
@@ -877,21 +900,21 @@ void sample_tree(const Scene &sc, const EdgeTree &tree, bool directional, const 
 
 // edge_tree.cpp:685-725
 void sample_shading_edges(const Scene &sc, const V3f &wi, const Isect &is, const SP<float> &sp, const Cone &cone,
-                          std::vector<EdgeW> &out, Philox &rng, Philox &rng_all) {
+                          std::vector<EdgeW> &out, Philox &rng, Philox &rng_all, uint32_t flags) {
     if (dot(wi, sp.fr[2]) < 0.f) return;
     if (sc.nrm_tree.nodes.empty()) {
-        sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, 1.f, out, rng, rng_all);
+        sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, 1.f, out, rng, rng_all, flags);
         return;
     }
     if (sc.sil_tree.nodes.empty()) {
-        sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, 1.f, out, rng, rng_all);
+        sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, 1.f, out, rng, rng_all, flags);
         return;
     }
     float ni = sc.nrm_tree.nodes[0].total_length, si = sc.sil_tree.nodes[0].total_length;
     float npmf = ni / (ni + si), spmf = si / (ni + si);
     float u = rng.next();
-    if (u < npmf) sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, npmf, out, rng, rng_all);
-    else sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, spmf, out, rng, rng_all);
+    if (u < npmf) sample_tree(sc, sc.nrm_tree, true, wi, is, sp, cone, npmf, out, rng, rng_all, flags);
+    else sample_tree(sc, sc.sil_tree, false, wi, is, sp, cone, spmf, out, rng, rng_all, flags);
 }
 
 struct EdgeResult {
@@ -936,15 +959,23 @@ EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, floa
     Frame<float> iso{{fx, fy, n}};
     M3 M, Minv;
     float m_pmf;
+    int comp;
     if (bsdf_component <= dpmf) {
         Minv = m3_from_frame(iso);
         M = inverse(Minv);
         m_pmf = dpmf;
+        comp = 0;
     } else {
         Minv = inverse(get_ltc_matrix(sc, mid, sp, wi)) * m3_from_frame(iso);
         M = inverse(Minv);
-        m_pmf = spmf;
+        m_pmf = OR_MUTANT == 2 ? dpmf : spmf;
+        comp = 1;
     }
+    // delta_ray.cpp:249, 296-330 evaluate the whole BSDF for a sample drawn
+    // by one component's density and divide by that component's pmf (each
+    // component then integrates the whole BSDF); without the quirk a sample
+    // counts its own component's part
+    if (flags & DR_QUIRK_EDGE_BSDF_SUM) comp = -1;
     V3f a = Minv * (v0 - sp.pos);
     V3f b = Minv * (v1 - sp.pos);
     if (a[2] <= 0.f && b[2] <= 0.f) return R;
@@ -995,6 +1026,7 @@ EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, floa
         return R;
     V3f eb = bsdf<float>(sc, mid, sp, wi, sdir, nullptr);
     if (sum(eb) < 1e-6f) return R;
+    if (comp >= 0) eb = bsdf<float>(sc, mid, sp, wi, sdir, nullptr, comp);
     V3f shr = v3(0.f, 0.f, 0.f), shl = v3(0.f, 0.f, 0.f);
     if (!direct_only) {
         shr = shade<float>(sc, -sdir, ir, spr, ls, ss, false, nullptr, nullptr, nullptr);
@@ -1005,7 +1037,7 @@ EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, floa
             float dsq = length_squared(dd);
             if (dsq > 0.f) {
                 sh = sh / std::max(dsq, 1e-2f);
-                sh = sh * std::fabs(dot(normalize(dd), hp.ng));
+                if (OR_MUTANT != 3) sh = sh * std::fabs(dot(normalize(dd), hp.ng));
             } else {
                 sh = v3(0.f, 0.f, 0.f);
             }
@@ -1013,8 +1045,8 @@ EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, floa
         fix(ir, spr, shr);
         fix(il, spl, shl);
     }
-    V3f er = eval<float>(sc, wi, is, sp, ir, spr, nullptr) + eb * shr;
-    V3f el = eval<float>(sc, wi, is, sp, il, spl, nullptr) + eb * shl;
+    V3f er = eval<float>(sc, wi, is, sp, ir, spr, nullptr, comp) + eb * shr;
+    V3f el = eval<float>(sc, wi, is, sp, il, spl, nullptr, comp) + eb * shl;
     auto too_close = [&](const Isect &h, const SP<float> &hp) {
         if (!h.valid()) return false;
         V3f dd = hp.pos - sp.pos;
@@ -1025,7 +1057,10 @@ EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, floa
     if (too_close(ir, spr) || too_close(il, spl)) return R;
     auto accum = [&](const SP<float> &hp, const V3f &ev) {
         M3 nj = normalize_jacobian(sample_p);
-        M3 ij = intersect_jacobian(sdir, hp.pos, hp.ng);
+        // the reference passes the absolute hit position (delta_ray.cpp:
+        // 329-331), i.e. a ray from the world origin; without the quirk the
+        // hit relative to the shading point
+        M3 ij = intersect_jacobian(sdir, (flags & DR_QUIRK_EDGE_HIT_JACOBIAN) ? hp.pos : hp.pos - sp.pos, hp.ng);
         float line_j = length(((ij * nj) * M) * wt) / (m_pmf * pdf(l));
         V3f p = sp.pos, x = hp.pos;
         V3f d0 = v0 - p, d1 = v1 - p;
@@ -1303,7 +1338,7 @@ V3f render_pixel_sample(const RenderCtx &rc, int xi, int yi, int s, Accum &acc, 
             Cone cone{p.spf.pos, normalize(dl), cos_a};
             edges.clear();
             Philox prng_all(o.seed, 0, sid, stream_point_all(p.bounce));
-            sample_shading_edges(sc, p.wi, p.is, p.spf, cone, edges, prng, prng_all);
+            sample_shading_edges(sc, p.wi, p.is, p.spf, cone, edges, prng, prng_all, o.flags);
             last_edge_count = (int)edges.size();
             for (int k = 0; k < (int)edges.size(); k++) {
                 const EdgeW &ew = edges[k];
@@ -1535,6 +1570,33 @@ static SP<float> probe_sp(const float *sp) {
     return p;
 }
 
+// Edge selection at one shading point (surface point of triangle `tri` of
+// shape `shape`, incoming direction wi, cone toward the light): over `trials`
+// independent runs of the two passes, the summed weight of every mesh edge
+// (sums[i] for the scene's i-th edge in (shape, v0, v1) order).  The
+// estimator is unbiased when every edge that can contribute averages 1.
+void or_probe_edge_weights(void *s, int shape, int tri, const float *sp, const float *wi, const float *cone,
+                           uint64_t seed, int64_t trials, uint32_t flags, double *sums) {
+    const Scene &sc = ((OScene *)s)->sc;
+    SP<float> p = probe_sp(sp);
+    Isect is{shape, tri};
+    Cone c{v3(cone[0], cone[1], cone[2]), v3(cone[3], cone[4], cone[5]), cone[6]};
+    V3f w = v3(wi[0], wi[1], wi[2]);
+    std::vector<EdgeW> out;
+    for (int64_t t = 0; t < trials; t++) {
+        Philox r(seed, 5, (uint64_t)t, 1), ra(seed, 5, (uint64_t)t, 2);
+        out.clear();
+        sample_shading_edges(sc, w, is, p, c, out, r, ra, flags);
+        for (const EdgeW &e : out) {
+            auto it = std::lower_bound(sc.edges.begin(), sc.edges.end(), e.e, [](const Edge &a, const Edge &b) {
+                return std::make_tuple(a.shape, a.v0, a.v1) < std::make_tuple(b.shape, b.v0, b.v1);
+            });
+            if (it != sc.edges.end() && it->shape == e.e.shape && it->v0 == e.e.v0 && it->v1 == e.e.v1)
+                sums[it - sc.edges.begin()] += e.w;
+        }
+    }
+}
+
 // out[0..2] bsdf, out[3] pdf
 void or_probe_bsdf(void *s, int mid, const float *sp, const float *wi, const float *wo, float *out) {
     const Scene &sc = ((OScene *)s)->sc;
@@ -1543,6 +1605,29 @@ void or_probe_bsdf(void *s, int mid, const float *sp, const float *wi, const flo
     V3f f = bsdf<float>(sc, mid, p, a, b, nullptr);
     for (int i = 0; i < 3; i++) out[i] = f[i];
     out[3] = bsdf_pdf<float>(sc, mid, p, a, b, nullptr);
+}
+
+// n directions wo (3 floats each) at one shading point: out[3 i .. 3 i + 2]
+void or_probe_bsdf_batch(void *s, int mid, const float *sp, const float *wi, const float *wo, int64_t n,
+                         float *out) {
+    const Scene &sc = ((OScene *)s)->sc;
+    SP<float> p = probe_sp(sp);
+    V3f a = v3(wi[0], wi[1], wi[2]);
+    for (int64_t i = 0; i < n; i++) {
+        V3f f = bsdf<float>(sc, mid, p, a, v3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), nullptr);
+        for (int k = 0; k < 3; k++) out[3 * i + k] = f[k];
+    }
+}
+
+// edge_tree.cpp:345-480 for a node box [lo, hi] seen from a shading point of
+// material mid (roughness from it), with the given reflectance luminances
+float or_probe_bsdf_bound(void *s, int mid, const float *sp, const float *wi, const float *lo, const float *hi,
+                          float dlum, float slum) {
+    const Scene &sc = ((OScene *)s)->sc;
+    EdgeNode n;
+    n.bounds.lo = v3(lo[0], lo[1], lo[2]);
+    n.bounds.hi = v3(hi[0], hi[1], hi[2]);
+    return bsdf_bound(sc, n, v3(wi[0], wi[1], wi[2]), mid, probe_sp(sp), dlum, slum);
 }
 
 // bs = (u0, u1, w); out[0..2] direction, out[3] is_specular

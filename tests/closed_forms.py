@@ -173,3 +173,196 @@ def occlusion_grad(camera, W, H, blocker, d_image, rho, radiance, light_y, light
                     exact[vi * 3 + c] += g
                     biased[vi * 3 + c] += fac * g
     return exact, biased
+
+
+# ------------------------------------------------------- two-bounce light --
+def gauss_panels(a, b, panels, order=8, grade=None):
+    """Composite Gauss-Legendre nodes and weights on [a, b]: `panels` panels,
+    uniform, or graded geometrically toward `grade` (= a or b) so that an
+    integrand varying on the scale of the distance to that end is resolved."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    if grade is None:
+        edges = np.linspace(a, b, panels + 1)
+    else:
+        # panel ends at distances L r^k from the graded end, r = 0.6
+        L = abs(b - a)
+        d = np.concatenate([[0.0], L * 0.6 ** np.arange(panels - 1, -1, -1)])
+        edges = grade + np.sign((a + b) / 2 - grade) * d
+        edges = np.sort(edges)
+    xs, ws = [], []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        xs.append(0.5 * (hi - lo) * x + 0.5 * (hi + lo))
+        ws.append(0.5 * (hi - lo) * w)
+    return np.concatenate(xs), np.concatenate(ws)
+
+
+def rect_quadrature(origin, eu, ev, nu, nv, grade_u=None, grade_v=None, order=8):
+    """Nodes (N, 3) and area weights (N,) of the rectangle origin + s eu + t ev,
+    s, t in [0, 1] (composite Gauss-Legendre, optionally graded toward s or
+    t = 0 / 1)."""
+    su, wu = gauss_panels(0.0, 1.0, nu, order, grade_u)
+    sv, wv = gauss_panels(0.0, 1.0, nv, order, grade_v)
+    S, T = np.meshgrid(su, sv, indexing="ij")
+    W = np.outer(wu, wv) * np.linalg.norm(np.cross(eu, ev))
+    P = origin[None, :] + S.reshape(-1, 1) * eu[None, :] + T.reshape(-1, 1) * ev[None, :]
+    return P, W.reshape(-1)
+
+
+def lambert_direct(P, n, rho, light_nodes, light_w, light_n, I):
+    """Reflected radiance of a Lambertian point set P (normal n) lit by an
+    unoccluded one-sided emitter (nodes, area weights, normal, radiance I):
+    rho/pi * sum_k I cos_p cos_l / r^2 w_k, with the BSDF's cut cos_p <= 5e-3
+    (material.h:271-273)."""
+    d = light_nodes[None, :, :] - P[:, None, :]
+    r2 = np.sum(d * d, axis=2)
+    r = np.sqrt(r2)
+    cp = np.sum(d * n, axis=2) / r
+    cl = -np.sum(d * light_n, axis=2) / r
+    f = np.where((cp > 5e-3) & (cl > 0.0), cp * cl / r2, 0.0)
+    return (rho / np.pi) * I * np.sum(f * light_w[None, :], axis=1)
+
+
+def lambert_gather(X, nx, rho, P, nP, LP, wP):
+    """One diffuse bounce: radiance at points X (normal nx, albedo rho) from
+    surface nodes P (normal nP, area weights wP) of outgoing radiance LP,
+    rho/pi sum L cos_x cos_p / r^2 w (cut cos_x <= 5e-3; P fully visible)."""
+    d = P[None, :, :] - X[:, None, :]
+    r2 = np.sum(d * d, axis=2)
+    r = np.sqrt(r2)
+    cx = np.sum(d * nx, axis=2) / r
+    cp = -np.sum(d * nP, axis=2) / r
+    f = np.where((cx > 5e-3) & (cp > 0.0), cx * cp / r2, 0.0)
+    return (rho / np.pi) * np.sum(f * (LP * wP)[None, :], axis=1)
+
+
+def camera_rays(camera, W, H, x, y, n, gauss=False):
+    """n x n sub-pixel rays of pixel (x, y) with camera.h sample_primary's
+    pinhole model: (origin, directions (n*n, 3)); midpoints, or with gauss
+    the Gauss-Legendre nodes (equal weights per axis are then not equal:
+    use gauss_pixel_weights)."""
+    c2w = np.asarray(camera.cam_to_world, np.float64)
+    fov = float(camera.fov_factor)
+    asp = float(camera.aspect_ratio)
+    if gauss:
+        k = 0.5 * (np.polynomial.legendre.leggauss(n)[0] + 1.0)
+    else:
+        k = (np.arange(n) + 0.5) / n
+    sx, sy = np.meshgrid((x + k) / W, (y + k) / H, indexing="xy")
+    dc = np.stack([fov * (sx.ravel() - 0.5) * -2.0, fov * (sy.ravel() - 0.5) * -2.0 / asp,
+                   np.ones(n * n)], axis=1)
+    dw = dc @ c2w[:3, :3].T
+    return c2w[:3, 3], dw / np.linalg.norm(dw, axis=1, keepdims=True)
+
+
+def gauss_pixel_weights(n):
+    """weights (n*n,) of camera_rays(..., gauss=True), summing to 1"""
+    w = 0.5 * np.polynomial.legendre.leggauss(n)[1]
+    return np.outer(w, w).ravel()
+
+
+# ------------------------------------------------ glossy occluded emitter --
+def triangle_rule(order=8):
+    """Collapsed Gauss-Legendre rule on the unit triangle (barycentric a, b):
+    nodes (N, 2) and weights summing to 1/2."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    x = 0.5 * (x + 1.0)
+    w = 0.5 * w
+    U, V = np.meshgrid(x, x, indexing="ij")
+    WU, WV = np.meshgrid(w, w, indexing="ij")
+    a = U
+    b = V * (1.0 - U)
+    return np.stack([a.ravel(), b.ravel()], 1), (WU * WV * (1.0 - U)).ravel()
+
+
+def polygon_nodes(poly3, rule):
+    """Quadrature nodes / area weights of a convex planar polygon (fan from
+    its first vertex)."""
+    tn, tw = rule
+    P, Wt = [], []
+    for k in range(1, len(poly3) - 1):
+        a, b, c = poly3[0], poly3[k], poly3[k + 1]
+        area2 = np.linalg.norm(np.cross(b - a, c - a))
+        P.append(a[None, :] + tn[:, :1] * (b - a)[None, :] + tn[:, 1:] * (c - a)[None, :])
+        Wt.append(tw * area2)
+    if not P:
+        return np.zeros((0, 3)), np.zeros(0)
+    return np.concatenate(P), np.concatenate(Wt)
+
+
+def occluded_light_polygon(p, blocker, light_y, light_half):
+    """The part of the square emitter {|x|, |z| <= light_half, y = light_y}
+    that the blocker triangle hides from p (3-D points)."""
+    B = np.asarray(blocker, np.float64)
+    inner = B.mean(axis=0)
+    poly = [(-light_half, -light_half), (light_half, -light_half), (light_half, light_half),
+            (-light_half, light_half)]
+    for a, b in BLOCKER_EDGES:
+        N = np.cross(B[a] - p, B[b] - p)
+        sgn = np.sign(np.dot(inner - p, N))
+        poly = clip(poly, lambda q, N=N, sgn=sgn: sgn * np.dot(np.array([q[0], light_y, q[1]]) - p, N))
+        if len(poly) < 3:
+            return []
+    return [np.array([x, light_y, z]) for x, z in poly]
+
+
+def glossy_occlusion_grad(camera, W, H, blocker, d_image, bsdf_at, I, light_y, light_half, n=2, eps=1e-5,
+                          order=8):
+    """dL/d(blocker vertices) of L = sum_p d_p . I_p for a ground point seen by
+    the camera under a square emitter (facing -y) partly hidden by the blocker:
+    I_p = integral over the visible emitter of f(wi, w) I cos_l / r^2 dA, f the
+    BSDF with its cosine (bsdf_at(p, wi, W) -> (N, 3)).  Only the hidden part
+    depends on the blocker, so dL = - sum d_p . d/dB integral over the hidden
+    polygon, by central differences of a fixed-order quadrature on it."""
+    B0 = np.asarray(blocker, np.float64)
+    rule = triangle_rule(order)
+    c2w = np.asarray(camera.cam_to_world, np.float64)
+    cam = c2w[:3, 3]
+    grad = np.zeros(9)
+
+    def hidden(p, wi, B):
+        poly = occluded_light_polygon(p, B, light_y, light_half)
+        if len(poly) < 3:
+            return np.zeros(3)
+        X, w = polygon_nodes(poly, rule)
+        d = X - p[None, :]
+        r2 = np.sum(d * d, axis=1)
+        dirs = d / np.sqrt(r2)[:, None]
+        cl = dirs[:, 1]  # emitter normal (0, -1, 0): cos = -n . (-dir)
+        f = bsdf_at(p, wi, dirs)
+        return np.sum(f * (I[None, :] * (cl / r2 * w)[:, None]), axis=0)
+
+    for x, y, p in ground_points(camera, W, H, n):
+        wi = (cam - p) / np.linalg.norm(cam - p)
+        dw = d_image[y, x].astype(np.float64) / (n * n)
+        for vi in range(3):
+            for c in range(3):
+                Bp, Bm = B0.copy(), B0.copy()
+                Bp[vi, c] += eps
+                Bm[vi, c] -= eps
+                g = (hidden(p, wi, Bp) - hidden(p, wi, Bm)) / (2.0 * eps)
+                grad[3 * vi + c] -= float(np.dot(dw, g))
+    return grad
+
+
+# ------------------------------------------------------------ node bound --
+def point_node_bound(p, n, wi, q, dlum, slum, exponent):
+    """edge_tree.cpp:345-480 for a node box shrunk to the point q, written
+    without its half-vector frame: the box's polar bound degenerates to
+    cos t' = sqrt((c + 1) / 2) with c = (q - p) . wi / |q - p|^2 (the
+    reference divides by the squared length, edge_tree.cpp:425-447), its
+    azimuth bound to the azimuth of q; the half-vector cosine bound is then
+    min(|sin t' (n . u)| + cos t' (n . wi), 1), u the unit part of q - p
+    orthogonal to wi."""
+    v = np.asarray(q, np.float64) - p
+    if np.dot(v, n) <= 0.0:
+        return 0.0
+    out = dlum / np.pi
+    if slum > 0.0:
+        c = np.dot(v, wi) / np.dot(v, v)
+        ct = np.sqrt((c + 1.0) / 2.0)
+        st = np.sqrt(max(1.0 - ct * ct, 0.0))
+        u = v - np.dot(v, wi) * wi
+        u = u / np.linalg.norm(u)
+        h = min(abs(st * np.dot(n, u)) + ct * np.dot(n, wi), 1.0)
+        out += slum * h ** exponent * (exponent + 2.0) / (2.0 * np.pi)
+    return out

@@ -12,7 +12,9 @@ from delta_ray_amd import delta_ray as dr
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# DR_ORACLE_SO: another build of the oracle (the planted-mistake builds of
+# tests/test_oracle_mutants.py)
+ORACLE_SO = os.environ.get("DR_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 _lib = None
 
@@ -25,7 +27,7 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(ORACLE_SO):
+    if not os.path.exists(ORACLE_SO) and not os.environ.get("DR_ORACLE_SO"):
         build()
     lib = ctypes.CDLL(ORACLE_SO)
     lib.or_load_ltc.restype = ctypes.c_int

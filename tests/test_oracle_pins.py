@@ -1,0 +1,493 @@
+"""Pins of the CPU oracle's parts that the reference's own files leave open
+(CPU only), each against a quantity computed independently in float64 NumPy:
+
+- two-bounce forward radiance (pathtrace.h:182-297 at max_bounces = 2):
+  direct and one-bounce indirect light in a Lambertian floor / wall corner
+  under a small one-sided emitter, by composite Gauss quadrature of the
+  truncated rendering equation (a wrong second-bounce throughput, pdf or MIS
+  weight fails it);
+- the glossy (LTC-specular) branch of the secondary-edge estimator
+  (delta_ray.cpp:86-110): a glossy ground under a partly hidden emitter, the
+  derivative of the BSDF-weighted visible emitter integral by quadrature of
+  the hidden polygon (the BSDF is the oracle's own, pinned bit for bit
+  against material.h by tests/test_oracle_ref.py);
+- the edge tree's selection weights (edge_tree.cpp:511-652): every edge that
+  can contribute must average a weight of 1 over the two passes;
+- continuation paths of edge samples (delta_ray.cpp:268-313): a wall that
+  sees a lit floor slab partly hidden by a black triangle, whose gradient is
+  the continuation term alone, against quadrature of the hidden floor area;
+- the node bound (edge_tree.cpp:345-480) against its closed forms: Lambert
+  only, and a node shrunk to a point, written without its half-vector frame.
+
+Three reference behaviours were found by these pins and are reproduced in
+parity mode, each behind a quirk bit of the render flags: the glossy branch
+counts the whole BSDF for either component (DR_QUIRK_EDGE_BSDF_SUM, twice the
+derivative), the side-hit Jacobian assumes rays from the world origin
+(DR_QUIRK_EDGE_HIT_JACOBIAN) and the cone pass's pmf bookkeeping
+under-weights edges (DR_QUIRK_CONE_PMF).  With the bits clear the estimator
+converges to the quadrature.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from delta_ray_amd import _abi, workloads
+from delta_ray_amd import delta_ray as dr
+from tests import closed_forms as cf
+
+# oracle-only knobs of oracle/oracle.cpp: the exact Dirac Jacobian and a fine
+# side offset of secondary-edge samples
+OR_EXACT_DIRAC = 1 << 16
+OR_FINE_OFFSET = 1 << 17
+EXACT_EDGES = OR_EXACT_DIRAC | OR_FINE_OFFSET
+
+
+# ------------------------------------------------------- two bounces ------
+_CORNER = dict(rhoF=0.6, rhoW=0.5, I=30.0, res=16)
+
+
+def _corner_scene():
+    c = _CORNER
+    fv, fi = workloads.quad([0, 0, 0], [0, 1, 0], 3.0)
+    wv, wi = workloads.quad([0, 1.5, 1.5], [0, 0, -1], 3.0)
+    lv, li = workloads.quad([0, 2.2, 0.2], [0, -1, 0], 0.25)
+    mf = workloads.diffuse_material([c["rhoF"]] * 3)
+    mw = workloads.diffuse_material([c["rhoW"]] * 3)
+    black = workloads.diffuse_material([0.0] * 3)
+    floor = dr.Shape(fv, fi, None, None, mf, None)
+    wall = dr.Shape(wv, wi, None, None, mw, None)
+    ls = dr.Shape(lv, li, None, None, black, None)
+    light = dr.Light(ls, np.array([c["I"]] * 3, np.float32))
+    ls.light = light
+    R = c["res"]
+    cam = workloads.make_camera([0, 1.2, -2.0], [0, 0.9, 1.5], [0, 1, 0], 50.0, (R, R))
+    return dict(camera=cam, shapes=[floor, wall, ls], materials=[mf, mw, black], lights=[light],
+                resolution=(R, R), max_bounces=2)
+
+
+def _first_hit(o, d):
+    """nearest of floor, wall, emitter along a ray: (t, 'F' | 'W' | 'L' | None)"""
+    best = (np.inf, None)
+    for kind, pt, nrm, lo, hi in (("F", (0, 0, 0), (0, 1, 0), (-1.5, -1, -1.5), (1.5, 1, 1.5)),
+                                  ("W", (0, 0, 1.5), (0, 0, -1), (-1.5, 0, 1.4), (1.5, 3, 1.6)),
+                                  ("L", (0, 2.2, 0.2), (0, -1, 0), (-0.125, 2.1, 0.075), (0.125, 2.3, 0.325))):
+        den = np.dot(d, nrm)
+        if abs(den) < 1e-12:
+            continue
+        t = np.dot(np.array(pt) - o, nrm) / den
+        x = o + t * d
+        if t > 0 and np.all(x >= np.array(lo)) and np.all(x <= np.array(hi)) and t < best[0]:
+            best = (t, kind)
+    return best
+
+
+def _corner_reference(panels):
+    """Per pixel (NaN where not tested): direct and one-bounce indirect
+    radiance averaged over the pixel by 4 x 4 Gauss-Legendre sub-pixel points
+    (the render's jitter is uniform over the pixel with the quirk bits
+    clear)."""
+    c = _CORNER
+    R = c["res"]
+    cam = _corner_scene()["camera"]
+    LN, LW = cf.rect_quadrature(np.array([-0.125, 2.2, 0.075]), np.array([0.25, 0, 0]), np.array([0, 0, 0.25]),
+                                2, 2, order=6)
+    ln = np.array([0.0, -1.0, 0.0])
+    nF, nW = np.array([0, 1.0, 0]), np.array([0, 0, -1.0])
+    nu, nv = panels
+    # both surfaces graded toward their common edge (y = 0, z = 1.5)
+    FP, FW = cf.rect_quadrature(np.array([-1.5, 0, -1.5]), np.array([3.0, 0, 0]), np.array([0, 0, 3.0]), nu, nv,
+                                grade_v=1.0)
+    WP, WW = cf.rect_quadrature(np.array([-1.5, 0, 1.5]), np.array([3.0, 0, 0]), np.array([0, 3.0, 0]), nu, nv,
+                                grade_v=0.0)
+    LdF = cf.lambert_direct(FP, nF, c["rhoF"], LN, LW, ln, c["I"])
+    LdW = cf.lambert_direct(WP, nW, c["rhoW"], LN, LW, ln, c["I"])
+    pw = cf.gauss_pixel_weights(4)
+    direct = np.full((R, R), np.nan)
+    indirect = np.full((R, R), np.nan)
+    for y in range(R):
+        for x in range(R):
+            o, D = cf.camera_rays(cam, R, R, x, y, 4, gauss=True)
+            hits = [_first_hit(o, d) for d in D]
+            kinds = {h[1] for h in hits}
+            if len(kinds) != 1 or None in kinds or "L" in kinds:
+                continue
+            kind = kinds.pop()
+            X = np.array([o + t * d for (t, _), d in zip(hits, D)])
+            # away from the corner (the gather integrand's scale) and the
+            # borders, and lit (the wall above the emitter's plane is dark)
+            if kind == "W":
+                if X[:, 1].min() < 0.4 or X[:, 1].max() > 2.0 or np.abs(X[:, 0]).max() > 1.2:
+                    continue
+                dd = cf.lambert_direct(X, nW, c["rhoW"], LN, LW, ln, c["I"])
+                ii = cf.lambert_gather(X, nW, c["rhoW"], FP, nF, LdF, FW)
+            else:
+                if X[:, 2].max() > 1.1 or X[:, 2].min() < -1.2 or np.abs(X[:, 0]).max() > 1.2:
+                    continue
+                dd = cf.lambert_direct(X, nF, c["rhoF"], LN, LW, ln, c["I"])
+                ii = cf.lambert_gather(X, nF, c["rhoF"], WP, nW, LdW, WW)
+            direct[y, x] = np.dot(dd, pw)
+            indirect[y, x] = np.dot(ii, pw)
+    return direct, indirect
+
+
+def test_two_bounce_forward_quadrature(oracle):
+    """B = 1 renders estimate the direct term; B = 2 minus B = 1 (same random
+    numbers: the first bounce draws the same) estimates the one-bounce
+    indirect term.  Each against quadrature, pixel by pixel and summed."""
+    d1, i1 = _corner_reference((12, 10))
+    d2, i2 = _corner_reference((16, 14))
+    m = ~np.isnan(d2)
+    assert m.sum() >= 100, m.sum()
+    # the quadrature has converged
+    assert np.max(np.abs(d1 - d2)[m] / d2[m]) < 1e-4
+    assert np.max(np.abs(i1 - i2)[m] / i2[m]) < 1e-4
+    w = _corner_scene()
+    osc = oracle.scene_from_workload(w)
+    R = _CORNER["res"]
+    K, spp = 8, 1024
+    b1, b2 = [], []
+    for s in range(K):
+        a, _, _ = osc.render(R, R, spp, 1, 100 + s, flags=0)
+        b, _, _ = osc.render(R, R, spp, 2, 100 + s, flags=0)
+        b1.append(a.mean(2))
+        b2.append(b.mean(2))
+    b1, b2 = np.array(b1), np.array(b2)
+    for name, est, ref in (("direct", b1, d2), ("indirect", b2 - b1, i2)):
+        mu = est.mean(0)
+        se = est.std(0, ddof=1) / np.sqrt(K)
+        e = (mu - ref)[m]
+        sum_z = e.sum() / np.sqrt((se[m] ** 2).sum())
+        # per pixel: 5 standard errors, the error floored at half the median
+        # relative error (a few pixels' seed spreads are small by chance)
+        floor = 0.5 * np.median(se[m] / ref[m]) * ref[m]
+        worst = np.max(np.abs(e) / np.maximum(se[m], floor))
+        print(name, "sum rel err %.2e  sum z %.2f  worst pixel z %.2f" % (e.sum() / ref[m].sum(), sum_z, worst))
+        assert abs(sum_z) < 4.5, (name, sum_z)
+        assert worst < 6.0, (name, worst)
+        assert abs(e.sum() / ref[m].sum()) < 0.01, name
+
+
+# ------------------------------------------------- glossy secondary edges --
+_SHADOW = dict(W=12, light_y=3.0, light_half=0.5, I=20.0,
+               blocker=[[-0.1, 1.4, -0.6], [0.5, 1.5, 0.0], [-0.3, 1.6, 0.5]])
+
+
+def _glossy_shadow_scene(diffuse=0.25, specular=0.5, roughness=0.3):
+    c = _SHADOW
+    W = c["W"]
+    cam = workloads.make_camera([0, 0.8, 0], [0, 0, 0], [0, 0, 1], 60.0, (W, W))
+    gv, gi = workloads.quad([0, 0, 0], [0, 1, 0], 10.0)
+    one = np.ones(2, np.float32)
+    gm = dr.Material(np.array([diffuse] * 3, np.float32), np.array([specular] * 3, np.float32),
+                     np.array([roughness], np.float32), one, one, one, False)
+    black = workloads.diffuse_material([0.0] * 3)
+    ground = dr.Shape(gv, gi, None, None, gm, None)
+    blk = dr.Shape(np.asarray(c["blocker"], np.float32), np.array([[0, 1, 2]], np.int32), None, None, black, None)
+    lv, li = workloads.quad([0, c["light_y"], 0], [0, -1, 0], 2 * c["light_half"])
+    ls = dr.Shape(lv, li, None, None, black, None)
+    light = dr.Light(ls, np.array([c["I"]] * 3, np.float32))
+    ls.light = light
+    return dict(camera=cam, shapes=[ground, blk, ls], materials=[gm, black], lights=[light], resolution=(W, W),
+                max_bounces=1)
+
+
+def _bsdf_batch(oracle, osc, mid):
+    lib = oracle.load()
+    lib.or_probe_bsdf_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+
+    def at(p, wi, dirs):
+        sp = np.zeros(17, np.float32)
+        sp[0:3] = p
+        sp[3:6] = (0, 1, 0)            # the ground's geometric normal
+        sp[6:15] = (1, 0, 0, 0, 0, -1, 0, 1, 0)  # a frame about +y
+        wi32 = np.asarray(wi, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        out = np.zeros((len(d), 3), np.float32)
+        lib.or_probe_bsdf_batch(osc.h, mid, sp.ctypes.data, wi32.ctypes.data, d.ctypes.data, len(d),
+                                out.ctypes.data)
+        return out.astype(np.float64)
+    return at
+
+
+@pytest.fixture(scope="module")
+def glossy_case(oracle):
+    c = _SHADOW
+    w = _glossy_shadow_scene()
+    osc = oracle.scene_from_workload(w)
+    W = c["W"]
+    d = np.random.default_rng(3).uniform(0.5, 1.5, size=(W, W, 3)).astype(np.float32)
+    exact = cf.glossy_occlusion_grad(w["camera"], W, W, c["blocker"], d, _bsdf_batch(oracle, osc, 0),
+                                     np.array([c["I"]] * 3, np.float64), c["light_y"], c["light_half"])
+    return w, osc, d, exact
+
+
+def _blocker_grad(osc, w, d, flags, spp=2048, seed=6):
+    W, H = w["resolution"]
+    _, g, _ = osc.render(W, H, spp, w["max_bounces"], seed, flags=flags, d_image=d)
+    off, _ = osc.sd.layout["shapes"][1]["vertices"]
+    return g[off:off + 9]
+
+
+def test_secondary_edges_glossy_exact(glossy_case):
+    """Glossy ground (diffuse and specular luminance 1:2, roughness 0.3): with
+    the quirk bits clear the estimator (both components' LTC line densities)
+    converges to the derivative of the glossy-weighted visible emitter."""
+    w, osc, d, exact = glossy_case
+    got = _blocker_grad(osc, w, d, EXACT_EDGES)
+    print("exact", exact, "\ngot", got)
+    np.testing.assert_array_less(np.abs(got - exact), 0.012 * np.abs(exact).max())
+
+
+def test_secondary_edges_glossy_reference_twice(glossy_case):
+    """The reference divides the whole BSDF by the chosen component's pmf
+    (delta_ray.cpp:86-101, 249, 326): each component's density then
+    integrates the whole BSDF, so the expectation is twice the derivative on
+    this two-component material."""
+    w, osc, d, exact = glossy_case
+    got = _blocker_grad(osc, w, d, EXACT_EDGES | _abi.DR_QUIRK_EDGE_BSDF_SUM)
+    np.testing.assert_array_less(np.abs(got - 2.0 * exact), 0.024 * np.abs(exact).max())
+
+
+# ------------------------------------------------- edge-tree weights ------
+def _wall_scene(blocker):
+    fv, fi = workloads.quad([0, 0, -1.1], [0, 1, 0], 1.8)
+    wv, wi = workloads.quad([0, 1.25, 1.0], [0, 0, -1], 2.5)
+    lv, li = workloads.quad([0, 2.5, -1.2], [0, -1, 0], 0.6)
+    mf = workloads.diffuse_material([0.8] * 3)
+    mw = workloads.diffuse_material([0.6] * 3)
+    black = workloads.diffuse_material([0.0] * 3)
+    slab = dr.Shape(fv, fi, None, None, mf, None)
+    wall = dr.Shape(wv, wi, None, None, mw, None)
+    blk = dr.Shape(np.asarray(blocker, np.float32), np.array([[0, 1, 2]], np.int32), None, None, black, None)
+    ls = dr.Shape(lv, li, None, None, black, None)
+    light = dr.Light(ls, np.array([60.0] * 3, np.float32))
+    ls.light = light
+    # the camera sees the wall from 0.86 to 2.5 high, well above the blocker
+    # (wall points at its height see it edge-on, where the reference's
+    # ad-hoc rejection of grazing side hits, delta_ray.cpp:296-313, drops
+    # samples)
+    cam = workloads.make_camera([0, 1.7, -2.8], [0, 1.7, 1.0], [0, 1, 0], 25.0, (16, 16))
+    return dict(camera=cam, shapes=[slab, wall, blk, ls], materials=[mf, mw, black], lights=[light],
+                resolution=(16, 16), max_bounces=2)
+
+
+_BLOCKER = [[-0.3, 0.45, 0.2], [0.35, 0.5, 0.35], [0.0, 0.55, -0.25]]
+
+
+def _edge_weights(oracle, osc, x1, cam_pos, flags, trials=100000):
+    lib = oracle.load()
+    lib.or_probe_edge_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64,
+                                          ctypes.c_uint32, ctypes.c_void_p]
+    wi = (cam_pos - x1) / np.linalg.norm(cam_pos - x1)
+    sp = np.zeros(17, np.float32)
+    sp[0:3] = x1
+    sp[3:6] = (0, 0, -1)
+    sp[6:15] = (1, 0, 0, 0, 1, 0, 0, 0, -1)  # frame of the wall (normal -z)
+    lc = np.array([0, 2.5, -1.2])
+    dl = lc - x1
+    tan = 0.3 * np.sqrt(2.0) / np.linalg.norm(dl)
+    cone = np.array(list(x1) + list(dl / np.linalg.norm(dl)) + [1.0 / np.sqrt(tan * tan + 1.0)], np.float32)
+    sums = np.zeros(osc.info()["edges"], np.float64)
+    wi32 = wi.astype(np.float32)  # (kept alive across the call)
+    lib.or_probe_edge_weights(osc.h, 1, 0, sp.ctypes.data, wi32.ctypes.data, cone.ctypes.data, 7, trials, flags,
+                              sums.ctypes.data)
+    return sums / trials
+
+
+def test_edge_tree_weights_average_one(oracle):
+    """From a wall point: the wall's own edges (in its plane) never count;
+    every other edge averages a weight of 1 over the cone and all passes.
+    With the reference's cone-pass bookkeeping (DR_QUIRK_CONE_PMF) an edge
+    reached through a node that holds the point and misses the light cone is
+    under-weighted."""
+    w = _wall_scene(_BLOCKER)
+    osc = oracle.scene_from_workload(w)
+    x1 = np.array([-0.5, 0.9, 1.0])
+    cam_pos = np.asarray(w["camera"].cam_to_world)[:3, 3].astype(np.float64)
+    mean = _edge_weights(oracle, osc, x1, cam_pos, 0)
+    wall = slice(4, 8)   # edges sorted by (shape, v0, v1): slab 0-3, wall 4-7
+    np.testing.assert_array_equal(mean[wall], 0.0)
+    others = np.r_[mean[:4], mean[8:]]
+    np.testing.assert_allclose(others, 1.0, atol=0.015)
+    quirk = _edge_weights(oracle, osc, x1, cam_pos, _abi.DR_QUIRK_CONE_PMF)
+    print("exact", mean.round(3), "\nreference", quirk.round(3))
+    assert quirk.min() < 0.9 * 1.0 and np.sum(np.abs(np.r_[quirk[:4], quirk[8:]] - 1.0) > 0.05) >= 1
+
+
+# -------------------------------------------------- continuation paths ----
+def _hidden_slab_grad(camera, W, H, blocker, d_image, n=2, eps=1e-4):
+    """dL/d(blocker translation x, y) of L = sum_p d_p . I_p for the wall
+    scene: the wall point x1 sees the lit slab except where the blocker hides
+    it, so only the hidden part depends on the blocker:
+    I(x1) = rho_w/pi sum (cos1 L_o(x2) cos2 / r^2) over the visible slab, L_o the
+    slab's direct radiance (the blocker casts no shadow on the slab)."""
+    LN, LW = cf.rect_quadrature(np.array([-0.3, 2.5, -1.5]), np.array([0.6, 0, 0]), np.array([0, 0, 0.6]), 2, 2,
+                                order=6)
+    ln = np.array([0, -1.0, 0])
+    nS, nW = np.array([0, 1.0, 0]), np.array([0, 0, -1.0])
+    rule = cf.triangle_rule(10)
+    c2w = np.asarray(camera.cam_to_world, np.float64)
+
+    def hidden(x1, B):
+        P = []
+        for v in B:
+            dv = v - x1
+            t = -x1[1] / dv[1]
+            P.append((x1[0] + t * dv[0], x1[2] + t * dv[2]))
+        a = (P[1][0] - P[0][0]) * (P[2][1] - P[0][1]) - (P[1][1] - P[0][1]) * (P[2][0] - P[0][0])
+        if a < 0:
+            P = [P[0], P[2], P[1]]
+        for f in (lambda q: q[0] + 0.9, lambda q: 0.9 - q[0], lambda q: q[1] + 2.0, lambda q: -0.2 - q[1]):
+            P = cf.clip(P, f)
+            if len(P) < 3:
+                return 0.0
+        X, wq = cf.polygon_nodes([np.array([x, 0.0, z]) for x, z in P], rule)
+        Lo = cf.lambert_direct(X, nS, 0.8, LN, LW, ln, 60.0)
+        dd = X - x1[None, :]
+        r2 = np.sum(dd * dd, 1)
+        r = np.sqrt(r2)
+        c1 = (dd @ nW) / r
+        c2 = -(dd @ nS) / r
+        return float(np.sum(0.6 / np.pi * np.where(c1 > 5e-3, c1, 0.0) * Lo * c2 / r2 * wq))
+
+    B0 = np.asarray(blocker, np.float64)
+    pw = cf.gauss_pixel_weights(n)
+    out = np.zeros(2)
+    for y in range(H):
+        for x in range(W):
+            o, D = cf.camera_rays(camera, W, H, x, y, n, gauss=True)
+            X1 = [o + ((1.0 - o[2]) / dv[2]) * dv for dv in D]   # the wall plane z = 1
+            wsum = float(d_image[y, x].astype(np.float64).sum())  # grey scene: the three channels alike
+            for k, axis in enumerate((0, 1)):
+                T = np.zeros((3, 3))
+                T[:, axis] = eps
+                g = np.dot(pw, [-(hidden(x1, B0 + T) - hidden(x1, B0 - T)) / (2 * eps) for x1 in X1])
+                out[k] += wsum * g
+    return out
+
+
+def test_edge_continuation_paths(oracle):
+    """The wall sees a floor slab lit by an emitter above; a black triangle
+    hides part of the slab from the wall but casts no shadow on the slab and
+    is not seen by the camera: its gradient is the secondary-edge samples'
+    continuation term alone (delta_ray.cpp:268-313).  With the quirk bits
+    clear (and the exact Dirac Jacobian) it converges to the quadrature; with
+    the reference's side-hit Jacobian (rays from the world origin) it
+    vanishes here, the slab's plane passing through the origin."""
+    w = _wall_scene(_BLOCKER)
+    W = 16
+    d = np.random.default_rng(42).uniform(0.5, 1.5, size=(W, W, 3)).astype(np.float32)
+    ref = _hidden_slab_grad(w["camera"], W, W, _BLOCKER, d)
+    osc = oracle.scene_from_workload(w)
+    off, _ = osc.sd.layout["shapes"][2]["vertices"]
+    K = 6
+    est = []
+    for s in range(K):
+        _, g, _ = osc.render(W, W, 1024, 2, 10 + s, flags=EXACT_EDGES, d_image=d)
+        gb = g[off:off + 9].reshape(3, 3).astype(np.float64)
+        est.append([gb[:, 0].sum(), gb[:, 1].sum()])
+    est = np.array(est)
+    mu, se = est.mean(0), est.std(0, ddof=1) / np.sqrt(K)
+    print("quadrature", ref, "estimate", mu, "+-", se)
+    # the B = 1 render does not depend on the blocker at all
+    _, g1, _ = osc.render(W, W, 256, 1, 5, flags=EXACT_EDGES, d_image=d)
+    assert np.all(g1[off:off + 9] == 0.0)
+    tol = 4.0 * se + 0.03 * np.abs(ref).max()
+    assert np.all(np.abs(mu - ref) < tol), (mu, ref, tol)
+    _, gq, _ = osc.render(W, W, 256, 2, 5, flags=EXACT_EDGES | _abi.DR_QUIRK_EDGE_HIT_JACOBIAN, d_image=d)
+    assert np.abs(gq[off:off + 9]).max() < 1e-5 * np.abs(ref).max()
+
+
+# -------------------------------------------------------------- bound -----
+def _bound_probe(oracle):
+    lib = oracle.load()
+    lib.or_probe_bsdf_bound.restype = ctypes.c_float
+    lib.or_probe_bsdf_bound.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_float]
+    return lib
+
+
+def test_node_bound_closed_forms(oracle):
+    """bsdf_bound (edge_tree.cpp:345-480):
+    - Lambert part: dlum/pi when some box corner is above the tangent plane,
+      else 0;
+    - a box shrunk to a point q: dlum/pi + slum (e+2)/(2 pi) h^e with
+      h = min(|sin t' (n . u)| + cos t' (n . wi), 1), cos t' = sqrt((c + 1)/2),
+      c = (q - p) . wi / |q - p|^2, u the unit part of q - p orthogonal to wi
+      (closed_forms.point_node_bound), on both sides of the azimuth."""
+    rough = 0.2
+    e = 2.0 / rough
+    one = np.ones(2, np.float32)
+    mat = dr.Material(np.array([0.3] * 3, np.float32), np.array([0.5] * 3, np.float32),
+                      np.array([rough], np.float32), one, one, one, False)
+    w = helpers_light_scene(mat)
+    osc = oracle.scene_from_workload(w)
+    lib = _bound_probe(oracle)
+    rng = np.random.default_rng(8)
+    sides = {True: 0, False: 0}
+    checked = 0
+    for trial in range(700):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        t = np.cross(n, [0.3, 1.0, 0.2])
+        t /= np.linalg.norm(t)
+        b = np.cross(n, t)
+        p = rng.uniform(-1, 1, size=3)
+        wi = rng.normal(size=3)
+        wi -= min(np.dot(wi, n), 0.0) * 2.0 * n  # on the normal's side
+        wi /= np.linalg.norm(wi)
+        if np.dot(wi, n) < 0.05:
+            continue
+        sp = np.zeros(17, np.float32)
+        sp[0:3] = p
+        sp[3:6] = n
+        sp[6:9], sp[9:12], sp[12:15] = t, b, n
+        dlum, slum = rng.uniform(0.1, 0.6), rng.uniform(0.0, 0.9) if trial % 3 else 0.0
+        # Lambert part with a random box
+        if slum == 0.0:
+            lo = rng.uniform(-2, 2, size=3)
+            hi = lo + rng.uniform(0.01, 1.0, size=3)
+            corners = np.array([[lo[0] if i & 1 == 0 else hi[0], lo[1] if i & 2 == 0 else hi[1],
+                                 lo[2] if i & 4 == 0 else hi[2]] for i in range(8)])
+            above = np.any((corners - p) @ n > 0)
+            wi32, lo32, hi32 = wi.astype(np.float32), lo.astype(np.float32), hi.astype(np.float32)
+            got = lib.or_probe_bsdf_bound(osc.h, 0, sp.ctypes.data, wi32.ctypes.data, lo32.ctypes.data,
+                                          hi32.ctypes.data, dlum, 0.0)
+            expect = dlum / np.pi if above else 0.0
+            assert abs(got - expect) <= 1e-6 * max(expect, 1e-30), (got, expect)
+            checked += 1
+            continue
+        # at distance >= 1 (so |c| <= 1: nearer points behind wi make the
+        # reference's half-angle cosine the square root of a negative number)
+        dq = rng.normal(size=3)
+        q = (p + rng.uniform(1.0, 3.0) * dq / np.linalg.norm(dq)).astype(np.float32).astype(np.float64)
+        expect = cf.point_node_bound(p.astype(np.float32).astype(np.float64), n.astype(np.float32),
+                                     wi.astype(np.float32), q, dlum, slum, e)
+        qq, wi32 = q.astype(np.float32), wi.astype(np.float32)
+        got = lib.or_probe_bsdf_bound(osc.h, 0, sp.ctypes.data, wi32.ctypes.data, qq.ctypes.data, qq.ctypes.data,
+                                      dlum, slum)
+        v = q - p
+        if np.dot(v, n) > 0:
+            u = v - np.dot(v, wi) * wi
+            sides[bool(np.dot(u, n - np.dot(n, wi) * wi) >= 0)] += 1
+        assert abs(got - expect) <= 2e-4 * max(expect, 1e-3), (trial, got, expect)
+        checked += 1
+    assert checked > 500 and sides[True] > 60 and sides[False] > 40, (checked, sides)
+
+
+def helpers_light_scene(mat):
+    """a one-triangle scene of material `mat` with a light (the probes read
+    the material from scene 0 of the oracle)"""
+    black = workloads.diffuse_material([0.0] * 3)
+    tri = dr.Shape(np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1]], np.float32), np.array([[0, 1, 2]], np.int32),
+                   None, None, mat, None)
+    lv, li = workloads.quad([0, 3, 0], [0, -1, 0], 1.0)
+    ls = dr.Shape(lv, li, None, None, black, None)
+    light = dr.Light(ls, np.array([1.0] * 3, np.float32))
+    ls.light = light
+    cam = workloads.make_camera([0, 2, -3], [0, 0, 0], [0, 1, 0], 45.0, (8, 8))
+    return dict(camera=cam, shapes=[tri, ls], materials=[mat, black], lights=[light], resolution=(8, 8),
+                max_bounces=1)
