@@ -4,10 +4,14 @@ the GPU must give what a scene built from those vertices gives.
 * Ray queries and forward images do not depend on the trees' topology (the
   closest hit is the one with the least t, ties to the smaller triangle id),
   so a refitted scene must match a freshly built one bit for bit there.
-* Scaling a mesh by 2 is exact in floating point and keeps every ordering the
-  builder's median splits see, so the refitted trees have the fresh build's
-  topology, and every adjoint term must match it up to the order of float
-  atomics (and the last bit of acosf in a normal cone).
+* Scaling every shape of a scene by a power of two about the origin is exact
+  in floating point and keeps every ordering and every comparison the
+  builders make (median splits, SAH costs, the coplanar-edge filter), so the
+  refitted trees have the fresh build's topology, and every adjoint term must
+  match it up to the order of float atomics (and the last bit of acosf in a
+  normal cone).  (Scaling one shape of several would reorder its edges
+  against the others' in the median splits: a fresh build of that scene has
+  other trees, and its Monte Carlo estimates differ from the refit's.)
 * When the filter of degenerate / coplanar edges changes its verdict, the
   library rebuilds the structures on the host: the result is a fresh build.
 """
@@ -34,6 +38,20 @@ def _with_vertices(w, verts, normals=None):
     n = s0.normals if normals is None else normals
     s = dr.Shape(verts, s0.indices, s0.uvs, n, s0.material, s0.light)
     return dict(w, shapes=[s] + list(w["shapes"][1:]))
+
+
+def _scaled(w, k):
+    """the workload with every shape scaled by k about the origin (lights
+    follow their shapes)"""
+    shapes, lights = [], []
+    for s in w["shapes"]:
+        ns = dr.Shape((k * np.asarray(s.vertices)).astype(np.float32), s.indices, s.uvs, s.normals,
+                      s.material, None)
+        if s.light is not None:
+            ns.light = dr.Light(ns, np.asarray(s.light.intensity))
+            lights.append(ns.light)
+        shapes.append(ns)
+    return dict(w, shapes=shapes, lights=lights)
 
 
 def _update(scene, verts, normals=None):
@@ -81,10 +99,10 @@ def test_refit_back_to_built_vertices_reproduces_scene(gpu):
     n0 = np.asarray(w["shapes"][0].normals).copy()
     ref = gpu_scene(w)
     sc = gpu_scene(w)
-    v1 = v0.copy()
-    v1[:, 1] += 0.05 * np.sin(3.0 * v0[:, 0])
-    n1 = workloads.vertex_normals(v1, np.asarray(w["shapes"][0].indices))
-    _update(sc, v1, n1)
+    # a move that keeps the coplanar-edge filter's verdicts (a generic one
+    # flips the verdict of nearly coplanar faces of the heightfield, and the
+    # library rebuilds), and its undo
+    _update(sc, 4.0 * v0, n0)
     _update(sc, v0, n0)
     info = sc.info()
     assert info["refits"] == 2 and info["rebuilds"] == 0, info
@@ -94,12 +112,13 @@ def test_refit_back_to_built_vertices_reproduces_scene(gpu):
 def test_refit_matches_fresh_build(gpu):
     print("case: refit scaled", flush=True)
     w = helpers.small_workload("heightfield", res=48, grid=24)
-    v0 = np.asarray(w["shapes"][0].vertices)
-    v2 = (2.0 * v0).astype(np.float32)
-    w2 = _with_vertices(w, v2)
+    w2 = _scaled(w, 2.0)
     sc = gpu_scene(w)
-    _update(sc, v2)
-    assert sc.info()["refits"] == 1
+    import torch
+    sc.update_vertices([(i, torch.from_numpy(np.asarray(s.vertices)).cuda(), None)
+                        for i, s in enumerate(w2["shapes"])])
+    info = sc.info()
+    assert info["refits"] == 1 and info["rebuilds"] == 0, info
     _compare_scenes(w2, sc, w2, gpu_scene(w2), 1e-5, 1e-6)
 
 
