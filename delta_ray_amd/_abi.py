@@ -14,6 +14,10 @@ LIB_PATH = os.path.join(_HERE, "libdelta_ray_amd.so")
 LTC_PATH = os.path.join(_HERE, "data", "ltc_tabM.f32")
 
 DR_OK = 0
+DR_ERROR_INVALID_ARGUMENT = 1
+DR_ERROR_OUT_OF_MEMORY = 2
+DR_ERROR_DEVICE = 3
+DR_ERROR_UNSUPPORTED = 4
 
 DR_QUIRK_JITTER_OVER_SPP = 1 << 0
 DR_QUIRK_TEXEL_SWAP = 1 << 1
@@ -147,6 +151,9 @@ ENTRY_POINTS = [
     ("dr_render_adjoint", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_render_options),
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p]),
+    ("dr_render_adjoint_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_render_options),
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p]),
     ("dr_intersect", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                     ctypes.c_int, ctypes.c_void_p]),
     ("dr_scene_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(dr_stats)]),

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -178,9 +179,12 @@ __global__ void __launch_bounds__(DR_BLOCK) k_path(const DrJob *__restrict__ job
 // exceed cap: the host grows the queue and renders again), cnt[1]: work
 // counter.
 #define DR_EMIT_GROUP 32
+// Points are fetched in the order f in [f_begin, f_end) (a pass's points are
+// f in [0, nslots); a part of them when a pass's selection runs in parts).
 __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ jobs, long long nslots, int maxb,
                                                      const PointRec *pts, EdgeEntry *q, unsigned long long *cnt,
-                                                     long long cap, int *counts, int refill) {
+                                                     long long cap, int *counts, int refill, long long f_begin,
+                                                     long long f_end) {
     const long long per_bounce = nslots / maxb;  // nslots is a multiple of maxb
     __shared__ float s_stack[DR_LDS_STACK * DR_LDS_WORDS * DR_BLOCK];
     const int lane = (int)__lane_id();
@@ -225,11 +229,11 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
                 base = __shfl(base, leader);
                 if (want) {
                     unsigned long long below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-                    long long f = (long long)(base + __popcll(below));
+                    long long f = f_begin + (long long)(base + __popcll(below));
                     // points in the order bounce-major: a wave's points are the
                     // same bounce of neighbouring samples and pixels, which
                     // walk similar tree paths (slots are bounce-minor)
-                    long long s = f < nslots ? (f % per_bounce) * maxb + f / per_bounce : f;
+                    long long s = f < f_end ? (f % per_bounce) * maxb + f / per_bounce : nslots;
                     if (s >= nslots) {
                         more = false;
                         want = false;
@@ -1239,7 +1243,7 @@ static dr_status enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs
             LaunchTimer t(tm, TK_SELECT, hs);
             long long blocks = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_select, g_select_blocks));
             k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, MAXB, w.pts, w.q, w.cnt, w.q_cap,
-                                                            w.counts, P.select_refill);
+                                                            w.counts, P.select_refill, 0ll, ps.nslots);
         };
         select();
         if (sync_select) {
@@ -1369,6 +1373,103 @@ static dr_status single_job(dr_scene *s, const dr_render_options *o, const float
     dr_status st = build_plan(&jb, 1, Tuning(), P);
     if (st != DR_OK) return st;
     return run_plan(P, s->ws, &s->tm, hs, false);
+}
+
+// ---------------------------------------------- allocation-free adjoint ---
+// dr_render_adjoint_noalloc keeps every device buffer beyond the caller's
+// inputs and outputs in `scratch`, a fixed array of this library (each GPU
+// the library runs on holds its own copy until the process ends), for
+// callers that cannot allocate.  Its passes are sized to fit the array; a
+// pass whose edge selection overflows the queue part of it is selected and
+// sampled in parts (a shading point's edges always fall in one part, so its
+// Dirac derivatives and the last-ordinal rule see all of them).
+#define DR_NOALLOC_BYTES (256ull << 20)
+#define DR_NOALLOC_QUEUE_PER_SLOT 1
+#define DR_NOALLOC_MAX_RANGES 4096
+static __device__ __align__(256) unsigned char scratch[DR_NOALLOC_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[256];  // device address of `scratch`, per device
+
+static dr_status scratch_address(void **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 256) return fail(DR_ERROR_DEVICE, "no current device");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(DR_ERROR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return DR_OK;
+}
+
+// slices of the scratch array (256-byte aligned)
+struct Carver {
+    char *base;
+    size_t off, cap;
+    template <class T>
+    T *take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T *p = (T *)(base + off);
+        off += n * sizeof(T);
+        return off <= cap ? p : nullptr;
+    }
+};
+
+template <int MAXB>
+static dr_status run_noalloc(const Plan &P, Workspace &w, hipStream_t hs) {
+    for (auto &z : P.zero) k_fill<<<256, 256, 0, hs>>>(z.first, z.second, 0.f);
+    k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 2), 2, 0.f);
+    double per = -1.0;  // selected edges per point of the last part (sizes the next one)
+    for (const Pass &ps : P.passes) {
+        const DrRange *R = w.ranges + ps.r0;
+        int nr = ps.r1 - ps.r0;
+        k_path<MAXB><<<(unsigned)ps.nblocks, DR_BLOCK, 0, hs>>>(w.jobs, R, nr, w.pts);
+        if (!ps.edges) continue;
+        k_fill<<<1024, 256, 0, hs>>>(w.pt_dp, (size_t)ps.nslots * 3, 0.f);
+        k_fill_i<<<1024, 256, 0, hs>>>(w.pt_lastj, (size_t)ps.nslots, -1);
+        long long part = per > 0.0 ? std::max(1ll, (long long)(0.9 * (double)w.q_cap / per)) : ps.nslots;
+        for (long long f0 = 0; f0 < ps.nslots;) {
+            long long f1 = std::min(ps.nslots, f0 + part);
+            k_fill<<<1, 64, 0, hs>>>((float *)w.cnt, 2 * 2, 0.f);       // queue length, work counter
+            k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 3), 2, 0.f);     // continuations
+            long long blocks = std::min<long long>(grid_of(f1 - f0), resident_blocks(k_select, g_select_blocks));
+            k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, MAXB, w.pts, w.q, w.cnt, w.q_cap,
+                                                            w.counts, P.select_refill, f0, f1);
+            unsigned long long len = 0;
+            if (hipMemcpyAsync(&len, w.cnt, sizeof(len), hipMemcpyDeviceToHost, hs) != hipSuccess ||
+                hipStreamSynchronize(hs) != hipSuccess)
+                return fail(DR_ERROR_DEVICE, "reading the edge queue length failed");
+            per = (double)len / (double)(f1 - f0);
+            if ((long long)len > w.q_cap) {
+                if (f1 - f0 <= 1)
+                    return fail(DR_ERROR_UNSUPPORTED, "a shading point selects more edges than the allocation-free "
+                                                      "queue holds");
+                part = std::max(1ll, std::min((f1 - f0) / 2, (long long)(0.9 * (double)w.q_cap / per)));
+                continue;
+            }
+            long long eb = resident_blocks(k_edge<MAXB>, g_edge_blocks[MAXB == 1 ? 0 : MAXB == 2 ? 1 : MAXB == 4 ? 2 : 3]);
+            k_edge<MAXB><<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.jobs, w.q, w.cnt, w.q_cap, w.pts, w.pt_dp, w.pt_lastj,
+                                                            w.entry_dp, w.cont);
+            long long sb = resident_blocks(k_edge_shade<MAXB>,
+                                           g_shade_blocks[MAXB == 1 ? 0 : MAXB == 2 ? 1 : MAXB == 4 ? 2 : 3]);
+            k_edge_shade<MAXB><<<(unsigned)sb, DR_BLOCK, 0, hs>>>(w.jobs, w.q, w.cnt, w.pts, w.pt_dp, w.pt_lastj,
+                                                                  w.entry_dp, w.cont);
+            if (ps.dp_last)
+                k_dp_last<<<(unsigned)eb, DR_BLOCK, 0, hs>>>(w.q, w.cnt, w.q_cap, w.entry_dp, w.pt_lastj, w.pt_dp);
+            f0 = f1;
+            if (per > 0.0) part = std::max(1ll, (long long)(0.9 * (double)w.q_cap / per));
+        }
+        k_dp<MAXB><<<(unsigned)ps.nblocks, DR_BLOCK, 0, hs>>>(w.jobs, R, nr, w.pts, w.pt_dp, w.counts);
+    }
+    if (P.prim_r1 > P.prim_r0)
+        k_primary<MAXB><<<(unsigned)P.prim_blocks, DR_BLOCK, 0, hs>>>(w.jobs, w.ranges + P.prim_r0,
+                                                                       P.prim_r1 - P.prim_r0);
+    if (P.fwd_r1 > P.fwd_r0)
+        k_forward<<<(unsigned)P.fwd_blocks, DR_BLOCK, 0, hs>>>(w.jobs, w.ranges + P.fwd_r0, P.fwd_r1 - P.fwd_r0);
+    return launch_check("render kernels");
 }
 
 extern "C" {
@@ -1740,6 +1841,68 @@ dr_status dr_render_adjoint(dr_scene *s, const dr_render_options *o, const float
     if (st != DR_OK) return st;
     if (!d_image || !image || !grads) return fail(DR_ERROR_INVALID_ARGUMENT, "d_image, image and grads are required");
     return single_job(s, o, d_image, image, grads, dx_image, (hipStream_t)stream);
+}
+
+dr_status dr_render_adjoint_noalloc(dr_scene *s, const dr_render_options *o, const float *d_image, float *image,
+                                    float *grads, float *dx_image, void *stream) {
+    if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
+    dr_status st = check_opts(o);
+    if (st != DR_OK) return st;
+    if (!d_image || !image || !grads) return fail(DR_ERROR_INVALID_ARGUMENT, "d_image, image and grads are required");
+    if (o->max_bounces > 8) return fail(DR_ERROR_UNSUPPORTED, "the adjoint supports max_bounces <= 8");
+    hipStream_t hs = (hipStream_t)stream;
+    void *base = nullptr;
+    st = scratch_address(&base);
+    if (st != DR_OK) return st;
+    dr_job jb;
+    std::memset(&jb, 0, sizeof(jb));
+    jb.scene = s;
+    jb.opts = *o;
+    jb.d_image = d_image;
+    jb.image = image;
+    jb.grads = grads;
+    jb.dx_image = dx_image;
+    // passes sized so that points and queue fit the array after the tables
+    const size_t fixed = 64 * 1024 + DR_NOALLOC_MAX_RANGES * sizeof(DrRange) + 4 * 256;
+    const size_t per_slot = sizeof(PointRec) + 3 * sizeof(float) + 2 * sizeof(int) +
+                            DR_NOALLOC_QUEUE_PER_SLOT * (sizeof(EdgeEntry) + sizeof(float4) + sizeof(int));
+    const int maxb = bucket(o->max_bounces);
+    Tuning tn;
+    tn.pass_samples = std::max<long long>(1024, (long long)((DR_NOALLOC_BYTES - fixed) / (per_slot * maxb)) & ~1023ll);
+    tn.min_threads = std::min<long long>(tn.min_threads, tn.pass_samples);
+    Plan P;
+    st = build_plan(&jb, 1, tn, P);
+    if (st != DR_OK) return st;
+    if (P.ranges.size() > DR_NOALLOC_MAX_RANGES)
+        return fail(DR_ERROR_UNSUPPORTED, "the render needs more passes than the allocation-free tables hold");
+    Carver c{(char *)base, 0, DR_NOALLOC_BYTES};
+    Workspace w;
+    w.cnt = c.take<unsigned long long>(4);
+    w.jobs = c.take<DrJob>(1);
+    w.ranges = c.take<DrRange>(DR_NOALLOC_MAX_RANGES);
+    long long slots = std::max<long long>(P.max_slots, 1);
+    w.pts = c.take<PointRec>(slots);
+    w.pt_dp = c.take<float>(3 * slots);
+    w.pt_lastj = c.take<int>(slots);
+    w.counts = c.take<int>(slots);
+    long long qn = DR_NOALLOC_QUEUE_PER_SLOT * slots;
+    w.q = c.take<EdgeEntry>(qn);
+    w.entry_dp = c.take<float4>(qn);
+    w.cont = c.take<int>(qn);
+    if (!w.cont) return fail(DR_ERROR_UNSUPPORTED, "the render does not fit the allocation-free scratch array");
+    w.slot_cap = slots;
+    w.q_cap = qn;
+    if (hipMemcpyAsync(w.jobs, P.jobs.data(), sizeof(DrJob), hipMemcpyHostToDevice, hs) != hipSuccess ||
+        hipMemcpyAsync(w.ranges, P.ranges.data(), P.ranges.size() * sizeof(DrRange), hipMemcpyHostToDevice, hs) !=
+            hipSuccess)
+        return fail(DR_ERROR_DEVICE, "job table upload failed");
+    if (P.maxb <= 1) st = run_noalloc<1>(P, w, hs);
+    else if (P.maxb <= 2) st = run_noalloc<2>(P, w, hs);
+    else if (P.maxb <= 4) st = run_noalloc<4>(P, w, hs);
+    else st = run_noalloc<8>(P, w, hs);
+    // the host tables must outlive their asynchronous upload
+    if (hipStreamSynchronize(hs) != hipSuccess && st == DR_OK) st = fail(DR_ERROR_DEVICE, "stream synchronisation failed");
+    return st;
 }
 
 int64_t dr_scene_edge_count(const dr_scene *s) {

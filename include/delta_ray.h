@@ -304,6 +304,23 @@ dr_status dr_render_adjoint(dr_scene *scene, const dr_render_options *opts,
                             const float *d_image, float *image, float *grads,
                             float *dx_image, void *stream);
 
+/* The same render as dr_render_adjoint for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It calls no hipMalloc / hipFree and no
+ * allocating library routine: every device buffer besides the caller's inputs and outputs lives
+ * in a fixed 256 MiB array of the library (one per GPU, held until the
+ * process ends), so passes are smaller and a pass whose edge selection
+ * exceeds the array's queue (one entry per shading point of a pass) is
+ * selected and sampled in parts.  Fails with
+ * DR_ERROR_UNSUPPORTED when a render does not fit (more than 4096 passes, or
+ * one shading point selecting more edges than the queue holds).  Calls that
+ * overlap on one GPU share the array and race: serialise them.  Synchronises
+ * the stream (queue lengths are read per pass), so it cannot be captured
+ * into a graph. */
+dr_status dr_render_adjoint_noalloc(dr_scene *scene, const dr_render_options *opts,
+                                    const float *d_image, float *image, float *grads,
+                                    float *dx_image, void *stream);
+
 /* Ray queries on the scene BVH (closest hit: tnear 1e-3, tfar inf,
  * scene.cpp:177-202; any hit: the caller's tmin/tmax, scene.cpp:147-175).
  * rays: device [n, 8] = org.xyz, tmin, dir.xyz, tmax.
