@@ -37,7 +37,11 @@ def _ptr(t):
 
 
 class EdgeProposal:
-    def __init__(self, scene, view=None, seed=0):
+    """use_network=False: the ablation without the network -- the applied
+    weights come from each edge's own statistics alone (edges without samples
+    keep the plain proposal's weight)."""
+
+    def __init__(self, scene, view=None, seed=0, use_network=True):
         import torch
         self.torch = torch
         self.lib = scene.lib
@@ -55,6 +59,7 @@ class EdgeProposal:
         self.mlp = h
         self.logits = torch.zeros(max(self.n, 1), dtype=torch.float32, device=dev)
         self.seed = seed
+        self.use_network = use_network
         self.trained_steps = 0
         self.log_w = np.zeros(self.n)  # current log-weights (all 1: the plain proposal)
 
@@ -98,6 +103,8 @@ class EdgeProposal:
         return y.float().contiguous(), w.float().contiguous()
 
     def train(self, steps=300, batch=1024, lr=3e-3):
+        if not self.use_network:
+            return 0.0
         y, w = self.targets()
         loss = ctypes.c_float(0.0)
         _abi.check(self.lib, self.lib.dr_mlp_train(self.mlp, _ptr(self.features), _ptr(y), _ptr(w), self.n, steps,
@@ -118,10 +125,13 @@ class EdgeProposal:
         (DR_LEARNED_EDGES | DR_EDGE_STATS renders) refine the weights: each
         round's RMS measures what the current proposal still under- or
         over-samples.  Weights are clipped to `clip`."""
-        _abi.check(self.lib, self.lib.dr_mlp_infer(self.mlp, _ptr(self.features), self.n, _ptr(self.logits), None),
-                   "dr_mlp_infer")
-        self.torch.cuda.synchronize()
-        pred = self.logits[:self.n].double().cpu().numpy()
+        if self.use_network:
+            _abi.check(self.lib, self.lib.dr_mlp_infer(self.mlp, _ptr(self.features), self.n, _ptr(self.logits),
+                                                       None), "dr_mlp_infer")
+            self.torch.cuda.synchronize()
+            pred = self.logits[:self.n].double().cpu().numpy()
+        else:
+            pred = np.zeros(self.n)
         y, _ = self.targets()
         y = y[:self.n].double().cpu().numpy()
         cnt = self.acc[:self.n, 1].cpu().numpy()

@@ -150,7 +150,10 @@ static void build_bvh(Scene &sc) {
     if (nt > 0) build_rec(sc, p, 0, nt);
 }
 
+thread_local uint64_t or_thread_rays = 0;
+
 Hit nearest_hit(const Scene &sc, const V3f &org, const V3f &dir, float tmin, float tmax) {
+    or_thread_rays++;
     Hit best;
     float best_t = tmax;
     int best_g = -1;
@@ -196,6 +199,7 @@ Hit nearest_hit(const Scene &sc, const V3f &org, const V3f &dir, float tmin, flo
 }
 
 bool any_hit(const Scene &sc, const V3f &org, const V3f &dir, float tmin, float tmax) {
+    or_thread_rays++;
     int nt = (int)sc.tri_shape.size();
     float u, v, t;
     if (!sc.use_bvh || sc.bvh.empty()) {

@@ -193,6 +193,9 @@ struct Scene {
 
 void build_scene(Scene &sc, const dr_scene_desc *d);
 void build_primary_edges(Scene &sc);
+// ray queries made by this thread (nearest_hit + any_hit), for the CPU
+// baseline's rays/s
+extern thread_local uint64_t or_thread_rays;
 Hit nearest_hit(const Scene &sc, const V3f &org, const V3f &dir, float tmin, float tmax);
 bool any_hit(const Scene &sc, const V3f &org, const V3f &dir, float tmin, float tmax);
 bool occluded(const Scene &sc, const V3f &org, const V3f &dir, float max_t);

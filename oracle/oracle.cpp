@@ -1186,6 +1186,7 @@ struct GradLayout {
 struct OScene {
     Scene sc;
     GradLayout lay;
+    uint64_t last_rays = 0;  // ray queries of the last or_render
 };
 
 struct Accum {
@@ -1447,6 +1448,9 @@ void or_scene_destroy(void *s) { delete (OScene *)s; }
 
 void or_scene_use_bvh(void *s, int use) { ((OScene *)s)->sc.use_bvh = use != 0; }
 
+// ray queries (closest + any hit) of the scene's last or_render
+uint64_t or_last_render_rays(void *s) { return ((OScene *)s)->last_rays; }
+
 // out[0..5] = edges, primary edges, silhouette nodes, normal nodes, bvh nodes, triangles
 void or_scene_info(void *s, int *out) {
     const Scene &sc = ((OScene *)s)->sc;
@@ -1505,6 +1509,7 @@ int or_render(void *s, const dr_render_options *o, const float *d_image, float *
     auto tile_worker = [&](int tid) {
         Tape tape;
         Accum &acc = accs[tid];
+        or_thread_rays = 0;
         while (true) {
             int t = next_tile.fetch_add(1);
             if (t >= ntx * nty) break;
@@ -1523,6 +1528,7 @@ int or_render(void *s, const dr_render_options *o, const float *d_image, float *
                     }
                 }
         }
+        acc.rays += or_thread_rays;
     };
     {
         std::vector<std::thread> th;
@@ -1536,17 +1542,21 @@ int or_render(void *s, const dr_render_options *o, const float *d_image, float *
         const int64_t chunk = 4096;
         auto edge_worker = [&](int tid) {
             Accum &acc = accs[tid];
+            or_thread_rays = 0;
             while (true) {
                 int64_t b = next.fetch_add(chunk);
                 if (b >= N) break;
                 int64_t e = std::min(b + chunk, N);
                 for (int64_t k = b; k < e; k++) render_primary_edge(rc, k, acc);
             }
+            acc.rays += or_thread_rays;
         };
         std::vector<std::thread> th;
         for (int i = 0; i < nthreads; i++) th.emplace_back(edge_worker, i);
         for (auto &t : th) t.join();
     }
+    os->last_rays = 0;
+    for (auto &a : accs) os->last_rays += a.rays;
     if (rc.adjoint && grads) {
         for (size_t i = 0; i < os->lay.total; i++) {
             double v = 0.0;

@@ -36,6 +36,8 @@ def load():
     lib.or_scene_create.argtypes = [ctypes.POINTER(_abi.dr_scene_desc)]
     lib.or_scene_destroy.argtypes = [ctypes.c_void_p]
     lib.or_scene_use_bvh.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.or_last_render_rays.restype = ctypes.c_uint64
+    lib.or_last_render_rays.argtypes = [ctypes.c_void_p]
     lib.or_scene_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     lib.or_grad_size.restype = ctypes.c_size_t
     lib.or_grad_size.argtypes = [ctypes.c_void_p]
@@ -70,6 +72,10 @@ class OracleScene:
 
     def use_bvh(self, on):
         self.lib.or_scene_use_bvh(self.h, 1 if on else 0)
+
+    def last_render_rays(self):
+        """ray queries (closest + any hit) of the last render"""
+        return int(self.lib.or_last_render_rays(self.h))
 
     def render(self, width, height, spp, max_bounces, seed, flags=_abi.DR_FLAGS_PARITY, d_image=None,
                nthreads=0, num_primary_edge_samples=0, want_dx=False):
