@@ -19,6 +19,9 @@ DR_ERROR_OUT_OF_MEMORY = 2
 DR_ERROR_DEVICE = 3
 DR_ERROR_UNSUPPORTED = 4
 
+DR_EDGES_FILTERED = 0
+DR_EDGES_ALL_CANDIDATES = 1
+
 DR_QUIRK_JITTER_OVER_SPP = 1 << 0
 DR_QUIRK_TEXEL_SWAP = 1 << 1
 DR_QUIRK_NORMAL_DN0 = 1 << 2
@@ -129,7 +132,7 @@ class dr_scene_info(ctypes.Structure):
                 ("num_primary_edges", ctypes.c_int),
                 ("silhouette_tree_nodes", ctypes.c_int),
                 ("normal_tree_nodes", ctypes.c_int), ("build_ms", ctypes.c_double),
-                ("refits", ctypes.c_int), ("rebuilds", ctypes.c_int)]
+                ("refits", ctypes.c_int), ("rebuilds", ctypes.c_int), ("edges_switched_off", ctypes.c_int)]
 
 
 class dr_shape_update(ctypes.Structure):
@@ -143,6 +146,7 @@ ENTRY_POINTS = [
     ("dr_scene_set_camera", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_camera)]),
     ("dr_scene_update_vertices", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_shape_update), ctypes.c_int,
                                                 ctypes.c_void_p]),
+    ("dr_scene_set_edge_policy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("dr_scene_update_materials", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_material), ctypes.c_int]),
     ("dr_scene_update_lights", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(dr_light), ctypes.c_int]),
     ("dr_scene_grad_size", ctypes.c_size_t, [ctypes.c_void_p]),

@@ -472,6 +472,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         }
         T.n_edges += nprims;
         for (int i = 0; i < nprims; i++) {
+            if (S.edge_on && !S.edge_on[(T.directional ? S.num_sil_edges : 0) + e.off + i]) continue;
             if (T.directional && !is_silhouette_nrm(S, sp.pos, e.off + i)) continue;
             if (mask & 1) emit(scale * cone_pmf / den, T.directional, e.off + i, T.jc++);
             if (mask & 2) emit(scale * all_pmf / den, T.directional, e.off + i, DR_ORD_ALL_PASS | T.ja++);

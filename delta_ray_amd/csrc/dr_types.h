@@ -125,6 +125,10 @@ struct DrSceneView {
     const DR_GLOBAL float *sil_geo;  // per silhouette-tree edge: v0, v1 (same layout)
     int num_sil_nodes, num_nrm_nodes;
     int num_sil_edges;
+    // per tree edge (silhouette tree, then normal tree): 0 when the
+    // degenerate / coplanar filter drops the edge on the current vertices;
+    // NULL when the trees hold only the edges the filter kept at the build
+    const DR_GLOBAL unsigned char *edge_on;
     float *edge_stat;  // per tree edge (silhouette tree, then normal tree): sum of |contribution|, count, sum of squares
     const DR_GLOBAL DrEdge *prim_edges;
     const DR_GLOBAL float *prim_func;

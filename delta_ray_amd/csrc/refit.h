@@ -25,6 +25,9 @@ struct RefitView {
     const DrEdge *cand;
     const int *cand_keep;
     int num_cand;
+    // edges holds every mesh edge (trees over all candidates): the primary
+    // edges take the filter's verdict on the current vertices
+    bool filter_edges;
 };
 
 struct EdgeTreeRefit {
@@ -36,6 +39,7 @@ struct EdgeTreeRefit {
     float *geo;           // 16 floats per edge
     float *on;            // 6 floats per edge (directional only)
     const float *weights; // learned per-edge weights or NULL
+    const unsigned char *flags;  // per edge: 0 = switched off by the filter (no length); NULL: all on
     const int *lv, *lv_off, *rng;
     int nlev;
     float *nsum, *wsum;   // scratch: 3 floats / 1 float per node
@@ -53,6 +57,10 @@ struct PrimaryOut {
 void refit_geometry(const RefitView &v, hipStream_t st);
 // adds to *changed (device) the candidate edges whose filter verdict changed
 void edge_filter(const RefitView &v, unsigned int *changed, hipStream_t st);
+// the filter's verdicts of `edges` as flags on[i] (1: kept); adds the edges
+// switched off to *off_count (device)
+void edge_flags(const RefitView &v, const DrEdge *edges, int n, unsigned char *on, unsigned int *off_count,
+                hipStream_t st);
 // geometry records, boxes, lengths, cones and learned lengths of one tree
 void refit_edge_tree(const RefitView &v, const EdgeTreeRefit &t, hipStream_t st);
 // primary-edge list and distribution for a camera

@@ -8,7 +8,10 @@
 //   k_refit_tris       triangle records (v0, e1, e2) in leaf order
 //   k_refit_bvh        BVH child boxes, bottom-up one level at a time
 //   k_edge_filter      the degenerate / coplanar edge filter (scene.cpp:80-96);
-//                      a change means the edge set changed and the host rebuilds
+//                      a change of its verdicts means the edge set changed and
+//                      the host rebuilds
+//   k_edge_flags       the same verdicts as per-edge flags, for trees built over
+//                      every mesh edge (vertex updates then never rebuild)
 //   k_refit_geo        per-tree-edge geometry records (endpoints, faces' third vertices)
 //   k_edge_normals     face normals of normal-tree edges (edge_tree.cpp:218-249)
 //   k_refit_etree      edge-tree boxes, lengths, normal sums, learned lengths
@@ -104,23 +107,37 @@ __global__ void __launch_bounds__(RF_TREE_BLOCK) k_refit_bvh(DrNode *nodes, cons
     }
 }
 
-// reference scene.cpp:80-96 on the new vertices; counts edges whose verdict
-// differs from the one the scene was built with
+// reference scene.cpp:80-96: is mesh edge e kept on the current vertices
+DR_INLINE bool filter_keeps(const DrEdge &e, const float *verts, const int *tri_vidx, const DrShape &sh) {
+    F3 dv = vtx(verts, sh.vert_off + e.v0) - vtx(verts, sh.vert_off + e.v1);
+    if (fabsf(dv.x) + fabsf(dv.y) + fabsf(dv.z) < 1e-10f) return false;
+    if (e.f1 != -1) {
+        float ang = fabsf(dot(face_normal(verts, tri_vidx, sh, e.f0), face_normal(verts, tri_vidx, sh, e.f1)));
+        if (ang == 1.f || ang == -1.f) return false;
+    }
+    return true;
+}
+
+// the filter on the new vertices; counts edges whose verdict differs from
+// the one the scene was built with
 __global__ void k_edge_filter(const DrEdge *cand, const int *keep, int n, const float *verts, const int *tri_vidx,
                               const DrShape *shapes, unsigned int *changed) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const DrEdge e = cand[i];
-    const DrShape sh = shapes[e.shape];
-    F3 dv = vtx(verts, sh.vert_off + e.v0) - vtx(verts, sh.vert_off + e.v1);
-    bool k = true;
-    if (fabsf(dv.x) + fabsf(dv.y) + fabsf(dv.z) < 1e-10f) {
-        k = false;
-    } else if (e.f1 != -1) {
-        float ang = fabsf(dot(face_normal(verts, tri_vidx, sh, e.f0), face_normal(verts, tri_vidx, sh, e.f1)));
-        if (ang == 1.f || ang == -1.f) k = false;
-    }
-    if ((k ? 1 : 0) != keep[i]) atomicAdd(changed, 1u);
+    if ((filter_keeps(e, verts, tri_vidx, shapes[e.shape]) ? 1 : 0) != keep[i]) atomicAdd(changed, 1u);
+}
+
+// the filter's verdicts as flags of the tree edges (trees over every mesh
+// edge); counts the edges switched off
+__global__ void k_edge_flags(const DrEdge *edges, int n, const float *verts, const int *tri_vidx,
+                             const DrShape *shapes, unsigned char *on, unsigned int *off_count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DrEdge e = edges[i];
+    bool k = filter_keeps(e, verts, tri_vidx, shapes[e.shape]);
+    on[i] = k ? 1 : 0;
+    if (!k) atomicAdd(off_count, 1u);
 }
 
 // per tree edge, 16 floats: endpoints, and for a normal-tree edge each face's
@@ -171,9 +188,11 @@ __global__ void k_edge_normals(float *on, const DrEdge *edges, int n, const floa
 // node array).  Leaf: box of its edges' endpoints, summed edge lengths, summed
 // face normals in edge order, summed learned lengths; interior: the union /
 // sums of its two children.  nsum: 3 floats per node; wsum: 1 per node.
+// flags (trees over every mesh edge): edges switched off count no length.
 __global__ void __launch_bounds__(RF_TREE_BLOCK) k_refit_etree(DrEdgeNode *nodes, const float *geo, const float *on,
-                                                               const float *w, const int *lv, const int *lv_off,
-                                                               int nlev, float *nsum, float *wsum) {
+                                                               const float *w, const unsigned char *flags,
+                                                               const int *lv, const int *lv_off, int nlev,
+                                                               float *nsum, float *wsum) {
     for (int l = 0; l < nlev; l++) {
         for (int k = lv_off[l] + (int)threadIdx.x; k < lv_off[l + 1]; k += blockDim.x) {
             int i = lv[k];
@@ -191,7 +210,9 @@ __global__ void __launch_bounds__(RF_TREE_BLOCK) k_refit_etree(DrEdgeNode *nodes
                     lo[2] = minf_(lo[2], minf_(a.z, b.z));
                     hi[0] = maxf_(hi[0], maxf_(a.x, b.x)); hi[1] = maxf_(hi[1], maxf_(a.y, b.y));
                     hi[2] = maxf_(hi[2], maxf_(a.z, b.z));
-                    float el = length(b - a);
+                    // an edge the filter switched off keeps its place in the
+                    // box and the normal cone (conservative) but adds no length
+                    float el = (!flags || flags[ei]) ? length(b - a) : 0.f;
                     len += el;
                     if (w) wl += maxf_(w[ei], 0.f) * el;
                     if (on) {
@@ -321,11 +342,15 @@ DR_INLINE bool is_silhouette(const float *verts, const int *tri_vidx, const DrSh
 }  // namespace
 
 __global__ void k_prim_len(const DrEdge *edges, int n, DrCamera cam, const float *verts, const int *tri_vidx,
-                           const DrShape *shapes, float *len) {
+                           const DrShape *shapes, bool filter, float *len) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const DrEdge e = edges[i];
     const DrShape sh = shapes[e.shape];
+    if (filter && !filter_keeps(e, verts, tri_vidx, sh)) {
+        len[i] = 0.f;
+        return;
+    }
     F3 org = xfm_point(cam.c2w, F3{0.f, 0.f, 0.f});
     float l = 0.f;
     F2 a, b;
@@ -432,6 +457,12 @@ void edge_filter(const RefitView &v, unsigned int *changed, hipStream_t st) {
                                                                        v.tri_vidx, v.shapes, changed);
 }
 
+void edge_flags(const RefitView &v, const DrEdge *edges, int n, unsigned char *on, unsigned int *off_count,
+                hipStream_t st) {
+    if (n > 0)
+        k_edge_flags<<<grid(n, RF_BLOCK), RF_BLOCK, 0, st>>>(edges, n, v.verts, v.tri_vidx, v.shapes, on, off_count);
+}
+
 void refit_edge_tree(const RefitView &v, const EdgeTreeRefit &t, hipStream_t st) {
     if (t.num_nodes == 0) return;
     k_refit_geo<<<grid(t.num_edges, RF_BLOCK), RF_BLOCK, 0, st>>>(t.geo, t.edges, t.num_edges, t.directional, v.verts,
@@ -439,8 +470,8 @@ void refit_edge_tree(const RefitView &v, const EdgeTreeRefit &t, hipStream_t st)
     if (t.directional)
         k_edge_normals<<<grid(t.num_edges, RF_BLOCK), RF_BLOCK, 0, st>>>(t.on, t.edges, t.num_edges, v.verts,
                                                                           v.normals, v.tri_vidx, v.shapes);
-    k_refit_etree<<<1, RF_TREE_BLOCK, 0, st>>>(t.nodes, t.geo, t.directional ? t.on : nullptr, t.weights, t.lv,
-                                               t.lv_off, t.nlev, t.nsum, t.wsum);
+    k_refit_etree<<<1, RF_TREE_BLOCK, 0, st>>>(t.nodes, t.geo, t.directional ? t.on : nullptr, t.weights, t.flags,
+                                               t.lv, t.lv_off, t.nlev, t.nsum, t.wsum);
     k_etree_cone<<<grid(64ll * t.num_nodes, RF_BLOCK), RF_BLOCK, 0, st>>>(
         t.nodes, t.num_nodes, t.directional ? t.on : nullptr, t.rng, t.nsum, t.wsum, t.weights != nullptr);
 }
@@ -448,7 +479,7 @@ void refit_edge_tree(const RefitView &v, const EdgeTreeRefit &t, hipStream_t st)
 void primary_edges(const RefitView &v, const DrCamera &cam, const PrimaryOut &o, hipStream_t st) {
     if (v.num_edges == 0) return;
     k_prim_len<<<grid(v.num_edges, RF_BLOCK), RF_BLOCK, 0, st>>>(v.edges, v.num_edges, cam, v.verts, v.tri_vidx,
-                                                                 v.shapes, o.len);
+                                                                 v.shapes, v.filter_edges, o.len);
     k_prim_build<<<1, RF_TREE_BLOCK, 0, st>>>(v.edges, o.len, v.num_edges, o.edges, o.func, o.cdf, o.count,
                                               o.func_int);
 }

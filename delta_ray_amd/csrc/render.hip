@@ -766,6 +766,7 @@ struct SceneStruct {
     float *nsum = nullptr, *wsum = nullptr;  // 3 / 1 floats per node of the larger tree
     float *prim_len = nullptr;            // one float per mesh edge
     float *edge_w = nullptr;              // learned per-edge weights (tree order), or NULL
+    unsigned char *edge_on = nullptr;     // per tree edge: the filter's verdict (trees over every mesh edge)
     size_t prim_cap = 0;                  // capacity of the primary-edge buffers (mesh edges)
     void *prim_bufs[3] = {nullptr, nullptr, nullptr};  // edges, func, cdf
 };
@@ -782,8 +783,9 @@ struct dr_scene {
     Workspace ws;
     double build_ms = 0.0;
     bool host_stale = false;        // device vertices newer than hs.verts / hs.normals
-    void *small = nullptr;          // device: changed-edge count, primary count, integral
+    void *small = nullptr;          // device, 4 words: edge-filter counts, or primary count and integral
     int refits = 0, rebuilds = 0;
+    int edges_off = 0;              // tree edges the filter drops on the current vertices (all-candidate trees)
     std::vector<dr_view *> views;   // views of this scene (their primary edges follow its vertices)
 };
 
@@ -890,6 +892,11 @@ static bool upload_structures(dr_scene *s) {
     if (!ok) return false;
     v.edge_stat = (float *)p;
     if (hipMemset(p, 0, 3 * std::max<size_t>(ne, 1) * sizeof(float)) != hipSuccess) return false;
+    v.edge_on = nullptr;
+    if (hs.all_candidates) {
+        if (!alloc_to(A, std::max<size_t>(ne, 1), (void **)&st.edge_on)) return false;
+        v.edge_on = (decltype(v.edge_on))st.edge_on;
+    }
     // primary edges: capacity of every mesh edge, filled from the host build
     st.prim_cap = hs.edges.size();
     if (!alloc_prim(st.prim_bufs, st.prim_cap)) return false;
@@ -1548,6 +1555,7 @@ static drrefit::RefitView refit_view(dr_scene *s) {
     r.cand = st.cand;
     r.cand_keep = st.cand_keep;
     r.num_cand = (int)s->hs.cand_edges.size();
+    r.filter_edges = s->hs.all_candidates;
     return r;
 }
 
@@ -1593,6 +1601,7 @@ static dr_status refit_edge_trees(dr_scene *s, hipStream_t hs) {
     t.geo = (float *)(v.sil_geo);
     t.on = nullptr;
     t.weights = st.edge_w;
+    t.flags = st.edge_on;
     t.lv = st.sil_lv;
     t.lv_off = st.sil_lv_off;
     t.rng = st.sil_rng;
@@ -1606,6 +1615,7 @@ static dr_status refit_edge_trees(dr_scene *s, hipStream_t hs) {
     t.geo = (float *)(v.nrm_geo);
     t.on = st.nrm_on;
     t.weights = st.edge_w ? st.edge_w + h.sil_edges.size() : nullptr;
+    t.flags = st.edge_on ? st.edge_on + h.sil_edges.size() : nullptr;
     t.lv = st.nrm_lv;
     t.lv_off = st.nrm_lv_off;
     t.rng = st.nrm_rng;
@@ -1623,6 +1633,35 @@ static dr_status sync_host_vertices(dr_scene *s) {
                                          hipMemcpyDeviceToHost) != hipSuccess))
         return fail(DR_ERROR_DEVICE, "vertex read-back failed");
     s->host_stale = false;
+    return DR_OK;
+}
+
+// Edge flags (trees over every mesh edge), edge trees and the scene camera's
+// primary edges for the current device vertices (synchronises the stream).
+static dr_status settle_edges(dr_scene *s, hipStream_t hs) {
+    const drh::HostScene &h = s->hs;
+    if (h.all_candidates) {
+        drrefit::RefitView rv = refit_view(s);
+        k_fill<<<1, 64, 0, hs>>>((float *)s->small, 1, 0.f);
+        unsigned int *off = (unsigned int *)s->small;
+        drrefit::edge_flags(rv, s->view.sil_edges, (int)h.sil_edges.size(), s->st.edge_on, off, hs);
+        drrefit::edge_flags(rv, s->view.nrm_edges, (int)h.nrm_edges.size(), s->st.edge_on + h.sil_edges.size(), off,
+                            hs);
+        unsigned int n_off = 0;
+        if (launch_check("edge flags") != DR_OK ||
+            hipMemcpyAsync(&n_off, s->small, sizeof(n_off), hipMemcpyDeviceToHost, hs) != hipSuccess ||
+            hipStreamSynchronize(hs) != hipSuccess)
+            return fail(DR_ERROR_DEVICE, "edge flags failed");
+        s->edges_off = (int)n_off;
+    }
+    dr_status st = refit_edge_trees(s, hs);
+    if (st != DR_OK) return st;
+    int n = 0;
+    float fi = 0.f;
+    st = device_primary(s, s->view.cam, s->st.prim_bufs, n, fi, hs);
+    if (st != DR_OK) return st;
+    s->view.num_prim_edges = n;
+    s->view.prim_func_int = fi;
     return DR_OK;
 }
 
@@ -1682,16 +1721,20 @@ dr_status dr_scene_update_vertices(dr_scene *s, const dr_shape_update *updates, 
         lights |= sh.light >= 0;
     }
     s->host_stale = true;
-    // the edge filter's verdicts on the new vertices
-    k_fill<<<1, 64, 0, hs>>>((float *)s->small, 1, 0.f);
     drrefit::RefitView rv = refit_view(s);
-    drrefit::edge_filter(rv, (unsigned int *)s->small, hs);
-    unsigned int changed = 0;
-    if (launch_check("edge filter") != DR_OK ||
-        hipMemcpyAsync(&changed, s->small, sizeof(changed), hipMemcpyDeviceToHost, hs) != hipSuccess ||
-        hipStreamSynchronize(hs) != hipSuccess)
-        return fail(DR_ERROR_DEVICE, "edge filter failed");
-    if (changed) {
+    bool rebuild = false;
+    if (!h.all_candidates) {
+        // the edge filter's verdicts on the new vertices
+        k_fill<<<1, 64, 0, hs>>>((float *)s->small, 1, 0.f);
+        drrefit::edge_filter(rv, (unsigned int *)s->small, hs);
+        unsigned int changed = 0;
+        if (launch_check("edge filter") != DR_OK ||
+            hipMemcpyAsync(&changed, s->small, sizeof(changed), hipMemcpyDeviceToHost, hs) != hipSuccess ||
+            hipStreamSynchronize(hs) != hipSuccess)
+            return fail(DR_ERROR_DEVICE, "edge filter failed");
+        rebuild = changed != 0;
+    }
+    if (rebuild) {
         // the edge set changed: rebuild the structures on the host
         dr_status st = sync_host_vertices(s);
         if (st != DR_OK) return st;
@@ -1701,15 +1744,9 @@ dr_status dr_scene_update_vertices(dr_scene *s, const dr_shape_update *updates, 
         s->rebuilds++;
     } else {
         drrefit::refit_geometry(rv, hs);
-        dr_status st = refit_edge_trees(s, hs);
-        if (st == DR_OK) st = launch_check("BVH refit");
+        dr_status st = launch_check("BVH refit");
+        if (st == DR_OK) st = settle_edges(s, hs);
         if (st != DR_OK) return st;
-        int n = 0;
-        float fi = 0.f;
-        st = device_primary(s, s->view.cam, s->st.prim_bufs, n, fi, hs);
-        if (st != DR_OK) return st;
-        s->view.num_prim_edges = n;
-        s->view.prim_func_int = fi;
         s->refits++;
     }
     if (lights) {
@@ -1719,6 +1756,26 @@ dr_status dr_scene_update_vertices(dr_scene *s, const dr_shape_update *updates, 
         if (!refresh_lights(s)) return fail(DR_ERROR_DEVICE, "light upload failed");
     }
     return refresh_views(s, hs);
+}
+
+dr_status dr_scene_set_edge_policy(dr_scene *s, int policy) {
+    if (!s || (policy != DR_EDGES_FILTERED && policy != DR_EDGES_ALL_CANDIDATES))
+        return fail(DR_ERROR_INVALID_ARGUMENT, "unknown edge policy");
+    drh::HostScene &h = s->hs;
+    bool all = policy == DR_EDGES_ALL_CANDIDATES;
+    if (all == h.all_candidates) return DR_OK;
+    dr_status st = sync_host_vertices(s);
+    if (st != DR_OK) return st;
+    h.all_candidates = all;
+    std::string err = drh::build_structures(h);
+    if (!err.empty()) return fail(DR_ERROR_INVALID_ARGUMENT, err);
+    if (!upload_structures(s)) return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation or copy failed");
+    s->edges_off = 0;
+    if (all) {
+        st = settle_edges(s, nullptr);
+        if (st != DR_OK) return st;
+    }
+    return refresh_views(s, nullptr);
 }
 
 dr_status dr_scene_update_materials(dr_scene *s, const dr_material *materials, int num_materials) {
@@ -1782,6 +1839,7 @@ dr_status dr_scene_get_info(const dr_scene *s, dr_scene_info *o) {
     o->build_ms = s->build_ms;
     o->refits = s->refits;
     o->rebuilds = s->rebuilds;
+    o->edges_switched_off = s->edges_off;
     return DR_OK;
 }
 

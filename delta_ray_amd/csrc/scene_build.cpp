@@ -762,7 +762,7 @@ std::string build_structures(HostScene &hs) {
         bool keep = edge_kept(hs, e);
         hs.cand_edges.push_back(e);
         hs.cand_keep.push_back(keep ? 1 : 0);
-        if (keep) hs.edges.push_back(e);
+        if (keep || hs.all_candidates) hs.edges.push_back(e);
     }
     build_primary_edges(hs);
     std::vector<DrEdge> sil, nrm;

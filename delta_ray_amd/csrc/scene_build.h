@@ -28,6 +28,9 @@ struct HostScene {
     std::vector<float> light_func, light_cdf;  // per light, concatenated
     Dist light_sel;
     float light_total_intensity = 0.f;
+    // trees over every mesh edge (the filter's verdicts become per-edge flags
+    // on the device) instead of the edges the filter keeps
+    bool all_candidates = false;
     std::vector<DrEdge> edges;  // all edges, sorted by (shape, v0, v1)
     std::vector<DrEdge> sil_edges, nrm_edges;
     std::vector<DrEdgeNode> sil_nodes, nrm_nodes;

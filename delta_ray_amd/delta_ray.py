@@ -329,6 +329,15 @@ class DeviceScene:
             ctypes.c_void_p(dx_image.data_ptr() if dx_image is not None else None), s),
             "dr_render_adjoint")
 
+    def set_edge_policy(self, policy):
+        """dr_scene_set_edge_policy: _abi.DR_EDGES_FILTERED (trees over the
+        edges the coplanar filter keeps, as the reference; a vertex update
+        that flips a verdict rebuilds on the host) or
+        _abi.DR_EDGES_ALL_CANDIDATES (trees over every mesh edge, the
+        filter's verdicts as device flags: vertex updates never rebuild)."""
+        _abi.check(self.lib, self.lib.dr_scene_set_edge_policy(self.handle, int(policy)),
+                   "dr_scene_set_edge_policy")
+
     def update_vertices(self, updates, stream=None):
         """updates: [(shape_index, vertices, normals or None)], torch float32
         CUDA tensors [N, 3] of the shapes' sizes (dr_scene_update_vertices:

@@ -14,7 +14,11 @@ Differences that follow from running on the GPU:
   texture sizes, light shapes) stays the same, a call only sends what changed
   -- new vertex positions are refitted on the device from the caller's CUDA
   tensors (dr_scene_update_vertices), materials, light intensities and the
-  camera are replaced -- and the device workspace is reused;
+  camera are replaced -- and the device workspace is reused.  The edge trees
+  hold every mesh edge with the coplanar filter's verdicts as device flags
+  (``RenderFunction.edge_policy``), so a refit never falls back to a host
+  rebuild; the estimator is the reference's in expectation, and
+  ``_abi.DR_EDGES_FILTERED`` gives the reference's trees sample for sample;
 * the image is returned on the GPU when the vertices live there, otherwise as a
   CPU tensor like the reference's;
 * the backward render uses the same random numbers as the forward render, so
@@ -85,6 +89,12 @@ class RenderFunction(torch.autograd.Function):
     """Render a scene (reference render_pytorch.py:7-333)."""
 
     flags = _abi.DR_FLAGS_PARITY
+    # edge trees over every mesh edge, with the coplanar filter's verdicts as
+    # device flags, so that vertex updates never rebuild on the host
+    # (optimiser steps flip the exact coplanar-face test of some nearly flat
+    # edge almost every step); _abi.DR_EDGES_FILTERED gives the reference's
+    # trees (and samples) exactly, rebuilding when a verdict flips
+    edge_policy = _abi.DR_EDGES_ALL_CANDIDATES
 
     @staticmethod
     def serialize_scene(cam, materials, shapes, lights, resolution, num_samples, max_bounces):
@@ -161,6 +171,7 @@ class RenderFunction(torch.autograd.Function):
             shape_objs[sid].light = l
             light_objs.append(l)
         scene = dr.DeviceScene(camera, shape_objs, materials, light_objs)
+        scene.set_edge_policy(RenderFunction.edge_policy)
         C.scene, C.key = scene, key
         C.indices = [_copy(sh[1]) for sh in shapes]
         C.uvs = [_copy(sh[2]) for sh in shapes]

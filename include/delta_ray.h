@@ -264,7 +264,7 @@ dr_status dr_scene_set_camera(dr_scene *scene, const dr_camera *camera);
  * When the new positions change which mesh edges the degenerate / coplanar
  * filter keeps (scene.cpp:80-96), the BVH, edges and edge trees are rebuilt
  * on the host instead (learned edge weights and edge statistics are then
- * cleared).  Light distributions are recomputed when an emissive shape
+ * cleared), unless the trees hold every mesh edge (dr_scene_set_edge_policy).  Light distributions are recomputed when an emissive shape
  * moves.  Ordered on `stream`; synchronises it (one read of the filter's
  * verdict and of the primary-edge count). */
 typedef struct dr_shape_update {
@@ -274,6 +274,24 @@ typedef struct dr_shape_update {
 } dr_shape_update;
 dr_status dr_scene_update_vertices(dr_scene *scene, const dr_shape_update *updates, int num_updates,
                                    void *stream);
+
+/* Which mesh edges the edge trees hold (no reference counterpart: the
+ * reference rebuilds per call).  DR_EDGES_FILTERED (the default): the edges
+ * the degenerate / coplanar filter keeps (scene.cpp:80-96), as the
+ * reference's trees; a vertex update that changes a verdict rebuilds on the
+ * host.  The filter drops an edge whose faces are coplanar to the last bit
+ * (|n0.n1| == 1 in float), and vertices moved by an optimiser step flip that
+ * verdict for some nearly flat edge almost every step.  DR_EDGES_ALL_CANDIDATES:
+ * the trees hold every mesh edge and the filter's verdicts are per-edge flags
+ * recomputed on the device at every vertex update (an edge switched off adds
+ * no length to its tree nodes and is never selected; it stays in the node
+ * boxes and normal cones), so updates never rebuild.  The estimator is the
+ * reference's in expectation; its samples differ from a fresh build's (other
+ * trees).  Switching rebuilds the structures on the host (learned edge
+ * weights and statistics are cleared). */
+#define DR_EDGES_FILTERED 0
+#define DR_EDGES_ALL_CANDIDATES 1
+dr_status dr_scene_set_edge_policy(dr_scene *scene, int policy);
 
 /* New material parameters (HOST arrays of the same sizes as the scene's) and
  * new light intensities (same lights, same shapes): what the reference reads
@@ -370,6 +388,8 @@ typedef struct dr_scene_info {
     double build_ms;
     int refits;              /* dr_scene_update_vertices calls refitted on the device */
     int rebuilds;            /* ... and rebuilt on the host (the edge set changed) */
+    int edges_switched_off;  /* DR_EDGES_ALL_CANDIDATES: tree edges the filter drops on the current
+                              * vertices (0 otherwise) */
 } dr_scene_info;
 dr_status dr_scene_get_info(const dr_scene *scene, dr_scene_info *out);
 

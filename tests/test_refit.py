@@ -161,3 +161,66 @@ def test_refit_moves_a_light(gpu):
     a, _, _ = gpu_render(sc, w, 4, 1, 3)
     b, _, _ = gpu_render(gpu_scene(w2), w2, 4, 1, 3)
     np.testing.assert_array_equal(a, b)
+
+
+def _mean_secondary(scene, w, seeds, d):
+    """mean and standard error (per entry) of the secondary-edge term over seeds"""
+    W, H = w["resolution"]
+    gs = []
+    for sd in seeds:
+        _, g, _ = gpu_render(scene, w, 16, w["max_bounces"], sd, flags=_abi.DR_FLAGS_PARITY | TERMS["secondary"],
+                             d_image=d)
+        gs.append(np.asarray(g, np.float64))
+    gs = np.array(gs)
+    return gs.mean(0), gs.std(0, ddof=1) / np.sqrt(len(gs))
+
+
+def _same_in_expectation(sa, sb, w, d, name):
+    ma, ea = _mean_secondary(sa, w, range(100, 132), d)
+    mb, eb = _mean_secondary(sb, w, range(200, 232), d)
+    se = np.sqrt(ea ** 2 + eb ** 2)
+    z = np.linalg.norm(ma - mb) / np.linalg.norm(se)
+    print("%s secondary: |mean difference| / |se| = %.3f, |mean| %.4g" % (name, z, np.linalg.norm(mb)), flush=True)
+    assert np.linalg.norm(mb) > 0 and z < 3.0
+
+
+def test_all_candidate_trees_refit_without_rebuild(gpu):
+    """Trees over every mesh edge (DR_EDGES_ALL_CANDIDATES), the coplanar
+    filter's verdicts as device flags: on the built vertices and after a move
+    that flips verdicts (which rebuilds a filtered scene), the forward image,
+    ray hits, interior and primary-edge terms equal a fresh filtered build's,
+    and the secondary-edge term agrees with it in expectation (other trees,
+    so other samples)."""
+    print("case: all-candidate trees", flush=True)
+    w = helpers.small_workload("heightfield", res=48, grid=24)
+    v0 = np.asarray(w["shapes"][0].vertices)
+    v1 = v0.copy()
+    v1[:, 1] += 1e-4 * np.random.default_rng(3).standard_normal(len(v0)).astype(np.float32)
+    w1 = _with_vertices(w, v1)
+    filtered = gpu_scene(w)
+    _update(filtered, v1)
+    assert filtered.info()["rebuilds"] == 1
+    allc = gpu_scene(w)
+    n_kept = allc.info()["num_edges"]
+    allc.set_edge_policy(_abi.DR_EDGES_ALL_CANDIDATES)
+    info = allc.info()
+    assert info["num_edges"] == n_kept + info["edges_switched_off"], (info, n_kept)
+    d = _dimg(w)
+    ref0 = gpu_scene(w)
+    _same_in_expectation(allc, ref0, w, d, "built vertices")
+    _update(allc, v1)
+    info = allc.info()
+    assert info["refits"] == 1 and info["rebuilds"] == 0, info
+    fresh = gpu_scene(w1)
+    assert info["num_edges"] == fresh.info()["num_edges"] + info["edges_switched_off"], (info, fresh.info())
+    ia, _, _ = gpu_render(allc, w1, 2, 2, 3)
+    ib, _, _ = gpu_render(fresh, w1, 2, 2, 3)
+    np.testing.assert_array_equal(ia, ib)
+    rays = helpers.random_rays(w1, 20000, np.random.default_rng(2))
+    np.testing.assert_array_equal(_hits(allc, rays), _hits(fresh, rays))
+    for term in ("interior", "primary"):
+        _, ga, _ = gpu_render(allc, w1, 2, 2, 5, flags=_abi.DR_FLAGS_PARITY | TERMS[term], d_image=d)
+        _, gb, _ = gpu_render(fresh, w1, 2, 2, 5, flags=_abi.DR_FLAGS_PARITY | TERMS[term], d_image=d)
+        print("term", term, flush=True)
+        helpers.assert_groups(w1, ga, gb, 1e-5, 1e-6)
+    _same_in_expectation(allc, fresh, w1, d, "moved vertices")
