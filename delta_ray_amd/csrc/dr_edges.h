@@ -173,24 +173,32 @@ DR_INLINE void bound_ctx(BoundCtx &B, F3 wi, const SPoint &sp, float roughness, 
     }
 }
 
+// Over the box's 8 corners c, a linear function f(c - p) ranges over
+// [f(lo - p) + sum_k min(0, f(ext_k)), f(lo - p) + sum_k max(0, f(ext_k))],
+// ext_k the box's extent along world axis k: the bounds below take the
+// corners' extreme values that way (the same values up to rounding).
 DR_INLINE float bsdf_bound(const DrEdgeNode &node, const SPoint &sp, const BoundCtx &B) {
-    float cos_max = 0.f;
-    for (int i = 0; i < 8; i++) {
-        F3 c = F3{(i & 1) ? node.hi[0] : node.lo[0], (i & 2) ? node.hi[1] : node.lo[1],
-                  (i & 4) ? node.hi[2] : node.lo[2]};
-        if (dot(sp.fr.n, c - sp.pos) > 0.f) { cos_max = 1.f; break; }
-    }
-    if (cos_max <= 0.f) return 0.f;
+    const F3 lo = F3{node.lo[0], node.lo[1], node.lo[2]};
+    const F3 ext = F3{node.hi[0], node.hi[1], node.hi[2]} - lo;
+    const F3 b = lo - sp.pos;
+    // some corner above the shading point's tangent plane (cos_max 1), else 0
+    const F3 n = sp.fr.n;
+    float up = dot(n, b) + fmaxf(n.x * ext.x, 0.f) + fmaxf(n.y * ext.y, 0.f) + fmaxf(n.z * ext.z, 0.f);
+    if (!(up > 0.f)) return 0.f;
     float phong_resp = 0.f;
     if (B.slum > 0.f) {
-        float blo[3] = {DR_INF, DR_INF, DR_INF}, bhi[3] = {-DR_INF, -DR_INF, -DR_INF};
-        for (int i = 0; i < 8; i++) {
-            F3 c = F3{(i & 1) ? node.hi[0] : node.lo[0], (i & 2) ? node.hi[1] : node.lo[1],
-                      (i & 4) ? node.hi[2] : node.lo[2]};
-            F3 l = to_local(B.hf, c - sp.pos);
-            blo[0] = fminf(blo[0], l.x); blo[1] = fminf(blo[1], l.y); blo[2] = fminf(blo[2], l.z);
-            bhi[0] = fmaxf(bhi[0], l.x); bhi[1] = fmaxf(bhi[1], l.y); bhi[2] = fmaxf(bhi[2], l.z);
-        }
+        // the box in the frame around wi: per local axis j, base + extents
+        const Frame &f = B.hf;
+        F3 l0 = to_local(f, b);
+        F3 ax = to_local(f, F3{ext.x, 0.f, 0.f}), ay = to_local(f, F3{0.f, ext.y, 0.f}),
+           az = to_local(f, F3{0.f, 0.f, ext.z});
+        float blo[3], bhi[3];
+        blo[0] = l0.x + fminf(ax.x, 0.f) + fminf(ay.x, 0.f) + fminf(az.x, 0.f);
+        bhi[0] = l0.x + fmaxf(ax.x, 0.f) + fmaxf(ay.x, 0.f) + fmaxf(az.x, 0.f);
+        blo[1] = l0.y + fminf(ax.y, 0.f) + fminf(ay.y, 0.f) + fminf(az.y, 0.f);
+        bhi[1] = l0.y + fmaxf(ax.y, 0.f) + fmaxf(ay.y, 0.f) + fmaxf(az.y, 0.f);
+        blo[2] = l0.z + fminf(ax.z, 0.f) + fminf(ay.z, 0.f) + fminf(az.z, 0.f);
+        bhi[2] = l0.z + fmaxf(ax.z, 0.f) + fmaxf(ay.z, 0.f) + fmaxf(az.z, 0.f);
         float cphm = 0.f;
         float hx = bhi[0];
         if (hx > 0.f) {
@@ -218,23 +226,31 @@ DR_INLINE float bsdf_bound(const DrEdgeNode &node, const SPoint &sp, const Bound
         float hn = fminf(fabsf(B.lx * sh * cphm) + B.lz * ch, 1.f);
         phong_resp = approx_pow01(hn, B.e) * B.e2 * (1.f / (2.f * DR_PI));
     }
-    return cos_max * (B.dterm + B.slum * phong_resp);
+    return B.dterm + B.slum * phong_resp;
 }
 
-// edge_tree.cpp:482-497
+// edge_tree.cpp:482-497: the node's edges can hold a silhouette seen from p
+// when the normal cone widened by the box's angular radius seen from p, va =
+// atan(r / |c - p|), reaches the plane perpendicular to the direction to the
+// box: |acos(-d.axis) - pi/2| <= angle + va.  With both sides' cosines (the
+// node stores cos(angle), the cosine of its normal cone's half angle): the
+// difference lies in [0, pi/2], so the test is sin(acos(-d.axis)) >=
+// cos(angle + va), which also holds whenever angle + va >= pi/2.  The same
+// verdicts as the angles' up to rounding at the boundary.
 DR_INLINE bool contains_silhouette(const DrEdgeNode &n, F3 p) {
     F3 lo = F3{n.lo[0], n.lo[1], n.lo[2]}, hi = F3{n.hi[0], n.hi[1], n.hi[2]};
     F3 c = (lo + hi) / 2.f;
     float r = length(hi - c);
     F3 d = c - p;
-    float len = length(d);
+    float len2 = length_sq(d);
+    float len = approx_sqrt(len2);
     float tv = r / len;
     if (tv >= 1.f) return true;
-    float va = atanf(tv);
-    d = d / len;
-    float ang = acosf(dot(-d, F3{n.axis[0], n.axis[1], n.axis[2]}));
-    float range = n.angle + va;
-    return ang - range <= DR_PI / 2.f && ang + range >= DR_PI / 2.f;
+    float ca = -dot(d, F3{n.axis[0], n.axis[1], n.axis[2]}) / len;
+    float s_ang = approx_sqrt(fmaxf(1.f - ca * ca, 0.f));
+    float cA = n.cos_angle, sA = approx_sqrt(fmaxf(1.f - cA * cA, 0.f));
+    // cos(angle + va) = (cos(angle) - sin(angle) tv) / sqrt(1 + tv^2)
+    return s_ang * approx_sqrt(1.f + tv * tv) >= cA - sA * tv;
 }
 
 DR_INLINE bool node_inside(const DrEdgeNode &n, F3 p) {
@@ -317,13 +333,17 @@ struct PlainStack {
     }
 };
 
-#define DR_LDS_STACK 12  // entries per lane in LDS (a multiple of 2)
+#define DR_LDS_STACK 12  // entries per lane in LDS
 #define DR_LDS_WORDS 4   // words per entry
 
 // Top DR_LDS_STACK entries of each lane in LDS, one word of an entry per
 // row of the block's lanes (conflict-free: lane l always uses bank l % 64);
 // entries below them in private memory.  Entries [lo, sp) are in LDS, entry i
-// in slot i % DR_LDS_STACK; [0, lo) in `spill`.
+// in slot i % DR_LDS_STACK; [0, lo) in `spill`.  A push onto a full LDS part
+// moves its bottom entry to `spill`; a pop below lo takes the entry from
+// `spill` directly, so an entry goes to private memory at most once and
+// comes back at most once (the walk is depth first: once entries below the
+// LDS part are being popped, everything above them has been).
 struct LdsStack {
     DR_LDS float *lds;  // the block's stack area + threadIdx.x (stride: blockDim.x)
     int stride;
@@ -346,20 +366,20 @@ struct LdsStack {
     }
     DR_INLINE void push(const TreeEntry &x, TreeAlt a, bool with_alt) {
         if (sp - lo == DR_LDS_STACK) {
-            for (int j = 0; j < DR_LDS_STACK / 2; j++) spill[lo + j] = get((lo + j) % DR_LDS_STACK);
-            lo += DR_LDS_STACK / 2;
+            spill[lo] = get(lo % DR_LDS_STACK);
+            lo++;
         }
         if (with_alt) alt[sp] = a;
         put(sp % DR_LDS_STACK, x);
         sp++;
     }
     DR_INLINE TreeEntry pop(TreeAlt &a, bool with_alt) {
-        if (sp == lo) {
-            lo -= DR_LDS_STACK / 2;
-            for (int j = 0; j < DR_LDS_STACK / 2; j++) put((lo + j) % DR_LDS_STACK, spill[lo + j]);
-        }
         --sp;
         if (with_alt) a = alt[sp];
+        if (sp < lo) {
+            lo = sp;
+            return spill[sp];
+        }
         return get(sp % DR_LDS_STACK);
     }
 };

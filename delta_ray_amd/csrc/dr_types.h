@@ -76,7 +76,7 @@ struct DrEdge {
 struct DrEdgeNode {
     float lo[3], hi[3];
     float axis[3];
-    float angle;
+    float cos_angle;  // cosine of the normal cone's half angle (least cosine of a face normal to the axis)
     int offset;   // leaf: first edge in the tree's edge list; interior: second child
     int nprims;   // 0: interior
     float total_length;

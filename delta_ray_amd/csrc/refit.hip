@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(RF_TREE_BLOCK) k_refit_etree(DrEdgeNode *nodes
     }
 }
 
-// A wave per node: axis = normalised normal sum, angle = acos of the least
+// A wave per node: axis = normalised normal sum, cos_angle = the least
 // cosine to it over the node's edges [rng[2i], rng[2i+1]) (the minimum does
 // not depend on the order; a NaN cosine never replaces it, as std::min);
 // learned length = summed weighted lengths scaled to the root's length.
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(RF_BLOCK) k_etree_cone(DrEdgeNode *nodes, int 
     for (int s = 32; s > 0; s >>= 1) cmin = minf_(cmin, __shfl_xor(cmin, s));
     if (lane == 0) {
         N.axis[0] = axis.x; N.axis[1] = axis.y; N.axis[2] = axis.z;
-        N.angle = acosf(cmin);
+        N.cos_angle = cmin;
         if (!learned) {
             N.pad = N.total_length;
         } else {

@@ -396,8 +396,8 @@ int build_edge_tree(const HostScene &hs, const std::vector<DrEdge> &edges, bool 
     // Bottom-up (children follow parents): interior offset = second child,
     // total length, the sum of the face normals (leaf: in edge order;
     // interior: left + right) and the node's range of ordered edges; the
-    // cone axis is the normalised sum, its angle the acos of the least cosine
-    // to it over the node's edges.  The device refit (dr_refit.h) repeats
+    // cone axis is the normalised sum, its cos_angle the least cosine to it
+    // over the node's edges.  The device refit (refit.hip) repeats
     // these steps in the same order, so a refit to the same vertices
     // reproduces this tree.
     int nn = (int)nodes.size();
@@ -430,7 +430,7 @@ int build_edge_tree(const HostScene &hs, const std::vector<DrEdge> &edges, bool 
             cmin = std::min(cmin, dot(on[2 * k + 1], axis));
         }
         n.axis[0] = axis.x; n.axis[1] = axis.y; n.axis[2] = axis.z;
-        n.angle = std::acos(cmin);
+        n.cos_angle = cmin;
     }
     // sampling length of the learned proposal: the plain length until
     // weights are set
@@ -923,7 +923,7 @@ void edge_features(const HostScene &hs, const DrCamera &cam, std::vector<float> 
             o[19] = dot(n1, ldir);
             const DrEdgeNode &lf = nodes[leaf[i]];
             o[20] = lf.total_length / root * (float)edges.size();
-            o[21] = lf.angle;
+            o[21] = std::acos(lf.cos_angle);
             o[22] = hs.shapes[e.shape].has_n ? 1.f : 0.f;
             o[23] = 1.f;
         }
