@@ -61,6 +61,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="heightfield")
+    ap.add_argument("--scene", metavar="XML", help="time a Mitsuba scene file instead (load_mitsuba), at its "
+                    "own resolution, --scene-spp samples per pixel and --scene-bounces bounces")
+    ap.add_argument("--scene-spp", type=int, default=16)
+    ap.add_argument("--scene-bounces", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU work of the bounded CPU sample (its spp is chosen from a 1-spp run)")
@@ -740,7 +744,11 @@ def main():
     if args.workload == "torch_step":
         bench_torch_step(args)
         return
-    w = make_workload(args.workload)
+    if args.scene:
+        from delta_ray_amd import workloads
+        w = workloads.mitsuba(args.scene, num_samples=args.scene_spp, max_bounces=args.scene_bounces)
+    else:
+        w = make_workload(args.workload)
     g = gpu_leg(w, args.steps, args.warmup, args.calibrate)
     value = g["value"]
     out = {"metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
@@ -760,7 +768,7 @@ def main():
                                          "primary-edge samples), fwd+adjoint, %.1f s, 8x8 pixel tiles from a "
                                          "shared work list" % (c["spp"], c["spp"] * W * H, c["seconds"])}
         out["extra"]["gpu_over_cpu"] = round(value / c["samples_per_s"], 1)
-    if not args.no_learned and args.workload == "heightfield":
+    if not args.no_learned and args.workload == "heightfield" and not args.scene:
         print("bench: learned-vs-plain leg", file=sys.stderr, flush=True)
         out["extra"]["learned_vs_plain"] = learned_leg(w, args.lv_bench_renders, args.lv_train_renders,
                                                        args.lv_train_steps, args.lv_rounds, args.lv_eps,

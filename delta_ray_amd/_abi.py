@@ -140,6 +140,12 @@ class dr_shape_update(ctypes.Structure):
 
 
 # entry points declared in include/delta_ray.h: (name, restype, argtypes)
+class dr_mesh_data(ctypes.Structure):
+    _fields_ = [("num_vertices", ctypes.c_int), ("num_triangles", ctypes.c_int),
+                ("vertices", _fp), ("normals", _fp), ("uvs", _fp), ("colors", _fp),
+                ("indices", ctypes.POINTER(ctypes.c_int))]
+
+
 ENTRY_POINTS = [
     ("dr_scene_create", ctypes.c_int, [ctypes.POINTER(dr_scene_desc), ctypes.POINTER(ctypes.c_void_p)]),
     ("dr_scene_destroy", None, [ctypes.c_void_p]),
@@ -187,6 +193,8 @@ ENTRY_POINTS = [
                                           ctypes.c_void_p]),
     ("dr_test_line_integral", ctypes.c_int, [ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
                                              ctypes.c_void_p]),
+    ("dr_load_serialized", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(dr_mesh_data)]),
+    ("dr_mesh_data_free", None, [ctypes.POINTER(dr_mesh_data)]),
     ("dr_last_error", ctypes.c_char_p, []),
 ]
 

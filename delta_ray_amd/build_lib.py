@@ -2,9 +2,10 @@
 
     python -m delta_ray_amd.build_lib        # or delta_ray_amd.build_lib.build()
 
-Compiles csrc/render.hip, csrc/mlp.hip and csrc/refit.hip with hipcc, csrc/scene_build.cpp with the host C++
-compiler and csrc/ltc_blob.S (the embedded LTC table), then links the shared
-library next to this file.  Everything is compiled with -ffp-contract=off so
+Compiles csrc/render.hip, csrc/mlp.hip and csrc/refit.hip with hipcc,
+csrc/scene_build.cpp and csrc/load_serialized.cpp with the host C++ compiler
+and csrc/ltc_blob.S (the embedded LTC table), then links the shared library
+next to this file (with zlib, for the .serialized reader).  Everything is compiled with -ffp-contract=off so
 that ray/triangle arithmetic rounds exactly like the CPU oracle's.
 """
 import os
@@ -51,6 +52,7 @@ def build(verbose=False):
         "mlp": os.path.join(BUILD, "mlp.o"),
         "refit": os.path.join(BUILD, "refit.o"),
         "scene": os.path.join(BUILD, "scene_build.o"),
+        "serialized": os.path.join(BUILD, "load_serialized.o"),
         "ltc": os.path.join(BUILD, "ltc_blob.o"),
     }
     steps = [
@@ -58,16 +60,17 @@ def build(verbose=False):
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "mlp.hip"), "-o", objs["mlp"]],
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "refit.hip"), "-o", objs["refit"]],
         [cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene_build.cpp"), "-o", objs["scene"]],
+        [cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "load_serialized.cpp"), "-o", objs["serialized"]],
         ["gcc", "-c", "-DLTC_FILE=\"%s\"" % LTC, os.path.join(CSRC, "ltc_blob.S"), "-o", objs["ltc"]],
     ]
-    with ThreadPoolExecutor(max_workers=5) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         outs = list(ex.map(_run, steps))
     if verbose:
         for o in outs:
             sys.stdout.write(o)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp,
-          objs["render"], objs["mlp"], objs["refit"], objs["scene"], objs["ltc"]])
+          objs["render"], objs["mlp"], objs["refit"], objs["scene"], objs["serialized"], objs["ltc"], "-lz"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -652,6 +652,8 @@ extern "C" const float dr_ltc_blob[];
 extern "C" const float dr_ltc_blob_end[];
 
 static thread_local std::string g_err;
+// the message of a failing entry point compiled in another file (dr_last_error)
+void dr_set_error(const std::string &msg) { g_err = msg; }
 
 static dr_status fail(dr_status s, const std::string &msg) {
     g_err = msg;

@@ -352,4 +352,46 @@ def batched(seed=4, num_scenes=64, views=4, min_tris=1000, max_tris=200000, res_
     return out
 
 
+def from_scene(camera, materials, shapes, lights, resolution, num_samples=4, max_bounces=2, name="scene"):
+    """A workload from scene classes (scene.Camera / Material / Shape / Light,
+    as load_mitsuba returns them).  The camera's matrices depend on its
+    aspect ratio only, so `resolution` may differ from the file's if the
+    aspect ratio is kept."""
+    def arr(t, dt=np.float32):
+        return None if t is None else np.asarray(t.detach().cpu().numpy() if hasattr(t, "detach") else t, dt)
+    if abs(resolution[0] / resolution[1] - camera.aspect_ratio) > 1e-6:
+        raise ValueError("resolution %s does not keep the camera's aspect ratio %g" % (resolution,
+                                                                                       camera.aspect_ratio))
+    cam = dr.Camera(arr(camera.cam_to_world), arr(camera.world_to_cam), arr(camera.sample_to_cam),
+                    arr(camera.cam_to_sample), float(arr(camera.fov_factor).reshape(-1)[0]),
+                    float(camera.aspect_ratio), float(arr(camera.clip_near).reshape(-1)[0]), bool(camera.fisheye))
+    mats = [dr.Material(arr(m.diffuse_reflectance), arr(m.specular_reflectance), arr(m.roughness),
+                        arr(m.diffuse_uv_scale), arr(m.specular_uv_scale), arr(m.roughness_uv_scale),
+                        bool(m.two_sided)) for m in materials]
+    out = []
+    for s in shapes:
+        if not 0 <= int(s.mat_id) < len(mats):
+            raise ValueError("a shape has no material (material id %d)" % int(s.mat_id))
+        out.append(dr.Shape(arr(s.vertices), arr(s.indices, np.int32), arr(s.uvs), arr(s.normals),
+                            mats[int(s.mat_id)], None))
+    ls = []
+    for l in lights:
+        light = dr.Light(out[int(l.shape_id)], arr(l.intensity))
+        out[int(l.shape_id)].light = light
+        ls.append(light)
+    return dict(name=name, camera=cam, shapes=out, materials=mats, lights=ls, resolution=tuple(resolution),
+                num_samples=num_samples, max_bounces=max_bounces,
+                params=[("shape", i, "vertices") for i in range(len(out))])
+
+
+def mitsuba(path, resolution=None, num_samples=4, max_bounces=2):
+    """A workload from a Mitsuba scene file (load_mitsuba), at the file's
+    resolution or another one of the same aspect ratio."""
+    from .load_mitsuba import load_mitsuba
+    import os
+    camera, materials, shapes, lights, res = load_mitsuba(path)
+    return from_scene(camera, materials, shapes, lights, tuple(resolution or res), num_samples, max_bounces,
+                      name=os.path.splitext(os.path.basename(path))[0])
+
+
 WORKLOADS = {"single_triangle": single_triangle, "shadow": shadow, "heightfield": heightfield}

@@ -506,6 +506,23 @@ dr_status dr_test_line_integral(int64_t num_samples, uint64_t seed, double *out,
  * the stream. */
 dr_status dr_test_tree_stack(int64_t ops, uint64_t seed, int64_t *out, void *stream);
 
+/* ----------------------------------------------------------- scene files
+ * Mitsuba ".serialized" meshes (reference load_serialized.cpp:1-277, a
+ * pybind11 module returning (indices, vertices, uvs, normals) there).  Reads
+ * shape `shape_index` of the file into host arrays that the library
+ * allocates; dr_mesh_data_free releases them.  Unflagged attributes are NULL.
+ * Colors are read (the file holds them) but no renderer input uses them. */
+typedef struct dr_mesh_data {
+    int num_vertices, num_triangles;
+    float *vertices; /* [num_vertices][3] */
+    float *normals;  /* [num_vertices][3] or NULL */
+    float *uvs;      /* [num_vertices][2] or NULL */
+    float *colors;   /* [num_vertices][3] or NULL */
+    int *indices;    /* [num_triangles][3] */
+} dr_mesh_data;
+dr_status dr_load_serialized(const char *filename, int shape_index, dr_mesh_data *out);
+void dr_mesh_data_free(dr_mesh_data *mesh);
+
 const char *dr_last_error(void);
 
 #ifdef __cplusplus
