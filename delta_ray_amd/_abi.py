@@ -36,9 +36,11 @@ DR_SKIP_PRIMARY_EDGES = 1 << 10
 DR_WRITE_DX_IMAGE = 1 << 11
 DR_LEARNED_EDGES = 1 << 12
 DR_EDGE_STATS = 1 << 13
+DR_QUIRK_CLIP_DERIVATIVE = 1 << 14
 DR_FLAGS_PARITY = (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP |
                    DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE |
-                   DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN | DR_QUIRK_CONE_PMF)
+                   DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN | DR_QUIRK_CONE_PMF |
+                   DR_QUIRK_CLIP_DERIVATIVE)
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)

@@ -813,22 +813,80 @@ DR_INLINE F3 shade(const DrSceneView &S, F3 wi, const Isect &pis, const SPoint &
 }
 
 // ------------------------------------------------------------ camera ------
-// camera.h:48-84 (pinhole)
-DR_INLINE void sample_primary(const DrCamera &c, F2 s, F3 &org, F3 &dir, F3 *ndir = nullptr) {
+// camera.h:48-84.  Pinhole: the film at z = 1, w = tan(fov / 2).  Fisheye
+// (equi-angular): the screen disk's radius r maps linearly to the angle
+// r pi / 2 from the view axis; a screen point outside the disk has no ray
+// (false; the reference makes a ray without direction, which hits nothing).
+// The fisheye direction does not depend on fov or aspect, so the camera's
+// derivative through it reaches cam_to_world only (through ndir).
+DR_INLINE F3 fisheye_dir(float x, float y) {
+    float r = sqrtf(x * x + y * y);
+    float phi = atan2f(y, x);
+    float theta = r * DR_PI / 2.f;
+    float sp = sinf(phi), cp = cosf(phi), st = sinf(theta), ct = cosf(theta);
+    return F3{-cp * st, -sp * st, ct};
+}
+DR_INLINE bool sample_primary(const DrCamera &c, F2 s, F3 &org, F3 &dir, F3 *ndir = nullptr) {
     org = xfm_point(c.c2w, F3{0.f, 0.f, 0.f});
-    float nx = (s.x - 0.5f) * -2.f;
-    float ny = (s.y - 0.5f) * -2.f / c.aspect;
-    F3 nd = normalize(F3{c.fov * nx, c.fov * ny, 1.f});
+    F3 nd;
+    if (c.fisheye) {
+        float x = 2.f * (s.x - 0.5f), y = 2.f * (s.y - 0.5f);
+        if (x * x + y * y > 1.f) {
+            dir = F3{0.f, 0.f, 0.f};
+            if (ndir) *ndir = dir;
+            return false;
+        }
+        nd = normalize(fisheye_dir(x, y));
+    } else {
+        float nx = (s.x - 0.5f) * -2.f;
+        float ny = (s.y - 0.5f) * -2.f / c.aspect;
+        nd = normalize(F3{c.fov * nx, c.fov * ny, 1.f});
+    }
     if (ndir) *ndir = nd;
     dir = xfm_vector(c.c2w, nd);
+    return true;
 }
+// camera.h:86-111
 DR_INLINE F3 unproject(const DrCamera &c, F2 s) {
+    if (c.fisheye) return fisheye_dir(2.f * (s.x - 0.5f), 2.f * (s.y - 0.5f));
     float nx = (s.x - 0.5f) * -2.f;
     float ny = (s.y - 0.5f) * -2.f / c.aspect;
     return F3{c.fov * nx, c.fov * ny, 1.f};
 }
+// camera.h:113-153: d unproject / d screen x (dx) and y (dy)
+DR_INLINE void d_unproject(const DrCamera &c, F2 s, F3 &dx, F3 &dy) {
+    if (c.fisheye) {
+        float x = 2.f * (s.x - 0.5f), y = 2.f * (s.y - 0.5f);
+        float r = sqrtf(x * x + y * y);
+        float phi = atan2f(y, x);
+        float theta = r * DR_PI / 2.f;
+        float sp = sinf(phi), cp = cosf(phi), st = sinf(theta), ct = cosf(theta);
+        float dxdphi = sp * st, dxdth = -cp * ct;
+        float dydphi = -cp * st, dydth = -sp * ct;
+        float dzdth = -st;
+        float dphidx = -y / (r * r), dphidy = x / (r * r);
+        float dthdx = (DR_PI / 2.f) * x / r, dthdy = (DR_PI / 2.f) * y / r;
+        dx = 2.f * F3{dxdphi * dphidx + dxdth * dthdx, dydphi * dphidx + dydth * dthdx, dzdth * dthdx};
+        dy = 2.f * F3{dxdphi * dphidy + dxdth * dthdy, dydphi * dphidy + dydth * dthdy, dzdth * dthdy};
+        return;
+    }
+    dx = F3{-2.f * c.fov, 0.f, 0.f};
+    dy = F3{0.f, -2.f * c.fov / c.aspect, 0.f};
+}
+// camera.h:156-178
 DR_INLINE F2 project_local(const DrCamera &c, F3 l) {
+    if (c.fisheye) {
+        F3 d = normalize(l);
+        float phi = atan2f(d.y, d.x);
+        float r = acosf(d.z) * 2.f / DR_PI;
+        return F2{0.5f * (-r * cosf(phi) + 1.f), 0.5f * (-r * sinf(phi) + 1.f)};
+    }
     return F2{(-l.x / (l.z * c.fov) + 1.f) * 0.5f, (-l.y / (l.z * c.fov * c.aspect) + 1.f) * 0.5f};
+}
+// camera.h:314-320
+DR_INLINE bool in_screen(const DrCamera &c, F2 p) {
+    if (!c.fisheye) return p.x >= 0.f && p.x < 1.f && p.y >= 0.f && p.y < 1.f;
+    return (p.x - 0.5f) * (p.x - 0.5f) + (p.y - 0.5f) * (p.y - 0.5f) < 0.25f;
 }
 // camera.h:216-242
 DR_INLINE bool project_edge(const DrCamera &c, F3 p0, F3 p1, F2 &s0, F2 &s1) {
@@ -847,7 +905,23 @@ DR_INLINE bool project_edge(const DrCamera &c, F3 p0, F3 p1, F2 &s0, F2 &s1) {
     s1 = project_local(c, b);
     return true;
 }
+// camera.h:180-214
 DR_INLINE F3 d_project_local(const DrCamera &c, F3 l, float dx, float dy) {
+    if (c.fisheye) {
+        F3 d = normalize(l);
+        float phi = atan2f(d.y, d.x);
+        float r = acosf(d.z) * 2.f / DR_PI;
+        float dr = -0.5f * (cosf(phi) * dx + sinf(phi) * dy);
+        float dphi = 0.5f * r * sinf(phi) * dx - 0.5f * r * cosf(phi) * dy;
+        float dtheta = dr * (2.f / DR_PI);
+        float d_cos_theta = -dtheta / sqrtf(1.f - d.z * d.z);
+        float a2 = d.x * d.x + d.y * d.y;
+        F3 dd = F3{-dphi * d.y / a2, dphi * d.x / a2, d_cos_theta};
+        // vector.h:454-464 in its arithmetic
+        float len = length(l);
+        float d_l = -dot(dd, l) / (len * len);
+        return (dd + d_l * l) / len;
+    }
     float zf = l.z * c.fov;
     float dxd0 = -1.f / zf;
     float dxd2 = c.fov * l.x / (zf * zf);
@@ -856,17 +930,21 @@ DR_INLINE F3 d_project_local(const DrCamera &c, F3 l, float dx, float dy) {
     float dyd2 = c.fov * c.aspect * l.y / (zfy * zfy);
     return F3{dx * dxd0, dy * dyd1, dx * dxd2 + dy * dyd2};
 }
-// camera.h:244-313 (clip with -(z + clip_near) as there)
-DR_INLINE void d_project(const DrCamera &c, F3 p0, F3 p1, float d0x, float d0y, float d1x, float d1y, float *dw2c) {
+// camera.h:244-313.  clip_quirk: the reference's clip, t = -(z + clip_near)
+// / dz with dt / d(dz) = z / dz^2 (DR_QUIRK_CLIP_DERIVATIVE); otherwise the
+// derivative of project_edge's clip, t = -(z - clip_near) / dz.
+DR_INLINE void d_project(const DrCamera &c, F3 p0, F3 p1, float d0x, float d0y, float d1x, float d1y, float *dw2c,
+                         bool clip_quirk) {
     F3 a = xfm_point(c.w2c, p0), b = xfm_point(c.w2c, p1);
+    const float cn = clip_quirk ? -c.clip_near : c.clip_near;  // t = -(z - cn) / dz
     F3 ca = a, cb = b;
     if (a.z < c.clip_near) {
         F3 d = a - b;
-        float t = -(b.z + c.clip_near) / d.z;
+        float t = -(b.z - cn) / d.z;
         ca = b + t * d;
     } else if (b.z < c.clip_near) {
         F3 d = b - a;
-        float t = -(a.z + c.clip_near) / d.z;
+        float t = -(a.z - cn) / d.z;
         cb = a + t * d;
     }
     F3 dca = d_project_local(c, ca, d0x, d0y);
@@ -874,23 +952,23 @@ DR_INLINE void d_project(const DrCamera &c, F3 p0, F3 p1, float d0x, float d0y, 
     F3 da = F3{0.f, 0.f, 0.f}, db = F3{0.f, 0.f, 0.f};
     if (a.z < c.clip_near) {
         F3 d = a - b;
-        float t = -(b.z + c.clip_near) / d.z;
+        float t = -(b.z - cn) / d.z;
         db += dca;
         float dt = dot(d, dca);
         F3 dd = t * dca;
         db.z += (-dt / d.z);
-        dd.z += dt * (b.z / (d.z * d.z));
+        dd.z += dt * ((clip_quirk ? b.z : b.z - cn) / (d.z * d.z));
         da += dd;
         db = db - dd;
         db += dcb;
     } else if (b.z < c.clip_near) {
         F3 d = b - a;
-        float t = -(a.z + c.clip_near) / d.z;
+        float t = -(a.z - cn) / d.z;
         da += dcb;
         float dt = dot(d, dcb);
         F3 dd = t * dcb;
         da.z += (-dt / d.z);
-        dd.z += dt * (a.z / (d.z * d.z));
+        dd.z += dt * ((clip_quirk ? a.z : a.z - cn) / (d.z * d.z));
         db += dd;
         da = da - dd;
         da += dca;

@@ -831,7 +831,7 @@ DR_INLINE void sample_primary_edge_point(const DrSceneView &S, const DrEdge &edg
     F3 ep3 = v0d + t * vd3;
     F3 elocal = v0l + t * v1l;
     F2 ep = project_local(cam, ep3);
-    if (!(ep.x >= 0.f && ep.x < 1.f && ep.y >= 0.f && ep.y < 1.f)) return;
+    if (!in_screen(cam, ep)) return;
     int xi = min((int)(ep.x * W), W - 1), yi = min((int)(ep.y * H), H - 1);
     F3 d_color = load3(d_image + ((size_t)yi * W + xi) * 3);
     float delta = 1e-4f * length(elocal);
@@ -839,14 +839,15 @@ DR_INLINE void sample_primary_edge_point(const DrSceneView &S, const DrEdge &edg
     F3 rdir = normalize(ep3 + delta * spn);
     F2 rpt = project_local(cam, rdir);
     F3 rorg, rd;
-    sample_primary(cam, rpt, rorg, rd);
     SPoint spr, spl;
-    Isect ir = intersect(S, rorg, rd, spr);
+    // (a fisheye side point outside the screen disk has no ray: a miss)
+    Isect ir = no_hit();
+    if (sample_primary(cam, rpt, rorg, rd)) ir = intersect(S, rorg, rd, spr);
     F3 ldir = normalize(ep3 - delta * spn);
     F2 lpt = project_local(cam, ldir);
     F3 lorg, ld;
-    sample_primary(cam, lpt, lorg, ld);
-    Isect il = intersect(S, lorg, ld, spl);
+    Isect il = no_hit();
+    if (sample_primary(cam, lpt, lorg, ld)) il = intersect(S, lorg, ld, spl);
     bool rc = ir.shape == edge.shape && (ir.tri == edge.f0 || ir.tri == edge.f1);
     bool lc = il.shape == edge.shape && (il.tri == edge.f0 || il.tri == edge.f1);
     if (!rc && !lc) return;
@@ -856,14 +857,17 @@ DR_INLINE void sample_primary_edge_point(const DrSceneView &S, const DrEdge &edg
     F3 xdiff = lpt.x > rpt.x ? -diff : diff;
     F3 ydiff = lpt.y > rpt.y ? -diff : diff;
     F3 edir = unproject(cam, ep);
-    F3 dx = F3{-2.f * cam.fov, 0.f, 0.f};
-    F3 dy = F3{0.f, -2.f * cam.fov / cam.aspect, 0.f};
+    // d unproject / d screen at both endpoints and at the sample
+    F3 dx0, dy0, dx1, dy1, dxe, dye;
+    d_unproject(cam, v0s, dx0, dy0);
+    d_unproject(cam, v1s, dx1, dy1);
+    d_unproject(cam, ep, dxe, dye);
     F3 c01 = cross(v0d, v1d);
-    float dadx = dot(dx, c01), dady = dot(dy, c01);
-    float da0x = fabsf(dot(cross(v1d, edir), dx));
-    float da0y = fabsf(dot(cross(v1d, edir), dy));
-    float da1x = fabsf(dot(cross(edir, v0d), dx));
-    float da1y = fabsf(dot(cross(edir, v0d), dy));
+    float dadx = dot(dxe, c01), dady = dot(dye, c01);
+    float da0x = fabsf(dot(cross(v1d, edir), dx0));
+    float da0y = fabsf(dot(cross(v1d, edir), dy0));
+    float da1x = fabsf(dot(cross(edir, v0d), dx1));
+    float da1y = fabsf(dot(cross(edir, v0d), dy1));
     float dirac_j = 1.f / sqrtf(dadx * dadx + dady * dady);
     F3 ep3d = v0d + (t + delta) * vd3;
     F2 epd = project_local(cam, ep3d);
@@ -890,8 +894,9 @@ DR_INLINE void sample_primary_edge_point(const DrSceneView &S, const DrEdge &edg
     R.dv0 = F3{g[0], g[1], g[2]};
     R.dv1 = F3{g[3], g[4], g[5]};
     for (int i = 0; i < 16; i++) R.dw2c[i] = 0.f;
-    // the exact camera derivative carries project_local's 0.5
-    const float cs = (flags & DR_QUIRK_PROJECT_SCALE) ? 1.f : 0.5f;
-    d_project(cam, v0, v1, cs * d0x, cs * d0y, cs * d1x, cs * d1y, R.dw2c);
+    // the exact camera derivative carries project_local's 0.5 (the pinhole
+    // d_project_local drops it, the fisheye one keeps it)
+    const float cs = ((flags & DR_QUIRK_PROJECT_SCALE) || cam.fisheye) ? 1.f : 0.5f;
+    d_project(cam, v0, v1, cs * d0x, cs * d0y, cs * d1x, cs * d1y, R.dw2c, (flags & DR_QUIRK_CLIP_DERIVATIVE) != 0);
     R.ok = true;
 }

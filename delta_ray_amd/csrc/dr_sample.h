@@ -31,7 +31,7 @@ DR_INLINE F3 forward_sample(const DrSceneView &S, const SampleCtx &c, int xi, in
     LightSample ls;
     pixel_sample_draws(c, xi, yi, s, rng, scr, ls);
     F3 org, dir;
-    sample_primary(S.cam, scr, org, dir);
+    if (!sample_primary(S.cam, scr, org, dir)) return F3{0.f, 0.f, 0.f};
     SPoint sp;
     Isect is = intersect(S, org, dir, sp);
     if (!valid(is)) return F3{0.f, 0.f, 0.f};
@@ -218,10 +218,11 @@ DR_INLINE bool trace_path(const DrSceneView &S, const SampleCtx &c, int xi, int 
     pixel_sample_draws(c, xi, yi, s, rng, scr, P.ls);
     SecondarySample ss[MAXB];
     for (int b = 0; b < c.nb; b++) ss[b] = draw_secondary(rng);
-    sample_primary(S.cam, scr, P.org, P.dir, &P.nd);
+    bool has_ray = sample_primary(S.cam, scr, P.org, P.dir, &P.nd);
     P.nused = 0;
     P.primary_light = -1;
     P.color = F3{0.f, 0.f, 0.f};
+    if (!has_ray) return false;
     // the differentiated path computes surface points as the tape does
     P.is[0] = intersect(S, P.org, P.dir, P.sp[0], true);
     if (!valid(P.is[0])) return false;

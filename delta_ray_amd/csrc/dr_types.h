@@ -85,7 +85,8 @@ struct DrEdgeNode {
 
 struct DrCamera {
     float c2w[16], w2c[16];
-    float fov, aspect, clip_near, pad;
+    float fov, aspect, clip_near;
+    int fisheye;  // equi-angular projection (the screen disk's radius maps linearly to the angle)
 };
 
 // Everything a kernel needs, passed by value.

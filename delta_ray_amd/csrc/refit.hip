@@ -280,6 +280,12 @@ __global__ void __launch_bounds__(RF_BLOCK) k_etree_cone(DrEdgeNode *nodes, int 
 namespace {
 // (xfm_point of dr_math.h has the host builder's arithmetic)
 DR_INLINE F2 project_local(const DrCamera &c, F3 l) {
+    if (c.fisheye) {
+        F3 d = normalize(l);
+        float phi = atan2f(d.y, d.x);
+        float r = acosf(d.z) * 2.f / DR_PI;
+        return F2{0.5f * (-r * cosf(phi) + 1.f), 0.5f * (-r * sinf(phi) + 1.f)};
+    }
     return F2{(-l.x / (l.z * c.fov) + 1.f) * 0.5f, (-l.y / (l.z * c.fov * c.aspect) + 1.f) * 0.5f};
 }
 DR_INLINE bool project_edge(const DrCamera &c, F3 p0, F3 p1, F2 &s0, F2 &s1) {

@@ -1501,7 +1501,7 @@ dr_status dr_scene_create(const dr_scene_desc *desc, dr_scene **out) {
     }
     if (!err.empty()) {
         delete s;
-        return fail(err.find("fisheye") != std::string::npos ? DR_ERROR_UNSUPPORTED : DR_ERROR_INVALID_ARGUMENT, err);
+        return fail(DR_ERROR_INVALID_ARGUMENT, err);
     }
     drh::HostScene &hs = s->hs;
     DrSceneView &v = s->view;
@@ -1690,7 +1690,6 @@ void dr_scene_destroy(dr_scene *s) {
 
 dr_status dr_scene_set_camera(dr_scene *s, const dr_camera *c) {
     if (!s || !c) return fail(DR_ERROR_INVALID_ARGUMENT, "null argument");
-    if (c->fisheye) return fail(DR_ERROR_UNSUPPORTED, "fisheye cameras are not supported");
     DrCamera cam = drh::make_camera(*c);
     int n = 0;
     float fi = 0.f;
@@ -2018,7 +2017,6 @@ dr_status dr_scene_edge_stats(dr_scene *s, float *out, int reset, void *stream) 
 dr_status dr_view_create(dr_scene *s, const dr_camera *c, dr_view **out) {
     if (!s || !c || !out) return fail(DR_ERROR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    if (c->fisheye) return fail(DR_ERROR_UNSUPPORTED, "fisheye cameras are not supported");
     dr_view *v = new (std::nothrow) dr_view();
     if (!v) return fail(DR_ERROR_OUT_OF_MEMORY, "host allocation failed");
     v->scene = s;

@@ -451,7 +451,14 @@ V3 xfm_point(const float *a, V3 p) {
 struct V2 {
     float x, y;
 };
+// camera.h:156-178
 V2 project_local(const DrCamera &c, V3 l) {
+    if (c.fisheye) {
+        V3 d = normalize(l);
+        float phi = std::atan2(d.y, d.x);
+        float r = std::acos(d.z) * 2.f / float(M_PI);
+        return V2{0.5f * (-r * std::cos(phi) + 1.f), 0.5f * (-r * std::sin(phi) + 1.f)};
+    }
     return V2{(-l.x / (l.z * c.fov) + 1.f) * 0.5f, (-l.y / (l.z * c.fov * c.aspect) + 1.f) * 0.5f};
 }
 bool project_edge(const DrCamera &c, V3 p0, V3 p1, V2 &s0, V2 &s1) {
@@ -557,7 +564,7 @@ DrCamera make_camera(const dr_camera &c) {
     o.fov = c.fov_factor;
     o.aspect = c.aspect_ratio;
     o.clip_near = c.clip_near;
-    o.pad = 0.f;
+    o.fisheye = c.fisheye ? 1 : 0;
     return o;
 }
 
@@ -565,7 +572,6 @@ void set_camera(HostScene &hs, const dr_camera &c) { hs.cam = make_camera(c); }
 
 std::string build_host_scene(const dr_scene_desc *d, HostScene &hs) {
     if (!d) return "null scene description";
-    if (d->camera.fisheye) return "fisheye cameras are not supported";
     if (d->num_shapes < 0 || d->num_materials < 0 || d->num_lights < 0) return "negative counts";
     if (d->num_lights == 0) return "the scene has no light (the reference indexes an empty light list)";
     set_camera(hs, d->camera);

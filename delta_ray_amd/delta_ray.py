@@ -132,8 +132,6 @@ class SceneDesc:
     """A dr_scene_desc plus the numpy arrays it points into (kept alive)."""
 
     def __init__(self, camera, shapes, materials, lights):
-        if camera.fisheye:
-            raise NotImplementedError("fisheye cameras are not supported by delta_ray_amd")
         keep = []
         d = _abi.dr_scene_desc()
         d.camera = SceneDesc.camera_struct(camera)
@@ -208,8 +206,6 @@ def _material_structs(materials):
 
 
 def _camera_struct(camera):
-    if camera.fisheye:
-        raise NotImplementedError("fisheye cameras are not supported by delta_ray_amd")
     c = _abi.dr_camera()
     for name in ("cam_to_world", "world_to_cam", "sample_to_cam", "cam_to_sample"):
         m = _abi.f32(getattr(camera, name))
@@ -219,7 +215,7 @@ def _camera_struct(camera):
     c.fov_factor = float(camera.fov_factor)
     c.aspect_ratio = float(camera.aspect_ratio)
     c.clip_near = float(camera.clip_near)
-    c.fisheye = 0
+    c.fisheye = 1 if camera.fisheye else 0
     return c
 
 

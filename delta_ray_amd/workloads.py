@@ -52,7 +52,7 @@ def perspective_matrix(fov, clip_near, clip_far):
 
 
 def make_camera(position, look_at, up, fov, resolution, clip_near=0.01, clip_far=10000.0,
-                cam_to_world=None):
+                cam_to_world=None, fisheye=False):
     """camera.py:7-47 -> a delta_ray.Camera."""
     c2w = look_at_matrix(position, look_at, up) if cam_to_world is None else np.asarray(cam_to_world, np.float64)
     w2c = np.linalg.inv(c2w)
@@ -65,7 +65,7 @@ def make_camera(position, look_at, up, fov, resolution, clip_near=0.01, clip_far
     s2c = np.linalg.inv(c2s)
     fov_factor = math.tan(math.radians(0.5 * fov))
     return dr.Camera(c2w.astype(np.float32), w2c.astype(np.float32), s2c.astype(np.float32),
-                     c2s.astype(np.float32), np.float32(fov_factor), aspect, clip_near, False)
+                     c2s.astype(np.float32), np.float32(fov_factor), aspect, clip_near, fisheye)
 
 
 def diffuse_material(albedo, specular=(0.0, 0.0, 0.0), roughness=(1.0,), two_sided=False):
@@ -123,6 +123,38 @@ def single_triangle(seed=1, resolution=(256, 256), num_samples=4, max_bounces=1)
     return dict(name="single_triangle", camera=cam, shapes=[tri, light_shape], materials=[grey, black],
                 lights=[light], resolution=tuple(resolution), num_samples=num_samples,
                 max_bounces=max_bounces, params=[("shape", 0, "vertices")])
+
+
+def _triangle_and_quad_light(tri, light_z, intensity, cam, resolution, num_samples, max_bounces, name):
+    grey = diffuse_material([0.5, 0.5, 0.5])
+    tv = np.asarray(tri, np.float32)
+    lv = np.array([[-1, -1, light_z], [1, -1, light_z], [-1, 1, light_z], [1, 1, light_z]], np.float32)
+    li = np.array([[0, 1, 2], [1, 3, 2]], np.int32)
+    t = dr.Shape(tv, np.array([[0, 1, 2]], np.int32), None, None, grey, None)
+    ls = dr.Shape(lv, li, None, None, grey, None)
+    light = dr.Light(ls, np.full(3, intensity, np.float32))
+    ls.light = light
+    return dict(name=name, camera=cam, shapes=[t, ls], materials=[grey], lights=[light],
+                resolution=tuple(resolution), num_samples=num_samples, max_bounces=max_bounces,
+                params=[("shape", 0, "vertices")])
+
+
+def single_triangle_fisheye(resolution=(256, 256), num_samples=4, max_bounces=1):
+    """The reference's fisheye test scene (test_single_triangle_fisheye.py):
+    a triangle at z = 0 seen from z = -1 through the equi-angular camera, a
+    quad light at z = -9."""
+    cam = make_camera([0, 0, -1], [0, 0, 0], [0, 1, 0], 45.0, resolution, fisheye=True)
+    return _triangle_and_quad_light([[-1.7, 1.0, 0.0], [1.0, 1.0, 0.0], [-0.5, -1.0, 0.0]], -9.0, 30.0, cam,
+                                    resolution, num_samples, max_bounces, "single_triangle_fisheye")
+
+
+def single_triangle_clipped(resolution=(256, 256), num_samples=4, max_bounces=1):
+    """The reference's near-plane test scene (test_single_triangle_clipped.py):
+    one vertex of the triangle lies behind the camera, so two of its edges
+    cross the near plane and are clipped before projection."""
+    cam = make_camera([0, 0, -5], [0, 0, 0], [0, 1, 0], 45.0, resolution)
+    return _triangle_and_quad_light([[-1.3, 1.0, 0.0], [1.0, 1.0, 0.0], [-0.5, -2.0, -7.0]], -7.0, 20.0, cam,
+                                    resolution, num_samples, max_bounces, "single_triangle_clipped")
 
 
 def shadow(seed=2, resolution=(256, 256), num_samples=16, max_bounces=1, light_intensity=10.0):

@@ -49,8 +49,10 @@ typedef enum dr_status {
 } dr_status;
 
 /* Camera (reference camera.h:12-21).  Matrices are row-major 4x4.
- * fov_factor = tan(fov/2).  Only the pinhole model is implemented; a scene
- * with fisheye != 0 is rejected with DR_ERROR_UNSUPPORTED. */
+ * fov_factor = tan(fov/2).  fisheye != 0: the equi-angular projection of
+ * camera.h:51-71 (the screen disk's radius maps linearly to the angle from
+ * the view axis, pi/2 at the rim; fov and aspect are not used); screen
+ * points outside the disk stay black. */
 typedef struct dr_camera {
     float cam_to_world[16];
     float world_to_cam[16];
@@ -127,9 +129,9 @@ enum {
      * last one is propagated (set_adjoint overwrites, autodiff.h:340-342,
      * delta_ray.cpp:977-982) */
     DR_QUIRK_DP_LAST = 1u << 3,
-    /* the world_to_cam gradient of primary edges is twice the derivative:
-     * d_project_local (camera.h:204-213) drops the 0.5 of project_local
-     * (camera.h:170-171) */
+    /* the world_to_cam gradient of primary edges seen by a pinhole camera is
+     * twice the derivative: d_project_local (camera.h:204-213) drops the 0.5
+     * of project_local (camera.h:170-171); its fisheye branch keeps it */
     DR_QUIRK_PROJECT_SCALE = 1u << 4,
     /* a secondary-edge sample weighs the whole BSDF by the pmf of the
      * component (diffuse / specular) whose LTC density drew it
@@ -164,11 +166,18 @@ enum {
     DR_LEARNED_EDGES = 1u << 12,
     /* accumulate per-edge statistics of the secondary-edge samples
      * (dr_scene_edge_stats) */
-    DR_EDGE_STATS = 1u << 13
+    DR_EDGE_STATS = 1u << 13,
+    /* the world_to_cam gradient of a primary edge clipped at the near plane
+     * differentiates another clip than the one drawn: d_project clips at
+     * t = -(z + clip_near) / dz where project clips at -(z - clip_near) / dz,
+     * and drops clip_near from dt / d(dz) (camera.h:231-236 vs 259-290);
+     * without it the clipped endpoint's derivative is the exact one */
+    DR_QUIRK_CLIP_DERIVATIVE = 1u << 14
 };
 #define DR_FLAGS_PARITY (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP | \
                          DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE | \
-                         DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN | DR_QUIRK_CONE_PMF)
+                         DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN | DR_QUIRK_CONE_PMF | \
+                         DR_QUIRK_CLIP_DERIVATIVE)
 
 typedef struct dr_render_options {
     int width;

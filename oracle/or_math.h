@@ -150,6 +150,14 @@ inline M3 inverse(const M3 &m) {
     return r;
 }
 
+// adjoint of normalize (vector.h:454-464, in its arithmetic)
+inline V3f d_normalize(const V3f &v, const V3f &d_out) {
+    float l = length(v);
+    float d_inv_l = dot(d_out, v);
+    float d_l = -d_inv_l / (l * l);
+    return (d_out + d_l * v) / l;
+}
+
 // d normalize(x) / dx
 inline M3 normalize_jacobian(const V3f &x) {
     float dsq = length_squared(x);

@@ -438,6 +438,7 @@ void build_scene(Scene &sc, const dr_scene_desc *d) {
     sc.cam.fov = c.fov_factor;
     sc.cam.aspect = c.aspect_ratio;
     sc.cam.clip_near = c.clip_near;
+    sc.cam.fisheye = c.fisheye != 0;
     for (int i = 0; i < d->num_materials; i++) {
         const dr_material &m = d->materials[i];
         Material o;

@@ -54,6 +54,7 @@ struct Light {
 struct Camera {
     M4T<float> c2w, w2c, s2c, c2s;
     float fov, aspect, clip_near;
+    bool fisheye = false;  // equi-angular projection (camera.h:51-71)
 };
 
 // pbrt-style piecewise-constant 1-D distribution (distribution.h:7-59)

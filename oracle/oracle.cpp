@@ -594,24 +594,95 @@ struct CamT {
     ADR fov, aspect;
 };
 
+// camera.h:48-84.  Pinhole: the film at z = 1, w = tan(fov / 2).  Fisheye
+// (equi-angular): the screen disk's radius r maps linearly to the angle
+// theta = r pi / 2 from the view axis; outside the disk the ray has no
+// direction (false: no ray).  The fisheye direction depends on the screen
+// position only, so its derivative reaches cam_to_world alone.
 template <class M, class T>
-void sample_primary(const M &c2w, const T &fov, const T &aspect, const V2<T> &s, V3<T> &org, V3<T> &dir) {
+bool sample_primary(const M &c2w, const T &fov, const T &aspect, bool fisheye, const V2<T> &s, V3<T> &org,
+                    V3<T> &dir) {
     org = xfm_point(c2w, v3<T>(T(0.f), T(0.f), T(0.f)));
+    if (fisheye) {
+        float x = 2.f * (val(s[0]) - 0.5f);
+        float y = 2.f * (val(s[1]) - 0.5f);
+        if (x * x + y * y > 1.f) {
+            dir = v3<T>(T(0.f), T(0.f), T(0.f));
+            return false;
+        }
+        float r = std::sqrt(x * x + y * y);
+        float phi = std::atan2(y, x);
+        float theta = r * float(M_PI) / 2.f;
+        float sp = std::sin(phi), cp = std::cos(phi), st = std::sin(theta), ct = std::cos(theta);
+        V3<T> d = v3<T>(T(-cp * st), T(-sp * st), T(ct));
+        dir = xfm_vector(c2w, normalize(d));
+        return true;
+    }
     T nx = (s[0] - 0.5f) * -2.f;
     T ny = (s[1] - 0.5f) * -2.f / aspect;
     V3<T> d = v3<T>(fov * nx, fov * ny, T(1.f));
     dir = xfm_vector(c2w, normalize(d));
+    return true;
+}
+inline bool sample_primary(const Camera &c, const V2f &s, V3f &org, V3f &dir) {
+    return sample_primary(c.c2w, c.fov, c.aspect, c.fisheye, s, org, dir);
 }
 
+// camera.h:86-111: the (unnormalised) local direction of a screen point
 inline V3f unproject(const Camera &c, const V2f &s) {
+    if (c.fisheye) {
+        float x = 2.f * (s[0] - 0.5f);
+        float y = 2.f * (s[1] - 0.5f);
+        float r = std::sqrt(x * x + y * y);
+        float phi = std::atan2(y, x);
+        float theta = r * float(M_PI) / 2.f;
+        float sp = std::sin(phi), cp = std::cos(phi), st = std::sin(theta), ct = std::cos(theta);
+        return v3(-cp * st, -sp * st, ct);
+    }
     float nx = (s[0] - 0.5f) * -2.f;
     float ny = (s[1] - 0.5f) * -2.f / c.aspect;
     return v3(c.fov * nx, c.fov * ny, 1.f);
 }
+// camera.h:113-153: d unproject / d screen x and y
+inline void d_unproject(const Camera &c, const V2f &s, V3f &dx, V3f &dy) {
+    if (c.fisheye) {
+        float x = 2.f * (s[0] - 0.5f);
+        float y = 2.f * (s[1] - 0.5f);
+        float r = std::sqrt(x * x + y * y);
+        float phi = std::atan2(y, x);
+        float theta = r * float(M_PI) / 2.f;
+        float sp = std::sin(phi), cp = std::cos(phi), st = std::sin(theta), ct = std::cos(theta);
+        float dxdphi = sp * st, dxdth = -cp * ct;
+        float dydphi = -cp * st, dydth = -sp * ct;
+        float dzdth = -st;
+        float dphidx = -y / (r * r), dphidy = x / (r * r);
+        float dthdx = (float(M_PI) / 2.f) * x / r, dthdy = (float(M_PI) / 2.f) * y / r;
+        dx = 2.f * v3(dxdphi * dphidx + dxdth * dthdx, dydphi * dphidx + dydth * dthdx, dzdth * dthdx);
+        dy = 2.f * v3(dxdphi * dphidy + dxdth * dthdy, dydphi * dphidy + dydth * dthdy, dzdth * dthdy);
+        return;
+    }
+    dx = v3(-2.f * c.fov, 0.f, 0.f);
+    dy = v3(0.f, -2.f * c.fov / c.aspect, 0.f);
+}
+// camera.h:156-178
 inline V2f project_local(const Camera &c, const V3f &l) {
+    if (c.fisheye) {
+        V3f d = normalize(l);
+        float cos_theta = d[2];
+        float phi = std::atan2(d[1], d[0]);
+        float theta = std::acos(cos_theta);
+        float r = theta * 2.f / float(M_PI);
+        return v2(0.5f * (-r * std::cos(phi) + 1.f), 0.5f * (-r * std::sin(phi) + 1.f));
+    }
     float x = (-l[0] / (l[2] * c.fov) + 1.f) * 0.5f;
     float y = (-l[1] / (l[2] * c.fov * c.aspect) + 1.f) * 0.5f;
     return v2(x, y);
+}
+// camera.h:314-320
+inline bool in_screen(const Camera &c, const V2f &p) {
+    if (!c.fisheye) return p[0] >= 0.f && p[0] < 1.f && p[1] >= 0.f && p[1] < 1.f;
+    float dist_sq = (p[0] - 0.5f) * (p[0] - 0.5f) + (p[1] - 0.5f) * (p[1] - 0.5f);
+    return dist_sq < 0.25f;
 }
 // camera.h:216-242
 bool project_edge(const Camera &c, V3f p0, V3f p1, V2f &s0, V2f &s1) {
@@ -630,7 +701,21 @@ bool project_edge(const Camera &c, V3f p0, V3f p1, V2f &s0, V2f &s1) {
     s1 = project_local(c, b);
     return true;
 }
+// camera.h:180-214
 inline V3f d_project_local(const Camera &c, const V3f &l, float dx, float dy) {
+    if (c.fisheye) {
+        V3f d = normalize(l);
+        float phi = std::atan2(d[1], d[0]);
+        float theta = std::acos(d[2]);
+        float r = theta * 2.f / float(M_PI);
+        float dr = -0.5f * (std::cos(phi) * dx + std::sin(phi) * dy);
+        float dphi = 0.5f * r * std::sin(phi) * dx - 0.5f * r * std::cos(phi) * dy;
+        float dtheta = dr * (2.f / float(M_PI));
+        float d_cos_theta = -dtheta / std::sqrt(1.f - d[2] * d[2]);
+        float a2 = d[0] * d[0] + d[1] * d[1];
+        V3f dd = v3(-dphi * d[1] / a2, dphi * d[0] / a2, d_cos_theta);
+        return d_normalize(l, dd);
+    }
     float zf = l[2] * c.fov;
     float dxd0 = -1.f / zf;
     float dxd2 = c.fov * l[0] / (zf * zf);
@@ -639,18 +724,22 @@ inline V3f d_project_local(const Camera &c, const V3f &l, float dx, float dy) {
     float dyd2 = c.fov * c.aspect * l[1] / (zfy * zfy);
     return v3(dx * dxd0, dy * dyd1, dx * dxd2 + dy * dyd2);
 }
-// camera.h:244-313 (clip uses -(z + clip_near) here; d_cam_to_sample untouched)
+// camera.h:244-313 (d_cam_to_sample untouched).  clip_quirk: the
+// reference's clip, t = -(z + clip_near) / dz with dt / d(dz) = z / dz^2
+// (DR_QUIRK_CLIP_DERIVATIVE); otherwise the derivative of project_edge's
+// clip, t = -(z - clip_near) / dz.
 void d_project(const Camera &c, const V3f &p0, const V3f &p1, float d0x, float d0y, float d1x, float d1y,
-               float dw2c[16]) {
+               float dw2c[16], bool clip_quirk = true) {
     V3f a = xfm_point(c.w2c, p0), b = xfm_point(c.w2c, p1);
+    const float cn = clip_quirk ? -c.clip_near : c.clip_near;  // t = -(z - cn) / dz
     V3f ca = a, cb = b;
     if (a[2] < c.clip_near) {
         V3f d = a - b;
-        float t = -(b[2] + c.clip_near) / d[2];
+        float t = -(b[2] - cn) / d[2];
         ca = b + t * d;
     } else if (b[2] < c.clip_near) {
         V3f d = b - a;
-        float t = -(a[2] + c.clip_near) / d[2];
+        float t = -(a[2] - cn) / d[2];
         cb = a + t * d;
     }
     V3f dca = d_project_local(c, ca, d0x, d0y);
@@ -658,23 +747,23 @@ void d_project(const Camera &c, const V3f &p0, const V3f &p1, float d0x, float d
     V3f da = v3(0.f, 0.f, 0.f), db = v3(0.f, 0.f, 0.f);
     if (a[2] < c.clip_near) {
         V3f d = a - b;
-        float t = -(b[2] + c.clip_near) / d[2];
+        float t = -(b[2] - cn) / d[2];
         db += dca;
         float dt = dot(d, dca);
         V3f dd = t * dca;
         db[2] += (-dt / d[2]);
-        dd[2] += dt * (b[2] / (d[2] * d[2]));
+        dd[2] += dt * ((clip_quirk ? b[2] : b[2] - cn) / (d[2] * d[2]));
         da += dd;
         db = db - dd;
         db += dcb;
     } else if (b[2] < c.clip_near) {
         V3f d = b - a;
-        float t = -(a[2] + c.clip_near) / d[2];
+        float t = -(a[2] - cn) / d[2];
         da += dcb;
         float dt = dot(d, dcb);
         V3f dd = t * dcb;
         da[2] += (-dt / d[2]);
-        dd[2] += dt * (a[2] / (d[2] * d[2]));
+        dd[2] += dt * ((clip_quirk ? a[2] : a[2] - cn) / (d[2] * d[2]));
         db += dd;
         da = da - dd;
         da += dca;
@@ -1088,8 +1177,6 @@ struct PrimaryResult {
     float dw2c[16];
 };
 
-inline bool in_screen(const V2f &p) { return p[0] >= 0.f && p[0] < 1.f && p[1] >= 0.f && p[1] < 1.f; }
-
 // delta_ray.cpp:377-557
 PrimaryResult sample_primary_edge_point(const Scene &sc, const Edge &edge, float eu, const LightSample &ls,
                                         const std::vector<SecondarySample> &ss, const float *d_image, int W, int H,
@@ -1109,7 +1196,7 @@ PrimaryResult sample_primary_edge_point(const Scene &sc, const Edge &edge, float
     V3f ep3 = v0d + t * vd3;
     V3f elocal = v0l + t * v1l;
     V2f ep = project_local(cam, ep3);
-    if (!in_screen(ep)) return R;
+    if (!in_screen(cam, ep)) return R;
     int xi = int(ep[0] * W), yi = int(ep[1] * H);
     const float *dc = &d_image[((size_t)yi * W + xi) * 3];
     V3f d_color = v3(dc[0], dc[1], dc[2]);
@@ -1118,14 +1205,13 @@ PrimaryResult sample_primary_edge_point(const Scene &sc, const Edge &edge, float
     V3f rdir = normalize(ep3 + delta * spn);
     V2f rpt = project_local(cam, rdir);
     V3f rorg, rd;
-    sample_primary(cam.c2w, cam.fov, cam.aspect, rpt, rorg, rd);
     SP<float> spr, spl;
-    Isect ir = intersect<float>(sc, rorg, rd, nullptr, spr);
+    // (a fisheye side point outside the screen disk has no ray: a miss)
+    Isect ir = sample_primary(cam, rpt, rorg, rd) ? intersect<float>(sc, rorg, rd, nullptr, spr) : Isect{};
     V3f ldir = normalize(ep3 - delta * spn);
     V2f lpt = project_local(cam, ldir);
     V3f lorg, ld;
-    sample_primary(cam.c2w, cam.fov, cam.aspect, lpt, lorg, ld);
-    Isect il = intersect<float>(sc, lorg, ld, nullptr, spl);
+    Isect il = sample_primary(cam, lpt, lorg, ld) ? intersect<float>(sc, lorg, ld, nullptr, spl) : Isect{};
     bool rc = ir.shape == edge.shape && (ir.tri == edge.f0 || ir.tri == edge.f1);
     bool lc = il.shape == edge.shape && (il.tri == edge.f0 || il.tri == edge.f1);
     if (!rc && !lc) return R;
@@ -1136,14 +1222,17 @@ PrimaryResult sample_primary_edge_point(const Scene &sc, const Edge &edge, float
     if (lpt[0] > rpt[0]) xdiff = -xdiff;
     if (lpt[1] > rpt[1]) ydiff = -ydiff;
     V3f edir = unproject(cam, ep);
-    V3f dx = v3(-2.f * cam.fov, 0.f, 0.f);
-    V3f dy = v3(0.f, -2.f * cam.fov / cam.aspect, 0.f);
+    // d unproject / d screen at both endpoints and at the sample
+    V3f dx0, dy0, dx1, dy1, dxe, dye;
+    d_unproject(cam, v0s, dx0, dy0);
+    d_unproject(cam, v1s, dx1, dy1);
+    d_unproject(cam, ep, dxe, dye);
     V3f c01 = cross(v0d, v1d);
-    float dadx = dot(dx, c01), dady = dot(dy, c01);
-    float da0x = std::fabs(dot(cross(v1d, edir), dx));
-    float da0y = std::fabs(dot(cross(v1d, edir), dy));
-    float da1x = std::fabs(dot(cross(edir, v0d), dx));
-    float da1y = std::fabs(dot(cross(edir, v0d), dy));
+    float dadx = dot(dxe, c01), dady = dot(dye, c01);
+    float da0x = std::fabs(dot(cross(v1d, edir), dx0));
+    float da0y = std::fabs(dot(cross(v1d, edir), dy0));
+    float da1x = std::fabs(dot(cross(edir, v0d), dx1));
+    float da1y = std::fabs(dot(cross(edir, v0d), dy1));
     float dirac_j = 1.f / std::sqrt(dadx * dadx + dady * dady);
     V3f ep3d = v0d + (t + delta) * vd3;
     V2f epd = project_local(cam, ep3d);
@@ -1168,9 +1257,10 @@ PrimaryResult sample_primary_edge_point(const Scene &sc, const Edge &edge, float
     }
     R.dv0 = v3(g[0], g[1], g[2]);
     R.dv1 = v3(g[3], g[4], g[5]);
-    // the exact camera derivative carries project_local's 0.5
-    float cs = (flags & DR_QUIRK_PROJECT_SCALE) ? 1.f : 0.5f;
-    d_project(cam, v0, v1, cs * d0x, cs * d0y, cs * d1x, cs * d1y, R.dw2c);
+    // the exact camera derivative carries project_local's 0.5 (the pinhole
+    // d_project_local drops it, the fisheye one keeps it)
+    float cs = ((flags & DR_QUIRK_PROJECT_SCALE) || cam.fisheye) ? 1.f : 0.5f;
+    d_project(cam, v0, v1, cs * d0x, cs * d0y, cs * d1x, cs * d1y, R.dw2c, (flags & DR_QUIRK_CLIP_DERIVATIVE) != 0);
     R.sx = ep[0];
     R.sy = ep[1];
     R.ok = true;
@@ -1256,7 +1346,7 @@ V3f render_pixel_sample(const RenderCtx &rc, int xi, int yi, int s, Accum &acc, 
     if (!rc.adjoint || is_zero(d_color)) {
         float x = float((xi + sub_x) / o.width), y = float((yi + sub_y) / o.height);
         V3f org, dir;
-        sample_primary(sc.cam.c2w, sc.cam.fov, sc.cam.aspect, v2(x, y), org, dir);
+        if (!sample_primary(sc.cam, v2(x, y), org, dir)) return v3(0.f, 0.f, 0.f);
         SP<float> sp;
         Isect is = intersect<float>(sc, org, dir, nullptr, sp);
         if (!is.valid()) return v3(0.f, 0.f, 0.f);
@@ -1272,7 +1362,7 @@ V3f render_pixel_sample(const RenderCtx &rc, int xi, int yi, int s, Accum &acc, 
     cam.aspect = ADR(sc.cam.aspect);
     ADR x = ADR((xi + sub_x) / o.width), y = ADR((yi + sub_y) / o.height);
     V3<ADR> org, dir;
-    sample_primary(cam.c2w, cam.fov, cam.aspect, v2(x, y), org, dir);
+    if (!sample_primary(cam.c2w, cam.fov, cam.aspect, sc.cam.fisheye, v2(x, y), org, dir)) return v3(0.f, 0.f, 0.f);
     Rec<ADR> rec;
     SP<ADR> sp;
     Isect pis = intersect<ADR>(sc, org, dir, &rec, sp);
@@ -1661,11 +1751,27 @@ void or_probe_ltc(void *s, int mid, const float *sp, const float *wi, float *out
 void or_probe_sample_primary(void *s, const float *screen, float *out) {
     const Camera &c = ((OScene *)s)->sc.cam;
     V3f org, dir;
-    sample_primary(c.c2w, c.fov, c.aspect, v2(screen[0], screen[1]), org, dir);
+    sample_primary(c, v2(screen[0], screen[1]), org, dir);
     for (int i = 0; i < 3; i++) {
         out[i] = org[i];
         out[3 + i] = dir[i];
     }
+}
+
+// camera functions at a screen point / local direction:
+// out = unproject[3], d_unproject dx[3], dy[3], project_local(local)[2],
+// d_project_local(local, dxy)[3], in_screen(screen)
+void or_probe_camera(void *s, const float *screen, const float *local, const float *dxy, float *out) {
+    const Camera &c = ((OScene *)s)->sc.cam;
+    V2f sp = v2(screen[0], screen[1]);
+    V3f u = unproject(c, sp), dx, dy;
+    d_unproject(c, sp, dx, dy);
+    V3f l = v3(local[0], local[1], local[2]);
+    V2f p = project_local(c, l);
+    V3f dl = d_project_local(c, l, dxy[0], dxy[1]);
+    const float vals[15] = {u[0], u[1], u[2], dx[0], dx[1], dx[2], dy[0], dy[1], dy[2], p[0], p[1],
+                            dl[0], dl[1], dl[2], in_screen(c, sp) ? 1.f : 0.f};
+    for (int i = 0; i < 15; i++) out[i] = vals[i];
 }
 
 // returns 0 when the segment is behind the near plane; out = s0[2], s1[2]
@@ -1683,6 +1789,16 @@ void or_probe_d_project(void *s, const float *p0, const float *p1, const float *
     const Camera &c = ((OScene *)s)->sc.cam;
     for (int i = 0; i < 16; i++) out[i] = 0.f;
     d_project(c, v3(p0[0], p0[1], p0[2]), v3(p1[0], p1[1], p1[2]), dxy[0], dxy[1], dxy[2], dxy[3], out);
+}
+
+// the same with the clip derivative chosen: clip_quirk != 0 the reference's,
+// 0 the derivative of project_edge's clip
+void or_probe_d_project_clip(void *s, const float *p0, const float *p1, const float *dxy, int clip_quirk,
+                             float *out) {
+    const Camera &c = ((OScene *)s)->sc.cam;
+    for (int i = 0; i < 16; i++) out[i] = 0.f;
+    d_project(c, v3(p0[0], p0[1], p0[2]), v3(p1[0], p1[1], p1[2]), dxy[0], dxy[1], dxy[2], dxy[3], out,
+              clip_quirk != 0);
 }
 
 int or_probe_cone_box(const float *lo, const float *hi, const float *org, const float *dir, float cos_angle) {

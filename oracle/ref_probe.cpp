@@ -6,7 +6,9 @@
 // that tests can compare the CPU restatement in oracle/ with them:
 //   intersect.h:71-90      Moller-Trumbore (u, v, t)
 //   material.h:257-489     bsdf, sample_bsdf, bsdf_pdf, get_ltc_matrix
-//   camera.h:48-313        sample_primary, project, d_project
+//   camera.h:48-320        sample_primary (pinhole and fisheye), unproject,
+//                          d_unproject, project_local, d_project_local,
+//                          project, d_project, in_screen
 //   aabb.cpp:98-311        AABB::intersect(Cone)
 //   line_clip.h            clip_line
 //   vector.h:690-733       intersect_jacobian, normalize_jacobian
@@ -54,8 +56,9 @@ SurfacePoint make_sp(const std::vector<float> &sp) {
     return p;
 }
 
-Camera make_camera(FArr c2w, FArr w2c, FArr s2c, FArr c2s, float fov, float aspect, float clip_near) {
-    return Camera{c2w, w2c, s2c, c2s, fov, aspect, clip_near, false};
+Camera make_camera(FArr c2w, FArr w2c, FArr s2c, FArr c2s, float fov, float aspect, float clip_near,
+                   bool fisheye = false) {
+    return Camera{c2w, w2c, s2c, c2s, fov, aspect, clip_near, fisheye};
 }
 
 }  // namespace
@@ -103,6 +106,35 @@ PYBIND11_MODULE(refprobe, m) {
         std::vector<float> out = vec(ray.org);
         auto d = vec(ray.dir);
         out.insert(out.end(), d.begin(), d.end());
+        return out;
+    });
+    m.def("sample_primary_fisheye", [](FArr c2w, FArr w2c, FArr s2c, FArr c2s, float fov, float aspect,
+                                       float clip_near, std::vector<float> screen) {
+        Camera cam = make_camera(c2w, w2c, s2c, c2s, fov, aspect, clip_near, true);
+        TCamera<float> tc(cam);
+        auto ray = sample_primary(tc, make_vector2(screen[0], screen[1]));
+        std::vector<float> out = vec(ray.org);
+        auto d = vec(ray.dir);
+        out.insert(out.end(), d.begin(), d.end());
+        return out;
+    });
+    // unproject, d_unproject (x, y), project_local, d_project_local, in_screen
+    m.def("camera_functions", [](FArr c2w, FArr w2c, FArr s2c, FArr c2s, float fov, float aspect, float clip_near,
+                                 bool fisheye, std::vector<float> screen, std::vector<float> local,
+                                 std::vector<float> dxy) {
+        Camera cam = make_camera(c2w, w2c, s2c, c2s, fov, aspect, clip_near, fisheye);
+        auto sp = make_vector2(screen[0], screen[1]);
+        auto u = unproject(cam, sp);
+        auto du = d_unproject(cam, sp);
+        auto p = project_local(cam, v3(local));
+        auto dl = d_project_local(cam, v3(local), dxy[0], dxy[1]);
+        std::vector<float> out = vec(u);
+        for (auto &x : vec(du.first)) out.push_back(x);
+        for (auto &x : vec(du.second)) out.push_back(x);
+        out.push_back(p[0]);
+        out.push_back(p[1]);
+        for (auto &x : vec(dl)) out.push_back(x);
+        out.push_back(in_screen(cam, sp) ? 1.f : 0.f);
         return out;
     });
     m.def("project", [](FArr c2w, FArr w2c, FArr s2c, FArr c2s, float fov, float aspect, float clip_near,

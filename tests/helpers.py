@@ -15,6 +15,10 @@ def small_workload(name, res=48, **kw):
         return workloads.heightfield(grid=kw.pop("grid", 24), resolution=(res, res), **kw)
     if name == "glossy_plane":
         return glossy_plane(res, **kw)
+    if name == "fisheye":
+        return workloads.single_triangle_fisheye(resolution=(res, res), **kw)
+    if name == "clipped":
+        return workloads.single_triangle_clipped(resolution=(res, res), **kw)
     raise KeyError(name)
 
 

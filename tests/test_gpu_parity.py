@@ -20,7 +20,11 @@ from tests.helpers import gpu_render, gpu_scene, rel_err, small_workload
 
 pytestmark = pytest.mark.gpu
 
-SCENES = ["single_triangle", "shadow", "heightfield", "glossy_plane"]
+# fisheye: the equi-angular camera; clipped: a triangle with a vertex behind
+# the camera (primary edges clipped at the near plane), the reference's
+# test_single_triangle_fisheye.py and test_single_triangle_clipped.py scenes
+SCENES = ["single_triangle", "shadow", "heightfield", "glossy_plane", "fisheye", "clipped"]
+NO_SECONDARY = ("single_triangle", "fisheye", "clipped")  # direct light, nothing to occlude it
 
 
 def _print_case(*a):
@@ -105,16 +109,50 @@ def _adjoint_case(oracle, w, term, spp, seed):
     return g, gref
 
 
+# The fisheye camera maps every screen point through atan2 / sin / cos
+# (ocml on the GPU, glibc in the oracle), so every camera ray -- not a rare
+# one -- differs in its last bits, and the secondary-edge estimator (heavy
+# tailed, with visibility decisions at light borders) turns that into a few
+# per cent per render (2-4 % measured at 48 x 48, 2 spp).  Its per-render bar
+# is therefore loose; test_fisheye_secondary_agrees_in_expectation checks that
+# the difference averages out (no bias), and the pinhole scenes keep the tight
+# per-render bar.
+LOOSE = {("fisheye", "secondary"): (0.1, 2e-2), ("fisheye", "secondary_exact"): (0.1, 2e-2)}
+
+
 @pytest.mark.parametrize("name", SCENES)
 @pytest.mark.parametrize("term", list(TERMS))
 def test_adjoint_matches_oracle(gpu, oracle, name, term):
     _print_case("adjoint", name, term)
     w = small_workload(name)
     g, gref = _adjoint_case(oracle, w, term, 2, 5)
-    assert np.abs(gref).sum() > 0 or term.startswith("secondary") and name == "single_triangle", \
+    assert np.abs(gref).sum() > 0 or term.startswith("secondary") and name in NO_SECONDARY, \
         "oracle gradient is zero"
     print("rel err (all)", rel_err(g, gref), "norm", float(np.linalg.norm(gref)))
-    helpers.assert_groups(w, g, gref, RTOL[term], FLOOR[term])
+    rtol, floor = LOOSE.get((name, term), (RTOL[term], FLOOR[term]))
+    helpers.assert_groups(w, g, gref, rtol, floor)
+
+
+def test_fisheye_secondary_agrees_in_expectation(gpu, oracle):
+    """The fisheye secondary-edge term over 24 seeds: the GPU's and the
+    oracle's means agree within their standard errors (|difference| / |se|
+    below 1; measured 0.03), while single renders differ by a few per cent."""
+    _print_case("fisheye secondary expectation")
+    w = small_workload("fisheye")
+    W, H = w["resolution"]
+    d = np.random.default_rng(3).normal(size=(H, W, 3)).astype(np.float32)
+    flags = _abi.DR_FLAGS_PARITY | TERMS["secondary"]
+    sc = gpu_scene(w)
+    osc = oracle.scene_from_workload(w)
+    G, O = [], []
+    for seed in range(100, 124):
+        G.append(gpu_render(sc, w, 2, 1, seed, flags=flags, d_image=d)[1].astype(np.float64))
+        O.append(osc.render(W, H, 2, 1, seed, flags=flags, d_image=d)[1].astype(np.float64))
+    G, O = np.array(G), np.array(O)
+    se = np.sqrt(G.var(0, ddof=1) / len(G) + O.var(0, ddof=1) / len(O))
+    z = np.linalg.norm(G.mean(0) - O.mean(0)) / np.linalg.norm(se)
+    print("|mean difference| / |se| = %.3f" % z, flush=True)
+    assert np.linalg.norm(O.mean(0)) > 0 and z < 1.0
 
 
 @pytest.mark.parametrize("term", list(TERMS))

@@ -17,7 +17,10 @@
   sees a lit floor slab partly hidden by a black triangle, whose gradient is
   the continuation term alone, against quadrature of the hidden floor area;
 - the node bound (edge_tree.cpp:345-480) against its closed forms: Lambert
-  only, and a node shrunk to a point, written without its half-vector frame.
+  only, and a node shrunk to a point, written without its half-vector frame;
+- the world_to_cam derivative of projected segments (camera.h:244-313) with
+  one end behind the near plane, pinhole and fisheye, against central
+  differences of the projection itself.
 
 Three reference behaviours were found by these pins and are reproduced in
 parity mode, each behind a quirk bit of the render flags: the glossy branch
@@ -25,7 +28,9 @@ counts the whole BSDF for either component (DR_QUIRK_EDGE_BSDF_SUM, twice the
 derivative), the side-hit Jacobian assumes rays from the world origin
 (DR_QUIRK_EDGE_HIT_JACOBIAN) and the cone pass's pmf bookkeeping
 under-weights edges (DR_QUIRK_CONE_PMF).  With the bits clear the estimator
-converges to the quadrature.
+converges to the quadrature.  The clip derivative pin found a fourth: the
+reference differentiates another near-plane clip than the one it draws
+(DR_QUIRK_CLIP_DERIVATIVE).
 """
 import ctypes
 
@@ -491,3 +496,73 @@ def helpers_light_scene(mat):
     cam = workloads.make_camera([0, 2, -3], [0, 0, 0], [0, 1, 0], 45.0, (8, 8))
     return dict(camera=cam, shapes=[tri, ls], materials=[mat, black], lights=[light], resolution=(8, 8),
                 max_bounces=1)
+
+
+# ---------------------------------------------------- near-plane clips ----
+@pytest.mark.parametrize("fisheye", [False, True])
+def test_clipped_projection_derivative(oracle, fisheye):
+    """d (screen endpoints) / d world_to_cam of a segment crossing the near
+    plane: the oracle's d_project without the clip quirk equals central
+    differences of project_edge (which clips at t = -(z - clip_near) / dz);
+    with the quirk (the reference's) it does not.  The pinhole
+    d_project_local drops project_local's 0.5, so its seeds are halved here,
+    as the renderer does outside parity mode; the fisheye one keeps it."""
+    lib = oracle.load()
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.or_probe_project.argtypes = [ctypes.c_void_p, fp, fp, fp]
+    lib.or_probe_project.restype = ctypes.c_int
+    lib.or_probe_d_project_clip.argtypes = [ctypes.c_void_p, fp, fp, fp, ctypes.c_int, fp]
+    lib.or_probe_d_project_clip.restype = None
+    base = workloads.make_camera([0.2, 0.1, -2.0], [0, 0, 0], [0, 1, 0], 50.0, (64, 64), clip_near=0.3,
+                                 fisheye=fisheye)
+
+    def scene_with(w2c):
+        cam = dr.Camera(base.cam_to_world, w2c.astype(np.float32), base.sample_to_cam, base.cam_to_sample,
+                        base.fov_factor, base.aspect_ratio, base.clip_near, fisheye)
+        w = workloads.single_triangle(resolution=(64, 64))
+        w["camera"] = cam
+        return oracle.scene_from_workload(w)
+
+    def arr(a):
+        a = np.ascontiguousarray(a, np.float32)
+        return a, a.ctypes.data_as(fp)
+
+    def project(osc, p0, p1):
+        out, po = arr(np.zeros(4))
+        k0, k1 = arr(p0), arr(p1)
+        assert lib.or_probe_project(osc.h, k0[1], k1[1], po) == 1
+        return out.astype(np.float64)
+
+    w2c = np.asarray(base.world_to_cam, np.float64)
+    osc0 = scene_with(w2c)
+    rng = np.random.default_rng(21 + fisheye)
+    checked = 0
+    for _ in range(12):
+        # p0 in front of the camera, p1 behind the near plane (local z < 0.3)
+        l0 = np.array([rng.uniform(-0.4, 0.4), rng.uniform(-0.4, 0.4), rng.uniform(1.0, 3.0)])
+        l1 = np.array([rng.uniform(-0.4, 0.4), rng.uniform(-0.4, 0.4), rng.uniform(-1.0, 0.1)])
+        c2w = np.linalg.inv(w2c)
+        p0 = (c2w @ np.r_[l0, 1])[:3]
+        p1 = (c2w @ np.r_[l1, 1])[:3]
+        dxy = rng.normal(size=4)
+        seed = dxy * (1.0 if fisheye else 0.5)
+        got = {}
+        for q in (0, 1):
+            out, po = arr(np.zeros(16))
+            k0, k1, kd = arr(p0), arr(p1), arr(seed)
+            lib.or_probe_d_project_clip(osc0.h, k0[1], k1[1], kd[1], q, po)
+            got[q] = out.astype(np.float64)
+        fd = np.zeros(16)
+        h = 1e-3
+        for k in range(16):
+            e = np.zeros(16)
+            e[k] = h
+            sp = project(scene_with(w2c + e.reshape(4, 4)), p0, p1)
+            sm = project(scene_with(w2c - e.reshape(4, 4)), p0, p1)
+            fd[k] = dxy @ (sp - sm) / (2 * h)
+        err = np.linalg.norm(got[0] - fd)
+        assert err <= 2e-2 * np.linalg.norm(fd) + 1e-4, (got[0], fd)
+        checked += 1
+        # the quirk differentiates another clip: visibly different here
+        assert np.linalg.norm(got[1] - fd) > 5 * max(err, 1e-3)
+    assert checked == 12

@@ -47,6 +47,7 @@ def olib(oracle):
         ("or_probe_sample_primary", [ctypes.c_void_p, fp, fp], None),
         ("or_probe_project", [ctypes.c_void_p, fp, fp, fp], ctypes.c_int),
         ("or_probe_d_project", [ctypes.c_void_p, fp, fp, fp, fp], None),
+        ("or_probe_camera", [ctypes.c_void_p, fp, fp, fp, fp], None),
         ("or_probe_cone_box", [fp, fp, fp, fp, ctypes.c_float], ctypes.c_int),
         ("or_probe_clip_line", [fp, fp, fp], ctypes.c_int),
         ("or_probe_jacobians", [fp, fp, fp, fp, fp], None),
@@ -254,6 +255,86 @@ def test_camera_functions_match_reference(ref, olib, camera_scene):
             z1 = (w2c @ np.r_[p1, 1])[2]
             n_clipped += (z0 < cam.clip_near) != (z1 < cam.clip_near)
     assert n_clipped > 10  # the clipped branches were exercised
+
+
+@pytest.fixture(scope="module")
+def fisheye_scene(oracle):
+    cam = workloads.make_camera([0.0, 0.0, -1.0], [0, 0, 0], [0, 1, 0], 45.0, (64, 64), fisheye=True)
+    w = workloads.single_triangle(resolution=(64, 64))
+    w["camera"] = cam
+    return cam, oracle.scene_from_workload(w)
+
+
+@pytest.mark.parametrize("fisheye", [False, True])
+def test_camera_screen_functions_match_reference(ref, olib, camera_scene, fisheye_scene, fisheye):
+    """camera.h:86-214, 314-320: unproject, d_unproject, project_local,
+    d_project_local and in_screen, pinhole and fisheye, bit for bit; and
+    fisheye sample_primary (51-71), including screen points outside the
+    disk (a ray without direction)."""
+    cam, osc = fisheye_scene if fisheye else camera_scene
+    args = _camera_args(cam)
+    rng = np.random.default_rng(16 + fisheye)
+    inside = 0
+    for _ in range(400):
+        scr = rng.uniform(-0.1, 1.1, size=2).astype(np.float32)
+        loc = rng.normal(size=3).astype(np.float32)
+        loc[2] = abs(loc[2]) + 0.1
+        dxy = rng.normal(size=2).astype(np.float32)
+        out, po = F(np.zeros(15, np.float32))
+        keep = [F(scr), F(loc), F(dxy)]
+        olib.or_probe_camera(osc.h, keep[0][1], keep[1][1], keep[2][1], po)
+        e = np.array(ref.camera_functions(*args, fisheye, scr.tolist(), loc.tolist(), dxy.tolist()), np.float32)
+        np.testing.assert_array_equal(out, e)
+        inside += int(e[14])
+        if fisheye:
+            o6, p6 = F(np.zeros(6, np.float32))
+            olib.or_probe_sample_primary(osc.h, keep[0][1], p6)
+            er = np.array(ref.sample_primary_fisheye(*args, scr.tolist()), np.float32)
+            np.testing.assert_array_equal(o6, er)
+    assert 50 < inside < 400
+
+
+def test_fisheye_maps_are_consistent(olib, fisheye_scene):
+    """What the mathematics of the equi-angular map fixes: project_local
+    inverts unproject inside the disk, the screen radius is linear in the
+    angle to the view axis (pi/2 at the rim), and d_unproject / d_project_local
+    match central differences of unproject / project_local."""
+    cam, osc = fisheye_scene
+    rng = np.random.default_rng(17)
+
+    def probe(scr, loc=(0.0, 0.0, 1.0), dxy=(0.0, 0.0)):
+        out, po = F(np.zeros(15, np.float64 if False else np.float32))
+        keep = [F(scr), F(loc), F(dxy)]
+        olib.or_probe_camera(osc.h, keep[0][1], keep[1][1], keep[2][1], po)
+        return out.astype(np.float64)
+    for _ in range(200):
+        r, a = np.sqrt(rng.uniform(0.01, 0.9)), rng.uniform(0, 2 * np.pi)
+        scr = np.array([0.5 + 0.5 * r * np.cos(a), 0.5 + 0.5 * r * np.sin(a)], np.float32)
+        o = probe(scr)
+        u = o[0:3]
+        theta = np.arccos(u[2] / np.linalg.norm(u))
+        assert abs(theta - r * np.pi / 2) < 2e-5
+        back = probe(scr, loc=u)[9:11]
+        np.testing.assert_allclose(back, scr, atol=2e-6)
+        h = 1e-3
+        for k, col in ((0, slice(3, 6)), (1, slice(6, 9))):
+            ep = scr.astype(np.float64).copy()
+            em = ep.copy()
+            ep[k] += h
+            em[k] -= h
+            fd = (probe(ep.astype(np.float32))[0:3] - probe(em.astype(np.float32))[0:3]) / (2 * h)
+            np.testing.assert_allclose(o[col], fd, rtol=2e-2, atol=2e-3)
+        # d_project_local: the adjoint of project_local at a local direction
+        loc = u * rng.uniform(0.5, 2.0)
+        dxy = rng.normal(size=2)
+        g = probe(scr, loc=loc, dxy=dxy)[11:14]
+        fd = np.zeros(3)
+        for k in range(3):
+            lp, lm = loc.copy(), loc.copy()
+            lp[k] += h
+            lm[k] -= h
+            fd[k] = dxy @ (probe(scr, loc=lp)[9:11] - probe(scr, loc=lm)[9:11]) / (2 * h)
+        np.testing.assert_allclose(g, fd, rtol=2e-2, atol=2e-3)
 
 
 def test_cone_box_matches_reference(ref, olib):
