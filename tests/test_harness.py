@@ -29,3 +29,24 @@ def test_optimisation_approaches_the_target(gpu, tmp_path, name, iters, shrink):
         from delta_ray_amd import image
         t = image.imread(out + "/target.exr")
         assert t.shape == (64, 64, 3) and t.sum() > 0
+
+
+def test_heightfield_gan_steps(gpu, tmp_path):
+    """Two WGAN-GP iterations (reference heightfield_gan.py): finite costs,
+    and the generator's gradient arrives through the renderer (its weights
+    move after the generator step)."""
+    import torch
+    from delta_ray_amd import heightfield_gan as hg
+    print("case: heightfield gan", flush=True)
+    torch.manual_seed(0)
+    G = hg.Generator().cuda()
+    render = hg.Renderer(torch.device("cuda"))
+    z = torch.randn(2, hg.NOISE, device="cuda")
+    imgs = render(G(z), [0.0, 1.0])
+    assert imgs.shape == (2, 1, hg.RES, hg.RES) and float(imgs.sum()) > 0
+    imgs.sum().backward()
+    g = G.out32.weight.grad
+    assert g is not None and float(g.abs().sum()) > 0
+    h = hg.train(iters=2, batch=2, out_dir=str(tmp_path), log=None)
+    assert all(np.isfinite(h["d_cost"])) and all(np.isfinite(h["g_cost"]))
+    assert (tmp_path / "generated_000001.png").exists()
