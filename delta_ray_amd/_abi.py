@@ -19,6 +19,8 @@ DR_ERROR_OUT_OF_MEMORY = 2
 DR_ERROR_DEVICE = 3
 DR_ERROR_UNSUPPORTED = 4
 
+DR_LTC_FIT_CONSISTENT = 1
+
 DR_EDGES_FILTERED = 0
 DR_EDGES_ALL_CANDIDATES = 1
 
@@ -197,6 +199,8 @@ ENTRY_POINTS = [
                                              ctypes.c_void_p]),
     ("dr_load_serialized", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(dr_mesh_data)]),
     ("dr_mesh_data_free", None, [ctypes.POINTER(dr_mesh_data)]),
+    ("dr_ltc_fit", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp,
+                                  ctypes.c_int, ctypes.c_int, _fp, _fp]),
     ("dr_last_error", ctypes.c_char_p, []),
 ]
 

@@ -3,7 +3,8 @@
     python -m delta_ray_amd.build_lib        # or delta_ray_amd.build_lib.build()
 
 Compiles csrc/render.hip, csrc/mlp.hip and csrc/refit.hip with hipcc,
-csrc/scene_build.cpp and csrc/load_serialized.cpp with the host C++ compiler
+csrc/scene_build.cpp, csrc/load_serialized.cpp and csrc/ltc_fit.cpp with the
+host C++ compiler
 and csrc/ltc_blob.S (the embedded LTC table), then links the shared library
 next to this file (with zlib, for the .serialized reader).  Everything is compiled with -ffp-contract=off so
 that ray/triangle arithmetic rounds exactly like the CPU oracle's.
@@ -53,6 +54,7 @@ def build(verbose=False):
         "refit": os.path.join(BUILD, "refit.o"),
         "scene": os.path.join(BUILD, "scene_build.o"),
         "serialized": os.path.join(BUILD, "load_serialized.o"),
+        "ltc_fit": os.path.join(BUILD, "ltc_fit.o"),
         "ltc": os.path.join(BUILD, "ltc_blob.o"),
     }
     steps = [
@@ -61,16 +63,17 @@ def build(verbose=False):
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "refit.hip"), "-o", objs["refit"]],
         [cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene_build.cpp"), "-o", objs["scene"]],
         [cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "load_serialized.cpp"), "-o", objs["serialized"]],
+        [cxx] + CXX_FLAGS + ["-pthread", "-c", os.path.join(CSRC, "ltc_fit.cpp"), "-o", objs["ltc_fit"]],
         ["gcc", "-c", "-DLTC_FILE=\"%s\"" % LTC, os.path.join(CSRC, "ltc_blob.S"), "-o", objs["ltc"]],
     ]
-    with ThreadPoolExecutor(max_workers=6) as ex:
+    with ThreadPoolExecutor(max_workers=7) as ex:
         outs = list(ex.map(_run, steps))
     if verbose:
         for o in outs:
             sys.stdout.write(o)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp,
-          objs["render"], objs["mlp"], objs["refit"], objs["scene"], objs["serialized"], objs["ltc"], "-lz"])
+          objs["render"], objs["mlp"], objs["refit"], objs["scene"], objs["serialized"], objs["ltc_fit"], objs["ltc"], "-lz", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -532,6 +532,24 @@ typedef struct dr_mesh_data {
 dr_status dr_load_serialized(const char *filename, int shape_index, dr_mesh_data *out);
 void dr_mesh_data_free(dr_mesh_data *mesh);
 
+/* ------------------------------------------------------------ LTC table
+ * Regenerates the table of linearly transformed cosines the secondary-edge
+ * sampler reads (the reference's fit/fitLTC.cpp:191-276 with the Blinn-Phong
+ * BRDF of fit/brdf_blinn_phong.h:10-66; the table the renderer embeds is the
+ * reference's ltc.inc, fitted that way).  tab: n x n entries of 9 floats
+ * (a 3x3 matrix row by row, as ltc.inc), entry (a, t) at index a + t * n for roughness index a
+ * (r = a / (n - 1)) and view angle index t; amplitude: n x n floats.  Fits
+ * rows [row_begin, row_end) at view angles [t_begin, t_end); a fit that does
+ * not start at (n - 1, 0) takes its first guesses from `guess` (a full
+ * table: the entry before each one), else guess may be NULL.  flags:
+ * DR_LTC_FIT_CONSISTENT fits the lobe the renderer evaluates at roughness r
+ * (Phong exponent 2 / r) instead of the reference's alpha = r^2 (exponent
+ * 2 / r^4).  nthreads <= 0: all host threads.  Host only; about a minute
+ * for the full 128 x 128 table on 16 threads. */
+#define DR_LTC_FIT_CONSISTENT 1
+dr_status dr_ltc_fit(int n, int row_begin, int row_end, int t_begin, int t_end, const float *guess, int flags,
+                     int nthreads, float *tab, float *amplitude);
+
 const char *dr_last_error(void);
 
 #ifdef __cplusplus
