@@ -10,7 +10,7 @@ buffers resident on the GPU before the timed region.
     python bench.py [--steps K] [--warmup W]
                     [--workload heightfield|shadow|single_triangle|batched|learned|torch_step]
     python bench.py --table OUT.json        # the three single-scene configs, fwd and fwd+adjoint, GPU and CPU
-    python bench.py --fill-baseline OUT.json  # writes that table into BASELINE.md (no GPU)
+    python bench.py --fill-baseline OUT.json RESULTS.md  # writes that table into RESULTS.md (no GPU)
 
 `--workload batched` times the uneven batch instead (64 scenes of 1k-200k
 triangles x 4 views, forward + adjoint each, one graph-captured batch per
@@ -43,7 +43,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "fwd+adjoint samples/s; time to gradient rel-RMSE 1e-2, learned vs plain"
 
-# algorithmic bytes (measurement section)
+# algorithmic bytes: the records each unit of work must read or write once
 B_RAY_IO = 32 + 16      # ray in (org, dir, tmin, tmax) + hit out (ids, u, v)
 B_NODE = 64             # one BVH2 node (two child boxes)
 B_TRI = 48              # one triangle (v0, e1, e2)
@@ -71,18 +71,18 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain leg")
     ap.add_argument("--table", metavar="OUT_JSON", help="time the three single-scene configs (fwd and "
                     "fwd+adjoint, GPU and CPU) and write them to OUT_JSON")
-    ap.add_argument("--fill-baseline", metavar="TABLE_JSON",
-                    help="write the results table of BASELINE.md from a --table run (no GPU)")
+    ap.add_argument("--fill-baseline", nargs=2, metavar=("TABLE_JSON", "MARKDOWN"),
+                    help="rewrite the results table of MARKDOWN from a --table run (no GPU)")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched workload")
     ap.add_argument("--no-autotune", action="store_true", help="batched workload: default pass sizes")
     ap.add_argument("--lv-res", type=int, default=128, help="learned workload: image size")
     ap.add_argument("--lv-spp", type=int, default=16, help="learned workload: samples per pixel")
     ap.add_argument("--lv-renders", type=int, default=512, help="learned workload: renders per estimator")
-    ap.add_argument("--lv-train-renders", type=int, default=16, help="learned: statistics renders per round")
+    ap.add_argument("--lv-train-renders", type=int, default=8, help="learned: statistics renders per round")
     ap.add_argument("--lv-train-steps", type=int, default=300, help="learned: MLP training steps per round")
     ap.add_argument("--lv-rounds", type=int, default=2, help="learned: rounds of statistics + training")
     ap.add_argument("--lv-eps", type=float, default=1e-2, help="learned: target relative RMSE")
-    ap.add_argument("--lv-bench-renders", type=int, default=48,
+    ap.add_argument("--lv-bench-renders", type=int, default=64,
                     help="renders per estimator of the learned-vs-plain leg of the default run")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known HBM traffic: "
@@ -130,6 +130,21 @@ def host_cpus():
 def make_workload(name):
     from delta_ray_amd import workloads
     return workloads.WORKLOADS[name]()
+
+
+def param_index(w):
+    """Indices, in the flat gradient buffer, of the parameters the workload
+    optimises (w["params"]: ("shape", i, "vertices" | "normals") or
+    ("material", i, "diffuse_reflectance" | "specular_reflectance" |
+    "roughness")); the rest of the scene (lights' quads, camera) is fixed."""
+    import numpy as np
+    from delta_ray_amd import delta_ray as dr
+    lay = dr.grad_layout(w["shapes"], w["materials"], w["lights"])
+    idx = []
+    for kind, i, name in w["params"]:
+        o, shape = (lay["shapes"] if kind == "shape" else lay["materials"])[i][name]
+        idx.append(np.arange(o, o + int(np.prod(shape))))
+    return np.concatenate(idx).astype(np.int64)
 
 
 def d_image_for(w, scene, dev):
@@ -269,26 +284,38 @@ def bench_batched(args):
 
 
 def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary", "full"), log="",
-                ablation=True):
+                ablation=True, flags=0):
     """Time to a gradient relative RMSE of `eps`, learned edge proposal vs
     the plain one, on workload `w`.  Both estimators are unbiased, so for
     each one the per-render variance V of the gradient (sum over parameters)
     and the time per render t give the time to reach relative RMSE eps as
-    t * V / (eps^2 |g|^2)  (n renders average to variance V / n); |g| is the
-    norm of the mean of all renders of both.  The secondary-edge estimator
-    is heavy-tailed (a rare sample through a tiny tree pmf carries a huge
-    weight), so V is estimated as the median over 8 or 16 groups of renders
-    of the groups' sample variances (median of means: what a run of typical
-    length sees); the plain sample variance over all renders, dominated by
-    the few largest renders, and tail percentiles of the per-render squared
-    error are reported beside it.  The learned time includes its training:
-    `rounds` rounds of DR_EDGE_STATS renders (statistics of the plain
-    estimator, then of the learned mixture), MLP training and applying the
-    weights.  Measured on the secondary-edge term, the one the proposal
-    changes, and (terms) on the full gradient (interior + secondary +
-    primary edges).  `ablation`: on the secondary-edge term, the same time
-    with weights from the per-edge statistics alone (no network), trained by
-    the same rounds."""
+    t * V / (eps^2 |g|^2)  (n renders average to variance V / n).
+
+    The gradient is the one an optimisation of the workload uses: its
+    parameters' entries (w["params"]; on the heightfield its vertices and
+    roughness texture, not the fixed light quads, whose vertex adjoints would
+    otherwise dominate |g|), and the proposal's statistics measure
+    contributions to those shapes.  Renders use `flags` (default 0: the exact
+    derivative).  With the reference's quirks (DR_FLAGS_PARITY) the
+    heightfield's vertex gradient is dominated by DR_QUIRK_EDGE_HIT_JACOBIAN,
+    whose factor (x . n) / ((x - p) . n) is unbounded where a side ray hits
+    the surface near the height of its origin: the estimate's variance is
+    then not finite and no relative RMSE is reached at the 1/sqrt(n) rate.
+
+    The estimator is still heavy-tailed (a rare sample through a tiny tree
+    pmf carries a large weight), so V is the median over 16 groups of
+    renders of the groups' sample variances, each around its own group's
+    mean (median of means: what a run of typical length sees), with a
+    bootstrap interval over the groups; |g| is the norm of the per-parameter
+    median of all groups' means.  The plain sample variance over all renders
+    and tail percentiles of the per-render squared error are reported
+    beside it.  The learned time includes its training: `rounds` rounds of
+    DR_EDGE_STATS renders (statistics of the plain estimator, then of the
+    learned mixture), MLP training and applying the weights.  Measured on
+    the secondary-edge term, the one the proposal changes, and (terms) on
+    the full gradient (interior + secondary + primary edges).  `ablation`:
+    on the secondary-edge term, the same with weights from the per-edge
+    statistics alone (no network), trained by the same rounds."""
     import numpy as np
     import torch
     from delta_ray_amd import _abi
@@ -302,34 +329,47 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
     img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
     g = torch.empty((scene.grad_size,), dtype=torch.float32, device=dev)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sel = torch.from_numpy(param_index(w)).to(dev)
 
-    def renders(flags, seeds):
-        out, ms = [], 0.0
-        for sd in seeds:
-            o = dr.DeviceScene.options(W, H, spp, nb, sd, _abi.DR_FLAGS_PARITY | flags)
+    def renders(fl, seeds):
+        out = torch.empty((len(seeds), len(sel)), dtype=torch.float64, device=dev)
+        ms = 0.0
+        for k, sd in enumerate(seeds):
+            o = dr.DeviceScene.options(W, H, spp, nb, sd, flags | fl)
             e0.record()
             scene.render_adjoint(o, d_img, img, g)
             e1.record()
             torch.cuda.synchronize()
             ms += e0.elapsed_time(e1)
-            out.append(g.double().cpu().numpy().copy())
-        return np.array(out), ms / len(seeds)
+            out[k] = g[sel].double()
+        return out, ms / len(seeds)
 
-    groups = 16 if n >= 128 else 8
+    groups = 16 if n >= 64 else 8
+    n = n - n % groups
+    rng = np.random.default_rng(0)
 
-    def var(a):
-        return float(((a - a.mean(0)) ** 2).sum(1).sum() / (len(a) - 1))
+    def group_stats(a):
+        k = a.shape[0] // groups
+        v = [float(((a[i * k:(i + 1) * k] - a[i * k:(i + 1) * k].mean(0)) ** 2).sum() / (k - 1))
+             for i in range(groups)]
+        means = torch.stack([a[i * k:(i + 1) * k].mean(0) for i in range(groups)])
+        return np.array(v), means
 
-    def mom_var(a):
-        m = len(a) // groups
-        return float(np.median([var(a[i * m:(i + 1) * m]) for i in range(groups)]))
+    def full_var(a):
+        return float(((a - a.mean(0)) ** 2).sum() / (a.shape[0] - 1))
 
     def tail(a):
-        e = ((a - a.mean(0)) ** 2).sum(1)
+        e = ((a - a.mean(0)) ** 2).sum(1).cpu().numpy()
         return [float("%.4g" % x) for x in np.percentile(e, [50, 90, 99, 100])]
+
+    def ratio_ci(va, vb):
+        r = [np.median(va[rng.integers(0, groups, groups)]) / np.median(vb[rng.integers(0, groups, groups)])
+             for _ in range(2000)]
+        return [round(float(x), 3) for x in np.percentile(r, [10, 90])]
 
     secondary = _abi.DR_SKIP_INTERIOR | _abi.DR_SKIP_PRIMARY_EDGES
     renders(secondary, [1])  # warm-up
+    param_shapes = sorted({i for kind, i, _ in w["params"] if kind == "shape"})
 
     def train(use_network):
         """rounds of statistics + training + applying the weights, timed as
@@ -337,12 +377,12 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
         rounds of the learned mixture itself"""
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        prop = EdgeProposal(scene, seed=7, use_network=use_network)
+        prop = EdgeProposal(scene, seed=7, use_network=use_network, stat_shapes=param_shapes)
         loss, wts = 0.0, None
         for rnd in range(rounds):
             prop.reset_stats()
-            flags = secondary | _abi.DR_EDGE_STATS | (_abi.DR_LEARNED_EDGES if rnd > 0 else 0)
-            renders(flags, range(50 + 100 * rnd, 50 + 100 * rnd + train_renders))
+            fl = secondary | _abi.DR_EDGE_STATS | (_abi.DR_LEARNED_EDGES if rnd > 0 else 0)
+            renders(fl, range(50 + 100 * rnd, 50 + 100 * rnd + train_renders))
             prop.collect()
             loss = prop.train(steps=train_steps, batch=1024)
             wts = prop.apply()
@@ -353,44 +393,51 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
     print("learned%s: trained in %.2f s" % (log, train_s), file=sys.stderr, flush=True)
     out = {}
     for term in terms:
-        flags = secondary if term == "secondary" else 0
+        fl = secondary if term == "secondary" else 0
         prop.set_weights(wts)
-        gp, tp = renders(flags, range(1000, 1000 + n))
-        gl, tl = renders(flags | _abi.DR_LEARNED_EDGES, range(100000, 100000 + n))
+        gp, tp = renders(fl, range(1000, 1000 + n))
+        gl, tl = renders(fl | _abi.DR_LEARNED_EDGES, range(100000, 100000 + n))
         print("learned%s: %s term measured" % (log, term), file=sys.stderr, flush=True)
-        mean = np.concatenate([gp, gl]).mean(0)
-        vp, vl = mom_var(gp), mom_var(gl)
-        g2 = max(float((mean ** 2).sum()) - 0.25 * (vp + vl) / n, 1e-30)
+        vpg, mp = group_stats(gp)
+        vlg, ml = group_stats(gl)
+        vp, vl = float(np.median(vpg)), float(np.median(vlg))
+        g2 = max(float((torch.cat([mp, ml]).median(0).values ** 2).sum()), 1e-30)
         t_plain = tp / 1e3 * vp / (eps ** 2 * g2)
         t_learned = tl / 1e3 * vl / (eps ** 2 * g2) + train_s
-        se = np.sqrt(gp.var(0, ddof=1) / n + gl.var(0, ddof=1) / n)
-        diff = np.abs(gp.mean(0) - gl.mean(0))
+        se = torch.sqrt(gp.var(0) / n + gl.var(0) / n)
+        diff = (gp.mean(0) - gl.mean(0)).abs()
         act = se > 0
         out[term] = {"time_to_eps_s_plain": round(t_plain, 3), "time_to_eps_s_learned": round(t_learned, 3),
                      "plain_over_learned_time": round(t_plain / t_learned, 4),
                      "variance_ratio_plain_over_learned": round(vp / vl, 4),
+                     "variance_ratio_10_90": ratio_ci(vpg, vlg),
                      "ms_per_render_plain": round(tp, 3), "ms_per_render_learned": round(tl, 3),
                      "rel_rmse_per_render_plain": round(float(np.sqrt(vp / g2)), 5),
                      "rel_rmse_per_render_learned": round(float(np.sqrt(vl / g2)), 5),
-                     "sample_variance_plain": float("%.4g" % var(gp)),
-                     "sample_variance_learned": float("%.4g" % var(gl)),
+                     "sample_variance_plain": float("%.4g" % full_var(gp)),
+                     "sample_variance_learned": float("%.4g" % full_var(gl)),
                      "sq_error_p50_p90_p99_max_plain": tail(gp), "sq_error_p50_p90_p99_max_learned": tail(gl),
-                     "mean_difference_over_se": round(float(np.linalg.norm(diff) / np.linalg.norm(se)), 3),
-                     "params_beyond_4se": round(float(np.mean(diff[act] / se[act] > 4.0)), 5)}
+                     "mean_difference_over_se": round(float(diff.norm() / se.norm()), 3),
+                     "params_beyond_4se": round(float((diff[act] / se[act] > 4.0).double().mean()), 5)}
         if term == "secondary" and ablation:
             # the per-edge statistics alone, no network: trained the same way
             _, _, _, ta_s = train(False)
-            ga, ta = renders(flags | _abi.DR_LEARNED_EDGES, range(200000, 200000 + n))
-            va = mom_var(ga)
+            ga, ta = renders(fl | _abi.DR_LEARNED_EDGES, range(200000, 200000 + n))
+            vag, _ = group_stats(ga)
+            va = float(np.median(vag))
             out[term]["ablation_stats_only"] = {
                 "time_to_eps_s": round(ta / 1e3 * va / (eps ** 2 * g2) + ta_s, 3),
                 "training_s": round(ta_s, 3),
                 "variance_ratio_plain_over_stats_only": round(vp / va, 4),
                 "variance_ratio_stats_only_over_learned": round(va / vl, 4),
+                "variance_ratio_stats_only_over_learned_10_90": ratio_ci(vag, vlg),
+                "sample_variance_stats_only": float("%.4g" % full_var(ga)),
                 "ms_per_render": round(ta, 3)}
             print("learned%s: ablation measured" % log, file=sys.stderr, flush=True)
     return {"workload": "%s %dx%d %dspp %d-bounce, %d renders per estimator" % (w["name"], W, H, spp, nb, n),
-            "eps": eps, "training_s": round(train_s, 3),
+            "eps": eps, "flags": "exact derivative" if flags == 0 else "0x%x" % flags,
+            "gradient": "parameters " + ", ".join("%s %d %s" % p for p in w["params"]),
+            "training_s": round(train_s, 3),
             "training": {"rounds": rounds, "stats_renders_per_round": train_renders,
                          "mlp_steps_per_round": train_steps, "final_loss": round(float(loss), 5),
                          "weight_p5_p50_p95": [round(float(x), 4) for x in np.percentile(wts, [5, 50, 95])]},
@@ -666,9 +713,9 @@ def bench_table(args):
     print(json.dumps(out), flush=True)
 
 
-def fill_baseline(path, base=None):
-    """Rewrite the results table of BASELINE.md (or of `base`) from a
-    --table JSON."""
+def fill_baseline(path, base):
+    """Rewrite the results table of the markdown file `base` (the table
+    after its "### Results table" heading) from a --table JSON."""
     with open(path) as f:
         t = json.load(f)
     cores = "%d (%s; nproc %s)" % (t["cpu_threads"], t["host"].get("model") or "?", t["host"].get("nproc"))
@@ -700,7 +747,6 @@ def fill_baseline(path, base=None):
                      "%.4g s vs %.4g s (%s) | — | %.2f× (target ≥3×) | — |" % (
                          s["time_to_eps_s_plain"], s["time_to_eps_s_learned"], lv["workload"],
                          s["plain_over_learned_time"]))
-    base = base or os.path.join(ROOT, "BASELINE.md")
     with open(base) as fh:
         text = fh.read()
     head = "### Results table (filled by the build's measurement harness, never by hand)"
@@ -726,7 +772,7 @@ def fill_baseline(path, base=None):
 def main():
     args = parse()
     if args.fill_baseline:
-        fill_baseline(args.fill_baseline)
+        fill_baseline(*args.fill_baseline)
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: multi-GPU runs are not part of this build "

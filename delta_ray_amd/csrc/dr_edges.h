@@ -422,7 +422,10 @@ DR_INLINE bool et_init(EdgeTraversal &T, Stack &stack, const DrSceneView &S, F3 
     if (dot(wi, sp.fr.n) < 0.f) return false;
     bool has_n = S.num_nrm_nodes > 0, has_s = S.num_sil_nodes > 0;
     if (!has_n && !has_s) return false;
-    float pmf = 1.f;
+    // pmf: the tree's probability under the point's proposal; pmf_o: under
+    // the other one (learned mixture: the proposals weigh the trees by their
+    // plain or learned root lengths)
+    float pmf = 1.f, pmf_o = 1.f;
     if (!has_n) {
         T.nodes = S.sil_nodes; T.edges = S.sil_edges; T.num_nodes = S.num_sil_nodes; T.directional = false;
     } else if (!has_s) {
@@ -430,13 +433,29 @@ DR_INLINE bool et_init(EdgeTraversal &T, Stack &stack, const DrSceneView &S, F3 
     } else {
         float ni = S.nrm_nodes[0].total_length, si = S.sil_nodes[0].total_length;
         float npmf = ni / (ni + si), spmf = si / (ni + si);
+        float npmf_o = npmf, spmf_o = spmf;
+        if (mode != 0) {
+            float nl = S.nrm_nodes[0].pad, sl = S.sil_nodes[0].pad;
+            float npl = npmf, spl = spmf;
+            if (nl + sl > 0.f) {
+                npl = nl / (nl + sl);
+                spl = sl / (nl + sl);
+            }
+            if (mode == 1) {
+                npmf_o = npmf; spmf_o = spmf; npmf = npl; spmf = spl;
+            } else {
+                npmf_o = npl; spmf_o = spl;
+            }
+        }
         float u = philox_next(T.rc);
         if (u < npmf) {
             T.nodes = S.nrm_nodes; T.edges = S.nrm_edges; T.num_nodes = S.num_nrm_nodes; T.directional = true;
             pmf = npmf;
+            pmf_o = npmf_o;
         } else {
             T.nodes = S.sil_nodes; T.edges = S.sil_edges; T.num_nodes = S.num_sil_nodes; T.directional = false;
             pmf = spmf;
+            pmf_o = spmf_o;
         }
     }
     bound_ctx(T.B, wi, sp, roughness, dlum, slum);
@@ -450,7 +469,7 @@ DR_INLINE bool et_init(EdgeTraversal &T, Stack &stack, const DrSceneView &S, F3 
     T.pops = 0;
     T.mode = mode;
     stack.clear();
-    stack.push(tree_entry(0, 3, root, root_bound, pmf, pmf, sp.pos), TreeAlt{pmf, pmf}, mode != 0);
+    stack.push(tree_entry(0, 3, root, root_bound, pmf, pmf, sp.pos), TreeAlt{pmf_o, pmf_o}, mode != 0);
     return true;
 }
 

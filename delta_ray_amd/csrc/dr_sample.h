@@ -418,7 +418,14 @@ DR_INLINE F3 stage_edge(const DrSceneView &S, const SampleCtx &c, GradSink &G, u
         G.add3(es.vgrad_off + 3ll * edge.v0, er.dv0);
         G.add3(es.vgrad_off + 3ll * edge.v1, er.dv1);
     }
-    if (mag) *mag = length(er.dv0) + length(er.dv1);
+    if (mag) {
+        // the sample's contribution to the gradients the statistics measure:
+        // the edge's vertex adjoints and the Dirac derivative at the shading
+        // point (which k_dp hands to that point's vertices)
+        float se = S.stat_shape ? S.stat_shape[edge.shape] : 1.f;
+        float spt = S.stat_shape ? S.stat_shape[p.shape] : 1.f;
+        *mag = se * (length(er.dv0) + length(er.dv1)) + spt * length(er.dp);
+    }
     return er.dp;
 }
 

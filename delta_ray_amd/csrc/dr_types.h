@@ -80,7 +80,7 @@ struct DrEdgeNode {
     int offset;   // leaf: first edge in the tree's edge list; interior: second child
     int nprims;   // 0: interior
     float total_length;
-    float pad;
+    float pad;    // the learned proposal's length: sum of weight x edge length (total_length without weights)
 };
 
 struct DrCamera {
@@ -131,6 +131,9 @@ struct DrSceneView {
     // NULL when the trees hold only the edges the filter kept at the build
     const DR_GLOBAL unsigned char *edge_on;
     float *edge_stat;  // per tree edge (silhouette tree, then normal tree): sum of |contribution|, count, sum of squares
+    // per shape: 1 when the statistics count gradients of its vertices, 0
+    // when not; NULL: every shape counts
+    const DR_GLOBAL float *stat_shape;
     const DR_GLOBAL DrEdge *prim_edges;
     const DR_GLOBAL float *prim_func;
     const DR_GLOBAL float *prim_cdf;

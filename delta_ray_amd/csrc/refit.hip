@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(RF_TREE_BLOCK) k_refit_etree(DrEdgeNode *nodes
 // A wave per node: axis = normalised normal sum, cos_angle = the least
 // cosine to it over the node's edges [rng[2i], rng[2i+1]) (the minimum does
 // not depend on the order; a NaN cosine never replaces it, as std::min);
-// learned length = summed weighted lengths scaled to the root's length.
+// learned length = summed weighted lengths.
 __global__ void __launch_bounds__(RF_BLOCK) k_etree_cone(DrEdgeNode *nodes, int n, const float *on, const int *rng,
                                                          const float *nsum, const float *wsum, bool learned) {
     const int lane = (int)__lane_id();
@@ -264,15 +264,9 @@ __global__ void __launch_bounds__(RF_BLOCK) k_etree_cone(DrEdgeNode *nodes, int 
     if (lane == 0) {
         N.axis[0] = axis.x; N.axis[1] = axis.y; N.axis[2] = axis.z;
         N.cos_angle = cmin;
-        if (!learned) {
-            N.pad = N.total_length;
-        } else {
-            float root_w = wsum[0];
-            float p = wsum[i];
-            // the host scales every node by the root's length / learned sum
-            // when that sum is positive
-            N.pad = root_w > 0.f ? p * (nodes[0].total_length / root_w) : p;
-        }
+        // unscaled, so that the two trees' learned totals also decide
+        // between the trees (unit weights give the plain lengths exactly)
+        N.pad = learned ? wsum[i] : N.total_length;
     }
 }
 

@@ -786,6 +786,7 @@ struct dr_scene {
     double build_ms = 0.0;
     bool host_stale = false;        // device vertices newer than hs.verts / hs.normals
     void *small = nullptr;          // device, 4 words: edge-filter counts, or primary count and integral
+    float *stat_shape = nullptr;    // device, one float per shape (dr_scene_set_stat_shapes)
     int refits = 0, rebuilds = 0;
     int edges_off = 0;              // tree edges the filter drops on the current vertices (all-candidate trees)
     std::vector<dr_view *> views;   // views of this scene (their primary edges follow its vertices)
@@ -1980,6 +1981,26 @@ dr_status dr_scene_edge_features(dr_scene *s, const dr_view *view, float *featur
     if (hipMemcpyAsync(features, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, hs) != hipSuccess ||
         hipStreamSynchronize(hs) != hipSuccess)
         return fail(DR_ERROR_DEVICE, "feature upload failed");
+    return DR_OK;
+}
+
+dr_status dr_scene_set_stat_shapes(dr_scene *s, const int *shape_ids, int n) {
+    if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
+    if (n < 0 || (n > 0 && !shape_ids)) return fail(DR_ERROR_INVALID_ARGUMENT, "bad shape list");
+    int ns = s->view.num_shapes;
+    for (int i = 0; i < n; i++)
+        if (shape_ids[i] < 0 || shape_ids[i] >= ns) return fail(DR_ERROR_INVALID_ARGUMENT, "shape index out of range");
+    if (n == 0) {
+        s->view.stat_shape = nullptr;  // (the array stays with the scene)
+        return DR_OK;
+    }
+    if (!s->stat_shape && !alloc_to(s->allocs, std::max(ns, 1) * sizeof(float), (void **)&s->stat_shape))
+        return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+    std::vector<float> m((size_t)std::max(ns, 1), 0.f);
+    for (int i = 0; i < n; i++) m[shape_ids[i]] = 1.f;
+    if (hipMemcpy(s->stat_shape, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(DR_ERROR_DEVICE, "mask upload failed");
+    s->view.stat_shape = (decltype(s->view.stat_shape))s->stat_shape;
     return DR_OK;
 }
 

@@ -849,27 +849,6 @@ void build_geo_records(HostScene &hs) {
     }
 }
 
-void set_learned_lengths(const HostScene &hs, std::vector<DrEdgeNode> &nodes, const std::vector<DrEdge> &ordered,
-                         const float *w) {
-    for (int i = (int)nodes.size() - 1; i >= 0; i--) {
-        DrEdgeNode &n = nodes[i];
-        if (n.nprims > 0) {
-            float s = 0.f;
-            for (int k = 0; k < n.nprims; k++) {
-                const DrEdge &ed = ordered[n.offset + k];
-                s += std::max(w[n.offset + k], 0.f) * length(getv(hs, ed.shape, ed.v1) - getv(hs, ed.shape, ed.v0));
-            }
-            n.pad = s;
-        } else {
-            n.pad = nodes[i + 1].pad + nodes[n.offset].pad;
-        }
-    }
-    if (!nodes.empty() && nodes[0].pad > 0.f) {
-        float scale = nodes[0].total_length / nodes[0].pad;
-        for (DrEdgeNode &n : nodes) n.pad *= scale;
-    }
-}
-
 void edge_features(const HostScene &hs, const DrCamera &cam, std::vector<float> &out) {
     const int F = 24;
     size_t n = hs.sil_edges.size() + hs.nrm_edges.size();

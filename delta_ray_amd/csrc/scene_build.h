@@ -69,11 +69,6 @@ void set_camera(HostScene &hs, const dr_camera &c);
 // length of camera silhouettes) for the scene's camera / for a given camera
 void build_primary_edges(HostScene &hs);
 void primary_edges_for(const HostScene &hs, const DrCamera &cam, std::vector<DrEdge> &edges, Dist &dist);
-// Per-node sampling lengths of the learned proposal (DrEdgeNode::pad): the
-// sum over the node's edges of weight x length, scaled so that the root's
-// equals its plain length.  w: one weight per edge of `ordered`.
-void set_learned_lengths(const HostScene &hs, std::vector<DrEdgeNode> &nodes, const std::vector<DrEdge> &ordered,
-                         const float *w);
 // DR_MLP_FEATURES floats per tree edge (silhouette tree, then normal tree)
 void edge_features(const HostScene &hs, const DrCamera &cam, std::vector<float> &out);
 

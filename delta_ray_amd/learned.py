@@ -8,14 +8,16 @@ length (dr_scene_set_edge_weights), and DR_LEARNED_EDGES renders mix that
 proposal with the plain one per shading point (probabilities 0.8 / 0.2): the
 two tree passes of the chosen proposal are weighted with the reference's
 power heuristic, and the proposals with the balance heuristic (a defensive
-mixture), so the estimator stays unbiased whatever the network predicts, the
+mixture; a proposal also picks between the silhouette and the normal tree by
+its own root lengths), so the estimator stays unbiased whatever the network predicts, the
 plain proposal remains part of every estimate, and uninformative weights
 give the plain estimator back.
 
 Training data come from the plain estimator itself: DR_EDGE_STATS renders
 accumulate, per edge, the number of secondary-edge samples and the sum and
-sum of squares of their |vertex-gradient contribution| (already divided by the
-selection probability).  For a proposal whose per-edge scale factors w_e
+sum of squares of their |gradient contribution| (already divided by the
+selection probability): the edge's vertex adjoints and the Dirac derivative
+at the shading point, counted for the shapes being optimised.  For a proposal whose per-edge scale factors w_e
 multiply the plain one, the second moment of the estimate is (to first
 order, when the points' totals vary little) smallest for w_e proportional to
 the root mean square of the edge's weighted contributions (sqrt(sum f^2/p / sum p), over the points that select it); the
@@ -39,9 +41,11 @@ def _ptr(t):
 class EdgeProposal:
     """use_network=False: the ablation without the network -- the applied
     weights come from each edge's own statistics alone (edges without samples
-    keep the plain proposal's weight)."""
+    keep the plain proposal's weight).  stat_shapes: indices of the shapes
+    whose gradients the proposal should serve (the optimised ones; None: all),
+    which decides what the statistics measure (dr_scene_set_stat_shapes)."""
 
-    def __init__(self, scene, view=None, seed=0, use_network=True):
+    def __init__(self, scene, view=None, seed=0, use_network=True, stat_shapes=None):
         import torch
         self.torch = torch
         self.lib = scene.lib
@@ -52,6 +56,9 @@ class EdgeProposal:
         vh = view.handle if view is not None else None
         _abi.check(self.lib, self.lib.dr_scene_edge_features(scene.handle, vh, _ptr(self.features), None),
                    "dr_scene_edge_features")
+        ids = np.ascontiguousarray(stat_shapes if stat_shapes is not None else [], np.int32)
+        _abi.check(self.lib, self.lib.dr_scene_set_stat_shapes(scene.handle, ids.ctypes.data if len(ids) else None,
+                                                               len(ids)), "dr_scene_set_stat_shapes")
         self.stats = torch.zeros((max(self.n, 1), 3), dtype=torch.float32, device=dev)
         self.acc = torch.zeros((max(self.n, 1), 3), dtype=torch.float64, device=dev)
         h = ctypes.c_void_p()
@@ -113,7 +120,7 @@ class EdgeProposal:
         self.trained_steps += steps
         return loss.value
 
-    def apply(self, clip=(0.05, 20.0), prior_count=16.0, temper=0.5):
+    def apply(self, clip=(1e-3, 1e3), prior_count=16.0, temper=0.5):
         """Network + statistics -> the scene's learned node lengths.  Per edge
         the step is (n * y + k * prediction) / (n + k): its own target y from
         n samples, shrunk toward the network's prediction with k =
@@ -124,7 +131,11 @@ class EdgeProposal:
         rounds of statistics gathered under the learned proposal itself
         (DR_LEARNED_EDGES | DR_EDGE_STATS renders) refine the weights: each
         round's RMS measures what the current proposal still under- or
-        over-samples.  Weights are clipped to `clip`."""
+        over-samples.  Weights are clipped to `clip`, a wide range: the
+        second moment of the secondary-edge estimate concentrates on few
+        edges (on the glossy heightfield, the light quads' edges and those
+        near them: 1 % of the edges hold 98 % of it), whose best weights are
+        hundreds of times the typical one."""
         if self.use_network:
             _abi.check(self.lib, self.lib.dr_mlp_infer(self.mlp, _ptr(self.features), self.n, _ptr(self.logits),
                                                        None), "dr_mlp_infer")

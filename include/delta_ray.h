@@ -494,10 +494,17 @@ dr_status dr_scene_edge_features(dr_scene *scene, const dr_view *view, float *fe
 dr_status dr_scene_set_edge_weights(dr_scene *scene, const float *weights);
 /* Statistics accumulated by DR_EDGE_STATS renders, three floats per edge
  * (device buffer `out`, may be NULL): over the edge's secondary-edge
- * samples, the sum of |vertex-gradient contribution| (weights included),
+ * samples, the sum of |gradient contribution| (weights included; what counts
+ * is set by dr_scene_set_stat_shapes),
  * the number of samples and the sum of squared contributions.  reset: clear
  * them afterwards. */
 dr_status dr_scene_edge_stats(dr_scene *scene, float *out, int reset, void *stream);
+/* The shapes whose gradients the statistics above measure (the parameters
+ * an optimisation changes): a sample's contribution is |dv0| + |dv1| of its
+ * edge when the edge's shape is listed, plus |dp| (the Dirac derivative at
+ * the shading point) when the shading point's shape is.  n = 0: every shape
+ * (the default).  No reference counterpart. */
+dr_status dr_scene_set_stat_shapes(dr_scene *scene, const int *shape_ids, int n);
 
 /* The reference's test_line_integral (delta_ray.cpp:1572-1836) on the GPU
  * with the device's LTC line sampler: four estimates of one line integral

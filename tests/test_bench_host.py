@@ -27,7 +27,7 @@ def test_cpu_leg_counts_samples_and_rays():
 
 
 def test_fill_baseline_writes_the_table(tmp_path):
-    base = tmp_path / "BASELINE.md"
+    base = tmp_path / "results.md"
     base.write_text("# x\n\n### Results table (filled by the build's measurement harness, never by hand)\n\n"
                     "| config | mode | a |\n|---|---|---|\n| C1 | fwd | TBD |\n\n## 3. next\n")
     roof = {"kernel": "k_select", "frac": 0.1, "traffic": 3e9, "achieved": 800.0}
