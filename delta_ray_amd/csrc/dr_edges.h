@@ -271,10 +271,13 @@ DR_INLINE float node_importance_len(const DrEdgeNode &n, float bound, F3 p, floa
 // depth first.  Here both passes walk together: a stack entry carries a pass
 // mask, a node both passes reach (all of them where the point lies inside the
 // boxes, which split in both passes) is expanded once, and each pass still
-// meets its nodes in its own depth-first order.  The cone pass draws from the
-// point's stream, the "all" pass from a stream of its own; selected edges get
-// ordinals counted per pass (bit 30 set for the "all" pass), which sort like
-// the reference's sequential order.  The traversal is explicit state advanced
+// meets its nodes in its own depth-first order.  A node's decisions are keyed
+// draws of the point's tree stream (block = node index; output 0 for the
+// cone pass, 1 for the "all" pass), so they do not depend on the order of the
+// walk; a selected edge's ordinal is its index in the tree with bit 30 set
+// for the "all" pass: the tree is stored in depth-first order (left child
+// first, leaves' edges in leaf order), so ordinals sort like the reference's
+// sequential order of selection.  The traversal is explicit state advanced
 // one node per et_step() so that a persistent kernel can give a lane a new
 // point as soon as its traversal ends; et_step() calls
 // `emit(weight, in_normal_tree, index, ordinal)` per selected edge.
@@ -391,8 +394,8 @@ struct EdgeTraversal {
     bool directional;
     BoundCtx B;
     Cone cone;
-    Philox rc, ra;  // draws of the cone pass / the "all" pass
-    int jc, ja;     // ordinals of the next edge of each pass
+    Philox rc;  // the point's stream (the tree choice)
+    Philox rt;  // the point's tree stream: keyed draws, block n = node n
     // proposal: 0 the reference's (node length = edge lengths); with the
     // learned mixture, 1 = this point uses the learned node lengths, 2 = the
     // plain ones (probabilities 0.8 / 0.2)
@@ -464,8 +467,6 @@ DR_INLINE bool et_init(EdgeTraversal &T, Stack &stack, const DrSceneView &S, F3 
     float root_bound = bsdf_bound(root, sp, T.B);
     T.n_inner = 0;
     T.n_edges = 0;
-    T.jc = 0;
-    T.ja = 0;
     T.pops = 0;
     T.mode = mode;
     stack.clear();
@@ -513,8 +514,10 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         for (int i = 0; i < nprims; i++) {
             if (S.edge_on && !S.edge_on[(T.directional ? S.num_sil_edges : 0) + e.off + i]) continue;
             if (T.directional && !is_silhouette_nrm(S, sp.pos, e.off + i)) continue;
-            if (mask & 1) emit(scale * cone_pmf / den, T.directional, e.off + i, T.jc++);
-            if (mask & 2) emit(scale * all_pmf / den, T.directional, e.off + i, DR_ORD_ALL_PASS | T.ja++);
+            // ordinal: the edge's index in its tree (increasing in the
+            // depth-first order of each pass), bit 30 for the "all" pass
+            if (mask & 1) emit(scale * cone_pmf / den, T.directional, e.off + i, e.off + i);
+            if (mask & 2) emit(scale * all_pmf / den, T.directional, e.off + i, DR_ORD_ALL_PASS | (e.off + i));
         }
         return true;
     }
@@ -588,11 +591,14 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         }
     }
     int lm = 0, rm = 0;
+    // this node's decisions: keyed draws of the point's tree stream
+    float uc = 0.f, ua = 0.f;
+    if (((mask & 1) && !cone_split) || ((mask & 2) && !all_split)) philox_keyed2(T.rt, (uint32_t)ni, uc, ua);
     if ((mask & 1) && ilc + irc > 1e-10f) {
         if (cone_split) {
             if (rcs) rm |= 1;
             if (lcs) lm |= 1;
-        } else if (philox_next(T.rc) < plc) {
+        } else if (uc < plc) {
             lm |= 1;
         } else {
             rm |= 1;
@@ -602,7 +608,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
         if (all_split) {
             if (rcs) rm |= 2;
             if (lcs) lm |= 2;
-        } else if (philox_next(T.ra) < pla) {
+        } else if (ua < pla) {
             lm |= 2;
         } else {
             rm |= 2;

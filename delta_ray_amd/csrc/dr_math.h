@@ -287,6 +287,15 @@ DR_INLINE void philox_refill(Philox &r) {
     r.c3++;
     r.idx = 0;
 }
+// the first two outputs of counter block `block` of r's stream, as uniforms
+// (independent of the draws made from r so far; r is not advanced)
+DR_INLINE void philox_keyed2(const Philox &r, uint32_t block, float &u0, float &u1) {
+    Philox t = r;
+    t.c3 = block;
+    philox_refill(t);
+    u0 = (float)(t.b0 >> 8) * (1.0f / 16777216.0f);
+    u1 = (float)(t.b1 >> 8) * (1.0f / 16777216.0f);
+}
 DR_INLINE float philox_next(Philox &r) {
     if (r.idx >= 4) philox_refill(r);
     uint32_t v = r.b0;
@@ -298,13 +307,15 @@ DR_INLINE float philox_next(Philox &r) {
 }
 #define DR_STREAM_MAIN 0u
 DR_INLINE uint32_t dr_stream_point(int b) { return 1u + (uint32_t)b; }
-// the edge-tree "all" pass at the point of bounce b
-DR_INLINE uint32_t dr_stream_point_all(int b) { return 0x100u + (uint32_t)b; }
+// the edge-tree decisions at the point of bounce b, keyed by node: block n
+// holds node n's draws (output 0: the cone pass, 1: the "all" pass), so a
+// node's decisions do not depend on the order in which the tree is walked
+DR_INLINE uint32_t dr_stream_point_tree(int b) { return 0x100u + (uint32_t)b; }
 // the choice between the plain and the learned edge proposal at that point
 DR_INLINE uint32_t dr_stream_point_mix(int b) { return 0x200u + (uint32_t)b; }
-// the selected edge of ordinal j (bit 30: the "all" pass; 27 bits of count)
-// at bounce b: bit 31 marks an edge stream, bits 28-30 the bounce, bit 27
-// the pass, bits 0-26 the count
+// the selected edge of ordinal j at bounce b (j: the edge's index in its
+// tree, below 2^27; bit 30: the "all" pass): bit 31 marks an edge stream,
+// bits 28-30 the bounce, bit 27 the pass, bits 0-26 the index
 DR_INLINE uint32_t dr_stream_edge(int b, int j) {
     return 0x80000000u | ((uint32_t)b << 28) | ((((uint32_t)j >> 30) & 1u) << 27) | ((uint32_t)j & 0x7ffffffu);
 }

@@ -380,7 +380,7 @@ DR_INLINE bool select_begin(const DrSceneView &S, const SampleCtx &c, const Poin
     philox_init(T.rc, c.seed, 0, p.sid, dr_stream_point(p.b));
     philox_next(T.rc);  // bsdf component of the point's edge sample
     philox_next(T.rc);  // position on the edge
-    philox_init(T.ra, c.seed, 0, p.sid, dr_stream_point_all(p.b));
+    philox_init(T.rt, c.seed, 0, p.sid, dr_stream_point_tree(p.b));
     int mode = 0;
     if (c.flags & DR_LEARNED_EDGES) {
         Philox mix;

@@ -63,23 +63,35 @@ struct Philox {
         return buf[idx++];
     }
     float next() { return (float)(next_u32() >> 8) * (1.0f / 16777216.0f); }
+    // output k (0-3) of counter block `block`, independent of the draws made
+    // so far (keyed draws: block = an edge-tree node)
+    float keyed(uint32_t block, int k) const {
+        Philox t = *this;
+        t.c3 = block;
+        t.refill();
+        return (float)(t.buf[k] >> 8) * (1.0f / 16777216.0f);
+    }
 };
 
 // Sub-streams of one pixel sample.
 //   STREAM_MAIN: jitter (2), light sample (4), per bounce: light sample (4),
 //                bsdf sample (3), russian roulette (1)
 //   stream_point(b): secondary shading point at bounce b: edge sample (2),
-//                edge-tree choice and the cone pass's traversal draws
-//   stream_point_all(b): the "all" pass's traversal draws at that point
-//   stream_edge(b, j): j-th selected edge at bounce b: light sample (4),
-//                per remaining bounce (8)
+//                edge-tree choice (1)
+//   stream_point_tree(b): the edge-tree decisions at that point, keyed by
+//                node: block n holds node n's draws, output 0 for the cone
+//                pass, output 1 for the "all" pass (so a node's decisions do
+//                not depend on the order in which the tree is walked)
+//   stream_edge(b, j): the selected edge of ordinal j at bounce b (j: the
+//                edge's index in its tree, bit 30 for the "all" pass): light
+//                sample (4), per remaining bounce (8)
 // A primary-edge sample uses STREAM_MAIN of pass 1: edge choice (1), edge
 // sample (2), light sample (4), per bounce (8).
 static const uint32_t STREAM_MAIN = 0;
 inline uint32_t stream_point(int b) { return 1u + (uint32_t)b; }
-inline uint32_t stream_point_all(int b) { return 0x100u + (uint32_t)b; }
+inline uint32_t stream_point_tree(int b) { return 0x100u + (uint32_t)b; }
 // bit 31: an edge stream; bits 28-30: bounce; bit 27: the "all" pass (bit 30
-// of the ordinal); bits 0-26: the ordinal's count
+// of the ordinal); bits 0-26: the edge's index in its tree
 inline uint32_t stream_edge(int b, int j) {
     return 0x80000000u | ((uint32_t)b << 28) | ((((uint32_t)j >> 30) & 1u) << 27) | ((uint32_t)j & 0x7ffffffu);
 }

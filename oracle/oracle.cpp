@@ -900,8 +900,7 @@ struct TraverseCtx {
     float dlum, slum;
     Cone cone;
     std::vector<EdgeW> *out;
-    Philox *rng;
-    int jc = 0, ja = 0;  // ordinals of the cone and the "all" pass
+    const Philox *rng;   // the point's tree stream: block n = node n's draws
     uint32_t flags = 0;  // render flags (DR_QUIRK_CONE_PMF)
 };
 
@@ -915,8 +914,11 @@ void sample_edge_rec(TraverseCtx &c, bool cone_mode, int node_id, float nb, floa
                             : all_pmf / (cone_pmf * cone_pmf + all_pmf * all_pmf);
         for (int i = 0; i < node.nprims; i++) {
             const Edge &e = t.edges[node.offset + i];
+            // ordinal: the edge's index in the tree (increasing in the
+            // depth-first order of the walk), bit 30 for the "all" pass
+            int idx = node.offset + i;
             if (is_silhouette(*c.sc, c.sp.pos, e))
-                c.out->push_back(EdgeW{e, w, cone_mode ? c.jc++ : (0x40000000 | c.ja++)});
+                c.out->push_back(EdgeW{e, w, cone_mode ? idx : (0x40000000 | idx)});
         }
         return;
     }
@@ -962,28 +964,29 @@ void sample_edge_rec(TraverseCtx &c, bool cone_mode, int node_id, float nb, floa
         if (lcs) sample_edge_rec(c, cone_mode, li, lb, plc * cone_pmf, pla * all_pmf);
         if (rcs) sample_edge_rec(c, cone_mode, ri, rb, prc * cone_pmf, pra * all_pmf);
     } else {
-        float u = c.rng->next();
+        float u = c.rng->keyed((uint32_t)node_id, cone_mode ? 0 : 1);
         bool left = cone_mode ? u < plc : u < pla;
         if (left) sample_edge_rec(c, cone_mode, li, lb, plc * cone_pmf, pla * all_pmf);
         else sample_edge_rec(c, cone_mode, ri, rb, prc * cone_pmf, pra * all_pmf);
     }
 }
 
-// The cone pass draws from `rng` (the point's stream), the "all" pass from
-// `rng_all` (a stream of its own), so that the two passes are independent
-// and may be walked together.
+// The decisions of both passes are keyed draws of the point's tree stream
+// (`rng_tree`: node n, output 0 for the cone pass and 1 for the "all" pass),
+// so that the passes are independent and the tree may be walked in any
+// order.
 void sample_tree(const Scene &sc, const EdgeTree &tree, bool directional, const V3f &wi, const Isect &is,
                  const SP<float> &sp, const Cone &cone, float pmf, std::vector<EdgeW> &out, Philox &rng,
-                 Philox &rng_all, uint32_t flags) {
+                 Philox &rng_tree, uint32_t flags) {
     if (tree.nodes.empty()) return;
     int mid = sc.shapes[is.shape].mat;
     const Material &m = sc.mats[mid];
     float dlum = luminance(get_diffuse<float>(m, sp.uv, nullptr));
     float slum = luminance(get_specular<float>(m, sp.uv, nullptr));
-    TraverseCtx c{&sc, &tree, directional, wi, mid, sp, dlum, slum, cone, &out, &rng, 0, 0, flags};
+    (void)rng;
+    TraverseCtx c{&sc, &tree, directional, wi, mid, sp, dlum, slum, cone, &out, &rng_tree, flags};
     float b = bsdf_bound(sc, tree.nodes[0], wi, mid, sp, dlum, slum);
     sample_edge_rec(c, true, 0, b, pmf, pmf);
-    c.rng = &rng_all;
     sample_edge_rec(c, false, 0, b, pmf, pmf);
 }
 
@@ -1428,7 +1431,7 @@ V3f render_pixel_sample(const RenderCtx &rc, int xi, int yi, int s, Accum &acc, 
             float cos_a = tan_a <= 1.f ? 1.f / std::sqrt(tan_a * tan_a + 1.f) : 0.f;
             Cone cone{p.spf.pos, normalize(dl), cos_a};
             edges.clear();
-            Philox prng_all(o.seed, 0, sid, stream_point_all(p.bounce));
+            Philox prng_all(o.seed, 0, sid, stream_point_tree(p.bounce));
             sample_shading_edges(sc, p.wi, p.is, p.spf, cone, edges, prng, prng_all, o.flags);
             last_edge_count = (int)edges.size();
             for (int k = 0; k < (int)edges.size(); k++) {
