@@ -476,7 +476,13 @@ template <int MAXB>
 __global__ void __launch_bounds__(DR_BLOCK) k_primary(const DrJob *__restrict__ jobs, const DrRange *__restrict__ R,
                                                       int nr) {
     __shared__ float s_w2c[16];
+    // vertex adjoints go through the workgroup's LDS cache: on a small scene
+    // every sample hits the same few vertices, and direct global atomics on
+    // them serialise (7.5 ms for 262 k samples of a single triangle)
+    __shared__ unsigned long long gkey[DR_GC_SLOTS_SMALL];
+    __shared__ float gval[DR_GC_SLOTS_SMALL];
     if (threadIdx.x < 16) s_w2c[threadIdx.x] = 0.f;
+    gc_init(gkey, gval, DR_GC_SLOTS_SMALL);
     __syncthreads();
     const DrRange rg = R[range_of_block(R, nr, blockIdx.x)];
     const DrJob &J = jobs[rg.job];
@@ -500,14 +506,18 @@ __global__ void __launch_bounds__(DR_BLOCK) k_primary(const DrJob *__restrict__ 
             PrimaryResult Rs;
             sample_primary_edge_point(S, e, eu, ls, ss, c.nb, J.d_image, c.W, c.H, w / c.spp, c.flags, Rs);
             if (Rs.ok) {
-                float *grads = J.grads;
+                GradSink G;
+                G.grads = J.grads;
+                G.cam = nullptr;
+                G.flags = c.flags;
+                G.color_mode = true;
+                G.gkey = gkey;
+                G.gval = gval;
+                G.gmask = DR_GC_SLOTS_SMALL - 1;
+                G.job = (unsigned)rg.job;
                 const DrShape &es = S.shapes[e.shape];
-                if (Rs.dv0.x != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0, Rs.dv0.x);
-                if (Rs.dv0.y != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0 + 1, Rs.dv0.y);
-                if (Rs.dv0.z != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v0 + 2, Rs.dv0.z);
-                if (Rs.dv1.x != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1, Rs.dv1.x);
-                if (Rs.dv1.y != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1 + 1, Rs.dv1.y);
-                if (Rs.dv1.z != 0.f) atomicAdd(grads + es.vgrad_off + 3ll * e.v1 + 2, Rs.dv1.z);
+                G.add3(es.vgrad_off + 3ll * e.v0, Rs.dv0);
+                G.add3(es.vgrad_off + 3ll * e.v1, Rs.dv1);
                 for (int i = 0; i < 16; i++)
                     if (Rs.dw2c[i] != 0.f) atomicAdd(&s_w2c[i], Rs.dw2c[i]);
             }
@@ -516,6 +526,7 @@ __global__ void __launch_bounds__(DR_BLOCK) k_primary(const DrJob *__restrict__ 
     }
     __syncthreads();
     if (threadIdx.x < 16 && s_w2c[threadIdx.x] != 0.f) atomicAdd(J.grads + 16 + threadIdx.x, s_w2c[threadIdx.x]);
+    gc_flush(gkey, gval, DR_GC_SLOTS_SMALL, [&](int j) { return jobs[j].grads; });
 }
 
 __global__ void k_intersect(DrSceneView S, const float *rays, long long n, int *hits, int any) {
