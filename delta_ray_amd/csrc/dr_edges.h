@@ -476,51 +476,52 @@ DR_INLINE bool et_init(EdgeTraversal &T, Stack &stack, const DrSceneView &S, F3 
 
 #define DR_LEARNED_SHARE 0.8f
 
-// One node.  Returns false when the traversal is done.
-template <class Stack, class Emit>
-DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, const SPoint &sp, Emit &emit) {
-    // every node is pushed at most once; the cap guards a malformed tree
-    if (stack.size() == 0 || T.pops >= T.num_nodes + 8) {
-        if (S.stats) {
-            atomicAdd(&S.stats[5], (unsigned long long)T.n_inner);
-            atomicAdd(&S.stats[6], (unsigned long long)T.n_edges);
-        }
-        return false;
-    }
-    T.pops++;
-    TreeAlt ea{0.f, 0.f};
-    TreeEntry e = stack.pop(ea, T.mode != 0);
-    if (e.empty()) return true;
-    const int ni = e.node(), mask = e.mask();
-    const int nprims = e.nprims();
+// Edge i of a leaf: emitted for each pass that reached the leaf when it is
+// a silhouette at the point.
+template <class Emit>
+DR_INLINE void et_leaf_edge(const EdgeTraversal &T, const TreeEntry &e, TreeAlt ea, const DrSceneView &S,
+                            const SPoint &sp, int i, Emit &emit) {
+    const int mask = e.mask();
     const float cone_pmf = e.cone_pmf(), all_pmf = e.all_pmf();
-    if (nprims > 0) {
-        // power heuristic over the two passes, which both run
-        // (edge_tree.cpp:528-536): pass t selected the edge with pmf p_t and
-        // weighs it p_t / (p_cone^2 + p_all^2).  With the learned mixture the
-        // point ran one proposal s of two (probabilities pi_s, pi_o); the
-        // passes' weights are scaled by q_s / (pi_s q_s + pi_o q_o), q = p_cone
-        // + p_all of each proposal (balance heuristic over the proposals: the
-        // defensive mixture, unbiased for any q, and exactly the plain
-        // estimator when both proposals agree).
-        float den = cone_pmf * cone_pmf + all_pmf * all_pmf;
-        float scale = 1.f;
-        if (T.mode != 0) {
-            float pi = T.mode == 1 ? DR_LEARNED_SHARE : 1.f - DR_LEARNED_SHARE;
-            float qs = cone_pmf + all_pmf, qo = ea.cone + ea.all;
-            scale = qs / (pi * qs + (1.f - pi) * qo);
-        }
-        T.n_edges += nprims;
-        for (int i = 0; i < nprims; i++) {
-            if (S.edge_on && !S.edge_on[(T.directional ? S.num_sil_edges : 0) + e.off + i]) continue;
-            if (T.directional && !is_silhouette_nrm(S, sp.pos, e.off + i)) continue;
-            // ordinal: the edge's index in its tree (increasing in the
-            // depth-first order of each pass), bit 30 for the "all" pass
-            if (mask & 1) emit(scale * cone_pmf / den, T.directional, e.off + i, e.off + i);
-            if (mask & 2) emit(scale * all_pmf / den, T.directional, e.off + i, DR_ORD_ALL_PASS | (e.off + i));
-        }
-        return true;
+    // power heuristic over the two passes, which both run
+    // (edge_tree.cpp:528-536): pass t selected the edge with pmf p_t and
+    // weighs it p_t / (p_cone^2 + p_all^2).  With the learned mixture the
+    // point ran one proposal s of two (probabilities pi_s, pi_o); the
+    // passes' weights are scaled by q_s / (pi_s q_s + pi_o q_o), q = p_cone
+    // + p_all of each proposal (balance heuristic over the proposals: the
+    // defensive mixture, unbiased for any q, and exactly the plain
+    // estimator when both proposals agree).
+    float den = cone_pmf * cone_pmf + all_pmf * all_pmf;
+    float scale = 1.f;
+    if (T.mode != 0) {
+        float pi = T.mode == 1 ? DR_LEARNED_SHARE : 1.f - DR_LEARNED_SHARE;
+        float qs = cone_pmf + all_pmf, qo = ea.cone + ea.all;
+        scale = qs / (pi * qs + (1.f - pi) * qo);
     }
+    if (S.edge_on && !S.edge_on[(T.directional ? S.num_sil_edges : 0) + e.off + i]) return;
+    if (T.directional && !is_silhouette_nrm(S, sp.pos, e.off + i)) return;
+    // ordinal: the edge's index in its tree (increasing in the depth-first
+    // order of each pass), bit 30 for the "all" pass
+    if (mask & 1) emit(scale * cone_pmf / den, T.directional, e.off + i, e.off + i);
+    if (mask & 2) emit(scale * all_pmf / den, T.directional, e.off + i, DR_ORD_ALL_PASS | (e.off + i));
+}
+
+// A leaf's edges in order.
+template <class Emit>
+DR_INLINE void et_leaf(EdgeTraversal &T, const TreeEntry &e, TreeAlt ea, const DrSceneView &S, const SPoint &sp,
+                       Emit &emit) {
+    const int nprims = e.nprims();
+    T.n_edges += nprims;
+    for (int i = 0; i < nprims; i++) et_leaf_edge(T, e, ea, S, sp, i, emit);
+}
+
+// An interior node: the children the passes go on to, at most two, in the
+// order a depth-first walk pushes them (right, then left).  Returns how
+// many (c0 first, then c1).
+DR_INLINE int et_expand(EdgeTraversal &T, const TreeEntry &e, TreeAlt ea, const DrSceneView &S, const SPoint &sp,
+                        TreeEntry &c0, TreeAlt &a0, TreeEntry &c1, TreeAlt &a1) {
+    const int ni = e.node(), mask = e.mask();
+    const float cone_pmf = e.cone_pmf(), all_pmf = e.all_pmf();
     int li = ni + 1, ri = e.off;
     const DrEdgeNode L = gptr(T.nodes)[li], R = gptr(T.nodes)[ri];
     T.n_inner++;
@@ -546,7 +547,7 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
             il2 = ll; ir2 = rl;
         }
     }
-    if (il + ir <= 1e-10f) return true;
+    if (il + ir <= 1e-10f) return 0;
     // the cone tests only matter while the cone pass can still reach the
     // node's subtree (its own mask bit or a nonzero pmf of either proposal);
     // otherwise every quantity they feed is multiplied by zero
@@ -614,18 +615,59 @@ DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, con
             rm |= 2;
         }
     }
+    int n = 0;
+    if (rm) {
+        c0 = tree_entry(ri, rm, R, rb, prc * cone_pmf, pra * all_pmf, sp.pos);
+        a0 = TreeAlt{prc2 * ea.cone, pra2 * ea.all};
+        n = 1;
+    }
+    if (lm) {
+        TreeEntry x = tree_entry(li, lm, L, lb, plc * cone_pmf, pla * all_pmf, sp.pos);
+        TreeAlt xa = TreeAlt{plc2 * ea.cone, pla2 * ea.all};
+        if (n == 0) {
+            c0 = x;
+            a0 = xa;
+        } else {
+            c1 = x;
+            a1 = xa;
+        }
+        n++;
+    }
+    return n;
+}
+
+// One node.  Returns false when the traversal is done.
+template <class Stack, class Emit>
+DR_INLINE bool et_step(EdgeTraversal &T, Stack &stack, const DrSceneView &S, const SPoint &sp, Emit &emit) {
+    // every node is pushed at most once; the cap guards a malformed tree
+    if (stack.size() == 0 || T.pops >= T.num_nodes + 8) {
+        if (S.stats) {
+            atomicAdd(&S.stats[5], (unsigned long long)T.n_inner);
+            atomicAdd(&S.stats[6], (unsigned long long)T.n_edges);
+        }
+        return false;
+    }
+    T.pops++;
+    TreeAlt ea{0.f, 0.f};
+    TreeEntry e = stack.pop(ea, T.mode != 0);
+    if (e.empty()) return true;
+    if (e.nprims() > 0) {
+        et_leaf(T, e, ea, S, sp, emit);
+        return true;
+    }
+    TreeEntry c0, c1;
+    TreeAlt a0{0.f, 0.f}, a1{0.f, 0.f};
+    int n = et_expand(T, e, ea, S, sp, c0, a0, c1, a1);
     // the builder bounds the tree depth so that the stack cannot fill (a
     // full stack would be counted in the statistics, not silently dropped)
-    if (rm && stack.size() < DR_TREE_STACK)
-        stack.push(tree_entry(ri, rm, R, rb, prc * cone_pmf, pra * all_pmf, sp.pos),
-                   TreeAlt{prc2 * ea.cone, pra2 * ea.all}, T.mode != 0);
-    else if (rm && S.stats)
-        atomicAdd(&S.stats[7], 1ull);
-    if (lm && stack.size() < DR_TREE_STACK)
-        stack.push(tree_entry(li, lm, L, lb, plc * cone_pmf, pla * all_pmf, sp.pos),
-                   TreeAlt{plc2 * ea.cone, pla2 * ea.all}, T.mode != 0);
-    else if (lm && S.stats)
-        atomicAdd(&S.stats[7], 1ull);
+    if (n > 0) {
+        if (stack.size() < DR_TREE_STACK) stack.push(c0, a0, T.mode != 0);
+        else if (S.stats) atomicAdd(&S.stats[7], 1ull);
+    }
+    if (n > 1) {
+        if (stack.size() < DR_TREE_STACK) stack.push(c1, a1, T.mode != 0);
+        else if (S.stats) atomicAdd(&S.stats[7], 1ull);
+    }
     return true;
 }
 

@@ -272,6 +272,270 @@ __global__ void __launch_bounds__(DR_BLOCK) k_select(const DrJob *__restrict__ j
     }
 }
 
+// B, breadth first: the same traversal (et_expand / et_leaf, the same keyed
+// draws and ordinals, so the same selected edges with the same weights) run
+// one tree level per launch over every point of the pass.  A lane takes one
+// frontier entry (a node some point's passes reached) and expands it, so all
+// lanes of a wave run the same code on full waves, where the per-point walk
+// leaves most lanes of a wave idle while the longest walk of its 64 points
+// finishes.  k_sel_start writes each point's traversal constants (SelPoint)
+// and its root entry; k_sel_level expands level d into level d + 1 (interior
+// children) and the leaf list; k_sel_leaves tests and emits the leaves'
+// edges.  Entries are appended with one atomic per wave; an append beyond a
+// buffer's capacity is dropped and the largest end is kept in cnt[4] (levels)
+// or cnt[5] (leaves), so the host grows the buffer and selects again.
+struct alignas(16) SelPoint {
+    F3 pos, n;
+    BoundCtx B;
+    F3 cdir;
+    float ccos;
+    uint32_t sid_lo, sid_hi;
+    int info;  // bit 0: normal tree; bits 1-2: proposal mode; bit 3: DR_QUIRK_CONE_PMF; bits 8-15: bounce
+    int job;
+};
+static_assert(sizeof(SelPoint) == 128, "SelPoint is 8 x 16 bytes");
+struct alignas(16) SelEntry {
+    int slot, code;
+    uint32_t cpmf, apmf;
+    int off;
+    float altc, alta;
+    int pad;
+};
+static_assert(sizeof(SelEntry) == 32, "SelEntry is 2 x 16 bytes");
+#define DR_SEL_LEAVES 64  // index of the leaf count among the level counters
+
+DR_INLINE SelEntry sel_entry(int slot, const TreeEntry &e, TreeAlt a) {
+    return SelEntry{slot, e.code, e.cpmf, e.apmf, e.off, a.cone, a.all, 0};
+}
+DR_INLINE TreeEntry sel_tree(const SelEntry &x) { return TreeEntry{x.code, x.cpmf, x.apmf, x.off}; }
+
+// the point's traversal state rebuilt from its SelPoint (what et_expand and
+// et_leaf read: tree, bound constants, light cone, proposal, keyed draws;
+// of the surface point only its position and normal)
+DR_INLINE void sel_ctx(const SelPoint &P, const DrJob &J, EdgeTraversal &T, SPoint &sp) {
+    const DrSceneView &S = J.S;
+    bool nrm = (P.info & 1) != 0;
+    T.nodes = nrm ? S.nrm_nodes : S.sil_nodes;
+    T.edges = nrm ? S.nrm_edges : S.sil_edges;
+    T.num_nodes = nrm ? S.num_nrm_nodes : S.num_sil_nodes;
+    T.directional = nrm;
+    T.B = P.B;
+    T.cone = Cone{P.pos, P.cdir, P.ccos};
+    T.mode = (P.info >> 1) & 3;
+    T.cone_pmf_quirk = (P.info & 8) != 0;
+    T.pops = 0;
+    T.n_inner = 0;
+    T.n_edges = 0;
+    uint64_t sid = (uint64_t)P.sid_lo | ((uint64_t)P.sid_hi << 32);
+    philox_init(T.rt, J.c.seed, 0, sid, dr_stream_point_tree((P.info >> 8) & 0xff));
+    sp.pos = P.pos;
+    sp.ng = P.n;
+    sp.fr = Frame{F3{0.f, 0.f, 0.f}, F3{0.f, 0.f, 0.f}, P.n};
+    sp.uv = F2{0.f, 0.f};
+}
+
+// Appends k (0-2) items per lane with one atomic per wave; every lane of the
+// wave must call it.  Returns the lane's first index; *end_max keeps the
+// largest end of an append.
+DR_INLINE long long wave_append(unsigned long long *counter, int k, unsigned long long *end_max) {
+    const int lane = (int)__lane_id();
+    unsigned long long m1 = __ballot(k >= 1), m2 = __ballot(k >= 2);
+    unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int before = __popcll(m1 & lt) + __popcll(m2 & lt);
+    int total = __popcll(m1) + __popcll(m2);
+    if (total == 0) return 0;
+    int leader = __ffsll((long long)m1) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) {
+        base = atomicAdd(counter, (unsigned long long)total);
+        atomicMax(end_max, base + (unsigned long long)total);
+    }
+    base = __shfl(base, leader);
+    return (long long)base + before;
+}
+
+// a tree root to the frontier or to the leaf list
+struct RootStack {
+    TreeEntry e;
+    TreeAlt a;
+    int sp;
+    DR_INLINE void clear() { sp = 0; }
+    DR_INLINE int size() const { return sp; }
+    DR_INLINE void push(const TreeEntry &x, TreeAlt al, bool) {
+        e = x;
+        a = al;
+        sp = 1;
+    }
+};
+
+__global__ void __launch_bounds__(DR_BLOCK) k_sel_start(const DrJob *__restrict__ jobs, long long nslots, int maxb,
+                                                        const PointRec *pts, SelPoint *spt, SelEntry *front,
+                                                        SelEntry *leaves, unsigned long long *lvl, long long fcap,
+                                                        long long lcap, unsigned long long *cnt, int *counts) {
+    const long long per_bounce = nslots / maxb;
+    const int lane = (int)__lane_id();
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long f0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); f0 < nslots; f0 += stride) {
+        long long f = f0 + lane;
+        int kf = 0, kl = 0;
+        SelEntry x{};
+        if (f < nslots) {
+            // bounce-major, as the depth-first selection: neighbouring entries
+            // are the same bounce of neighbouring samples
+            long long s = (f % per_bounce) * maxb + f / per_bounce;
+            counts[s] = 0;
+            const PointRec &p = pts[s];
+            if (p.valid) {
+                const DrJob &J = jobs[p.job];
+                EdgeTraversal T;
+                RootStack st;
+                st.sp = 0;
+                if (select_begin(J.S, J.c, p, T, st) && st.sp == 1 && !st.e.empty()) {
+                    SelPoint P;
+                    P.pos = p.sp.pos;
+                    P.n = p.sp.fr.n;
+                    P.B = T.B;
+                    P.cdir = p.cone_dir;
+                    P.ccos = p.cone_cos;
+                    P.sid_lo = (uint32_t)p.sid;
+                    P.sid_hi = (uint32_t)(p.sid >> 32);
+                    P.info = (T.directional ? 1 : 0) | (T.mode << 1) | (T.cone_pmf_quirk ? 8 : 0) | (p.b << 8);
+                    P.job = p.job;
+                    spt[s] = P;
+                    x = sel_entry((int)s, st.e, st.a);
+                    if (st.e.nprims() > 0) kl = 1;
+                    else kf = 1;
+                }
+            }
+        }
+        long long bf = wave_append(lvl, kf, &cnt[4]);
+        long long bl = wave_append(lvl + DR_SEL_LEAVES, kl, &cnt[5]);
+        if (kf && bf < fcap) front[bf] = x;
+        if (kl && bl < lcap) leaves[bl] = x;
+    }
+}
+
+__global__ void __launch_bounds__(DR_BLOCK) k_sel_level(const DrJob *__restrict__ jobs, const SelPoint *spt,
+                                                        const SelEntry *in, const unsigned long long *in_cnt,
+                                                        SelEntry *out, unsigned long long *out_cnt, SelEntry *leaves,
+                                                        unsigned long long *leaf_cnt, long long fcap, long long lcap,
+                                                        unsigned long long *cnt) {
+    const long long n = std::min<long long>((long long)*in_cnt, fcap);
+    const int lane = (int)__lane_id();
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long b0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); b0 < n; b0 += stride) {
+        long long i = b0 + lane;
+        SelEntry x0{}, x1{};
+        bool v0 = false, v1 = false, l0 = false, l1 = false;
+        if (i < n) {
+            const SelEntry E = in[i];
+            const TreeEntry te = sel_tree(E);
+            const SelPoint P = spt[E.slot];
+            const DrJob &J = jobs[P.job];
+            EdgeTraversal T;
+            SPoint sp;
+            sel_ctx(P, J, T, sp);
+            TreeEntry c0, c1;
+            TreeAlt a0{0.f, 0.f}, a1{0.f, 0.f};
+            int k = et_expand(T, te, TreeAlt{E.altc, E.alta}, J.S, sp, c0, a0, c1, a1);
+            if (J.S.stats) atomicAdd(&J.S.stats[5], 1ull);
+            // children whose bound is 0 take nothing (the depth-first walk
+            // pops and drops them)
+            if (k > 0 && !c0.empty()) {
+                x0 = sel_entry(E.slot, c0, a0);
+                v0 = true;
+                l0 = c0.nprims() > 0;
+            }
+            if (k > 1 && !c1.empty()) {
+                x1 = sel_entry(E.slot, c1, a1);
+                v1 = true;
+                l1 = c1.nprims() > 0;
+            }
+        }
+        int kf = (v0 && !l0) + (v1 && !l1), kl = (v0 && l0) + (v1 && l1);
+        long long bf = wave_append(out_cnt, kf, &cnt[4]);
+        long long bl = wave_append(leaf_cnt, kl, &cnt[5]);
+        if (v0) {
+            if (l0) {
+                if (bl < lcap) leaves[bl] = x0;
+                bl++;
+            } else {
+                if (bf < fcap) out[bf] = x0;
+                bf++;
+            }
+        }
+        if (v1) {
+            if (l1) {
+                if (bl < lcap) leaves[bl] = x1;
+            } else {
+                if (bf < fcap) out[bf] = x1;
+            }
+        }
+    }
+}
+
+// One leaf per lane; its edges in turn (as many rounds as the wave's largest
+// leaf), each round's selections appended with one atomic per wave.
+__global__ void __launch_bounds__(DR_BLOCK) k_sel_leaves(const DrJob *__restrict__ jobs, const SelPoint *spt,
+                                                         const SelEntry *leaves, const unsigned long long *leaf_cnt,
+                                                         long long lcap, EdgeEntry *q, unsigned long long *cnt,
+                                                         long long cap, int *counts) {
+    const long long n = std::min<long long>((long long)*leaf_cnt, lcap);
+    const int lane = (int)__lane_id();
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long b0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); b0 < n; b0 += stride) {
+        long long i = b0 + lane;
+        SelEntry E{};
+        TreeEntry te{0, 0u, 0x80000000u, 0};
+        int nprims = 0;
+        if (i < n) {
+            E = leaves[i];
+            te = sel_tree(E);
+            nprims = te.nprims();
+        }
+        int rounds = nprims;
+        for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
+        SelPoint P;
+        EdgeTraversal T;
+        SPoint sp;
+        const DrJob *J = nullptr;
+        if (nprims > 0) {
+            P = spt[E.slot];
+            J = &jobs[P.job];
+            sel_ctx(P, *J, T, sp);
+        }
+        int calls = 0;
+        for (int r = 0; r < rounds; r++) {
+            EdgeEntry y0{}, y1{};
+            int ke = 0;
+            if (r < nprims) {
+                int got = 0;
+                auto emit = [&](float w, bool nrm, int idx, int ord) {
+                    calls++;
+                    if (w != 0.f) {
+                        EdgeEntry y{E.slot, idx | (nrm ? (int)0x80000000 : 0), w, ord};
+                        if (got == 0) y0 = y;
+                        else y1 = y;
+                        got++;
+                    }
+                };
+                et_leaf_edge(T, te, TreeAlt{E.altc, E.alta}, J->S, sp, r, emit);
+                ke = got;
+            }
+            long long bq = wave_append(&cnt[0], ke, &cnt[2]);
+            if (ke > 0 && bq < cap) q[bq] = y0;
+            if (ke > 1 && bq + 1 < cap) q[bq + 1] = y1;
+        }
+        if (nprims > 0 && calls > 0) {
+            if (J->dx) atomicAdd(&counts[E.slot], calls);
+            if (J->S.stats) {
+                atomicAdd(&J->S.stats[3], (unsigned long long)calls);
+            }
+        }
+        if (nprims > 0 && J->S.stats) atomicAdd(&J->S.stats[6], (unsigned long long)nprims);
+    }
+}
+
 // C: one selected edge per iteration of a grid-stride loop over the queue
 // (its length is read on the device, so no host round trip is needed).  Only
 // about one edge sample in five passes the connectivity test of its two side
@@ -727,7 +991,7 @@ struct LaunchTimer {
 struct Tuning {
     long long pass_samples = 1ll << 22;
     long long min_threads = 1ll << 20;
-    int select_refill = 64;
+    int select_refill = 64;  // 1-64: depth first with that refill threshold; 0: breadth first
 };
 
 // Device working set of the pipeline, grown on demand, and the hipGraph of
@@ -742,8 +1006,15 @@ struct Workspace {
     float4 *entry_dp = nullptr;
     int *cont = nullptr;                     // queue entries left for the continuation launch
     long long q_cap = 0;
-    unsigned long long *cnt = nullptr;       // queue length, work counter, largest queue length, continuations
-    unsigned long long *cnt_host = nullptr;  // pinned
+    // queue length, work counter, largest queue length, continuations,
+    // largest frontier end, largest leaf-list end (breadth-first selection)
+    unsigned long long *cnt = nullptr;
+    unsigned long long *cnt_host = nullptr;  // pinned, 8 words
+    SelPoint *spt = nullptr;                 // breadth-first selection: per point slot
+    SelEntry *fr[2] = {nullptr, nullptr};    // frontier of the current / next level
+    SelEntry *lv = nullptr;                  // leaves reached
+    unsigned long long *lvl = nullptr;       // entries per level (0-63), leaves (DR_SEL_LEAVES)
+    long long spt_cap = 0, f_cap = 0, l_cap = 0;
     DrJob *jobs = nullptr;
     int jobs_cap = 0;
     DrRange *ranges = nullptr;
@@ -755,7 +1026,8 @@ struct Workspace {
 };
 
 static void free_workspace(Workspace &w) {
-    void *ps[] = {w.pts, w.pt_dp, w.pt_lastj, w.counts, w.q, w.entry_dp, w.cont, w.cnt, w.jobs, w.ranges};
+    void *ps[] = {w.pts, w.pt_dp, w.pt_lastj, w.counts, w.q, w.entry_dp, w.cont, w.cnt, w.jobs, w.ranges,
+                  w.spt, w.fr[0], w.fr[1], w.lv, w.lvl};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (w.cnt_host) (void)hipHostFree(w.cnt_host);
@@ -1032,6 +1304,7 @@ struct Plan {
     long long prim_blocks = 0, fwd_blocks = 0, max_slots = 0;
     int maxb = 1;
     int select_refill = 64;
+    int sel_depth = 0;  // deepest edge-tree node of the adjoint jobs' scenes (levels of the breadth-first selection)
     std::vector<std::pair<float *, size_t>> zero;  // buffers cleared first
     std::vector<unsigned long long> key;           // what a captured graph depends on
 };
@@ -1081,6 +1354,7 @@ static dr_status build_plan(const dr_job *jobs, int n, const Tuning &tn, Plan &P
             P.zero.push_back({jb.grads, sc->hs.grad_size});
         }
         if (adj) P.maxb = std::max(P.maxb, bucket(jb.opts.max_bounces));
+        if (adj) P.sel_depth = std::max(P.sel_depth, std::max(sc->hs.sil_depth, sc->hs.nrm_depth));
         P.jobs.push_back(dj);
         unsigned long long k[] = {(unsigned long long)(uintptr_t)sc, sc->serial, (unsigned long long)sc->hs.grad_size,
                                   (unsigned long long)(uintptr_t)jb.view,
@@ -1151,7 +1425,10 @@ static dr_status build_plan(const dr_job *jobs, int n, const Tuning &tn, Plan &P
     }
     P.fwd_r1 = (int)P.ranges.size();
     P.fwd_blocks = blocks;
-    P.select_refill = tn.select_refill;
+    // the breadth-first selection keeps a count per level for 63 levels;
+    // deeper trees are walked depth first
+    P.select_refill = tn.select_refill == 0 && P.sel_depth > 62 ? 64 : tn.select_refill;
+    P.key.push_back((unsigned long long)P.sel_depth);
     P.key.push_back((unsigned long long)tn.pass_samples);
     P.key.push_back((unsigned long long)tn.min_threads);
     P.key.push_back((unsigned long long)tn.select_refill);
@@ -1159,6 +1436,9 @@ static dr_status build_plan(const dr_job *jobs, int n, const Tuning &tn, Plan &P
 }
 
 static dr_status ensure_workspace_(Workspace &w, const Plan &P);
+static bool has_edges(const Plan &P);
+static dr_status grow_frontier(Workspace &w, long long n);
+static dr_status grow_leaves(Workspace &w, long long n);
 static dr_status ensure_workspace(Workspace &w, const Plan &P) {
     unsigned long long e0 = g_alloc_epoch;
     dr_status st = ensure_workspace_(w, P);
@@ -1167,9 +1447,31 @@ static dr_status ensure_workspace(Workspace &w, const Plan &P) {
 }
 static dr_status ensure_workspace_(Workspace &w, const Plan &P) {
     if (!w.cnt) {
-        if (hipMalloc((void **)&w.cnt, 4 * sizeof(unsigned long long)) != hipSuccess ||
-            hipHostMalloc((void **)&w.cnt_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        if (hipMalloc((void **)&w.cnt, 8 * sizeof(unsigned long long)) != hipSuccess ||
+            hipHostMalloc((void **)&w.cnt_host, 8 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
             return fail(DR_ERROR_OUT_OF_MEMORY, "adjoint counters");
+    }
+    if (P.select_refill == 0 && has_edges(P)) {
+        // breadth-first selection: per point slot its constants; frontier and
+        // leaf lists start at 6 and 12 entries per slot (the heightfield at 2
+        // bounces tests about 9 leaves per point) and grow on overflow
+        if (!w.lvl && !grow((void **)&w.lvl, 72 * sizeof(unsigned long long)))
+            return fail(DR_ERROR_OUT_OF_MEMORY, "selection counters");
+        if (P.max_slots > w.spt_cap) {
+            if (!grow((void **)&w.spt, P.max_slots * sizeof(SelPoint))) {
+                w.spt_cap = 0;
+                return fail(DR_ERROR_OUT_OF_MEMORY, "selection point buffer");
+            }
+            w.spt_cap = P.max_slots;
+        }
+        if (6 * P.max_slots > w.f_cap) {
+            dr_status st = grow_frontier(w, 6 * P.max_slots);
+            if (st != DR_OK) return st;
+        }
+        if (12 * P.max_slots > w.l_cap) {
+            dr_status st = grow_leaves(w, 12 * P.max_slots);
+            if (st != DR_OK) return st;
+        }
     }
     if (P.max_slots > w.slot_cap) {
         long long n = P.max_slots;
@@ -1203,6 +1505,25 @@ static dr_status ensure_workspace_(Workspace &w, const Plan &P) {
     return DR_OK;
 }
 
+static dr_status grow_frontier(Workspace &w, long long n) {
+    w.epoch++;
+    if (!grow((void **)&w.fr[0], n * sizeof(SelEntry)) || !grow((void **)&w.fr[1], n * sizeof(SelEntry))) {
+        w.f_cap = 0;
+        return fail(DR_ERROR_OUT_OF_MEMORY, "selection frontier");
+    }
+    w.f_cap = n;
+    return DR_OK;
+}
+static dr_status grow_leaves(Workspace &w, long long n) {
+    w.epoch++;
+    if (!grow((void **)&w.lv, n * sizeof(SelEntry))) {
+        w.l_cap = 0;
+        return fail(DR_ERROR_OUT_OF_MEMORY, "selection leaf list");
+    }
+    w.l_cap = n;
+    return DR_OK;
+}
+
 static dr_status grow_queue(Workspace &w, long long n) {
     w.epoch++;
     if (!grow((void **)&w.q, n * sizeof(EdgeEntry)) || !grow((void **)&w.entry_dp, n * sizeof(float4)) ||
@@ -1215,6 +1536,7 @@ static dr_status grow_queue(Workspace &w, long long n) {
 }
 
 static long long g_select_blocks = 0, g_edge_blocks[4] = {0, 0, 0, 0}, g_shade_blocks[4] = {0, 0, 0, 0};
+static long long g_sel_blocks[3] = {0, 0, 0};
 
 // All launches of a plan, stream-ordered, without host synchronisation (so
 // they can be captured in a hipGraph).  The tables are already on the device.
@@ -1224,16 +1546,45 @@ static dr_status grow_queue(Workspace &w, long long n);
 // read the queue length and, if the queue overflowed, grow it and run the
 // selection of that pass again (it only writes the queue and the per-point
 // counts, so a second run replaces the first).
+// Grows what overflowed (edge queue; breadth-first frontier and leaf list)
+// after the counters cnt[0..5] were read into w.cnt_host; grew: whether
+// something did.
+static dr_status grow_overflowed(Workspace &w, long long qlen, bool bfs, bool &grew) {
+    grew = false;
+    if (qlen > w.q_cap) {
+        dr_status st = grow_queue(w, qlen + qlen / 4);
+        if (st != DR_OK) return st;
+        grew = true;
+    }
+    if (bfs) {
+        long long f = (long long)w.cnt_host[4], l = (long long)w.cnt_host[5];
+        // at least twice as large: entries a full list dropped never reached
+        // the next levels, which may then overflow in turn
+        if (f > w.f_cap) {
+            dr_status st = grow_frontier(w, std::max(2 * f, 2 * w.f_cap));
+            if (st != DR_OK) return st;
+            grew = true;
+        }
+        if (l > w.l_cap) {
+            dr_status st = grow_leaves(w, std::max(2 * l, 2 * w.l_cap));
+            if (st != DR_OK) return st;
+            grew = true;
+        }
+    }
+    return DR_OK;
+}
+
 template <class Launch>
-static dr_status settle_selection(Workspace &w, hipStream_t hs, Launch &&select) {
-    for (int attempt = 0; attempt < 3; attempt++) {
-        if (hipMemcpyAsync(w.cnt_host, w.cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) != hipSuccess ||
+static dr_status settle_selection(Workspace &w, hipStream_t hs, bool bfs, Launch &&select) {
+    for (int attempt = 0; attempt < 8; attempt++) {
+        if (hipMemcpyAsync(w.cnt_host, w.cnt, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) !=
+                hipSuccess ||
             hipStreamSynchronize(hs) != hipSuccess)
             return fail(DR_ERROR_DEVICE, "reading the edge queue length failed");
-        long long len = (long long)*w.cnt_host;
-        if (len <= w.q_cap) return DR_OK;
-        dr_status st = grow_queue(w, len + len / 4);
+        bool grew = false;
+        dr_status st = grow_overflowed(w, (long long)w.cnt_host[0], bfs, grew);
         if (st != DR_OK) return st;
+        if (!grew) return DR_OK;
         k_fill<<<1, 64, 0, hs>>>((float *)w.cnt, 4, 0.f);
         select();
     }
@@ -1246,7 +1597,8 @@ static dr_status settle_selection(Workspace &w, hipStream_t hs, Launch &&select)
 template <int MAXB>
 static dr_status enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs, bool sync_select) {
     for (auto &z : P.zero) k_fill<<<256, 256, 0, hs>>>(z.first, z.second, 0.f);
-    k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 2), 2, 0.f);
+    k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 2), 8, 0.f);  // largest queue, continuations, largest lists
+    const bool bfs = P.select_refill == 0;
     const int bi = MAXB == 1 ? 0 : MAXB == 2 ? 1 : MAXB == 4 ? 2 : 3;
     for (const Pass &ps : P.passes) {
         const DrRange *R = w.ranges + ps.r0;
@@ -1262,13 +1614,28 @@ static dr_status enqueue(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs
         k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 3), 2, 0.f);  // continuations
         auto select = [&]() {
             LaunchTimer t(tm, TK_SELECT, hs);
-            long long blocks = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_select, g_select_blocks));
-            k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, MAXB, w.pts, w.q, w.cnt, w.q_cap,
-                                                            w.counts, P.select_refill, 0ll, ps.nslots);
+            if (bfs) {
+                k_fill<<<1, 256, 0, hs>>>((float *)w.lvl, 2 * 72, 0.f);
+                long long b0 = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_sel_start, g_sel_blocks[0]));
+                k_sel_start<<<(unsigned)b0, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, MAXB, w.pts, w.spt, w.fr[0], w.lv,
+                                                               w.lvl, w.f_cap, w.l_cap, w.cnt, w.counts);
+                long long b1 = resident_blocks(k_sel_level, g_sel_blocks[1]);
+                for (int d = 0; d < P.sel_depth; d++)
+                    k_sel_level<<<(unsigned)b1, DR_BLOCK, 0, hs>>>(w.jobs, w.spt, w.fr[d & 1], w.lvl + d,
+                                                                   w.fr[(d + 1) & 1], w.lvl + d + 1, w.lv,
+                                                                   w.lvl + DR_SEL_LEAVES, w.f_cap, w.l_cap, w.cnt);
+                long long b2 = resident_blocks(k_sel_leaves, g_sel_blocks[2]);
+                k_sel_leaves<<<(unsigned)b2, DR_BLOCK, 0, hs>>>(w.jobs, w.spt, w.lv, w.lvl + DR_SEL_LEAVES, w.l_cap,
+                                                                w.q, w.cnt, w.q_cap, w.counts);
+            } else {
+                long long blocks = std::min<long long>(grid_of(ps.nslots), resident_blocks(k_select, g_select_blocks));
+                k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, MAXB, w.pts, w.q, w.cnt, w.q_cap,
+                                                                w.counts, P.select_refill, 0ll, ps.nslots);
+            }
         };
         select();
         if (sync_select) {
-            dr_status st = settle_selection(w, hs, select);
+            dr_status st = settle_selection(w, hs, bfs, select);
             if (st != DR_OK) return st;
         }
         long long eb = resident_blocks(k_edge<MAXB>, g_edge_blocks[bi]);
@@ -1324,7 +1691,7 @@ static bool has_edges(const Plan &P) {
 static dr_status run_plan(const Plan &P, Workspace &w, Timing *tm, hipStream_t hs, bool use_graph) {
     dr_status st = ensure_workspace(w, P);
     if (st != DR_OK) return st;
-    for (int attempt = 0; attempt < 3; attempt++) {
+    for (int attempt = 0; attempt < 8; attempt++) {
         if (hipMemcpyAsync(w.jobs, P.jobs.data(), P.jobs.size() * sizeof(DrJob), hipMemcpyHostToDevice, hs) !=
                 hipSuccess ||
             (!P.ranges.empty() && hipMemcpyAsync(w.ranges, P.ranges.data(), P.ranges.size() * sizeof(DrRange),
@@ -1335,10 +1702,12 @@ static dr_status run_plan(const Plan &P, Workspace &w, Timing *tm, hipStream_t h
             // workspace's reallocation count
             std::vector<unsigned long long> key = P.key;
             const void *bufs[] = {w.pts, w.pt_dp, w.pt_lastj, w.counts, w.q, w.entry_dp, w.cont, w.cnt, w.jobs,
-                                  w.ranges};
+                                  w.ranges, w.spt, w.fr[0], w.fr[1], w.lv, w.lvl};
             for (const void *b : bufs) key.push_back((unsigned long long)(uintptr_t)b);
             key.push_back((unsigned long long)w.q_cap);
             key.push_back((unsigned long long)w.slot_cap);
+            key.push_back((unsigned long long)w.f_cap);
+            key.push_back((unsigned long long)w.l_cap);
             key.push_back(w.epoch);
             if (!w.exec || key != w.graph_key) {
                 if (w.exec) (void)hipGraphExecDestroy(w.exec);
@@ -1368,14 +1737,14 @@ static dr_status run_plan(const Plan &P, Workspace &w, Timing *tm, hipStream_t h
         st = launch_check("render kernels");
         if (st != DR_OK) return st;
         if (!has_edges(P)) return DR_OK;
-        if (hipMemcpyAsync(w.cnt_host, w.cnt + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) !=
+        if (hipMemcpyAsync(w.cnt_host, w.cnt, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, hs) !=
                 hipSuccess ||
             hipStreamSynchronize(hs) != hipSuccess)
             return fail(DR_ERROR_DEVICE, "reading the edge queue length failed");
-        long long len = (long long)*w.cnt_host;
-        if (len <= w.q_cap) return DR_OK;
-        st = grow_queue(w, len + len / 4);
+        bool grew = false;
+        st = grow_overflowed(w, (long long)w.cnt_host[2], P.select_refill == 0, grew);
         if (st != DR_OK) return st;
+        if (!grew) return DR_OK;
     }
     return fail(DR_ERROR_DEVICE, "edge queue kept overflowing");
 }
@@ -1457,8 +1826,10 @@ static dr_status run_noalloc(const Plan &P, Workspace &w, hipStream_t hs) {
             k_fill<<<1, 64, 0, hs>>>((float *)w.cnt, 2 * 2, 0.f);       // queue length, work counter
             k_fill<<<1, 64, 0, hs>>>((float *)(w.cnt + 3), 2, 0.f);     // continuations
             long long blocks = std::min<long long>(grid_of(f1 - f0), resident_blocks(k_select, g_select_blocks));
+            // (the depth-first selection: its working set fits the array)
             k_select<<<(unsigned)blocks, DR_BLOCK, 0, hs>>>(w.jobs, ps.nslots, MAXB, w.pts, w.q, w.cnt, w.q_cap,
-                                                            w.counts, P.select_refill, f0, f1);
+                                                            w.counts, P.select_refill > 0 ? P.select_refill : 64,
+                                                            f0, f1);
             unsigned long long len = 0;
             if (hipMemcpyAsync(&len, w.cnt, sizeof(len), hipMemcpyDeviceToHost, hs) != hipSuccess ||
                 hipStreamSynchronize(hs) != hipSuccess)
@@ -2101,7 +2472,7 @@ dr_status dr_batch_set_tuning(dr_batch *b, const dr_tuning *t) {
         return fail(DR_ERROR_INVALID_ARGUMENT, "bad tuning");
     b->tune.pass_samples = t->pass_samples;
     b->tune.min_threads = t->min_threads;
-    b->tune.select_refill = t->select_refill > 0 ? t->select_refill : Tuning().select_refill;
+    b->tune.select_refill = t->select_refill;
     return DR_OK;
 }
 
@@ -2141,7 +2512,7 @@ dr_status dr_batch_autotune(dr_batch *b, const dr_job *jobs, int num_jobs, void 
         }
     }
     // then the refill threshold of the edge selection, at the chosen pass sizes
-    static const int kRefill[] = {8, 16, 32};
+    static const int kRefill[] = {8, 16, 32, 0};
     if (has_edges(P)) {
         for (int rf : kRefill) {
             if (st != DR_OK) break;

@@ -469,9 +469,10 @@ class Batch:
         return {"pass_samples": t.pass_samples, "min_threads": t.min_threads, "select_refill": t.select_refill,
                 "best_ms": t.best_ms, "candidates": t.candidates}
 
-    def set_tuning(self, pass_samples, min_threads, select_refill=0):
-        """select_refill: idle lanes an edge-selection wave gathers before
-        starting new shading points (0 = default)."""
+    def set_tuning(self, pass_samples, min_threads, select_refill=64):
+        """select_refill: 1-64 = depth-first edge selection (the default 64),
+        with the idle lanes a wave gathers before starting new shading
+        points; 0 = breadth first."""
         t = _abi.dr_tuning(pass_samples, min_threads, select_refill, 0.0, 0)
         _abi.check(self.lib, self.lib.dr_batch_set_tuning(self.handle, ctypes.byref(t)), "dr_batch_set_tuning")
 

@@ -442,10 +442,12 @@ enum { DR_BATCH_GRAPH = 1 };
 dr_status dr_render_batch(dr_batch *batch, const dr_job *jobs, int num_jobs, int flags, void *stream);
 
 /* Pass sizes of the pipeline: pixel samples per pass and the least number
- * of threads a job is split into; select_refill: idle lanes (1-64) a wave
- * of the edge-selection kernel gathers before it starts new shading points
- * (0 = the default, 64).  dr_batch_autotune renders the job list with each
- * candidate (outputs are overwritten) and keeps the fastest. */
+ * of threads a job is split into; select_refill: 1-64 = the depth-first
+ * persistent edge selection, whose waves gather that many idle lanes before
+ * starting new shading points (64 by default), 0 = the breadth-first
+ * selection (one tree level per launch over every shading point of a pass;
+ * the same selected edges).  dr_batch_autotune renders the job list with
+ * each candidate (outputs are overwritten) and keeps the fastest. */
 typedef struct dr_tuning {
     int64_t pass_samples;
     int64_t min_threads;

@@ -67,8 +67,9 @@ def test_batch_matches_single_renders(gpu):
 @pytest.mark.parametrize("refill", [0, 1, 64])
 def test_batch_small_passes_match(gpu, refill):
     """Jobs split over many passes (tiny pass budget) give the same result,
-    whatever number of idle lanes the edge selection gathers before it starts
-    new shading points (the schedule changes, the estimate does not)."""
+    whatever the edge selection: breadth first (refill 0) or depth first
+    with 1 or 64 idle lanes gathered before new shading points start (the
+    schedule changes, the selected edges and the estimate do not)."""
     import torch
     jobs = _job_list(torch, gpu)
     batch = dr.Batch()
@@ -135,10 +136,11 @@ def test_autotune_picks_a_candidate(gpu):
     jobs = _job_list(torch, gpu)[:3]
     batch = dr.Batch()
     t = batch.autotune(jobs)
-    # 4 x 3 pass sizes, then 3 more refill thresholds at the best of them
-    assert t["candidates"] == 15 and t["best_ms"] > 0
+    # 4 x 3 pass sizes, then 3 more refill thresholds and the breadth-first
+    # selection at the best of them
+    assert t["candidates"] == 16 and t["best_ms"] > 0
     assert t["pass_samples"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
-    assert t["select_refill"] in (8, 16, 32, 64)
+    assert t["select_refill"] in (0, 8, 16, 32, 64)
     batch.render(jobs)
     torch.cuda.synchronize()
     for j in jobs:
