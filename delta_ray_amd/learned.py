@@ -158,3 +158,48 @@ class EdgeProposal:
         self.log_w = np.log(np.maximum(w.astype(np.float64), 1e-30))
         _abi.check(self.lib, self.lib.dr_scene_set_edge_weights(self.scene.handle, w.ctypes.data),
                    "dr_scene_set_edge_weights")
+
+
+def main():
+    """Time the proposal's network on the edges of the glossy heightfield
+    (128 x 128 grid): `--reps` inferences of every edge (dr_mlp_infer, bf16
+    on the matrix cores) and one training call of `--steps` Adam steps.
+
+        python -m delta_ray_amd.learned --reps 200 --steps 300
+    """
+    import argparse
+    import json
+    import time
+    import torch
+    from . import delta_ray as dr
+    from . import workloads
+    ap = argparse.ArgumentParser(description=main.__doc__.split("\n\n")[0])
+    ap.add_argument("--reps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=300)
+    a = ap.parse_args()
+    torch.cuda.init()
+    w = workloads.heightfield()
+    scene = dr.DeviceScene(w["camera"], w["shapes"], w["materials"], w["lights"])
+    prop = EdgeProposal(scene, seed=1)
+    y = torch.randn(prop.n, device="cuda")
+    wt = torch.ones(prop.n, device="cuda")
+    loss = ctypes.c_float(0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _abi.check(prop.lib, prop.lib.dr_mlp_train(prop.mlp, _ptr(prop.features), _ptr(y), _ptr(wt), prop.n, a.steps,
+                                               1024, 3e-3, 1, ctypes.byref(loss), None), "dr_mlp_train")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(a.reps):
+        _abi.check(prop.lib, prop.lib.dr_mlp_infer(prop.mlp, _ptr(prop.features), prop.n, _ptr(prop.logits), None),
+                   "dr_mlp_infer")
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    flops = 2.0 * (_abi.DR_MLP_FEATURES * 64 + 64 * 64 + 64) * prop.n
+    ms = (t2 - t1) / a.reps * 1e3
+    print(json.dumps({"edges": prop.n, "infer_ms": round(ms, 4), "infer_tflops": round(flops / (ms * 1e-3) / 1e12, 4),
+                      "train_ms": round((t1 - t0) * 1e3, 2), "train_steps": a.steps, "loss": loss.value}))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,7 +1,8 @@
 """Summarise rocprofv3 results of bench.py runs (rocpd SQLite databases).
 
-    python -m delta_ray_amd.prof_summary --kt KT.db [--fetch F.db --write W.db] [--sq SQ.db]
+    python -m delta_ray_amd.prof_summary --kt KT.db [--fetch F.db --write W.db] [--sq SQ.db] [--sq2 SQ2.db]
                                          [--warmup W --steps K] [--copy-bytes N]
+    python -m delta_ray_amd.prof_summary --mfma MFMA.db     # the network kernels (python -m delta_ray_amd.learned)
 
 bench.py renders, in order: two forward renders (the d_image of the loss),
 `warmup` + `steps` forward+adjoint renders (the timed ones are the last
@@ -14,6 +15,14 @@ with the ones bench.py measures with HIP events.
 HBM traffic: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the
 same counters read for the `--calibrate` copy of bench.py (a device-to-device
 copy of `copy-bytes` bytes: it reads and writes that many bytes).
+
+Second SQ pass: active lanes per VALU instruction = SQ_THREAD_CYCLES_VALU /
+SQ_INSTS_VALU; the fractions of wave-cycles issuing VALU (SQ_ACTIVE_INST_VALU)
+and waiting (SQ_WAIT_ANY) over SQ_WAVE_CYCLES.
+
+MFMA: per network kernel, bf16 matrix flops = SQ_INSTS_VALU_MFMA_MOPS_BF16 x
+512, over the dispatch's duration; matrix-core busy fraction =
+SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 1024 SIMDs).
 """
 import argparse
 import collections
@@ -73,7 +82,9 @@ def _copy_kib(rows, copy_bytes):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--kt", required=True)
+    ap.add_argument("--kt")
+    ap.add_argument("--sq2")
+    ap.add_argument("--mfma")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--sq")
@@ -83,6 +94,12 @@ def main():
     ap.add_argument("--pmc-steps", type=int, default=1)
     ap.add_argument("--copy-bytes", type=int, default=(64 << 20) * 4)
     a = ap.parse_args()
+    if a.mfma:
+        mfma_summary(a.mfma)
+        if not a.kt:
+            return
+    if not a.kt:
+        raise SystemExit("--kt (or --mfma) is required")
 
     disp = _dispatches(a.kt)
     win = _window(disp, a.warmup, a.steps)
@@ -151,6 +168,58 @@ def main():
             print("  %-14s %10.0f %12.0f %10.0f %10.0f %10.0f %10.0f" % (
                 k, w, cs.get("SQ_INSTS_VALU", 0) / w, cs.get("SQ_INSTS_SALU", 0) / w, cs.get("SQ_INSTS_VMEM_RD", 0) / w,
                 cs.get("SQ_INSTS_VMEM_WR", 0) / w, cs.get("SQ_INSTS_LDS", 0) / w))
+    if a.sq2:
+        sq2_summary(a.sq2, a.pmc_warmup, a.pmc_steps)
+
+
+def _per_dispatch(db):
+    con = sqlite3.connect(db)
+    rows = con.execute("select dispatch_id, kernel_name, counter_name, value, start, end from counters_collection "
+                       "order by start").fetchall()
+    con.close()
+    disp = collections.OrderedDict()
+    for d, k, c, v, s, e in rows:
+        x = disp.setdefault(d, [_short(k), s, e, {}])
+        x[3][c] = x[3].get(c, 0.0) + v
+    return disp
+
+
+def sq2_summary(db, warmup, steps):
+    disp = _per_dispatch(db)
+    wset = _window([(d, k, s, e) for d, (k, s, e, _) in disp.items()], warmup, steps)
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    for d, (k, s, e, cs) in disp.items():
+        if d in wset:
+            for c, v in cs.items():
+                per[k][c] += v
+    print("\nSecond SQ pass (fractions of wave-cycles; active lanes per VALU instruction, of 64):")
+    print("  %-14s %10s %12s %9s" % ("kernel", "lanes/VALU", "VALU-active", "waiting"))
+    for k, cs in sorted(per.items(), key=lambda kv: -kv[1].get("SQ_INSTS_VALU", 0)):
+        if cs.get("SQ_INSTS_VALU", 0) < 1e8 or cs.get("SQ_WAVE_CYCLES", 0) <= 0:
+            continue
+        print("  %-14s %10.1f %12.3f %9.3f" % (k, cs["SQ_THREAD_CYCLES_VALU"] / cs["SQ_INSTS_VALU"],
+                                               cs["SQ_ACTIVE_INST_VALU"] / cs["SQ_WAVE_CYCLES"],
+                                               cs["SQ_WAIT_ANY"] / cs["SQ_WAVE_CYCLES"]))
+
+
+def mfma_summary(db):
+    disp = _per_dispatch(db)
+    per = collections.defaultdict(lambda: [0, 0.0, 0.0, 0.0, 0.0])
+    for d, (k, s, e, cs) in disp.items():
+        if "mlp" not in k:
+            continue
+        t = per[k]
+        t[0] += 1
+        t[1] += (e - s) * 1e-9
+        t[2] += cs.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0) * 512
+        t[3] += cs.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0)
+        t[4] += cs.get("GRBM_GUI_ACTIVE", 0.0)
+    print("MFMA, network kernels (bf16; peak 2.5 PFLOP/s dense):")
+    print("  %-14s %8s %10s %12s %12s %10s" % ("kernel", "launches", "avg_us", "MFLOP/launch", "TFLOP/s", "MFMA busy"))
+    for k, (n, sec, fl, busy, gui) in sorted(per.items()):
+        print("  %-14s %8d %10.1f %12.3f %12.3f %10.4f" % (k, n, sec / n * 1e6, fl / n / 1e6,
+                                                          fl / sec / 1e12 if sec > 0 else 0.0,
+                                                          busy / (gui * 1024) if gui > 0 else 0.0))
 
 
 if __name__ == "__main__":
