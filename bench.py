@@ -82,7 +82,7 @@ def parse():
     ap.add_argument("--lv-train-steps", type=int, default=300, help="learned: MLP training steps per round")
     ap.add_argument("--lv-rounds", type=int, default=2, help="learned: rounds of statistics + training")
     ap.add_argument("--lv-eps", type=float, default=1e-2, help="learned: target relative RMSE")
-    ap.add_argument("--lv-bench-renders", type=int, default=64,
+    ap.add_argument("--lv-bench-renders", type=int, default=96,
                     help="renders per estimator of the learned-vs-plain leg of the default run")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known HBM traffic: "
@@ -342,6 +342,8 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
             torch.cuda.synchronize()
             ms += e0.elapsed_time(e1)
             out[k] = g[sel].double()
+            if (k + 1) % 32 == 0:
+                print("learned%s: %d of %d renders" % (log, k + 1, len(seeds)), file=sys.stderr, flush=True)
         return out, ms / len(seeds)
 
     groups = 16 if n >= 64 else 8

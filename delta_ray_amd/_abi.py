@@ -39,10 +39,11 @@ DR_WRITE_DX_IMAGE = 1 << 11
 DR_LEARNED_EDGES = 1 << 12
 DR_EDGE_STATS = 1 << 13
 DR_QUIRK_CLIP_DERIVATIVE = 1 << 14
+DR_QUIRK_EDGE_EMITTER_FULL = 1 << 15
 DR_FLAGS_PARITY = (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP |
                    DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE |
                    DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN | DR_QUIRK_CONE_PMF |
-                   DR_QUIRK_CLIP_DERIVATIVE)
+                   DR_QUIRK_CLIP_DERIVATIVE | DR_QUIRK_EDGE_EMITTER_FULL)
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -136,7 +137,8 @@ class dr_scene_info(ctypes.Structure):
                 ("num_primary_edges", ctypes.c_int),
                 ("silhouette_tree_nodes", ctypes.c_int),
                 ("normal_tree_nodes", ctypes.c_int), ("build_ms", ctypes.c_double),
-                ("refits", ctypes.c_int), ("rebuilds", ctypes.c_int), ("edges_switched_off", ctypes.c_int)]
+                ("refits", ctypes.c_int), ("rebuilds", ctypes.c_int), ("edges_switched_off", ctypes.c_int),
+                ("bvh_depth", ctypes.c_int), ("edge_tree_depth", ctypes.c_int)]
 
 
 class dr_shape_update(ctypes.Structure):

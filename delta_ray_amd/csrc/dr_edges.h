@@ -675,6 +675,21 @@ struct EdgeResult {
     F3 dp, dv0, dv1;
 };
 
+// The power-heuristic weight of BSDF sampling for the emitter point hp (hit
+// h) seen from sp, as the path estimate weighs an emitter a BSDF-sampled
+// direction hits (pathtrace.h:139-150): pdfs in solid angle.
+DR_INLINE float emitter_bsdf_mis(const DrSceneView &S, int mid, const SPoint &sp, F3 wi, const Isect &h,
+                                 const SPoint &hp) {
+    F3 d = hp.pos - sp.pos;
+    float dsq = length_sq(d);
+    if (dsq <= 0.f) return 0.f;
+    F3 wo = normalize(d);
+    float pdf = bsdf_pdf(S, mid, sp, wi, wo);
+    if (pdf <= 0.f) return 0.f;
+    float lpdf = light_pdf(S, h) * dsq / fabsf(dot(wo, hp.ng));
+    return sq(pdf) / (sq(lpdf) + sq(pdf));
+}
+
 // delta_ray.cpp:45-365: one point on one silhouette edge, seen from a
 // shading point; LTC line sampling, two rays across the edge, Dirac
 // Jacobian.  ss / nss: bounces after the edge hit (empty when direct only).
@@ -844,8 +859,19 @@ DR_INLINE EdgeResult sample_shading_edge_point(const DrSceneView &S, float bsdf_
     }
     int rl = valid(ir) ? S.shapes[ir.shape].light : -1;
     int ll = valid(il) ? S.shapes[il.shape].light : -1;
-    F3 er = eval_light(S, wi, is, sp, rl, spr.pos, spr.fr.n, comp) + eb * shr;
-    F3 el = eval_light(S, wi, is, sp, ll, spl.pos, spl.fr.n, comp) + eb * shl;
+    F3 er_e = eval_light(S, wi, is, sp, rl, spr.pos, spr.fr.n, comp);
+    F3 el_e = eval_light(S, wi, is, sp, ll, spl.pos, spl.fr.n, comp);
+    if (!(quirks & DR_QUIRK_EDGE_EMITTER_FULL)) {
+        // on an emitter's own outline (the side that hits one of the edge's
+        // faces is the emitter) only the BSDF-sampled share of its light has
+        // a boundary: the light-sampled share is an area integral over the
+        // emitter.  Light seen past another object's edge keeps its full
+        // weight (both shares are occluded there).
+        if (rl >= 0 && rc) er_e = er_e * emitter_bsdf_mis(S, mid, sp, wi, ir, spr);
+        if (ll >= 0 && lc) el_e = el_e * emitter_bsdf_mis(S, mid, sp, wi, il, spl);
+    }
+    F3 er = er_e + eb * shr;
+    F3 el = el_e + eb * shl;
     if (valid(ir)) {
         F3 dd = spr.pos - sp.pos;
         if (length(dd) * fabsf(dot(normalize(dd), spr.ng)) < 1e-3f) return R;

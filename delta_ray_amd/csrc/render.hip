@@ -2224,6 +2224,8 @@ dr_status dr_scene_get_info(const dr_scene *s, dr_scene_info *o) {
     o->refits = s->refits;
     o->rebuilds = s->rebuilds;
     o->edges_switched_off = s->edges_off;
+    o->bvh_depth = s->hs.bvh_depth;
+    o->edge_tree_depth = std::max(s->hs.sil_depth, s->hs.nrm_depth);
     return DR_OK;
 }
 

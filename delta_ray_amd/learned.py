@@ -132,10 +132,9 @@ class EdgeProposal:
         (DR_LEARNED_EDGES | DR_EDGE_STATS renders) refine the weights: each
         round's RMS measures what the current proposal still under- or
         over-samples.  Weights are clipped to `clip`, a wide range: the
-        second moment of the secondary-edge estimate concentrates on few
-        edges (on the glossy heightfield, the light quads' edges and those
-        near them: 1 % of the edges hold 98 % of it), whose best weights are
-        hundreds of times the typical one."""
+        second moment of the secondary-edge estimate can concentrate on few
+        edges, whose best weights are then hundreds of times the typical
+        one."""
         if self.use_network:
             _abi.check(self.lib, self.lib.dr_mlp_infer(self.mlp, _ptr(self.features), self.n, _ptr(self.logits),
                                                        None), "dr_mlp_infer")

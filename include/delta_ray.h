@@ -172,12 +172,21 @@ enum {
      * t = -(z + clip_near) / dz where project clips at -(z - clip_near) / dz,
      * and drops clip_near from dt / d(dz) (camera.h:231-236 vs 259-290);
      * without it the clipped endpoint's derivative is the exact one */
-    DR_QUIRK_CLIP_DERIVATIVE = 1u << 14
+    DR_QUIRK_CLIP_DERIVATIVE = 1u << 14,
+    /* a secondary-edge sample counts the whole emission an emitter sends
+     * through one side of the edge (delta_ray.cpp:289-295), while the
+     * render's estimate of that light is split by MIS between light sampling
+     * (an integral over the emitter's area, whose derivative the interior
+     * adjoint carries) and BSDF sampling (the only part with a boundary at
+     * the emitter's outline): emitter outlines count about twice; without it
+     * the emission seen across an emitter's own edge is weighted by the BSDF
+     * strategy's MIS weight in that direction (pathtrace.h:139-150) */
+    DR_QUIRK_EDGE_EMITTER_FULL = 1u << 15
 };
 #define DR_FLAGS_PARITY (DR_QUIRK_JITTER_OVER_SPP | DR_QUIRK_TEXEL_SWAP | \
                          DR_QUIRK_NORMAL_DN0 | DR_QUIRK_DP_LAST | DR_QUIRK_PROJECT_SCALE | \
                          DR_QUIRK_EDGE_BSDF_SUM | DR_QUIRK_EDGE_HIT_JACOBIAN | DR_QUIRK_CONE_PMF | \
-                         DR_QUIRK_CLIP_DERIVATIVE)
+                         DR_QUIRK_CLIP_DERIVATIVE | DR_QUIRK_EDGE_EMITTER_FULL)
 
 typedef struct dr_render_options {
     int width;
@@ -399,6 +408,9 @@ typedef struct dr_scene_info {
     int rebuilds;            /* ... and rebuilt on the host (the edge set changed) */
     int edges_switched_off;  /* DR_EDGES_ALL_CANDIDATES: tree edges the filter drops on the current
                               * vertices (0 otherwise) */
+    int bvh_depth;           /* deepest triangle-BVH node (root 0); the builder keeps it within the
+                              * traversal stack (64 entries) */
+    int edge_tree_depth;     /* deepest node of the silhouette and normal edge trees */
 } dr_scene_info;
 dr_status dr_scene_get_info(const dr_scene *scene, dr_scene_info *out);
 

@@ -1137,8 +1137,33 @@ EdgeResult sample_shading_edge_point(const Scene &sc, float bsdf_component, floa
         fix(ir, spr, shr);
         fix(il, spl, shl);
     }
-    V3f er = eval<float>(sc, wi, is, sp, ir, spr, nullptr, comp) + eb * shr;
-    V3f el = eval<float>(sc, wi, is, sp, il, spl, nullptr, comp) + eb * shl;
+    V3f er_e = eval<float>(sc, wi, is, sp, ir, spr, nullptr, comp);
+    V3f el_e = eval<float>(sc, wi, is, sp, il, spl, nullptr, comp);
+    if (!(flags & DR_QUIRK_EDGE_EMITTER_FULL)) {
+        // Without the quirk, on an emitter's own outline (the side whose hit
+        // is one of the edge's faces is an emitter) the emission that side
+        // sees is weighted by the BSDF strategy's MIS weight in that
+        // direction (the weight shade() gives a BSDF-sampled emitter hit,
+        // pathtrace.h:139-150): only that share of the light has a boundary
+        // at the emitter's outline; the light-sampled share is an area
+        // integral whose derivative the interior adjoint carries.  Light seen
+        // past another object's edge is occluded in both shares: full weight.
+        auto mis = [&](const Isect &h, const SP<float> &hp, bool own_face) {
+            if (!own_face || !h.valid() || sc.shapes[h.shape].light < 0) return 1.f;
+            V3f d = hp.pos - sp.pos;
+            float dsq = length_squared(d);
+            if (dsq <= 0.f) return 0.f;
+            V3f wo = normalize(d);
+            float pdf = bsdf_pdf<float>(sc, mid, sp, wi, wo, nullptr);
+            if (pdf <= 0.f) return 0.f;
+            float lpdf = light_pdf<float>(sc, h, nullptr) * dsq / std::fabs(dot(wo, hp.ng));
+            return square(pdf) / (square(lpdf) + square(pdf));
+        };
+        er_e = er_e * mis(ir, spr, rc);
+        el_e = el_e * mis(il, spl, lc);
+    }
+    V3f er = er_e + eb * shr;
+    V3f el = el_e + eb * shl;
     auto too_close = [&](const Isect &h, const SP<float> &hp) {
         if (!h.valid()) return false;
         V3f dd = hp.pos - sp.pos;

@@ -165,3 +165,62 @@ def test_adjoint_matches_oracle_full_heightfield_mesh(gpu, oracle, term):
     assert np.abs(gref).sum() > 0
     print("rel err (all)", rel_err(g, gref), "norm", float(np.linalg.norm(gref)))
     helpers.assert_groups(w, g, gref, RTOL[term], FLOOR[term])
+
+
+def _geometric_mesh(n=150):
+    """n triangles along x at 0.8^k (k = 0 .. n-1), each of size 0.8^k / 8:
+    most of them crowd one end of the binned-SAH build's bins, which then
+    splits off a few triangles per level, far deeper than the traversal
+    stack without the builder's depth bound"""
+    from delta_ray_amd import workloads
+    from delta_ray_amd import delta_ray as dr
+    v, ind = [], []
+    for k in range(n):
+        c, h = 0.8 ** k, 0.8 ** k / 8
+        v += [[c - h, -h, 0.0], [c + h, -h, 0.0], [c, h, 0.0]]
+        ind.append([3 * k, 3 * k + 1, 3 * k + 2])
+    v, ind = np.array(v, np.float32), np.array(ind, np.int32)
+    grey = workloads.diffuse_material([0.5, 0.5, 0.5])
+    mesh = dr.Shape(v, ind, None, None, grey, None)
+    lv = np.array([[-1, -1, 5], [1, -1, 5], [-1, 1, 5]], np.float32)
+    ls = dr.Shape(lv, np.array([[0, 1, 2]], np.int32), None, None, grey, None)
+    light = dr.Light(ls, np.full(3, 10.0, np.float32))
+    ls.light = light
+    cam = workloads.make_camera([0.5, 0, 3], [0.5, 0, 0], [0, 1, 0], 45.0, (16, 16))
+    return dict(name="geometric", camera=cam, shapes=[mesh, ls], materials=[grey], lights=[light],
+                resolution=(16, 16), num_samples=1, max_bounces=1, params=[("shape", 0, "vertices")]), v
+
+
+def test_deep_bvh_mesh_intersects_exactly(gpu, oracle):
+    """Geometrically spaced triangles (a degenerate input for a SAH build):
+    the builder's depth bound keeps the triangle BVH within the traversal
+    stack, so every ray down onto each triangle finds it, bit for bit as the
+    oracle's brute force, and no traversal stack overflows."""
+    import torch
+    _print_case("intersect", "geometric mesh")
+    w, v = _geometric_mesh()
+    n = len(v) // 3
+    cen = v.reshape(n, 3, 3).mean(1)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:2] = cen[:, 0:2]
+    rays[:, 2] = 1.0
+    rays[:, 3] = 0.0
+    rays[:, 6] = -1.0
+    rays[:, 7] = 10.0
+    osc = oracle.scene_from_workload(w)
+    osc.use_bvh(False)
+    ref = osc.intersect(rays)
+    sc = gpu_scene(w)
+    sc.stats(True)
+    r = torch.from_numpy(rays).cuda()
+    h = torch.zeros((n, 4), dtype=torch.int32, device="cuda")
+    sc.intersect(r, h)
+    torch.cuda.synchronize()
+    got = h.cpu().numpy()
+    assert (ref[:, 0] == 0).sum() >= n - 2, "rays should hit the mesh"
+    np.testing.assert_array_equal(got, ref)
+    st = sc.stats(False)
+    assert st["stack_overflows"] == 0
+    info = sc.info()
+    print("bvh depth", info["bvh_depth"])
+    assert info["bvh_depth"] <= 64

@@ -20,7 +20,10 @@
   only, and a node shrunk to a point, written without its half-vector frame;
 - the world_to_cam derivative of projected segments (camera.h:244-313) with
   one end behind the near plane, pinhole and fisheye, against central
-  differences of the projection itself.
+  differences of the projection itself;
+- the derivative with respect to an emitter's own position (a light quad
+  lighting a triangle): the sum of the interior, secondary-edge and
+  primary-edge terms against central differences of the rendered image.
 
 Three reference behaviours were found by these pins and are reproduced in
 parity mode, each behind a quirk bit of the render flags: the glossy branch
@@ -30,7 +33,11 @@ derivative), the side-hit Jacobian assumes rays from the world origin
 under-weights edges (DR_QUIRK_CONE_PMF).  With the bits clear the estimator
 converges to the quadrature.  The clip derivative pin found a fourth: the
 reference differentiates another near-plane clip than the one it draws
-(DR_QUIRK_CLIP_DERIVATIVE).
+(DR_QUIRK_CLIP_DERIVATIVE).  The emitter pin found a fifth: the reference's
+edge samples count an emitter's own outline with its whole emission, while
+the light-sampled share of that light is an area integral whose derivative
+the interior adjoint already carries (DR_QUIRK_EDGE_EMITTER_FULL: about
+twice the derivative).
 """
 import ctypes
 
@@ -566,3 +573,48 @@ def test_clipped_projection_derivative(oracle, fisheye):
         # the quirk differentiates another clip: visibly different here
         assert np.linalg.norm(got[1] - fd) > 5 * max(err, 1e-3)
     assert checked == 12
+
+
+# ------------------------------------------------- an emitter's outline ----
+def test_emitter_translation_matches_finite_differences(oracle):
+    """The single-triangle scene's light quad (a two-triangle emitter)
+    translated along x, y, z: d(sum of the image)/d(translation) from the
+    adjoint (interior + secondary + primary edges, summed over the light's
+    vertices) against central differences of the forward render with the
+    same random numbers.  Exact mode: within 1 % of the difference quotient;
+    with the reference's whole-emission outline (DR_QUIRK_EDGE_EMITTER_FULL)
+    the secondary-edge term counts the outline again (about twice)."""
+    res, spp, seeds, h, which = 64, 32, range(31, 39), 0.03, 1
+    w = workloads.single_triangle(resolution=(res, res))
+    d = np.ones((res, res, 3), np.float32)
+
+    def shifted(t):
+        shapes, lights = list(w["shapes"]), []
+        s = shapes[which]
+        ns = dr.Shape((np.asarray(s.vertices) + t).astype(np.float32), s.indices, s.uvs, s.normals, s.material,
+                      None)
+        for l in w["lights"]:
+            sh = ns if l.shape is s else l.shape
+            nl = dr.Light(sh, l.intensity)
+            sh.light = nl
+            lights.append(nl)
+        shapes[which] = ns
+        return dict(w, shapes=shapes, lights=lights)
+
+    def loss(t):
+        o = oracle.scene_from_workload(shifted(t))
+        return sum(float(o.render(res, res, spp, 1, sd, flags=0)[0].astype(np.float64).sum()) for sd in seeds)
+
+    fd = np.array([(loss(h * np.eye(3)[k]) - loss(-h * np.eye(3)[k])) / (2 * h) for k in range(3)])
+    osc = oracle.scene_from_workload(w)
+    lay = dr.SceneDesc(w["camera"], w["shapes"], w["materials"], w["lights"]).layout
+    off, shp = lay["shapes"][which]["vertices"]
+
+    def adjoint(extra):
+        g = sum(osc.render(res, res, spp, 1, sd, flags=extra, d_image=d)[1] for sd in seeds)
+        return g[off:off + int(np.prod(shp))].reshape(-1, 3).sum(0)
+    exact = adjoint(0)
+    ref = adjoint(_abi.DR_QUIRK_EDGE_EMITTER_FULL)
+    print("fd", fd, "exact", exact, "whole-emission outline", ref)
+    assert np.linalg.norm(exact - fd) < 1e-2 * np.linalg.norm(fd)
+    assert np.linalg.norm(ref - fd) > 0.5 * np.linalg.norm(fd)
