@@ -188,6 +188,7 @@ ENTRY_POINTS = [
     ("dr_scene_set_edge_weights", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("dr_scene_edge_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     ("dr_scene_set_stat_shapes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    ("dr_scene_build_bvh_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("dr_mlp_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     ("dr_mlp_destroy", None, [ctypes.c_void_p]),
     ("dr_mlp_params", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -2,7 +2,7 @@
 
     python -m delta_ray_amd.build_lib        # or delta_ray_amd.build_lib.build()
 
-Compiles csrc/render.hip, csrc/mlp.hip and csrc/refit.hip with hipcc,
+Compiles csrc/render.hip, csrc/mlp.hip, csrc/refit.hip and csrc/lbvh.hip with hipcc,
 csrc/scene_build.cpp, csrc/load_serialized.cpp and csrc/ltc_fit.cpp with the
 host C++ compiler
 and csrc/ltc_blob.S (the embedded LTC table), then links the shared library
@@ -52,6 +52,7 @@ def build(verbose=False):
         "render": os.path.join(BUILD, "render.o"),
         "mlp": os.path.join(BUILD, "mlp.o"),
         "refit": os.path.join(BUILD, "refit.o"),
+        "lbvh": os.path.join(BUILD, "lbvh.o"),
         "scene": os.path.join(BUILD, "scene_build.o"),
         "serialized": os.path.join(BUILD, "load_serialized.o"),
         "ltc_fit": os.path.join(BUILD, "ltc_fit.o"),
@@ -61,6 +62,7 @@ def build(verbose=False):
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "render.hip"), "-o", objs["render"]],
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "mlp.hip"), "-o", objs["mlp"]],
         [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "refit.hip"), "-o", objs["refit"]],
+        [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "lbvh.hip"), "-o", objs["lbvh"]],
         [cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene_build.cpp"), "-o", objs["scene"]],
         [cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "load_serialized.cpp"), "-o", objs["serialized"]],
         [cxx] + CXX_FLAGS + ["-pthread", "-c", os.path.join(CSRC, "ltc_fit.cpp"), "-o", objs["ltc_fit"]],
@@ -73,7 +75,7 @@ def build(verbose=False):
             sys.stdout.write(o)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp,
-          objs["render"], objs["mlp"], objs["refit"], objs["scene"], objs["serialized"], objs["ltc_fit"], objs["ltc"], "-lz", "-lpthread"])
+          objs["render"], objs["mlp"], objs["refit"], objs["lbvh"], objs["scene"], objs["serialized"], objs["ltc_fit"], objs["ltc"], "-lz", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -19,6 +19,7 @@
 // skipped).
 #include "../../include/delta_ray.h"
 #include "dr_sample.h"
+#include "lbvh.h"
 #include "refit.h"
 #include "scene_build.h"
 
@@ -2365,6 +2366,54 @@ dr_status dr_scene_edge_features(dr_scene *s, const dr_view *view, float *featur
     if (hipMemcpyAsync(features, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, hs) != hipSuccess ||
         hipStreamSynchronize(hs) != hipSuccess)
         return fail(DR_ERROR_DEVICE, "feature upload failed");
+    return DR_OK;
+}
+
+dr_status dr_scene_build_bvh_device(dr_scene *s, double *build_ms, void *stream) {
+    if (!s) return fail(DR_ERROR_INVALID_ARGUMENT, "null scene");
+    hipStream_t hs = (hipStream_t)stream;
+    dr_status st = sync_host_vertices(s);  // the structures are uploaded from the host copy
+    if (st != DR_OK) return st;
+    int n = (int)s->hs.tri_shape.size();
+    if (n < 2) {
+        if (build_ms) *build_ms = 0.0;
+        return DR_OK;  // a single leaf: the host tree is that already
+    }
+    drlbvh::Layout L = drlbvh::layout(n);
+    void *scratch = nullptr;
+    DrNode *dn = nullptr;
+    DrTri *dt = nullptr;
+    if (hipMalloc(&scratch, L.total) != hipSuccess || hipMalloc((void **)&dn, (size_t)(n - 1) * sizeof(DrNode)) != hipSuccess ||
+        hipMalloc((void **)&dt, (size_t)n * sizeof(DrTri)) != hipSuccess) {
+        (void)hipFree(scratch);
+        (void)hipFree(dn);
+        return fail(DR_ERROR_OUT_OF_MEMORY, "device BVH build buffers");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, hs);
+    hipError_t e = drlbvh::build((const float *)s->view.verts, (const int *)s->view.tri_vidx, n, dn, dt, scratch,
+                                 L.total, hs);
+    (void)hipEventRecord(e1, hs);
+    std::vector<DrNode> kn((size_t)n - 1);
+    std::vector<DrTri> tris((size_t)n);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(kn.data(), dn, kn.size() * sizeof(DrNode), hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(tris.data(), dt, tris.size() * sizeof(DrTri), hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    float ms = 0.f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    (void)hipFree(dn);
+    (void)hipFree(dt);
+    if (e != hipSuccess) return fail(DR_ERROR_DEVICE, "device BVH build failed");
+    std::string why = drh::adopt_device_bvh(s->hs, kn, tris);
+    if (!why.empty()) return fail(DR_ERROR_UNSUPPORTED, why);
+    if (!upload_structures(s)) return fail(DR_ERROR_OUT_OF_MEMORY, "device allocation or copy failed");
+    if (build_ms) *build_ms = ms;
     return DR_OK;
 }
 

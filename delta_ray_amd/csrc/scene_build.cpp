@@ -737,6 +737,49 @@ void tree_levels(int n, Children children, std::vector<int> &order, std::vector<
 }
 }  // namespace
 
+std::string adopt_device_bvh(HostScene &hs, const std::vector<DrNode> &built, std::vector<DrTri> &tris) {
+    int m = (int)built.size();
+    if (m == 0) return "empty device BVH";
+    // relabel in preorder (a parent before its children, the left subtree
+    // before the right one), as the host builder lays nodes out
+    std::vector<int> map(m, -1), order;
+    order.reserve(m);
+    std::vector<int> stack = {0};
+    while (!stack.empty()) {
+        int i = stack.back();
+        stack.pop_back();
+        if (i < 0 || i >= m || map[i] >= 0) return "device BVH is not a tree";
+        map[i] = (int)order.size();
+        order.push_back(i);
+        for (int sl = 1; sl >= 0; sl--)
+            if (built[i].d[2 + sl] == 0) stack.push_back(built[i].d[sl]);
+    }
+    if ((int)order.size() != m) return "device BVH is not a tree";
+    std::vector<DrNode> nodes(m);
+    for (int k = 0; k < m; k++) {
+        DrNode N = built[order[k]];
+        for (int sl = 0; sl < 2; sl++)
+            if (N.d[2 + sl] == 0) N.d[sl] = map[N.d[sl]];
+        nodes[k] = N;
+    }
+    std::vector<int> lv, off;
+    tree_levels(m, [&](int i, int *c) {
+        int k = 0;
+        for (int sl = 0; sl < 2; sl++)
+            if (nodes[i].d[2 + sl] == 0 && nodes[i].d[sl] > 0) c[k++] = nodes[i].d[sl];
+        return k;
+    }, lv, off);
+    // the ray traversal stack holds at most one far child per interior level
+    int levels = (int)off.size() - 1;
+    if (levels > DR_STACK) return "device BVH deeper than the traversal stack";
+    hs.nodes.swap(nodes);
+    hs.tris.swap(tris);
+    hs.bvh_lv.swap(lv);
+    hs.bvh_lv_off.swap(off);
+    hs.bvh_depth = levels;
+    return "";
+}
+
 // Triangle BVH, mesh edges (reference scene.cpp:54-97), edge trees, their
 // per-edge geometry records, the primary-edge distribution and the refit
 // tables, from the shapes' current vertices.

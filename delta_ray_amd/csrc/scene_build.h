@@ -63,6 +63,11 @@ std::string build_host_scene(const dr_scene_desc *desc, HostScene &hs);
 void build_lights(HostScene &hs);
 std::string build_structures(HostScene &hs);
 void build_geo_records(HostScene &hs);
+// A triangle BVH built on the device (internal nodes, root 0, in any order;
+// tris in leaf order) in place of the host-built one: relabelled in
+// preorder, with refit levels.  Returns an empty string, or why it was
+// refused (the host tree is then kept).
+std::string adopt_device_bvh(HostScene &hs, const std::vector<DrNode> &built, std::vector<DrTri> &tris);
 DrCamera make_camera(const dr_camera &c);
 void set_camera(HostScene &hs, const dr_camera &c);
 // Primary-edge list and distribution (pmf proportional to the clipped screen

@@ -295,6 +295,14 @@ class DeviceScene:
             self.lib.dr_scene_destroy(h)
             self.handle = None
 
+    def build_bvh_device(self, stream=None):
+        """Replace the triangle BVH with one built on the device
+        (dr_scene_build_bvh_device); returns the device build's time in ms."""
+        ms = ctypes.c_double(0.0)
+        _abi.check(self.lib, self.lib.dr_scene_build_bvh_device(self.handle, ctypes.byref(ms), _stream_ptr(stream)),
+                   "dr_scene_build_bvh_device")
+        return ms.value
+
     def info(self):
         i = _abi.dr_scene_info()
         _abi.check(self.lib, self.lib.dr_scene_get_info(self.handle, ctypes.byref(i)), "dr_scene_get_info")

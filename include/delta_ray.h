@@ -394,6 +394,18 @@ typedef struct dr_stats {
 } dr_stats;
 dr_status dr_scene_stats(dr_scene *scene, int enable, dr_stats *out);
 
+/* Replace the scene's triangle BVH (built on the host, binned SAH, up to 4
+ * triangles per leaf) with one built on the device: a linear BVH (Karras,
+ * HPG 2012: Morton codes of the triangles' box centres, radix sort, binary
+ * radix tree, boxes bottom up), one triangle per leaf.  Ray hits are the
+ * same (closest t, ties to the smaller triangle id); traversal costs differ.
+ * Refits keep it; a host rebuild (the edge set changed) goes back to the
+ * host builder.  Structures derived from the BVH are re-uploaded, so call it
+ * before setting learned edge weights.  build_ms (may be NULL): the device
+ * build's time.  No reference counterpart (the reference builds Embree's BVH
+ * on the host, scene.cpp:100-135).  Synchronises the stream. */
+dr_status dr_scene_build_bvh_device(dr_scene *scene, double *build_ms, void *stream);
+
 /* Scene sizes (host). */
 typedef struct dr_scene_info {
     int num_triangles;
