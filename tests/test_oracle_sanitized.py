@@ -55,7 +55,10 @@ def test_oracle_under_sanitizers(tmp_path):
     with open(so, "rb") as f:
         blob = f.read()
     assert b"__asan_" in blob and b"__ubsan_" in blob, "the oracle was built without the sanitizers"
-    env = dict(os.environ, LD_PRELOAD=rt, DR_ORACLE_SO=so,
+    # the sanitizer runtime first, ahead of whatever the environment already
+    # preloads (kept as it is)
+    pre = os.environ.get("LD_PRELOAD", "")
+    env = dict(os.environ, LD_PRELOAD=rt + (":" + pre if pre else ""), DR_ORACLE_SO=so,
                ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:exitcode=23",
                UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1:exitcode=24")
     r = subprocess.run([sys.executable, "-c", CHILD % ROOT], capture_output=True, text=True, timeout=900, env=env)
