@@ -336,7 +336,10 @@ struct PlainStack {
     }
 };
 
-#define DR_LDS_STACK 12  // entries per lane in LDS
+// entries per lane in LDS: 10 x 16 B x 256 lanes = 40 KB per workgroup, so
+// that four workgroups (4 waves per SIMD) share a CU's 160 KB; 12 entries
+// (48 KB, 3 waves per SIMD) made k_select 8 % slower
+#define DR_LDS_STACK 10
 #define DR_LDS_WORDS 4   // words per entry
 
 // Top DR_LDS_STACK entries of each lane in LDS, one word of an entry per
