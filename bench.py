@@ -84,6 +84,10 @@ def parse():
     ap.add_argument("--lv-eps", type=float, default=1e-2, help="learned: target relative RMSE")
     ap.add_argument("--lv-bench-renders", type=int, default=96,
                     help="renders per estimator of the learned-vs-plain leg of the default run")
+    ap.add_argument("--lv-blocks", type=int, default=1,
+                    help="learned: disjoint seed blocks of renders per estimator (each of --lv-renders, or "
+                         "--lv-bench-renders in the default and --table runs); variances pool the blocks' "
+                         "groups, and each block's ratios are reported too")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a kernel of known HBM traffic: "
                          "256 MiB read + 256 MiB written) to scale hardware byte counters")
@@ -284,7 +288,7 @@ def bench_batched(args):
 
 
 def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary", "full"), log="",
-                ablation=True, flags=0):
+                ablation=True, flags=0, blocks=1):
     """Time to a gradient relative RMSE of `eps`, learned edge proposal vs
     the plain one, on workload `w`.  Both estimators are unbiased, so for
     each one the per-render variance V of the gradient (sum over parameters)
@@ -313,7 +317,10 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
     DR_EDGE_STATS renders (statistics of the plain estimator, then of the
     learned mixture), MLP training and applying the weights.  Measured on
     the secondary-edge term, the one the proposal changes, and (terms) on
-    the full gradient (interior + secondary + primary edges).  `ablation`:
+    the full gradient (interior + secondary + primary edges).  `blocks`:
+    each estimator renders that many disjoint seed blocks of n renders, cut
+    into 16 groups each; V pools all blocks' groups, and each block's own
+    variance ratios are reported beside it.  `ablation`:
     on the secondary-edge term, the same with weights from the per-edge
     statistics alone (no network), trained by the same rounds."""
     import numpy as np
@@ -348,6 +355,8 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
 
     groups = 16 if n >= 64 else 8
     n = n - n % groups
+    per_block = groups
+    groups = per_block * blocks
     rng = np.random.default_rng(0)
 
     def group_stats(a):
@@ -397,8 +406,8 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
     for term in terms:
         fl = secondary if term == "secondary" else 0
         prop.set_weights(wts)
-        gp, tp = renders(fl, range(1000, 1000 + n))
-        gl, tl = renders(fl | _abi.DR_LEARNED_EDGES, range(100000, 100000 + n))
+        gp, tp = renders(fl, range(1000, 1000 + blocks * n))
+        gl, tl = renders(fl | _abi.DR_LEARNED_EDGES, range(100000, 100000 + blocks * n))
         print("learned%s: %s term measured" % (log, term), file=sys.stderr, flush=True)
         vpg, mp = group_stats(gp)
         vlg, ml = group_stats(gl)
@@ -406,7 +415,7 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
         g2 = max(float((torch.cat([mp, ml]).median(0).values ** 2).sum()), 1e-30)
         t_plain = tp / 1e3 * vp / (eps ** 2 * g2)
         t_learned = tl / 1e3 * vl / (eps ** 2 * g2) + train_s
-        se = torch.sqrt(gp.var(0) / n + gl.var(0) / n)
+        se = torch.sqrt(gp.var(0) / gp.shape[0] + gl.var(0) / gl.shape[0])
         diff = (gp.mean(0) - gl.mean(0)).abs()
         act = se > 0
         out[term] = {"time_to_eps_s_plain": round(t_plain, 3), "time_to_eps_s_learned": round(t_learned, 3),
@@ -421,10 +430,14 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
                      "sq_error_p50_p90_p99_max_plain": tail(gp), "sq_error_p50_p90_p99_max_learned": tail(gl),
                      "mean_difference_over_se": round(float(diff.norm() / se.norm()), 3),
                      "params_beyond_4se": round(float((diff[act] / se[act] > 4.0).double().mean()), 5)}
+        if blocks > 1:
+            out[term]["variance_ratio_per_block"] = [
+                round(float(np.median(vpg[b * per_block:(b + 1) * per_block]) /
+                            np.median(vlg[b * per_block:(b + 1) * per_block])), 3) for b in range(blocks)]
         if term == "secondary" and ablation:
             # the per-edge statistics alone, no network: trained the same way
             _, _, _, ta_s = train(False)
-            ga, ta = renders(fl | _abi.DR_LEARNED_EDGES, range(200000, 200000 + n))
+            ga, ta = renders(fl | _abi.DR_LEARNED_EDGES, range(200000, 200000 + blocks * n))
             vag, _ = group_stats(ga)
             va = float(np.median(vag))
             out[term]["ablation_stats_only"] = {
@@ -435,8 +448,14 @@ def learned_leg(w, n, train_renders, train_steps, rounds, eps, terms=("secondary
                 "variance_ratio_stats_only_over_learned_10_90": ratio_ci(vag, vlg),
                 "sample_variance_stats_only": float("%.4g" % full_var(ga)),
                 "ms_per_render": round(ta, 3)}
+            if blocks > 1:
+                out[term]["ablation_stats_only"]["variance_ratio_plain_over_stats_only_per_block"] = [
+                    round(float(np.median(vpg[b * per_block:(b + 1) * per_block]) /
+                                np.median(vag[b * per_block:(b + 1) * per_block])), 3) for b in range(blocks)]
             print("learned%s: ablation measured" % log, file=sys.stderr, flush=True)
-    return {"workload": "%s %dx%d %dspp %d-bounce, %d renders per estimator" % (w["name"], W, H, spp, nb, n),
+    return {"workload": "%s %dx%d %dspp %d-bounce, %d renders per estimator" % (w["name"], W, H, spp, nb,
+                                                                                   blocks * n)
+            + (" (%d seed blocks of %d)" % (blocks, n) if blocks > 1 else ""),
             "eps": eps, "flags": "exact derivative" if flags == 0 else "0x%x" % flags,
             "gradient": "parameters " + ", ".join("%s %d %s" % p for p in w["params"]),
             "training_s": round(train_s, 3),
@@ -451,7 +470,8 @@ def bench_learned(args):
     --lv-spp with --lv-renders renders per estimator, as its own JSON line."""
     from delta_ray_amd import workloads
     w = workloads.heightfield(resolution=(args.lv_res, args.lv_res), num_samples=args.lv_spp)
-    r = learned_leg(w, args.lv_renders, args.lv_train_renders, args.lv_train_steps, args.lv_rounds, args.lv_eps)
+    r = learned_leg(w, args.lv_renders, args.lv_train_renders, args.lv_train_steps, args.lv_rounds, args.lv_eps,
+                    blocks=args.lv_blocks)
     sec = r["secondary"]
     res_line = {"metric": "time to gradient rel-RMSE 1e-2 (secondary-edge term), learned proposal",
                 "value": sec["time_to_eps_s_learned"], "unit": "s", "n_gpus": 1, "steps": args.lv_renders,
@@ -709,7 +729,8 @@ def bench_table(args):
     if not args.no_learned:
         w = workloads.heightfield()
         out["learned_vs_plain"] = learned_leg(w, args.lv_bench_renders, args.lv_train_renders, args.lv_train_steps,
-                                              args.lv_rounds, args.lv_eps, terms=("secondary",), log=" (C3)")
+                                              args.lv_rounds, args.lv_eps, terms=("secondary",), log=" (C3)",
+                                              blocks=args.lv_blocks)
     with open(args.table, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out), flush=True)
@@ -820,7 +841,7 @@ def main():
         print("bench: learned-vs-plain leg", file=sys.stderr, flush=True)
         out["extra"]["learned_vs_plain"] = learned_leg(w, args.lv_bench_renders, args.lv_train_renders,
                                                        args.lv_train_steps, args.lv_rounds, args.lv_eps,
-                                                       terms=("secondary",))
+                                                       terms=("secondary",), blocks=args.lv_blocks)
     print(json.dumps(out), flush=True)
 
 
