@@ -824,8 +824,9 @@ __global__ void k_intersect(DrSceneView S, const float *rays, long long n, int *
 // cosine lobe; (2) uniform, glossy; (3) the LTC line sampler, glossy.
 // out: per estimator the sum and the sum of squares (double).
 // dr_test_tree_stack: every lane runs `ops` random pushes and pops on the
-// selection kernel's LDS stack and on a plain array stack side by side;
-// out[0] counts pops that differ, out[1] the deepest stack reached.
+// selection kernel's LDS stack and on a plain array stack side by side (odd
+// waves with the alternative pmfs, even waves without); out[0] counts pops
+// that differ, out[1] the deepest stack reached.
 __global__ void __launch_bounds__(DR_BLOCK) k_tree_stack(long long ops, uint64_t seed, unsigned long long *out) {
     __shared__ float s_stack[DR_LDS_STACK * DR_LDS_WORDS * DR_BLOCK];
     LdsStack A;
@@ -838,8 +839,12 @@ __global__ void __launch_bounds__(DR_BLOCK) k_tree_stack(long long ops, uint64_t
     philox_init(r, seed, 4, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, 0);
     unsigned long long bad = 0;
     int deepest = 0;
+    // with or without the alternative pmfs for a lane's whole run, as a walk
+    // of the selection kernel keeps its proposal mode: an entry pushed
+    // without them must not be popped with them (the stacks' slots for them
+    // would hold whatever private memory held before)
+    const bool alt = (threadIdx.x >> 6) & 1;
     for (long long k = 0; k < ops; k++) {
-        bool alt = (k >> 6) & 1;
         float u = philox_next(r);
         bool push = B.size() == 0 || (B.size() < DR_TREE_STACK && u < 0.55f);
         if (push) {
